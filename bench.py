@@ -1,0 +1,193 @@
+"""Benchmark: energy-batched RGF + Keldysh on one MI355X.
+
+Workload (the one the headline metric is quoted on): N = 64 principal layers
+of b = 512, complex128, 256 energies on linspace(-1, 1), eta = 1e-4, lesser
+self-energies i f_{L,R}(E) Gamma_{L,R} with kT = 0.025, mu_L = +0.1,
+mu_R = -0.1.  One step = one pass over the whole energy grid: G^R and G^<
+diagonal blocks and T(E) for all 256 energies, outputs resident in HBM.
+Inputs (H, Gamma, energies) are generated by the seeded generator and moved
+to HBM before the timed region.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--wave 64]
+Prints one JSON line.  Extra fields: `roofline` for the dominant kernel (the
+FP64-MFMA batched ZGEMM, timed with HIP events on the stream it runs on, at
+the sweep's shape) and `cpu_baseline` (the scipy/OpenBLAS restatement of the
+reference's call sequence on the host cores, one energy, which also gives
+the sampled relative error of the GPU result against FP64 on the CPU).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "libnegf.jl_amd"))
+
+METRIC = "energies/s RGF+Keldysh b=512,N=64 @relerr≤1e-8 vs FP64; %MFMA peak"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--wave", type=int, default=0, help="energies per wave (0 = from free memory)")
+    ap.add_argument("--energies", type=int, default=256)
+    ap.add_argument("--N", type=int, default=64)
+    ap.add_argument("--b", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gemm-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def gemm_roofline(torch, api, b, batch, reps):
+    """Average duration of the batched FP64-MFMA ZGEMM at the sweep's shape
+    (b x b x b, `batch` instances, one operand broadcast like the shared
+    coupling blocks), timed with HIP events on the launching stream."""
+    from libnegf_amd._lib import call, cplx
+    s = torch.cuda.current_stream()
+    A = torch.randn((b * b,), dtype=torch.complex128, device="cuda")
+    B = torch.randn((batch, b * b), dtype=torch.complex128, device="cuda")
+    D = torch.empty((batch, b * b), dtype=torch.complex128, device="cuda")
+    args = lambda: (b"N", b"N", b, b, b, cplx(1), C.c_void_p(A.data_ptr()), b, 0,
+                    C.c_void_p(B.data_ptr()), b, b * b, cplx(0), C.c_void_p(D.data_ptr()), b,
+                    b * b, batch, C.c_void_p(s.cuda_stream))
+    for _ in range(3):
+        call("negf_zgemm_strided_batched", *args())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        call("negf_zgemm_strided_batched", *args())
+    e1.record(s)
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    flops = 8.0 * b ** 3 * batch
+    ach = flops / dt / 1e12
+    return {"kernel": "zgemm_mfma_kernel<N,N> (FP64 MFMA 16x16x4)", "bound": "mfma",
+            "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "launch": f"{b}x{b}x{b} complex128 x {batch}", "avg_launch_us": round(dt * 1e6, 1),
+            "flops_per_launch": flops}
+
+
+def cpu_baseline(dev, E, eta, kT, muL, muR, gpu_out):
+    """One energy of the oracle (the reference's call sequence through scipy's
+    OpenBLAS 0.3.29) on the host cores; also the relative error of the GPU
+    result for the same energy."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, str(ROOT))
+    from oracle.rgf import bndiag_of_inv_rgf_local, rgf_upward, first_row_chain
+    from oracle.keldysh import keldyshndiag
+    from libnegf_amd.synthetic import assemble_A, fermi
+    cores = min(os.cpu_count() or 1, 16)
+    with threadpool_limits(limits=cores, user_api="blas"):
+        t0 = time.perf_counter()
+        A = assemble_A(dev, E, eta)
+        buf = rgf_upward(A)
+        G = bndiag_of_inv_rgf_local(A, upward=buf)
+        N, b = dev.nblocks, dev.b
+        Sd = [np.zeros((b, b), complex, order="F") for _ in range(N)]
+        Sd[0] = np.asfortranarray(fermi(E, muL, kT) * dev.gamma_L)
+        Sd[-1] = np.asfortranarray(Sd[-1] + fermi(E, muR, kT) * dev.gamma_R)
+        Su = [np.zeros((b, b), complex, order="F") for _ in range(N - 1)]
+        gd, _ = keldyshndiag(A, Sd, Su, upward=buf)
+        g1n = first_row_chain(A, G, buf)[-1]
+        T = float(np.real(np.trace(dev.gamma_L @ g1n @ dev.gamma_R @ g1n.conj().T)))
+        dt = time.perf_counter() - t0
+    gr, gl, tg = gpu_out
+    err = 0.0
+    for i in range(N):
+        blk = gr[i * b * b:(i + 1) * b * b].reshape(b, b).T
+        err = max(err, np.linalg.norm(blk - G[(i, i)]) / np.linalg.norm(G[(i, i)]))
+        blk = gl[i * b * b:(i + 1) * b * b].reshape(b, b).T
+        ref = 1j * gd[i]
+        err = max(err, np.linalg.norm(blk - ref) / max(np.linalg.norm(ref), 1e-300))
+    terr = abs(tg - T) / max(abs(T), 1e-300)
+    return ({"value": round(1.0 / dt, 5), "unit": "energies/s", "cores": cores, "kind": "port",
+             "sample": f"1 energy (E={E:+.4f}) of the same workload: RGF + RKD + T through "
+                       f"scipy/OpenBLAS 0.3.29, {dt:.1f} s"},
+            {"max_block_relerr": float(err), "transmission_relerr": float(terr), "energy": E})
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); energies shard "
+              "across GPUs with no communication, see the design notes", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from libnegf_amd import api, synthetic, flops_rgf_keldysh_energy
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
+    N, b, nE = a.N, a.b, a.energies
+    eta, kT, muL, muR = cfg["eta"], cfg["kT"], cfg["mu_L"], cfg["mu_R"]
+    dev = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
+    Es = synthetic.energy_grid(nE, cfg["lo"], cfg["hi"])
+    H = api.DeviceHamiltonianGPU(dev)
+    E_dev = torch.tensor(Es, dtype=torch.float64, device="cuda")
+    GR = torch.empty((nE, N * b * b), dtype=torch.complex128, device="cuda")
+    GL = torch.empty_like(GR)
+    T = torch.empty(nE, dtype=torch.float64, device="cuda")
+    free, total = torch.cuda.mem_get_info()
+    wave = a.wave
+    if wave == 0:
+        wave = min(nE, 64)
+        while wave > 1 and api.energies_workspace_bytes(N, b, nE, wave) > 0.85 * free:
+            wave //= 2
+    print(f"# N={N} b={b} energies={nE} wave={wave} free={free/2**30:.1f}GiB", file=sys.stderr,
+          flush=True)
+
+    def step():
+        api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave, check=False)
+
+    for w in range(a.warmup):
+        step()
+        torch.cuda.synchronize()
+        print(f"# warmup {w} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step()
+        torch.cuda.synchronize()
+        print(f"# step {k} done {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    eps = nE / dt
+    fl = flops_rgf_keldysh_energy(N, b)
+    roof = gemm_roofline(torch, api, b, wave, a.gemm_reps)
+    roof["path_tflops"] = round(eps * fl / 1e12, 3)
+    roof["path_frac_of_fp64_peak"] = round(eps * fl / 1e12 / FP64_PEAK_TFLOPS, 4)
+    info = torch.zeros(1)
+    cpu = None
+    acc = None
+    if not a.no_cpu_baseline:
+        k = nE // 2 + nE // 8
+        gpu_out = (GR[k].cpu().numpy(), GL[k].cpu().numpy(), float(T[k].item()))
+        cpu, acc = cpu_baseline(dev, float(Es[k]), eta, kT, muL, muR, gpu_out)
+    out = {
+        "metric": METRIC, "value": round(eps, 4), "unit": "energies/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (f64)",
+        "data": "synthetic",
+        "config": {"workload": f"rgf_keldysh_batched: N={N} b={b} energies={nE} "
+                               f"linspace(-1,1) eta={eta} kT={kT} muL={muL} muR={muR}",
+                   "form": "plain FP64", "wave": wave, "flops_per_energy": fl},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy_vs_fp64_cpu": acc,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * negf_hip.h -- C ABI of the MI355X (gfx950) block-tridiagonal NEGF library.
+ *
+ * This is the drop-in boundary for LibNEGF.jl's compute backend: a new
+ * src/backend_amd.jl ccalls these entry points where backend_cpu.jl calls
+ * LAPACK/BLAS, and the sweep-level entry points replace whole
+ * matinvertndiag_* / keldyshndiag calls.  Every citation below is to the
+ * reference file and line whose behaviour the entry point reproduces.
+ *
+ * Conventions
+ *   - complex128 = two doubles (re, im), i.e. C99 `double complex` / Julia
+ *     ComplexF64, column-major, exactly the reference's Matrix{ComplexF64}.
+ *   - All matrix pointers are DEVICE pointers; `stream` is a hipStream_t
+ *     (NULL = default stream).  Calls are asynchronous on that stream.
+ *   - `*_strided_batched`: instance k of an operand lives at ptr + k*stride
+ *     (elements); stride 0 broadcasts one operand to every instance.
+ *   - Return value: NEGF_OK (0) or a negative NEGF_* error.  Numerical
+ *     singularity is reported per instance through a device `info` array,
+ *     LAPACK style: 0 = ok, k > 0 = the 1-based column (per-op) or layer
+ *     (sweeps) of the first exactly-zero pivot.  The reference prints and
+ *     calls exit() there (src/backend_cpu.jl:151-155); the Julia shim maps
+ *     info > 0 to that behaviour.
+ *
+ * Packed block-tridiagonal layout (the sweeps): N blocks of sizes b_0..b_{N-1}
+ *   diag  : A_ii        (b_i x b_i)      at element offset sum_{j<i} b_j^2
+ *   upper : A_{i,i+1}   (b_i x b_{i+1})  at element offset sum_{j<i} b_j b_{j+1}
+ *   lower : A_{i+1,i}   (b_{i+1} x b_i)  same offsets as upper
+ * every block column-major with leading dimension = its row count.  This is
+ * BlockMatrix with ndiag = 3 (src/blockmatrix.jl:10-17) flattened into three
+ * device arrays; `lower` = NULL means Hermitian storage (isHermitian = true,
+ * A_{i+1,i} = A_{i,i+1}^H, src/blockmatrix.jl:105-109).  Outputs use the
+ * same layout with tight batch strides (nd = sum b_i^2, no = sum b_i b_{i+1}).
+ */
+#ifndef NEGF_HIP_H
+#define NEGF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEGF_OK 0
+#define NEGF_BAD_ARG (-1)
+#define NEGF_TOO_LARGE (-2)
+#define NEGF_HIP_ERROR (-3)
+#define NEGF_NO_MEMORY (-4)
+
+/* Library version (major*10000 + minor*100 + patch). */
+int negf_version(void);
+/* Static description of a status code. */
+const char* negf_status_string(int status);
+/* Frees the cached sweep workspaces (the next sweep reallocates). */
+int negf_workspace_release(void);
+
+/* ------------------------------------------------------------------ per-op */
+
+/* C = alpha op(A) op(B) + beta C, op in {'N','T','C'} (zgemm).
+ * Replaces be_gemm! (src/backend_cpu.jl:208-219). */
+int negf_zgemm(char transA, char transB, int32_t m, int32_t n, int32_t k,
+               const double alpha[2], const void* A, int64_t lda,
+               const void* B, int64_t ldb, const double beta[2],
+               void* C, int64_t ldc, void* stream);
+int negf_zgemm_strided_batched(char transA, char transB, int32_t m, int32_t n, int32_t k,
+                               const double alpha[2], const void* A, int64_t lda, int64_t strideA,
+                               const void* B, int64_t ldb, int64_t strideB,
+                               const double beta[2], void* C, int64_t ldc, int64_t strideC,
+                               int32_t batch, void* stream);
+
+/* LU with partial pivoting in place, LAPACK layout; ipiv (device, n int32,
+ * 1-based) and info (device, 1 int32).  Replaces the getrf inside be_lu!
+ * (src/backend_cpu.jl:147-169) and be_getrf! (:255-265, which discards ipiv). */
+int negf_zgetrf(int32_t n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* stream);
+int negf_zgetrf_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA,
+                                int32_t* ipiv, int64_t strideP, int32_t* info,
+                                int32_t batch, void* stream);
+
+/* B <- op(LU)^{-1} B, trans in {'N','T','C'} (zgetrs).
+ * Replaces be_mldivide! (src/backend_cpu.jl:193-206; the copy Min->Mout is
+ * the caller's negf_zcopy). */
+int negf_zgetrs(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
+                const int32_t* ipiv, void* B, int64_t ldb, void* stream);
+int negf_zgetrs_strided_batched(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
+                                int64_t strideA, const int32_t* ipiv, int64_t strideP,
+                                void* B, int64_t ldb, int64_t strideB, int32_t batch,
+                                void* stream);
+
+/* Triangular solve with n or m right-hand sides (ztrsm).
+ * Replaces be_trsm! (src/backend_cpu.jl:241-253). */
+int negf_ztrsm(char side, char uplo, char transA, char diag, int32_t m, int32_t n,
+               const double alpha[2], const void* A, int64_t lda, void* B, int64_t ldb,
+               void* stream);
+int negf_ztrsm_strided_batched(char side, char uplo, char transA, char diag, int32_t m,
+                               int32_t n, const double alpha[2], const void* A, int64_t lda,
+                               int64_t strideA, void* B, int64_t ldb, int64_t strideB,
+                               int32_t batch, void* stream);
+
+/* Inverse from the LU factors in place (zgetri); `work` is a device buffer of
+ * n*n complex128 per instance.  ipiv = NULL means identity pivots, which is
+ * be_getri! (src/backend_cpu.jl:229-239). */
+int negf_zgetri(int32_t n, void* A, int64_t lda, const int32_t* ipiv, void* work, void* stream);
+int negf_zgetri_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA,
+                                const int32_t* ipiv, int64_t strideP, void* work,
+                                int32_t batch, void* stream);
+
+/* Explicit inverse by Gauss-Jordan with partial pivoting: Ainv = A^{-1}
+ * (A untouched).  The AMD backend's be_lu! stores this inverse in its LU
+ * holder, so be_mldivide! becomes one negf_zgemm (src/backend_cpu.jl:147-206);
+ * also be_inv (:143-145). */
+int negf_zinv_strided_batched(int32_t n, const void* A, int64_t lda, int64_t strideA,
+                              void* Ainv, int64_t ldinv, int64_t strideInv, int32_t batch,
+                              int32_t* info, void* stream);
+
+/* B = A^H (m x n -> n x m).  Replaces be_ctranspose! (src/backend_cpu.jl:58-68). */
+int negf_zadjoint_strided_batched(int32_t m, int32_t n, const void* A, int64_t lda,
+                                  int64_t strideA, void* B, int64_t ldb, int64_t strideB,
+                                  int32_t batch, void* stream);
+/* B = A.  Replaces be_copy_in_hw! (src/backend_cpu.jl:42-44). */
+int negf_zcopy_strided_batched(int32_t m, int32_t n, const void* A, int64_t lda,
+                               int64_t strideA, void* B, int64_t ldb, int64_t strideB,
+                               int32_t batch, void* stream);
+/* A[i,j] = (i == j ? diag_value : value).  be_fill! (value = diag_value),
+ * be_zero_array, be_identity (src/backend_cpu.jl:50-56, 86-88). */
+int negf_zfill_strided_batched(int32_t m, int32_t n, const double value[2],
+                               const double diag_value[2], void* A, int64_t lda,
+                               int64_t strideA, int32_t batch, void* stream);
+
+/* ------------------------------------------------------------------ sweeps */
+
+/* band(A^{-1}) of a block-tridiagonal A by RGF, batched over `batch`
+ * instances of identical block sizes (host array block_sizes[nblocks]).
+ * Outputs (may be NULL to skip): g_diag (nd per instance), g_upper / g_lower
+ * (no per instance).  info: device int32[batch] or NULL.
+ * Replaces bndiag_of_inv_rgf_local! (src/matinvertndiag_rgf.jl:114-200). */
+int negf_rgf(int32_t nblocks, const int32_t* block_sizes,
+             const void* diag, int64_t diag_bstride,
+             const void* upper, int64_t upper_bstride,
+             const void* lower, int64_t lower_bstride,
+             void* g_diag, void* g_upper, void* g_lower,
+             int32_t batch, int32_t* info, void* stream);
+
+/* Recursive Keldysh: band(A^{-1} Sigma A^{-H}) for a Hermitian
+ * block-tridiagonal Sigma given in Hermitian storage (diag + upper); output
+ * gn_diag / gn_upper in the same storage.  The optional RGF outputs reuse
+ * the shared upward pass; gamma_L / gamma_R (b_0 x b_0, b_{N-1} x b_{N-1},
+ * may be NULL) add T = Re Tr(Gamma_L G_{1N} Gamma_R G_{1N}^H) into
+ * transmission[batch] (device).
+ * Replaces keldyshndiag! (src/keldyshndiag.jl:120-131 -> keldyshndiag_v3!
+ * :166-318) and, with g_* given, the preceding bndiag_of_inv_rgf_local!. */
+int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes,
+                 const void* diag, int64_t diag_bstride,
+                 const void* upper, int64_t upper_bstride,
+                 const void* lower, int64_t lower_bstride,
+                 const void* sigma_diag, int64_t sigma_diag_bstride,
+                 const void* sigma_upper, int64_t sigma_upper_bstride,
+                 void* g_diag, void* g_upper, void* g_lower,
+                 void* gn_diag, void* gn_upper,
+                 const void* gamma_L, int64_t gamma_L_bstride,
+                 const void* gamma_R, int64_t gamma_R_bstride,
+                 double* transmission,
+                 int32_t batch, int32_t* info, void* stream);
+
+/* Energy-batched NEGF workload on one device Hamiltonian (no reference
+ * counterpart; the reference runs one energy per call): uniform blocks of
+ * size b, H_diag (N blocks, Hermitian), H_upper (N-1 blocks H_{i,i+1}),
+ * contact broadenings gamma_L, gamma_R (b x b, wide band).  For every energy
+ * E_e (device array) with A(E) = (E + i eta) I - H + (i/2)(Gamma_L (+) Gamma_R):
+ *   GR_diag   : diagonal blocks of G^R = A^{-1}        (nE * N*b*b, or NULL)
+ *   Gless_diag: diagonal blocks of G^< = G^R Sigma^< G^A with
+ *               Sigma^< = i f_L Gamma_L (+) i f_R Gamma_R   (or NULL)
+ *   transmission: T(E) = Tr(Gamma_L G^R_{1N} Gamma_R G^A_{N1})  (or NULL)
+ *   f(E) = 1 / (1 + exp((E - mu) / kT)).
+ * Energies run in waves of `wave` (0 = pick from free memory).  info:
+ * device int32[nE] or NULL (layer of the first singular Schur complement). */
+int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
+                              const void* H_diag, const void* H_upper,
+                              const void* gamma_L, const void* gamma_R,
+                              const double* energies, int32_t n_energies,
+                              double eta, double mu_L, double mu_R, double kT,
+                              void* GR_diag, void* Gless_diag, double* transmission,
+                              int32_t* info, int32_t wave, void* stream);
+
+/* Bytes of device workspace the energy-batched call uses for a given wave
+ * (for sizing `wave` against free memory). */
+int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
+                                            int32_t wave);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEGF_HIP_H */

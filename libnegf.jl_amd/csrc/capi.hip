@@ -1,0 +1,326 @@
+// extern "C" entry points declared in include/negf_hip.h: argument checks,
+// workspace management, and dispatch to the gfx950 kernels.
+#include "../../include/negf_hip.h"
+#include "negf_internal.hpp"
+#include "sweeps.hpp"
+
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+using namespace negf;
+
+namespace {
+
+inline zc zz(const double* v) { return v ? zmake(v[0], v[1]) : zmake(0.0, 0.0); }
+inline hipStream_t st(void* s) { return (hipStream_t)s; }
+
+// Cached device workspace per (device, stream); grows on demand.
+struct WsKey {
+  int dev;
+  void* stream;
+  int tag;   // 0: sweep / inverse workspace, 1: identity pivots
+  bool operator<(const WsKey& o) const {
+    if (dev != o.dev) return dev < o.dev;
+    if (stream != o.stream) return stream < o.stream;
+    return tag < o.tag;
+  }
+};
+struct WsBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<WsKey, WsBuf> g_ws;
+
+int workspace(void* stream, size_t bytes, void** out, int tag = 0) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return NEGF_HIP_ERROR;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  WsBuf& b = g_ws[WsKey{dev, stream, tag}];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      if (hipStreamSynchronize(st(stream)) != hipSuccess) return NEGF_HIP_ERROR;
+      (void)hipFree(b.p);
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      return NEGF_NO_MEMORY;
+    }
+    b.bytes = bytes;
+  }
+  *out = b.p;
+  return NEGF_OK;
+}
+
+bool valid_op(char c) { return c == 'N' || c == 'T' || c == 'C' || c == 'n' || c == 't' || c == 'c'; }
+int opc(char c) { return (c == 'N' || c == 'n') ? OP_N : ((c == 'T' || c == 't') ? OP_T : OP_C); }
+
+bool valid_sizes(int32_t nblocks, const int32_t* sizes) {
+  if (nblocks <= 0 || !sizes) return false;
+  for (int i = 0; i < nblocks; ++i)
+    if (sizes[i] <= 0 || sizes[i] > 2048) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int negf_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+const char* negf_status_string(int status) {
+  switch (status) {
+    case NEGF_OK: return "ok";
+    case NEGF_BAD_ARG: return "invalid argument";
+    case NEGF_TOO_LARGE: return "problem size beyond the supported limits";
+    case NEGF_HIP_ERROR: return "HIP runtime error";
+    case NEGF_NO_MEMORY: return "out of device memory";
+  }
+  return status > 0 ? "singular pivot" : "unknown status";
+}
+
+int negf_workspace_release(void) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : g_ws)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  g_ws.clear();
+  return NEGF_OK;
+}
+
+// ------------------------------------------------------------------- per-op
+int negf_zgemm_strided_batched(char transA, char transB, int32_t m, int32_t n, int32_t k,
+                               const double alpha[2], const void* A, int64_t lda, int64_t strideA,
+                               const void* B, int64_t ldb, int64_t strideB, const double beta[2],
+                               void* C, int64_t ldc, int64_t strideC, int32_t batch,
+                               void* stream) {
+  if (!valid_op(transA) || !valid_op(transB) || !alpha || !beta) return NEGF_BAD_ARG;
+  GemmArgs g{};
+  g.opA = opc(transA); g.opB = opc(transB);
+  g.m = m; g.n = n; g.k = k;
+  g.alpha = zz(alpha); g.beta = zz(beta);
+  g.A = (const zc*)A; g.lda = lda; g.strideA = strideA;
+  g.B = (const zc*)B; g.ldb = ldb; g.strideB = strideB;
+  g.C = (const zc*)C; g.ldc = ldc; g.strideC = strideC;
+  g.D = (zc*)C; g.ldd = ldc; g.strideD = strideC;
+  g.batch = batch;
+  return zgemm_launch(g, st(stream));
+}
+
+int negf_zgemm(char transA, char transB, int32_t m, int32_t n, int32_t k, const double alpha[2],
+               const void* A, int64_t lda, const void* B, int64_t ldb, const double beta[2],
+               void* C, int64_t ldc, void* stream) {
+  return negf_zgemm_strided_batched(transA, transB, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C,
+                                    ldc, 0, 1, stream);
+}
+
+int negf_zgetrf_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA, int32_t* ipiv,
+                                int64_t strideP, int32_t* info, int32_t batch, void* stream) {
+  if (!A || !ipiv || !info) return n == 0 || batch == 0 ? NEGF_OK : NEGF_BAD_ARG;
+  return zgetrf_launch(n, (zc*)A, lda, strideA, ipiv, strideP, info, batch, st(stream));
+}
+
+int negf_zgetrf(int32_t n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* stream) {
+  return negf_zgetrf_strided_batched(n, A, lda, 0, ipiv, 0, info, 1, stream);
+}
+
+int negf_zgetrs_strided_batched(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
+                                int64_t strideA, const int32_t* ipiv, int64_t strideP, void* B,
+                                int64_t ldb, int64_t strideB, int32_t batch, void* stream) {
+  if (!valid_op(trans)) return NEGF_BAD_ARG;
+  if ((!A || !ipiv || !B) && n > 0 && nrhs > 0 && batch > 0) return NEGF_BAD_ARG;
+  return zgetrs_launch(trans, n, nrhs, (const zc*)A, lda, strideA, ipiv, strideP, (zc*)B, ldb,
+                       strideB, batch, st(stream));
+}
+
+int negf_zgetrs(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
+                const int32_t* ipiv, void* B, int64_t ldb, void* stream) {
+  return negf_zgetrs_strided_batched(trans, n, nrhs, A, lda, 0, ipiv, 0, B, ldb, 0, 1, stream);
+}
+
+int negf_ztrsm_strided_batched(char side, char uplo, char transA, char diag, int32_t m, int32_t n,
+                               const double alpha[2], const void* A, int64_t lda, int64_t strideA,
+                               void* B, int64_t ldb, int64_t strideB, int32_t batch,
+                               void* stream) {
+  if (!alpha) return NEGF_BAD_ARG;
+  if ((!A || !B) && m > 0 && n > 0 && batch > 0) return NEGF_BAD_ARG;
+  return ztrsm_launch(side, uplo, transA, diag, m, n, zz(alpha), (const zc*)A, lda, strideA,
+                      (zc*)B, ldb, strideB, batch, st(stream));
+}
+
+int negf_ztrsm(char side, char uplo, char transA, char diag, int32_t m, int32_t n,
+               const double alpha[2], const void* A, int64_t lda, void* B, int64_t ldb,
+               void* stream) {
+  return negf_ztrsm_strided_batched(side, uplo, transA, diag, m, n, alpha, A, lda, 0, B, ldb, 0, 1,
+                                    stream);
+}
+
+int negf_zgetri_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA,
+                                const int32_t* ipiv, int64_t strideP, void* work, int32_t batch,
+                                void* stream) {
+  if (n < 0 || batch < 0) return NEGF_BAD_ARG;
+  if (n == 0 || batch == 0) return NEGF_OK;
+  if (!A || !work) return NEGF_BAD_ARG;
+  const int32_t* ip = ipiv;
+  void* ident = nullptr;
+  if (!ip) {
+    // identity pivots (be_getri!): 1..n, shared by all instances
+    std::vector<int32_t> h(n);
+    for (int i = 0; i < n; ++i) h[i] = i + 1;
+    int rc = workspace(stream, sizeof(int32_t) * (size_t)n, &ident, 1);
+    if (rc) return rc;
+    if (hipMemcpyAsync(ident, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st(stream)) !=
+        hipSuccess)
+      return NEGF_HIP_ERROR;
+    if (hipStreamSynchronize(st(stream)) != hipSuccess) return NEGF_HIP_ERROR;
+    ip = (const int32_t*)ident;
+    strideP = 0;
+  }
+  return zgetri_launch(n, (zc*)A, lda, strideA, ip, strideP, batch, (zc*)work, st(stream));
+}
+
+int negf_zgetri(int32_t n, void* A, int64_t lda, const int32_t* ipiv, void* work, void* stream) {
+  return negf_zgetri_strided_batched(n, A, lda, 0, ipiv, 0, work, 1, stream);
+}
+
+int negf_zinv_strided_batched(int32_t n, const void* A, int64_t lda, int64_t strideA, void* Ainv,
+                              int64_t ldinv, int64_t strideInv, int32_t batch, int32_t* info,
+                              void* stream) {
+  if (n < 0 || batch < 0) return NEGF_BAD_ARG;
+  if (n == 0 || batch == 0) return NEGF_OK;
+  if (!A || !Ainv || !info) return NEGF_BAD_ARG;
+  if (n > 2048) return NEGF_TOO_LARGE;
+  void* w = nullptr;
+  int rc = workspace(stream, zinv_work_bytes(n, batch), &w);
+  if (rc) return rc;
+  InvWork iw;
+  zinv_work_carve(w, n, batch, &iw);
+  return zinv_launch(n, (const zc*)A, lda, strideA, (zc*)Ainv, ldinv, strideInv, batch, iw, info,
+                     st(stream));
+}
+
+int negf_zadjoint_strided_batched(int32_t m, int32_t n, const void* A, int64_t lda,
+                                  int64_t strideA, void* B, int64_t ldb, int64_t strideB,
+                                  int32_t batch, void* stream) {
+  if (lda < (m > 1 ? m : 1) || ldb < (n > 1 ? n : 1)) return NEGF_BAD_ARG;
+  return zadjoint_launch(m, n, (const zc*)A, lda, strideA, (zc*)B, ldb, strideB, batch,
+                         st(stream));
+}
+
+int negf_zcopy_strided_batched(int32_t m, int32_t n, const void* A, int64_t lda, int64_t strideA,
+                               void* B, int64_t ldb, int64_t strideB, int32_t batch,
+                               void* stream) {
+  if (lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1)) return NEGF_BAD_ARG;
+  return zcopy2d_launch(m, n, (const zc*)A, lda, strideA, (zc*)B, ldb, strideB, batch,
+                        st(stream));
+}
+
+int negf_zfill_strided_batched(int32_t m, int32_t n, const double value[2],
+                               const double diag_value[2], void* A, int64_t lda, int64_t strideA,
+                               int32_t batch, void* stream) {
+  if (!value || !diag_value || lda < (m > 1 ? m : 1)) return NEGF_BAD_ARG;
+  return zfill_launch(m, n, zz(value), zz(diag_value), (zc*)A, lda, strideA, batch, st(stream));
+}
+
+// ------------------------------------------------------------------- sweeps
+int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                 int64_t diag_bstride, const void* upper, int64_t upper_bstride, const void* lower,
+                 int64_t lower_bstride, const void* sigma_diag, int64_t sigma_diag_bstride,
+                 const void* sigma_upper, int64_t sigma_upper_bstride, void* g_diag,
+                 void* g_upper, void* g_lower, void* gn_diag, void* gn_upper,
+                 const void* gamma_L, int64_t gamma_L_bstride, const void* gamma_R,
+                 int64_t gamma_R_bstride, double* transmission, int32_t batch, int32_t* info,
+                 void* stream) {
+  if (!valid_sizes(nblocks, block_sizes) || batch < 0) return NEGF_BAD_ARG;
+  if (batch == 0) return NEGF_OK;
+  if (!diag || (nblocks > 1 && !upper)) return NEGF_BAD_ARG;
+  if ((g_upper || g_lower) && !g_diag) return NEGF_BAD_ARG;
+  const bool want_rkd = gn_diag != nullptr;
+  if (want_rkd && (!sigma_diag || (nblocks > 1 && (!sigma_upper || !gn_upper))))
+    return NEGF_BAD_ARG;
+  const bool want_T = transmission != nullptr;
+  if (want_T && (!gamma_L || !gamma_R)) return NEGF_BAD_ARG;
+  Layout L(nblocks, block_sizes);
+  void* w = nullptr;
+  // the per-sweep info accumulator lives behind the sweep workspace
+  const size_t sw = sweep_work_bytes(L, batch, want_rkd);
+  int rc = workspace(stream, sw + sizeof(int32_t) * (size_t)batch + 256, &w);
+  if (rc) return rc;
+  int32_t* inf = info ? info : (int32_t*)((char*)w + sw);
+  BtdIn A{(const zc*)diag, diag_bstride, (const zc*)upper, upper_bstride, (const zc*)lower,
+          lower_bstride};
+  BtdOut G{(zc*)g_diag, (zc*)g_upper, (zc*)g_lower};
+  BtdHerm S{(const zc*)sigma_diag, sigma_diag_bstride, (const zc*)sigma_upper,
+            sigma_upper_bstride};
+  BtdOut Gn{(zc*)gn_diag, (zc*)gn_upper, nullptr};
+  GammaIn gam{(const zc*)gamma_L, gamma_L_bstride, (const zc*)gamma_R, gamma_R_bstride};
+  return sweep_rgf(L, A, G, want_rkd ? &S : nullptr, want_rkd ? &Gn : nullptr,
+                   want_T ? &gam : nullptr, transmission, batch, inf, w, st(stream));
+}
+
+int negf_rgf(int32_t nblocks, const int32_t* block_sizes, const void* diag, int64_t diag_bstride,
+             const void* upper, int64_t upper_bstride, const void* lower, int64_t lower_bstride,
+             void* g_diag, void* g_upper, void* g_lower, int32_t batch, int32_t* info,
+             void* stream) {
+  if (!g_diag) return NEGF_BAD_ARG;
+  return negf_rgf_rkd(nblocks, block_sizes, diag, diag_bstride, upper, upper_bstride, lower,
+                      lower_bstride, nullptr, 0, nullptr, 0, g_diag, g_upper, g_lower, nullptr,
+                      nullptr, nullptr, 0, nullptr, 0, nullptr, batch, info, stream);
+}
+
+int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
+                                            int32_t wave) {
+  if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0) return 0;
+  std::vector<int32_t> sz(nblocks, b);
+  Layout L(nblocks, sz.data());
+  return (int64_t)(energies_work_bytes(L, n_energies, wave) + sizeof(zc) * (size_t)L.nd + 4096);
+}
+
+int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
+                              const void* H_upper, const void* gamma_L, const void* gamma_R,
+                              const double* energies, int32_t n_energies, double eta, double mu_L,
+                              double mu_R, double kT, void* GR_diag, void* Gless_diag,
+                              double* transmission, int32_t* info, int32_t wave, void* stream) {
+  if (nblocks <= 0 || b <= 0 || b > 2048 || n_energies < 0 || wave < 0) return NEGF_BAD_ARG;
+  if (n_energies == 0) return NEGF_OK;
+  if (!H_diag || (nblocks > 1 && !H_upper) || !gamma_L || !gamma_R || !energies)
+    return NEGF_BAD_ARG;
+  if (!(kT > 0.0)) return NEGF_BAD_ARG;
+  std::vector<int32_t> sz(nblocks, b);
+  Layout L(nblocks, sz.data());
+  if (wave == 0) {
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return NEGF_HIP_ERROR;
+    wave = n_energies < 64 ? n_energies : 64;
+    while (wave > 1 &&
+           (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave) >
+               freeb / 10 * 9)
+      wave /= 2;
+  }
+  if (wave > n_energies) wave = n_energies;
+  const size_t ew = energies_work_bytes(L, n_energies, wave);
+  void* w = nullptr;
+  int rc = workspace(stream, ew + sizeof(zc) * (size_t)L.nd + 4096, &w);
+  if (rc) return rc;
+  hipStream_t s = st(stream);
+  // H' = H - (i/2) Gamma on the contact layers, behind the sweep workspace
+  zc* Hp = (zc*)((char*)w + (ew + 255) / 256 * 256);
+  rc = zcopy2d_launch((int)L.nd, 1, (const zc*)H_diag, L.nd, 0, Hp, L.nd, 0, 1, s);
+  if (rc) return rc;
+  rc = zaxpy_launch(b, b, zmake(0.0, -0.5), nullptr, 0, (const zc*)gamma_L, b, 0, Hp, b, 0, 1, s);
+  if (rc) return rc;
+  rc = zaxpy_launch(b, b, zmake(0.0, -0.5), nullptr, 0, (const zc*)gamma_R, b, 0,
+                    Hp + L.od[nblocks - 1], b, 0, 1, s);
+  if (rc) return rc;
+  EnergyIn in{Hp, (const zc*)H_upper, (const zc*)gamma_L, (const zc*)gamma_R, energies,
+              n_energies, eta, mu_L, mu_R, kT};
+  EnergyOut out{(zc*)GR_diag, (zc*)Gless_diag, transmission, info};
+  return sweep_energies(L, in, out, wave, w, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+//
+// Storage convention (same as the reference's Matrix{ComplexF64} blocks):
+// column-major, interleaved complex128 (re, im) = HIP double2 (x = re, y = im).
+// A "batch" of instances is addressed as base + idx * stride (stride 0
+// broadcasts one block to every instance, used for energy-independent blocks).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+namespace negf {
+
+typedef double2 zc;
+
+// op codes for gemm/trsm operands, as the BLAS characters 'N', 'T', 'C'
+enum Op : int { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+__host__ __device__ inline zc zmake(double r, double i) { zc z; z.x = r; z.y = i; return z; }
+__host__ __device__ inline zc zadd(zc a, zc b) { return zmake(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline zc zsub(zc a, zc b) { return zmake(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline zc zmul(zc a, zc b) {
+  return zmake(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline zc zconj(zc a) { return zmake(a.x, -a.y); }
+__host__ __device__ inline double zabs1(zc a) { return fabs(a.x) + fabs(a.y); }
+// 1/z with Smith's scaling (R. L. Smith, "Algorithm 116: Complex division",
+// CACM 5(8), 1962), which avoids overflow/underflow of |z|^2.
+__host__ __device__ inline zc zrcp(zc z) {
+  if (fabs(z.y) <= fabs(z.x)) {
+    double r = z.y / z.x, d = z.x + z.y * r;
+    return zmake(1.0 / d, -r / d);
+  } else {
+    double r = z.x / z.y, d = z.y + z.x * r;
+    return zmake(r / d, -1.0 / d);
+  }
+}
+
+struct Status {
+  static constexpr int OK = 0;
+  static constexpr int BAD_ARG = -1;
+  static constexpr int TOO_LARGE = -2;
+  static constexpr int HIP_ERROR = -3;
+  static constexpr int NO_MEMORY = -4;
+};
+
+// ----------------------------------------------------------------------------
+// Batched ZGEMM on FP64 MFMA:  D = alpha * op(A) op(B) + beta * C  (+ shift I)
+// C may alias D (in-place update); C is not read when beta == 0.
+// `shift` (optional, may be null) adds shift[idx * shift_stride] to every
+// diagonal element of instance idx, which builds A(E) = zI - ... in the
+// epilogue of the Schur-complement GEMM.
+struct GemmArgs {
+  int opA, opB;
+  int m, n, k;
+  zc alpha, beta;
+  const zc* A; long long lda, strideA;
+  const zc* B; long long ldb, strideB;
+  const zc* C; long long ldc, strideC;
+  zc* D; long long ldd, strideD;
+  const zc* shift; long long shift_stride;
+  int batch;
+};
+int zgemm_launch(const GemmArgs& g, hipStream_t s);
+
+// Several independent GEMMs of one shape in one launch (fills the GPU when
+// each problem alone is a handful of tiles).
+int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Batched explicit inverse by blocked Gauss-Jordan with partial pivoting.
+// Ainv[idx] = inv(A[idx]).  `X` is the in-place work matrix (n x n, ld n,
+// one per instance) and may be the same buffer as A when A may be destroyed.
+struct InvWork {
+  zc* X;        // n*n*batch
+  zc* W;        // n*nb*batch
+  zc* R;        // nb*n*batch
+  int* piv;     // n*batch
+  int* used;    // n*batch
+};
+size_t zinv_work_bytes(int n, int batch);
+void zinv_work_carve(void* base, int n, int batch, InvWork* w);
+// A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the
+// 1-based column of the first exactly-zero pivot.
+int zinv_launch(int n, const zc* A, long long lda, long long strideA,
+                zc* Ainv, long long ldinv, long long strideInv, int batch,
+                const InvWork& w, int* info, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Batched LU with partial pivoting (LAPACK zgetrf layout, 1-based ipiv),
+// triangular solves and inversion from the factors (per-op drop-in).
+int zgetrf_launch(int n, zc* A, long long lda, long long strideA, int* ipiv,
+                  long long strideP, int* info, int batch, hipStream_t s);
+int ztrsm_launch(char side, char uplo, char trans, char diag, int m, int n, zc alpha,
+                 const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                 long long strideB, int batch, hipStream_t s);
+int zlaswp_launch(int ncols, zc* B, long long ldb, long long strideB, const int* ipiv,
+                  long long strideP, int k1, int k2, int incx, int batch, hipStream_t s);
+int zgetrs_launch(char trans, int n, int nrhs, const zc* A, long long lda, long long strideA,
+                  const int* ipiv, long long strideP, zc* B, long long ldb, long long strideB,
+                  int batch, hipStream_t s);
+int zgetri_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
+                  long long strideP, int batch, zc* work, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Elementwise helpers (all batched, column-major).
+int zadjoint_launch(int m, int n, const zc* A, long long lda, long long strideA, zc* B,
+                    long long ldb, long long strideB, int batch, hipStream_t s);
+int zcopy2d_launch(int m, int n, const zc* A, long long lda, long long strideA, zc* B,
+                   long long ldb, long long strideB, int batch, hipStream_t s);
+int zfill_launch(int m, int n, zc value, zc diag_value, zc* A, long long lda,
+                 long long strideA, int batch, hipStream_t s);
+// out[idx] = Re sum_ij P[i,j] conj(Q[i,j]) over an m x n block.
+int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP, const zc* Q,
+                    long long ldq, long long strideQ, double* out, int batch, hipStream_t s);
+// B = s * A with s = scale_dev[idx * scale_stride] when scale_dev is given
+// (device array, stride 0 broadcasts), else the host value `value`.
+int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
+                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                  long long strideB, int batch, hipStream_t s);
+
+}  // namespace negf

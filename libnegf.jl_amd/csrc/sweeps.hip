@@ -1,0 +1,493 @@
+// Block-tridiagonal NEGF sweeps, batched over instances (energies), built from
+// the MFMA GEMM and the Gauss-Jordan diagonal-block inverse.
+//
+// Every step of a sweep is one launch over all instances of the batch, so a
+// wave of energies turns each b x b block operation into a batched GEMM with
+// (wave * tiles) workgroups.  The sequence per layer follows the reference's
+// algorithms with the LU solves replaced by products with the explicit
+// inverse g_i = S_i^{-1} (computed once per layer and reused by every later
+// pass):
+//
+//   RGF upward  (src/matinvertndiag_rgf.jl:143-162)
+//     g_{N-1} = A_{N-1,N-1}^{-1};  for i = N-2..0:
+//       t_i = A_{i,i+1} g_{i+1},  l_{i+1} = g_{i+1} A_{i+1,i},
+//       g_i = (A_ii - A_{i,i+1} l_{i+1})^{-1}
+//   RGF downward (:169-182)
+//     G_00 = g_0;  G_{i-1,i} = -G_{i-1,i-1} t_{i-1},  G_{i,i-1} = -l_i G_{i-1,i-1},
+//     G_ii = g_i - l_i G_{i-1,i}
+//   RKD (src/keldyshndiag.jl:221-318): upward, central (X recursion), the
+//     diagonal re-scaling g_i S_ii g_i^H, downward, as in keldyshndiag_v3!.
+//   Transmission: G_{0,j} = -G_{0,j-1} t_{j-1} (:185-199),
+//     T = Re Tr(Gamma_L G_{0,N-1} Gamma_R G_{0,N-1}^H).
+#include "negf_internal.hpp"
+#include "sweeps.hpp"
+
+#include <vector>
+
+namespace negf {
+
+namespace {
+
+inline zc Z(double r, double i = 0.0) { return zmake(r, i); }
+const zc ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
+
+struct M {           // one block position in a batch: base pointer, ld, batch stride
+  const zc* p;
+  long long ld, bs;
+};
+struct MW {
+  zc* p;
+  long long ld, bs;
+};
+inline M ro(const MW& w) { return M{w.p, w.ld, w.bs}; }
+
+int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D, int batch,
+         hipStream_t s, const zc* shift = nullptr, long long shift_stride = 0) {
+  GemmArgs g{};
+  g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
+  g.alpha = alpha; g.beta = beta;
+  g.A = A.p; g.lda = A.ld; g.strideA = A.bs;
+  g.B = B.p; g.ldb = B.ld; g.strideB = B.bs;
+  g.C = C.p; g.ldc = C.ld; g.strideC = C.bs;
+  g.D = D.p; g.ldd = D.ld; g.strideD = D.bs;
+  g.shift = shift; g.shift_stride = shift_stride;
+  g.batch = batch;
+  return zgemm_launch(g, s);
+}
+
+GemmArgs ga(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D,
+            int batch) {
+  GemmArgs g{};
+  g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
+  g.alpha = alpha; g.beta = beta;
+  g.A = A.p; g.lda = A.ld; g.strideA = A.bs;
+  g.B = B.p; g.ldb = B.ld; g.strideB = B.bs;
+  g.C = C.p; g.ldc = C.ld; g.strideC = C.bs;
+  g.D = D.p; g.ldd = D.ld; g.strideD = D.bs;
+  g.batch = batch;
+  return g;
+}
+
+#define RC(x)                 \
+  do {                        \
+    int _rc = (x);            \
+    if (_rc) return _rc;      \
+  } while (0)
+
+__global__ void fold_info_kernel(const int* layer_info, int* sweep_info, int layer, int batch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < batch && layer_info[i] != 0 && sweep_info[i] == 0) sweep_info[i] = layer + 1;
+}
+
+__global__ void zero_int_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// z_e = E_e + i eta, fL_e = f(E_e; mu_L), fR_e = f(E_e; mu_R) as complex scales,
+// fermi = 1 / (1 + exp((E - mu) / kT)) in the overflow-safe tanh form.
+__global__ void energy_scalars_kernel(int n, const double* E, double eta, double muL, double muR,
+                                      double kT, zc* z, zc* fL, zc* fR) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double e = E[i];
+  z[i] = zmake(e, eta);
+  fL[i] = zmake(0.5 * (1.0 - tanh(0.5 * (e - muL) / kT)), 0.0);
+  fR[i] = zmake(0.5 * (1.0 - tanh(0.5 * (e - muR) / kT)), 0.0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+Layout::Layout(int N_, const int* sizes) : N(N_), b(sizes, sizes + N_) {
+  od.resize(N + 1);
+  oo.resize(N > 0 ? N : 1);
+  od[0] = 0;
+  for (int i = 0; i < N; ++i) od[i + 1] = od[i] + (long long)b[i] * b[i];
+  oo[0] = 0;
+  for (int i = 0; i + 1 < N; ++i) oo[i + 1] = oo[i] + (long long)b[i] * b[i + 1];
+  nd = od[N];
+  no = N > 1 ? oo[N - 1] : 0;
+  bmax = 0;
+  for (int x : b) bmax = x > bmax ? x : bmax;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace layout of one sweep wave (all in units of zc unless stated).
+struct SweepWork {
+  zc* g;      // N diag blocks (packed diag layout) per instance
+  zc* t;      // N-1 offdiag (b_i x b_{i+1}) per instance
+  zc* l;      // N-1 offdiag (b_{i+1} x b_i) per instance
+  zc* T1;     // bmax^2 per instance
+  zc* T2;     // bmax^2 per instance
+  zc* X;      // bmax^2 per instance (RKD X recursion)
+  zc* ones;   // 1 element (the shift that adds I)
+  int* linfo; // batch
+  InvWork inv;
+};
+
+size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh) {
+  const size_t zb = sizeof(zc);
+  size_t bytes = 0;
+  auto add = [&](size_t b) { bytes += (b + 255) / 256 * 256; };
+  add(zb * L.nd * batch);
+  add(zb * L.no * batch);
+  add(zb * L.no * batch);
+  const size_t bb = (size_t)L.bmax * L.bmax;
+  add(zb * bb * batch);
+  add(zb * bb * batch);
+  add(zb * bb * batch);
+  add(zb);
+  add(sizeof(int) * (size_t)batch);
+  add(zinv_work_bytes(L.bmax, batch));
+  (void)keldysh;
+  return bytes + 4096;
+}
+
+static void carve(void* base, const Layout& L, int batch, SweepWork* w) {
+  char* p = (char*)base;
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += (b + 255) / 256 * 256;
+    return r;
+  };
+  const size_t zb = sizeof(zc);
+  w->g = (zc*)take(zb * L.nd * batch);
+  w->t = (zc*)take(zb * L.no * batch);
+  w->l = (zc*)take(zb * L.no * batch);
+  const size_t bb = (size_t)L.bmax * L.bmax;
+  w->T1 = (zc*)take(zb * bb * batch);
+  w->T2 = (zc*)take(zb * bb * batch);
+  w->X = (zc*)take(zb * bb * batch);
+  w->ones = (zc*)take(zb);
+  w->linfo = (int*)take(sizeof(int) * (size_t)batch);
+  zinv_work_carve(p, L.bmax, batch, &w->inv);
+}
+
+// ---------------------------------------------------------------------------
+// Upward pass.  Source of the matrix blocks:
+//   Adiag(i): A_ii   (or, with H-form, -H'_ii with the energy shift z added)
+//   Aup(i):   A_{i,i+1};  Alow(i): A_{i+1,i} (opLow = OP_N), or the adjoint of
+//   Aup(i) when the matrix is given in Hermitian-coupling form (opLow = OP_C,
+//   sign sLow).
+struct Source {
+  // diag blocks
+  const zc* diag; long long diag_bs;
+  const zc* up; long long up_bs;
+  const zc* low; long long low_bs;   // may be null: A_{i+1,i} = sLow * A_{i,i+1}^H
+  zc sLow;
+  // energy form: A_ii = shift - diag (diag holds H'_ii); shift per instance
+  const zc* shift; long long shift_stride;
+  zc sUp;      // A_{i,i+1} = sUp * up
+};
+
+static int upward(const Layout& L, const Source& S, const SweepWork& w, int batch, int* info,
+                  hipStream_t s) {
+  const int N = L.N;
+  const long long gbs = L.nd, obs = L.no;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+  auto diagM = [&](int i) { return M{S.diag + L.od[i], L.b[i], S.diag_bs}; };
+  auto invert = [&](int i, const zc* Xsrc, long long ldx, long long xs) -> int {
+    RC(zinv_launch(L.b[i], Xsrc, ldx, xs, w.g + L.od[i], L.b[i], gbs, batch, w.inv, w.linfo, s));
+    hipLaunchKernelGGL(fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, w.linfo, info,
+                       i, batch);
+    return Status::OK;
+  };
+  // bottom layer
+  {
+    const int i = N - 1, bi = L.b[i];
+    if (S.shift) {
+      // X = shift * I - H'_{N-1}
+      MW X{w.inv.X, bi, (long long)bi * bi};
+      RC(gemm(OP_N, OP_N, bi, bi, 0, ONE, M{nullptr, bi, 0}, M{nullptr, 1, 0}, MONE, diagM(i), X,
+              batch, s, S.shift, S.shift_stride));
+      RC(invert(i, w.inv.X, bi, (long long)bi * bi));
+    } else {
+      RC(invert(i, S.diag + L.od[i], bi, S.diag_bs));
+    }
+  }
+  for (int i = N - 2; i >= 0; --i) {
+    const int bi = L.b[i], bj = L.b[i + 1];
+    M up{S.up + L.oo[i], bi, S.up_bs};
+    M gj{w.g + L.od[i + 1], bj, gbs};
+    MW t{w.t + L.oo[i], bi, obs};
+    MW l{w.l + L.oo[i], bj, obs};
+    GemmArgs two[2];
+    // t_i = A_{i,i+1} g_{i+1}
+    two[0] = ga(OP_N, OP_N, bi, bj, bj, S.sUp, up, gj, ZERO, M{nullptr, 1, 0}, t, batch);
+    // l_{i+1} = g_{i+1} A_{i+1,i}
+    if (S.low) {
+      M low{S.low + L.oo[i], bj, S.low_bs};
+      two[1] = ga(OP_N, OP_N, bj, bi, bj, ONE, gj, low, ZERO, M{nullptr, 1, 0}, l, batch);
+    } else {
+      two[1] = ga(OP_N, OP_C, bj, bi, bj, S.sLow, gj, up, ZERO, M{nullptr, 1, 0}, l, batch);
+    }
+    if (bi == bj) {
+      RC(zgemm_group_launch(two, 2, s));
+    } else {
+      RC(zgemm_launch(two[0], s));
+      RC(zgemm_launch(two[1], s));
+    }
+    // X = A_ii - A_{i,i+1} l_{i+1}
+    MW X{w.inv.X, bi, (long long)bi * bi};
+    zc a = zmake(-S.sUp.x, -S.sUp.y);
+    if (S.shift) {
+      RC(gemm(OP_N, OP_N, bi, bi, bj, a, up, ro(l), MONE, diagM(i), X, batch, s, S.shift,
+              S.shift_stride));
+    } else {
+      RC(gemm(OP_N, OP_N, bi, bi, bj, a, up, ro(l), ONE, diagM(i), X, batch, s));
+    }
+    RC(invert(i, w.inv.X, bi, (long long)bi * bi));
+  }
+  return Status::OK;
+}
+
+// Downward RGF.  Writes G_ii into gd (packed diag layout, batch stride gd_bs)
+// and G_{i,i+1} / G_{i+1,i} into gu / gl (packed offdiag, may be null: then
+// two bmax^2 scratch blocks T1/T2 hold them one layer at a time).
+static int downward(const Layout& L, const SweepWork& w, int batch, zc* gd, long long gd_bs,
+                    zc* gu, zc* gl, long long go_bs, hipStream_t s) {
+  const int N = L.N;
+  const long long gbs = L.nd, obs = L.no;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  RC(zcopy2d_launch(L.b[0], L.b[0], w.g, L.b[0], gbs, gd, L.b[0], gd_bs, batch, s));
+  for (int i = 1; i < N; ++i) {
+    const int bp = L.b[i - 1], bi = L.b[i];
+    M Gpp{gd + L.od[i - 1], bp, gd_bs};
+    MW Gup = gu ? MW{gu + L.oo[i - 1], bp, go_bs} : MW{w.T1, bp, xbs};
+    MW Glo = gl ? MW{gl + L.oo[i - 1], bi, go_bs} : MW{w.T2, bi, xbs};
+    M t{w.t + L.oo[i - 1], bp, obs};
+    M l{w.l + L.oo[i - 1], bi, obs};
+    GemmArgs two[2];
+    two[0] = ga(OP_N, OP_N, bp, bi, bp, MONE, Gpp, t, ZERO, M{nullptr, 1, 0}, Gup, batch);
+    two[1] = ga(OP_N, OP_N, bi, bp, bp, MONE, l, Gpp, ZERO, M{nullptr, 1, 0}, Glo, batch);
+    if (bp == bi) {
+      RC(zgemm_group_launch(two, 2, s));
+    } else {
+      RC(zgemm_launch(two[0], s));
+      RC(zgemm_launch(two[1], s));
+    }
+    // G_ii = g_i - l_i G_{i-1,i}
+    MW Gii{gd + L.od[i], bi, gd_bs};
+    RC(gemm(OP_N, OP_N, bi, bi, bp, MONE, l, ro(Gup), ONE, M{w.g + L.od[i], bi, gbs}, Gii, batch,
+            s));
+  }
+  return Status::OK;
+}
+
+// Transmission T = Re Tr(Gamma_L G_{0,N-1} Gamma_R G_{0,N-1}^H), with the
+// first block row from G_{0,j} = -G_{0,j-1} t_{j-1} starting at G_00 = g_0.
+static int transmission(const Layout& L, const SweepWork& w, int batch, M gammaL, M gammaR,
+                        double* T, hipStream_t s) {
+  const int N = L.N;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  const int b0 = L.b[0], bl = L.b[N - 1];
+  // row chain ping-pongs between X and T1 (T2 is free here)
+  MW cur{w.X, b0, xbs};
+  RC(zcopy2d_launch(b0, b0, w.g, b0, L.nd, cur.p, b0, xbs, batch, s));
+  for (int j = 1; j < N; ++j) {
+    MW nxt{cur.p == w.X ? w.T1 : w.X, b0, xbs};
+    RC(gemm(OP_N, OP_N, b0, L.b[j], L.b[j - 1], MONE, ro(cur), M{w.t + L.oo[j - 1], L.b[j - 1], L.no},
+            ZERO, M{nullptr, 1, 0}, nxt, batch, s));
+    cur = nxt;
+  }
+  // P = Gamma_L G (b0 x bl) into T2; Q = G Gamma_R (b0 x bl) into the other buffer
+  MW P{w.T2, b0, xbs};
+  MW Q{cur.p == w.X ? w.T1 : w.X, b0, xbs};
+  GemmArgs two[2];
+  two[0] = ga(OP_N, OP_N, b0, bl, b0, ONE, gammaL, ro(cur), ZERO, M{nullptr, 1, 0}, P, batch);
+  two[1] = ga(OP_N, OP_N, b0, bl, bl, ONE, ro(cur), gammaR, ZERO, M{nullptr, 1, 0}, Q, batch);
+  if (b0 == bl) {
+    RC(zgemm_group_launch(two, 2, s));
+  } else {
+    RC(zgemm_launch(two[0], s));
+    RC(zgemm_launch(two[1], s));
+  }
+  return zdotc_re_launch(b0, bl, P.p, b0, xbs, Q.p, b0, xbs, T, batch, s);
+}
+
+// RKD passes on Sd (packed diag, batch stride sd_bs) and Su (packed offdiag,
+// stride su_bs), in place: input Sigma (Hermitian storage), output the band of
+// A^{-1} Sigma A^{-H}.  lowSrc/sLow give A_{i+1,i} for the X recursion.
+static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs, zc* Su,
+               long long su_bs, const Source& S, hipStream_t s) {
+  const int N = L.N;
+  const long long gbs = L.nd, obs = L.no;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
+  auto su = [&](int i) { return MW{Su + L.oo[i], L.b[i], su_bs}; };
+  auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
+  auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], obs}; };
+  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], obs}; };  // l_i = S_i^{-1} A_{i,i-1}
+  // upward RKD (keldyshndiag.jl:230-234)
+  for (int i = N - 2; i >= 0; --i) {
+    const int bi = L.b[i], bj = L.b[i + 1];
+    GemmArgs two[2];
+    // Sd_i -= t_i Su_i^H ; Su_i -= t_i Sd_{i+1}
+    two[0] = ga(OP_N, OP_C, bi, bi, bj, MONE, t(i), ro(su(i)), ONE, ro(sd(i)), sd(i), batch);
+    two[1] = ga(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ONE, ro(su(i)), su(i), batch);
+    if (bi == bj) {
+      RC(zgemm_group_launch(two, 2, s));
+    } else {
+      RC(zgemm_launch(two[0], s));
+      RC(zgemm_launch(two[1], s));
+    }
+    // Sd_i -= Su_i t_i^H
+    RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, ro(su(i)), t(i), ONE, ro(sd(i)), sd(i), batch, s));
+  }
+  // central RKD (keldyshndiag.jl:256-301)
+  if (N > 1) {
+    const int b0 = L.b[0], b1 = L.b[1];
+    MW T1{w.T1, b0, xbs};
+    RC(gemm(OP_N, OP_N, b0, b1, b0, ONE, g(0), ro(su(0)), ZERO, M{nullptr, 1, 0}, T1, batch, s));
+    RC(gemm(OP_N, OP_C, b0, b1, b1, ONE, ro(T1), g(1), ZERO, M{nullptr, 1, 0}, su(0), batch, s));
+  }
+  {
+    const int b0 = L.b[0];
+    RC(zfill_launch(b0, b0, ZERO, ONE, w.X, b0, xbs, batch, s));
+  }
+  for (int i = 1; i + 1 < N; ++i) {
+    const int bp = L.b[i - 1], bi = L.b[i], bn = L.b[i + 1];
+    MW T1{w.T1, bp, xbs}, T2{w.T2, bp, xbs};
+    // T1 = X_{i-1} t_{i-1}   (bp x bi)
+    RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, M{w.X, bp, xbs}, t(i - 1), ZERO, M{nullptr, 1, 0}, T1,
+            batch, s));
+    // T2 = g_{i-1} T1        (bp x bi)
+    RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, g(i - 1), ro(T1), ZERO, M{nullptr, 1, 0}, T2, batch, s));
+    // X_i = A_{i,i-1} T2 + I (bi x bi)
+    MW Xn{w.X, bi, xbs};
+    if (S.low) {
+      RC(gemm(OP_N, OP_N, bi, bi, bp, ONE, M{S.low + L.oo[i - 1], bi, S.low_bs}, ro(T2), ZERO,
+              M{nullptr, 1, 0}, Xn, batch, s, w.ones, 0));
+    } else {
+      RC(gemm(OP_C, OP_N, bi, bi, bp, S.sLow, M{S.up + L.oo[i - 1], bp, S.up_bs}, ro(T2), ZERO,
+              M{nullptr, 1, 0}, Xn, batch, s, w.ones, 0));
+    }
+    // T1 = g_i X_i ; T2 = T1 Su_i ; Su_i = T2 g_{i+1}^H
+    MW T1b{w.T1, bi, xbs}, T2b{w.T2, bi, xbs};
+    RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(Xn), ZERO, M{nullptr, 1, 0}, T1b, batch, s));
+    RC(gemm(OP_N, OP_N, bi, bn, bi, ONE, ro(T1b), ro(su(i)), ZERO, M{nullptr, 1, 0}, T2b, batch,
+            s));
+    RC(gemm(OP_N, OP_C, bi, bn, bn, ONE, ro(T2b), g(i + 1), ZERO, M{nullptr, 1, 0}, su(i), batch,
+            s));
+  }
+  // diagonal re-scaling: Sd_i = g_i Sd_i g_i^H
+  for (int i = 0; i < N; ++i) {
+    const int bi = L.b[i];
+    MW T1{w.T1, bi, xbs};
+    RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(sd(i)), ZERO, M{nullptr, 1, 0}, T1, batch, s));
+    RC(gemm(OP_N, OP_C, bi, bi, bi, ONE, ro(T1), g(i), ZERO, M{nullptr, 1, 0}, sd(i), batch, s));
+  }
+  // downward RKD (keldyshndiag.jl:313-317), l = l_{i+1}
+  for (int i = 0; i + 1 < N; ++i) {
+    const int bi = L.b[i], bj = L.b[i + 1];
+    GemmArgs two[2];
+    // Sd_{i+1} -= Su_i^H l^H ; Su_i -= Sd_i l^H
+    two[0] = ga(OP_C, OP_C, bj, bj, bi, MONE, ro(su(i)), l(i + 1), ONE, ro(sd(i + 1)), sd(i + 1),
+                batch);
+    two[1] = ga(OP_N, OP_C, bi, bj, bi, MONE, ro(sd(i)), l(i + 1), ONE, ro(su(i)), su(i), batch);
+    if (bi == bj) {
+      RC(zgemm_group_launch(two, 2, s));
+    } else {
+      RC(zgemm_launch(two[0], s));
+      RC(zgemm_launch(two[1], s));
+    }
+    // Sd_{i+1} -= l Su_i
+    RC(gemm(OP_N, OP_N, bj, bj, bi, MONE, l(i + 1), ro(su(i)), ONE, ro(sd(i + 1)), sd(i + 1), batch,
+            s));
+  }
+  return Status::OK;
+}
+
+// ---------------------------------------------------------------------------
+// Public sweep drivers (called by the C-ABI layer with validated arguments).
+
+int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
+              const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
+              hipStream_t s) {
+  SweepWork w;
+  carve(work, L, batch, &w);
+  const zc one = ONE;
+  RC((int)hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) == 0
+         ? Status::OK
+         : Status::HIP_ERROR);
+  Source S{};
+  S.diag = A.diag; S.diag_bs = A.diag_bs;
+  S.up = A.up; S.up_bs = A.up_bs;
+  S.low = A.low; S.low_bs = A.low_bs;
+  S.sLow = ONE; S.sUp = ONE;
+  RC(upward(L, S, w, batch, info, s));
+  if (G.diag) RC(downward(L, w, batch, G.diag, L.nd, G.up, G.low, L.no, s));
+  if (gam && T)
+    RC(transmission(L, w, batch, M{gam->gammaL, L.b[0], gam->gammaL_bs},
+                    M{gam->gammaR, L.b[L.N - 1], gam->gammaR_bs}, T, s));
+  if (sigma && gn) {
+    // copy Sigma into the outputs and run RKD in place there
+    RC(zcopy2d_launch((int)L.nd, 1, sigma->diag, L.nd, sigma->diag_bs, gn->diag, L.nd, L.nd, batch,
+                      s));
+    if (L.N > 1)
+      RC(zcopy2d_launch((int)L.no, 1, sigma->up, L.no, sigma->up_bs, gn->up, L.no, L.no, batch, s));
+    RC(rkd(L, w, batch, gn->diag, L.nd, gn->up, L.no, S, s));
+  }
+  return Status::OK;
+}
+
+int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
+                   void* work, hipStream_t s) {
+  const int N = L.N, b = L.b[0];
+  const int nE = in.nE;
+  // device scalars for all energies live at the tail of the workspace
+  const size_t sw = sweep_work_bytes(L, wave, true);
+  char* tail = (char*)work + sw;
+  zc* z = (zc*)tail;
+  zc* fL = z + nE;
+  zc* fR = fL + nE;
+  zc* Sdw = fR + nE;                         // wave * nd
+  zc* Suw = Sdw + (size_t)wave * L.nd;       // wave * no
+  int* einfo = (int*)(Suw + (size_t)wave * L.no);  // nE, when the caller passes none
+  hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
+                     in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
+  SweepWork w;
+  carve(work, L, wave, &w);
+  const zc one = ONE;
+  if (hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
+    return Status::HIP_ERROR;
+  const long long bb = (long long)b * b;
+  for (int e0 = 0; e0 < nE; e0 += wave) {
+    const int nb = (nE - e0) < wave ? (nE - e0) : wave;
+    Source S{};
+    S.diag = in.Hp_diag; S.diag_bs = 0;        // H' (contacts folded in), shared
+    S.up = in.H_up; S.up_bs = 0;
+    S.low = nullptr; S.low_bs = 0;
+    S.sUp = MONE;                              // A_{i,i+1} = -H_{i,i+1}
+    S.sLow = MONE;                             // A_{i+1,i} = -H_{i,i+1}^H
+    S.shift = z + e0; S.shift_stride = 1;
+    int* info = out.info ? out.info + e0 : einfo + e0;
+    RC(upward(L, S, w, nb, info, s));
+    // G^R diagonal (off-diagonal blocks through scratch)
+    zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Sdw;
+    RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
+    if (out.T) RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
+    if (out.Gless_diag) {
+      // Sigma^n = fL Gamma_L (+) fR Gamma_R, zero elsewhere
+      RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
+      if (N > 1) RC(zfill_launch((int)L.no, 1, ZERO, ZERO, Suw, L.no, L.no, nb, s));
+      RC(zscale_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
+      // (a single layer carries both contacts: accumulate)
+      RC(zaxpy_launch(b, b, ZERO, fR + e0, 1, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
+      RC(rkd(L, w, nb, Sdw, L.nd, Suw, L.no, S, s));
+      // G^< = i G^n
+      RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,
+                       out.Gless_diag + (size_t)e0 * L.nd, L.nd, L.nd, nb, s));
+    }
+  }
+  return Status::OK;
+}
+
+size_t energies_work_bytes(const Layout& L, int nE, int wave) {
+  return sweep_work_bytes(L, wave, true) + sizeof(zc) * (3 * (size_t)nE) +
+         sizeof(zc) * (size_t)wave * (L.nd + L.no) + sizeof(int) * (size_t)nE + 4096;
+}
+
+}  // namespace negf

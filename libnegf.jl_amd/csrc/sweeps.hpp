@@ -1,0 +1,65 @@
+// Sweep-level drivers: packed block-tridiagonal layout and entry points.
+#pragma once
+
+#include "negf_internal.hpp"
+#include <vector>
+
+namespace negf {
+
+// Packed block-tridiagonal layout: diag blocks D_i (b_i x b_i) at od[i];
+// off-diagonal blocks (A_{i,i+1}: b_i x b_{i+1}; A_{i+1,i}: b_{i+1} x b_i) at
+// oo[i]; every block column-major with ld = its row count.
+struct Layout {
+  int N;
+  std::vector<int> b;
+  std::vector<long long> od, oo;
+  long long nd, no;
+  int bmax;
+  Layout(int N, const int* sizes);
+};
+
+struct BtdIn {                 // A: diag / upper / lower (lower may be null = Hermitian)
+  const zc* diag; long long diag_bs;
+  const zc* up; long long up_bs;
+  const zc* low; long long low_bs;
+};
+struct BtdOut {                // outputs with tight batch strides (nd / no)
+  zc* diag; zc* up; zc* low;
+};
+struct BtdHerm {               // Hermitian block-tridiagonal Sigma (diag + upper)
+  const zc* diag; long long diag_bs;
+  const zc* up; long long up_bs;
+};
+struct GammaIn {
+  const zc* gammaL; long long gammaL_bs;
+  const zc* gammaR; long long gammaR_bs;
+};
+struct EnergyIn {
+  const zc* Hp_diag;           // H'_ii = H_ii - (i/2) Gamma on the contact layers
+  const zc* H_up;              // H_{i,i+1}
+  const zc* gammaL; const zc* gammaR;
+  const double* energies;      // device, nE
+  int nE;
+  double eta, muL, muR, kT;
+};
+struct EnergyOut {
+  zc* GR_diag;                 // nE * nd or null
+  zc* Gless_diag;              // nE * nd or null
+  double* T;                   // nE or null
+  int* info;                   // nE or null
+};
+
+size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
+size_t energies_work_bytes(const Layout& L, int nE, int wave);
+
+int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
+              const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
+              hipStream_t s);
+int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
+                   void* work, hipStream_t s);
+
+int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
+                 const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                 long long strideB, int batch, hipStream_t s);
+
+}  // namespace negf

@@ -1,0 +1,177 @@
+// Memory-bound helpers on complex128 blocks (batched, column-major):
+// adjoint (be_ctranspose!, src/backend_cpu.jl:58-68), 2-D copy
+// (be_copy_in_hw!), fill / identity (be_fill!, be_identity, be_zero_array),
+// scaling, and the trace reduction of the transmission.
+#include "negf_internal.hpp"
+#include "sweeps.hpp"
+
+namespace negf {
+
+namespace {
+
+// B = A^H through a 32x32 LDS tile (coalesced reads and writes).
+__global__ __launch_bounds__(256) void adjoint_kernel(int m, int n, const zc* __restrict__ A,
+                                                      long long lda, long long strideA,
+                                                      zc* __restrict__ B, long long ldb,
+                                                      long long strideB) {
+  __shared__ zc tile[32][33];
+  const int inst = blockIdx.z;
+  const zc* Ai = A + inst * strideA;
+  zc* Bi = B + inst * strideB;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;  // tile of A: rows i0.., cols j0..
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int i = i0 + tx, j = j0 + jj;
+    if (i < m && j < n) tile[jj][tx] = Ai[i + (long long)j * lda];
+  }
+  __syncthreads();
+  // B is n x m: B[j, i] = conj(A[i, j])
+  for (int ii = ty; ii < 32; ii += 8) {
+    const int j = j0 + tx, i = i0 + ii;
+    if (i < m && j < n) Bi[j + (long long)i * ldb] = zconj(tile[tx][ii]);
+  }
+}
+
+__global__ void copy2d_kernel(int m, int n, const zc* __restrict__ A, long long lda,
+                              long long strideA, zc* __restrict__ B, long long ldb,
+                              long long strideB) {
+  const int inst = blockIdx.z;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j0 = blockIdx.y * 16 + (threadIdx.x >> 6) * 4;
+  if (i >= m) return;
+  const zc* Ai = A + inst * strideA;
+  zc* Bi = B + inst * strideB;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int j = j0 + jj;
+    if (j < n) Bi[i + (long long)j * ldb] = Ai[i + (long long)j * lda];
+  }
+}
+
+__global__ void fill_kernel(int m, int n, zc value, zc dvalue, zc* __restrict__ A, long long lda,
+                            long long strideA) {
+  const int inst = blockIdx.z;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j0 = blockIdx.y * 16 + (threadIdx.x >> 6) * 4;
+  if (i >= m) return;
+  zc* Ai = A + inst * strideA;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int j = j0 + jj;
+    if (j < n) Ai[i + (long long)j * lda] = (i == j) ? dvalue : value;
+  }
+}
+
+// B = s A (accumulate: B += s A)
+__global__ void scale_kernel(int m, int n, zc value, const zc* __restrict__ sdev, long long sstride,
+                             const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                             long long strideB, bool accumulate) {
+  const int inst = blockIdx.z;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j0 = blockIdx.y * 16 + (threadIdx.x >> 6) * 4;
+  if (i >= m) return;
+  const zc sc = sdev ? sdev[inst * sstride] : value;
+  const zc* Ai = A + inst * strideA;
+  zc* Bi = B + inst * strideB;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int j = j0 + jj;
+    if (j < n) {
+      zc v = zmul(sc, Ai[i + (long long)j * lda]);
+      if (accumulate) v = zadd(v, Bi[i + (long long)j * ldb]);
+      Bi[i + (long long)j * ldb] = v;
+    }
+  }
+}
+
+// One workgroup per instance; fixed reduction order (deterministic).
+__global__ __launch_bounds__(256) void dotc_re_kernel(int m, int n, const zc* __restrict__ P,
+                                                      long long ldp, long long strideP,
+                                                      const zc* __restrict__ Q, long long ldq,
+                                                      long long strideQ, double* out) {
+  __shared__ double red[256];
+  const int inst = blockIdx.x;
+  const zc* Pi = P + inst * strideP;
+  const zc* Qi = Q + inst * strideQ;
+  double acc = 0.0;
+  const long long total = (long long)m * n;
+  for (long long e = threadIdx.x; e < total; e += 256) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    const zc p = Pi[i + j * ldp], q = Qi[i + j * ldq];
+    acc += p.x * q.x + p.y * q.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[inst] = red[0];
+}
+
+inline int ok() { return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR; }
+
+}  // namespace
+
+int zadjoint_launch(int m, int n, const zc* A, long long lda, long long strideA, zc* B,
+                    long long ldb, long long strideB, int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(adjoint_kernel, dim3((m + 31) / 32, (n + 31) / 32, batch), dim3(256), 0, s, m,
+                     n, A, lda, strideA, B, ldb, strideB);
+  return ok();
+}
+
+int zcopy2d_launch(int m, int n, const zc* A, long long lda, long long strideA, zc* B,
+                   long long ldb, long long strideB, int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(copy2d_kernel, dim3((m + 63) / 64, (n + 15) / 16, batch), dim3(256), 0, s, m,
+                     n, A, lda, strideA, B, ldb, strideB);
+  return ok();
+}
+
+int zfill_launch(int m, int n, zc value, zc diag_value, zc* A, long long lda, long long strideA,
+                 int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(fill_kernel, dim3((m + 63) / 64, (n + 15) / 16, batch), dim3(256), 0, s, m, n,
+                     value, diag_value, A, lda, strideA);
+  return ok();
+}
+
+int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
+                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                  long long strideB, int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(scale_kernel, dim3((m + 63) / 64, (n + 15) / 16, batch), dim3(256), 0, s, m, n,
+                     value, scale_dev, scale_stride, A, lda, strideA, B, ldb, strideB, false);
+  return ok();
+}
+
+int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
+                 const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                 long long strideB, int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(scale_kernel, dim3((m + 63) / 64, (n + 15) / 16, batch), dim3(256), 0, s, m, n,
+                     value, scale_dev, scale_stride, A, lda, strideA, B, ldb, strideB, true);
+  return ok();
+}
+
+int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP, const zc* Q,
+                    long long ldq, long long strideQ, double* out, int batch, hipStream_t s) {
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  if (batch == 0) return Status::OK;
+  hipLaunchKernelGGL(dotc_re_kernel, dim3(batch), dim3(256), 0, s, m, n, P, ldp, strideP, Q, ldq,
+                     strideQ, out);
+  return ok();
+}
+
+}  // namespace negf

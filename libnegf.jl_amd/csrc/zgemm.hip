@@ -1,0 +1,261 @@
+// Batched complex128 GEMM on gfx950 FP64 MFMA (v_mfma_f64_16x16x4_f64).
+//
+//   D[idx] = alpha * op(A[idx]) * op(B[idx]) + beta * C[idx]  (+ shift[idx] * I)
+//
+// Complex products as one real GEMM over a doubled K' = 2K ("K'-space"):
+//   Re D = sum_k' A-[i,k'] B'[k',j],  A-[i,2k] = Re a, A-[i,2k+1] = -Im a,
+//                                     B'[2k,j] = Re b, B'[2k+1,j] = Im b
+//   Im D = sum_k' A'[i,k'] B''[k',j], A'[i,2k] = Re a, A'[i,2k+1] =  Im a,
+//                                     B''[2k,j] = Im b, B''[2k+1,j] = Re b
+// so the interleaved (re, im) storage maps straight onto the MFMA K lanes:
+// one LDS row per K' index, the im-rows of B read with k' ^ 1, and the
+// A- sign flip a per-lane constant.  Two MFMAs per 16x16 output block and two
+// complex K steps.
+//
+// Tiling: a 256-thread workgroup computes a 64x64 complex tile of D; its four
+// waves each own 32x32 (2x2 MFMA blocks, re and im accumulators: 32 doubles).
+// K is staged 16 complex at a time through LDS (K'-major rows, padded), the
+// next stage prefetched into registers while the MFMAs run.  op(A) / op(B)
+// transposition and conjugation happen in the global->LDS staging, so the MFMA
+// loop is identical for all nine N/T/C combinations.
+//
+// Launch: one flat grid over (problem, instance, tile).  Workgroups are
+// dispatched round-robin over the 8 XCDs; the index is remapped so that the
+// tiles of one instance run on one XCD and share its L2.
+#include "negf_internal.hpp"
+
+namespace negf {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int KP = 2 * TK;        // K'-space rows per stage
+constexpr int LDS_S = TM + 2;     // padded LDS row, doubles
+constexpr int MAXG = 4;           // problems per grouped launch
+
+struct GemmGroup {
+  GemmArgs p[MAXG];
+  int count;
+  int tiles_m, tiles_n;
+  long long tiles_per_problem;    // tiles_m * tiles_n * batch
+  long long total;                // tiles_per_problem * count
+  long long grid_padded;          // multiple of 8
+};
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
+  // XCD-aware remap of the flat workgroup index.
+  const long long bid = blockIdx.x;
+  const long long per = G.grid_padded >> 3;
+  const long long logical = (bid & 7) * per + (bid >> 3);
+  if (logical >= G.total) return;
+  const int pidx = (int)(logical / G.tiles_per_problem);
+  const long long rem = logical - (long long)pidx * G.tiles_per_problem;
+  const int tiles_inst = G.tiles_m * G.tiles_n;
+  const int inst = (int)(rem / tiles_inst);
+  const int tile = (int)(rem - (long long)inst * tiles_inst);
+  const int tm = tile % G.tiles_m, tn = tile / G.tiles_m;
+  const GemmArgs& g = G.p[pidx];
+  const int m = g.m, n = g.n, k = g.k;
+  const int i0 = tm * TM, j0 = tn * TN;
+
+  const zc* __restrict__ A = g.A + inst * g.strideA;
+  const zc* __restrict__ B = g.B + inst * g.strideB;
+  const long long lda = g.lda, ldb = g.ldb;
+
+  __shared__ double As[KP][LDS_S];
+  __shared__ double Bs[KP][LDS_S];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  dbl4 acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc_re[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+      acc_im[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  zc ra[4], rb[4];
+
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int i, kk;
+      if (OPA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * q; }
+      else             { kk = tid & 15; i = (tid >> 4) + 16 * q; }
+      const int gi = i0 + i, gk = k0 + kk;
+      zc v = zmake(0.0, 0.0);
+      if (gi < m && gk < k) v = (OPA == OP_N) ? A[gi + gk * lda] : A[gk + gi * lda];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int j, kk;
+      if (OPB == OP_N) { kk = tid & 15; j = (tid >> 4) + 16 * q; }
+      else             { j = tid & 63; kk = (tid >> 6) + 4 * q; }
+      const int gj = j0 + j, gk = k0 + kk;
+      zc v = zmake(0.0, 0.0);
+      if (gj < n && gk < k) v = (OPB == OP_N) ? B[gk + gj * ldb] : B[gj + gk * ldb];
+      rb[q] = v;
+    }
+  };
+  auto store_stage = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int i, kk;
+      if (OPA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * q; }
+      else             { kk = tid & 15; i = (tid >> 4) + 16 * q; }
+      As[2 * kk][i] = ra[q].x;
+      As[2 * kk + 1][i] = (OPA == OP_C) ? -ra[q].y : ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int j, kk;
+      if (OPB == OP_N) { kk = tid & 15; j = (tid >> 4) + 16 * q; }
+      else             { j = tid & 63; kk = (tid >> 6) + 4 * q; }
+      Bs[2 * kk][j] = rb[q].x;
+      Bs[2 * kk + 1][j] = (OPB == OP_C) ? -rb[q].y : rb[q].y;
+    }
+  };
+
+  const int nk = (k + TK - 1) / TK;
+  const int kq = lane >> 4;           // K' row within a 4-row MFMA step
+  const double asign = (kq & 1) ? -1.0 : 1.0;
+  const int li = lane & 15;
+
+  load_stage(0);
+  store_stage();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load_stage((kt + 1) * TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 2; ++ks) {
+      const int kr = 4 * ks + kq;
+      double a[2], am[2], b1[2], b2[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        a[r] = As[kr][wm * 32 + r * 16 + li];
+        am[r] = asign * a[r];
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        b1[c] = Bs[kr][wn * 32 + c * 16 + li];
+        b2[c] = Bs[kr ^ 1][wn * 32 + c * 16 + li];
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[r], b1[c], acc_re[r][c], 0, 0, 0);
+          acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b2[c], acc_im[r][c], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nk) {
+      __syncthreads();
+      store_stage();
+      __syncthreads();
+    }
+  }
+
+  // Epilogue.  Output layout of the 16x16 f64 MFMA: lane l, element r holds
+  // row 4*(l/16) + r, column l % 16.
+  zc* __restrict__ D = g.D + inst * g.strideD;
+  const zc* __restrict__ C = g.C ? g.C + inst * g.strideC : nullptr;
+  const bool use_c = (g.beta.x != 0.0 || g.beta.y != 0.0) && C != nullptr;
+  zc sh = zmake(0.0, 0.0);
+  const bool use_shift = g.shift != nullptr;
+  if (use_shift) sh = g.shift[inst * g.shift_stride];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = j0 + wn * 32 + c * 16 + li;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + wm * 32 + r * 16 + 4 * kq + e;
+        if (i < m && j < n) {
+          const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
+          zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
+          if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
+          if (use_shift && i == j) out = zadd(out, sh);
+          D[i + j * g.ldd] = out;
+        }
+      }
+    }
+}
+
+template <int OPA, int OPB>
+void launch_t(const GemmGroup& G, hipStream_t s) {
+  hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB>), dim3((unsigned)G.grid_padded), dim3(256), 0, s,
+                     G);
+}
+
+int check_args(const GemmArgs& g) {
+  if (g.m < 0 || g.n < 0 || g.k < 0 || g.batch < 0) return Status::BAD_ARG;
+  if (g.opA < 0 || g.opA > 2 || g.opB < 0 || g.opB > 2) return Status::BAD_ARG;
+  const int rowsA = g.opA == OP_N ? g.m : g.k;
+  const int rowsB = g.opB == OP_N ? g.k : g.n;
+  if (g.m > 0 && g.n > 0 && g.k > 0) {
+    if (g.lda < (rowsA > 1 ? rowsA : 1) || g.ldb < (rowsB > 1 ? rowsB : 1)) return Status::BAD_ARG;
+    if (!g.A || !g.B) return Status::BAD_ARG;
+  }
+  if (g.m > 0 && g.n > 0) {
+    if (g.ldd < (g.m > 1 ? g.m : 1) || !g.D) return Status::BAD_ARG;
+    if ((g.beta.x != 0.0 || g.beta.y != 0.0) && (!g.C || g.ldc < (g.m > 1 ? g.m : 1)))
+      return Status::BAD_ARG;
+  }
+  return Status::OK;
+}
+
+}  // namespace
+
+int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
+  if (count <= 0) return Status::OK;
+  if (count > MAXG) {
+    int rc = zgemm_group_launch(g, MAXG, s);
+    if (rc) return rc;
+    return zgemm_group_launch(g + MAXG, count - MAXG, s);
+  }
+  for (int i = 0; i < count; ++i) {
+    int rc = check_args(g[i]);
+    if (rc) return rc;
+    if (g[i].m != g[0].m || g[i].n != g[0].n || g[i].k != g[0].k || g[i].opA != g[0].opA ||
+        g[i].opB != g[0].opB || g[i].batch != g[0].batch)
+      return Status::BAD_ARG;
+  }
+  const GemmArgs& g0 = g[0];
+  if (g0.m == 0 || g0.n == 0 || g0.batch == 0) return Status::OK;
+  GemmGroup G;
+  for (int i = 0; i < count; ++i) G.p[i] = g[i];
+  G.count = count;
+  // k == 0: D = beta C (+shift); the kernel handles it (no K stages, zero acc).
+  G.tiles_m = (g0.m + TM - 1) / TM;
+  G.tiles_n = (g0.n + TN - 1) / TN;
+  G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * g0.batch;
+  G.total = G.tiles_per_problem * count;
+  G.grid_padded = (G.total + 7) / 8 * 8;
+  if (G.grid_padded > 0x7fffffffLL) return Status::TOO_LARGE;
+  const int key = g0.opA * 3 + g0.opB;
+  switch (key) {
+    case 0: launch_t<OP_N, OP_N>(G, s); break;
+    case 1: launch_t<OP_N, OP_T>(G, s); break;
+    case 2: launch_t<OP_N, OP_C>(G, s); break;
+    case 3: launch_t<OP_T, OP_N>(G, s); break;
+    case 4: launch_t<OP_T, OP_T>(G, s); break;
+    case 5: launch_t<OP_T, OP_C>(G, s); break;
+    case 6: launch_t<OP_C, OP_N>(G, s); break;
+    case 7: launch_t<OP_C, OP_T>(G, s); break;
+    case 8: launch_t<OP_C, OP_C>(G, s); break;
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+int zgemm_launch(const GemmArgs& g, hipStream_t s) { return zgemm_group_launch(&g, 1, s); }
+
+}  // namespace negf

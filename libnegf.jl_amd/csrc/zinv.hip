@@ -1,0 +1,272 @@
+// Batched explicit inverse of complex128 blocks by blocked Gauss-Jordan
+// elimination with partial pivoting (the diagonal-block inversion of RGF).
+//
+// Algorithm (per instance, n x n work matrix X, column panels of NB):
+//   panel kernel (one workgroup per instance, the n x NB panel in registers,
+//   RPT rows per thread):  NB exchange steps; step j picks the pivot row p_j
+//   among rows not yet pivoted by max |re| + |im| (LAPACK's izamax rule, so
+//   the pivot sequence is getrf's), scales it, eliminates column j from every
+//   other row (Gauss-Jordan: rows above too).  The composite row operation T
+//   of the NB steps acts on every other column c as
+//       X[:,c] <- X[:,c] + (P - E) X[p_J, c],
+//   P = the finished panel, E = unit columns at the pivot rows.  The kernel
+//   writes W = P - E, gathers R = X[p_J, :] (identity in the panel columns)
+//   and sets X[:,J] = E, so one MFMA GEMM  X <- X + W R  finishes the step.
+//   No physical row swaps: after all panels X[p_j, k] = inv(A)[j, p_k], and
+//   the final unscramble kernel writes inv(A)[r, c] = X[p_r, q_c] with
+//   q = p^{-1}.
+// Flops: 8 n^3 (the nominal getrf + getri count).
+#include "negf_internal.hpp"
+#include <climits>
+
+namespace negf {
+
+namespace {
+
+constexpr int PT = 256;   // threads per panel workgroup
+
+template <int RPT, int NB>
+__global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restrict__ X,
+                                                      zc* __restrict__ W, zc* __restrict__ R,
+                                                      int* __restrict__ piv,
+                                                      int* __restrict__ used,
+                                                      int* __restrict__ info) {
+  const int inst = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  zc* Xi = X + (long long)inst * n * n;
+  zc* Wi = W + (long long)inst * n * NB;
+  zc* Ri = R + (long long)inst * NB * n;
+  int* usedi = used + (long long)inst * n;
+  const int nbe = (n - k0) < NB ? (n - k0) : NB;
+
+  __shared__ double red_v[PT / 64];
+  __shared__ int red_i[PT / 64];
+  __shared__ zc prow[NB];
+  __shared__ int s_pc[NB];
+
+  zc P[RPT][NB];
+  bool usd[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + PT * q;
+    usd[q] = true;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) P[q][c] = zmake(0.0, 0.0);
+    if (r < n) {
+      usd[q] = usedi[r] != 0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nbe) P[q][c] = Xi[r + (long long)(k0 + c) * n];
+    }
+  }
+
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj) {
+    if (jj < nbe) {
+      // pivot search: max |re|+|im| over rows not yet pivoted, lowest index on ties
+      double bv = -1.0;
+      int bi = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = tid + PT * q;
+        if (r < n && !usd[q]) {
+          const double v = zabs1(P[q][jj]);
+          if (v > bv) { bv = v; bi = r; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+      __syncthreads();
+      bv = red_v[0]; bi = red_i[0];
+#pragma unroll
+      for (int w = 1; w < PT / 64; ++w) {
+        const double ov = red_v[w];
+        const int oi = red_i[w];
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      const int p = bi;
+      if (tid == 0) {
+        s_pc[jj] = p;
+        if (bv <= 0.0 && info[inst] == 0) info[inst] = k0 + jj + 1;
+      }
+      // the owner of row p publishes the scaled pivot row
+      if ((p % PT) == tid) {
+        const int qp = p / PT;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          if (q == qp) {
+            const zc pv = P[q][jj];
+            const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) prow[c] = (c == jj) ? inv : zmul(P[q][c], inv);
+          }
+        }
+      }
+      __syncthreads();
+      const zc inv = prow[jj];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = tid + PT * q;
+        if (r < n) {
+          if (r == p) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P[q][c] = prow[c];
+            usd[q] = true;
+          } else {
+            const zc f = P[q][jj];
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+              if (c != jj) P[q][c] = zsub(P[q][c], zmul(f, prow[c]));
+            P[q][jj] = zmake(-(f.x * inv.x - f.y * inv.y), -(f.x * inv.y + f.y * inv.x));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // W = P - E ; X[:, J] = E ; used flags back
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + PT * q;
+    if (r < n) {
+      usedi[r] = usd[q] ? 1 : 0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < nbe) {
+          const bool e = (r == s_pc[c]);
+          zc w = P[q][c];
+          if (e) w.x -= 1.0;
+          Wi[r + (long long)c * n] = w;
+          Xi[r + (long long)(k0 + c) * n] = zmake(e ? 1.0 : 0.0, 0.0);
+        }
+      }
+    }
+  }
+  if (tid < nbe) piv[(long long)inst * n + k0 + tid] = s_pc[tid];
+  // R = X[p_J, :] outside the panel, identity inside (ld NB)
+  for (int idx = tid; idx < NB * n; idx += PT) {
+    const int c = idx % NB, col = idx / NB;
+    if (c < nbe) {
+      zc v;
+      if (col >= k0 && col < k0 + nbe) v = zmake(col - k0 == c ? 1.0 : 0.0, 0.0);
+      else v = Xi[s_pc[c] + (long long)col * n];
+      Ri[idx] = v;
+    }
+  }
+}
+
+__global__ void gj_init_kernel(int n, int* used, int* info, int batch) {
+  const long long total = (long long)n * batch;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    used[i] = 0;
+    if (i < batch) info[i] = 0;
+  }
+}
+
+// out[r, c] = X[p_r, q_c], q = p^{-1}; grid.x = column chunks of 64, grid.y = instance
+__global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __restrict__ X,
+                                                            const int* __restrict__ piv,
+                                                            zc* __restrict__ out, long long ldo,
+                                                            long long strideO) {
+  extern __shared__ int sh[];
+  int* p = sh;
+  int* qinv = sh + n;
+  const int inst = blockIdx.y;
+  const int* pv = piv + (long long)inst * n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) p[j] = pv[j];
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) qinv[p[j]] = j;
+  __syncthreads();
+  const zc* Xi = X + (long long)inst * n * n;
+  zc* Oi = out + inst * strideO;
+  const int c0 = blockIdx.x * 64;
+  const int c1 = (c0 + 64 < n) ? c0 + 64 : n;
+  for (int c = c0; c < c1; ++c) {
+    const zc* col = Xi + (long long)qinv[c] * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) Oi[r + c * ldo] = col[p[r]];
+  }
+}
+
+template <int RPT, int NB>
+int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
+        long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+  if (A != w.X || lda != n || strideA != (long long)n * n) {
+    int rc = zcopy2d_launch(n, n, A, lda, strideA, w.X, n, (long long)n * n, batch, s);
+    if (rc) return rc;
+  }
+  {
+    long long total = (long long)n * batch;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gj_init_kernel, dim3(blocks), dim3(256), 0, s, n, w.used, info, batch);
+  }
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nbe = (n - k0) < NB ? (n - k0) : NB;
+    hipLaunchKernelGGL((gj_panel_kernel<RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, w.X, w.W,
+                       w.R, w.piv, w.used, info);
+    GemmArgs g{};
+    g.opA = OP_N; g.opB = OP_N;
+    g.m = n; g.n = n; g.k = nbe;
+    g.alpha = zmake(1.0, 0.0); g.beta = zmake(1.0, 0.0);
+    g.A = w.W; g.lda = n; g.strideA = (long long)n * NB;
+    g.B = w.R; g.ldb = NB; g.strideB = (long long)NB * n;
+    g.C = w.X; g.ldc = n; g.strideC = (long long)n * n;
+    g.D = w.X; g.ldd = n; g.strideD = (long long)n * n;
+    g.batch = batch;
+    int rc = zgemm_launch(g, s);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
+                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+int nb_for(int n) { return n <= 512 ? 16 : (n <= 1024 ? 8 : 4); }
+
+}  // namespace
+
+size_t zinv_work_bytes(int n, int batch) {
+  const size_t nb = nb_for(n);
+  size_t b = 0;
+  b += sizeof(zc) * (size_t)n * n * batch;   // X
+  b += sizeof(zc) * (size_t)n * nb * batch;  // W
+  b += sizeof(zc) * nb * (size_t)n * batch;  // R
+  b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
+  return b + 1024;
+}
+
+void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
+  const size_t nb = nb_for(n);
+  char* p = (char*)base;
+  auto take = [&](size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) / 256 * 256;
+    return r;
+  };
+  w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
+  w->W = (zc*)take(sizeof(zc) * (size_t)n * nb * batch);
+  w->R = (zc*)take(sizeof(zc) * nb * (size_t)n * batch);
+  w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
+  w->used = (int*)take(sizeof(int) * (size_t)n * batch);
+}
+
+int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
+                long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+  if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || batch == 0) return Status::OK;
+  if (n > 2048) return Status::TOO_LARGE;
+  if (n <= 256) return run<1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 512) return run<2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 1024) return run<4, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  return run<8, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+}
+
+}  // namespace negf

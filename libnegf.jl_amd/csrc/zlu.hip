@@ -1,0 +1,482 @@
+// Batched LU with partial pivoting, triangular solves, getrs and getri on
+// complex128 blocks: the per-op kernels behind be_lu!/be_mldivide!/be_getrf!/
+// be_trsm!/be_getri! (src/backend_cpu.jl:147-265).
+//
+// zgetrf: right-looking, NB-column panels.  The panel (rows k0..n-1 of NB
+//   columns) lives in registers of one 256-thread workgroup per instance;
+//   each column takes the izamax pivot (max |re|+|im|, first index on ties),
+//   swaps through LDS, scales by the reciprocal and applies the rank-1
+//   update inside the panel.  Then the row interchanges are applied to the
+//   columns left and right of the panel, U12 = L11^{-1} A12 by the blocked
+//   trsm, and A22 -= L21 U12 on the MFMA GEMM.  ipiv is LAPACK's (1-based).
+// ztrsm: blocked, 16-wide diagonal blocks solved by one thread per right-hand
+//   side (row or column) with the block in LDS, the rest as MFMA GEMMs.
+#include "negf_internal.hpp"
+#include <climits>
+
+namespace negf {
+
+namespace {
+
+constexpr int PT = 256;
+constexpr int TB = 16;   // trsm diagonal block
+
+template <int RPT, int NB>
+__global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restrict__ A,
+                                                      long long lda, long long strideA,
+                                                      int* __restrict__ ipiv, long long strideP,
+                                                      int* __restrict__ info) {
+  const int inst = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  zc* Ai = A + inst * strideA;
+  int* ip = ipiv + inst * strideP;
+  const int nbe = (n - k0) < NB ? (n - k0) : NB;
+
+  __shared__ double red_v[PT / 64];
+  __shared__ int red_i[PT / 64];
+  __shared__ zc rowcur[NB];
+  __shared__ zc rowpiv[NB];
+
+  zc P[RPT][NB];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = k0 + tid + PT * q;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      P[q][c] = (r < n && c < nbe) ? Ai[r + (long long)(k0 + c) * lda] : zmake(0.0, 0.0);
+  }
+
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj) {
+    if (jj < nbe) {
+      const int cur = k0 + jj;
+      double bv = -1.0;
+      int bi = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = k0 + tid + PT * q;
+        if (r < n && r >= cur) {
+          const double v = zabs1(P[q][jj]);
+          if (v > bv) { bv = v; bi = r; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+      __syncthreads();
+      bv = red_v[0]; bi = red_i[0];
+#pragma unroll
+      for (int w = 1; w < PT / 64; ++w) {
+        const double ov = red_v[w];
+        const int oi = red_i[w];
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      const int p = bi;
+      if (tid == 0) {
+        ip[cur] = p + 1;
+        if (bv <= 0.0 && info[inst] == 0) info[inst] = cur + 1;
+      }
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = k0 + tid + PT * q;
+        if (r == cur) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) rowcur[c] = P[q][c];
+        }
+        if (r == p) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) rowpiv[c] = P[q][c];
+        }
+      }
+      __syncthreads();
+      const zc pv = rowpiv[jj];
+      const bool nz = (pv.x != 0.0 || pv.y != 0.0);
+      const zc inv = nz ? zrcp(pv) : zmake(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = k0 + tid + PT * q;
+        if (r < n && r >= cur) {
+          if (r == cur) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P[q][c] = rowpiv[c];
+          } else {
+            if (r == p) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) P[q][c] = rowcur[c];
+            }
+            if (nz) {
+              const zc l = zmul(P[q][jj], inv);
+              P[q][jj] = l;
+#pragma unroll
+              for (int c = 0; c < NB; ++c)
+                if (c > jj) P[q][c] = zsub(P[q][c], zmul(l, rowpiv[c]));
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = k0 + tid + PT * q;
+    if (r < n) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nbe) Ai[r + (long long)(k0 + c) * lda] = P[q][c];
+    }
+  }
+}
+
+__global__ void laswp_kernel(int ncols, zc* __restrict__ B, long long ldb, long long strideB,
+                             const int* __restrict__ ipiv, long long strideP, int k1, int k2,
+                             int incx) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int inst = blockIdx.y;
+  if (col >= ncols) return;
+  zc* Bc = B + inst * strideB + (long long)col * ldb;
+  const int* ip = ipiv + inst * strideP;
+  if (incx > 0) {
+    for (int i = k1; i < k2; ++i) {
+      const int p = ip[i] - 1;
+      if (p != i) { zc t = Bc[i]; Bc[i] = Bc[p]; Bc[p] = t; }
+    }
+  } else {
+    for (int i = k2 - 1; i >= k1; --i) {
+      const int p = ip[i] - 1;
+      if (p != i) { zc t = Bc[i]; Bc[i] = Bc[p]; Bc[p] = t; }
+    }
+  }
+}
+
+__global__ void zero_info_kernel(int* info, int batch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < batch) info[i] = 0;
+}
+
+// Effective triangle T = op(A)[o:o+nb, o:o+nb] staged into LDS as T[i][j].
+__device__ inline zc op_elem(const zc* A, long long lda, int op, int i, int j) {
+  if (op == OP_N) return A[i + (long long)j * lda];
+  zc v = A[j + (long long)i * lda];
+  return op == OP_C ? zconj(v) : v;
+}
+
+// Left side: X = T^{-1} B on rows [o, o+nb) of B, one thread per column.
+// lower: forward substitution, else backward.
+__global__ __launch_bounds__(256) void trsm_left_diag_kernel(int o, int nb, int ncols,
+                                                             const zc* __restrict__ A,
+                                                             long long lda, long long strideA,
+                                                             int op, bool lower, bool unit,
+                                                             zc* __restrict__ B, long long ldb,
+                                                             long long strideB) {
+  __shared__ zc T[TB][TB + 1];
+  const int inst = blockIdx.y;
+  const zc* Ai = A + inst * strideA;
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int i = e % TB, j = e / TB;
+    T[i][j] = (i < nb && j < nb) ? op_elem(Ai, lda, op, o + i, o + j) : zmake(0.0, 0.0);
+  }
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncols) return;
+  zc* Bc = B + inst * strideB + (long long)col * ldb + o;
+  zc x[TB];
+#pragma unroll
+  for (int i = 0; i < TB; ++i) x[i] = i < nb ? Bc[i] : zmake(0.0, 0.0);
+  if (lower) {
+#pragma unroll
+    for (int i = 0; i < TB; ++i) {
+      if (i < nb) {
+        zc s = x[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s = zsub(s, zmul(T[i][k], x[k]));
+        x[i] = unit ? s : zmul(s, zrcp(T[i][i]));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = TB - 1; i >= 0; --i) {
+      if (i < nb) {
+        zc s = x[i];
+#pragma unroll
+        for (int k = i + 1; k < TB; ++k)
+          if (k < nb) s = zsub(s, zmul(T[i][k], x[k]));
+        x[i] = unit ? s : zmul(s, zrcp(T[i][i]));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TB; ++i)
+    if (i < nb) Bc[i] = x[i];
+}
+
+// Right side: X = B T^{-1} on columns [o, o+nb) of B, one thread per row.
+// upper: forward over columns, else backward.
+__global__ __launch_bounds__(256) void trsm_right_diag_kernel(int o, int nb, int nrows,
+                                                              const zc* __restrict__ A,
+                                                              long long lda, long long strideA,
+                                                              int op, bool upper, bool unit,
+                                                              zc* __restrict__ B, long long ldb,
+                                                              long long strideB) {
+  __shared__ zc T[TB][TB + 1];
+  const int inst = blockIdx.y;
+  const zc* Ai = A + inst * strideA;
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int i = e % TB, j = e / TB;
+    T[i][j] = (i < nb && j < nb) ? op_elem(Ai, lda, op, o + i, o + j) : zmake(0.0, 0.0);
+  }
+  __syncthreads();
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  zc* Br = B + inst * strideB + row + (long long)o * ldb;
+  zc x[TB];
+#pragma unroll
+  for (int j = 0; j < TB; ++j) x[j] = j < nb ? Br[(long long)j * ldb] : zmake(0.0, 0.0);
+  if (upper) {
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+      if (j < nb) {
+        zc s = x[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s = zsub(s, zmul(x[k], T[k][j]));
+        x[j] = unit ? s : zmul(s, zrcp(T[j][j]));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = TB - 1; j >= 0; --j) {
+      if (j < nb) {
+        zc s = x[j];
+#pragma unroll
+        for (int k = j + 1; k < TB; ++k)
+          if (k < nb) s = zsub(s, zmul(x[k], T[k][j]));
+        x[j] = unit ? s : zmul(s, zrcp(T[j][j]));
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TB; ++j)
+    if (j < nb) Br[(long long)j * ldb] = x[j];
+}
+
+int op_code(char c) {
+  switch (c) {
+    case 'N': case 'n': return OP_N;
+    case 'T': case 't': return OP_T;
+    case 'C': case 'c': return OP_C;
+  }
+  return -1;
+}
+
+// GEMM helper: B_sub -= opA_sub * X_sub (all column-major pointers).
+int gemm_sub(int opA, int m, int n, int k, const zc* A, long long lda, long long strideA,
+             const zc* Bx, long long ldbx, long long strideBx, int opB, zc* C, long long ldc,
+             long long strideC, int batch, hipStream_t s) {
+  GemmArgs g{};
+  g.opA = opA; g.opB = opB;
+  g.m = m; g.n = n; g.k = k;
+  g.alpha = zmake(-1.0, 0.0); g.beta = zmake(1.0, 0.0);
+  g.A = A; g.lda = lda; g.strideA = strideA;
+  g.B = Bx; g.ldb = ldbx; g.strideB = strideBx;
+  g.C = C; g.ldc = ldc; g.strideC = strideC;
+  g.D = C; g.ldd = ldc; g.strideD = strideC;
+  g.batch = batch;
+  return zgemm_launch(g, s);
+}
+
+template <int RPT, int NB>
+int getrf_run(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
+              int* info, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(zero_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nbe = (n - k0) < NB ? (n - k0) : NB;
+    hipLaunchKernelGGL((lu_panel_kernel<RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, A, lda,
+                       strideA, ipiv, strideP, info);
+    // interchanges on the columns left and right of the panel
+    if (k0 > 0)
+      hipLaunchKernelGGL(laswp_kernel, dim3((k0 + 255) / 256, batch), dim3(256), 0, s, k0, A, lda,
+                         strideA, ipiv, strideP, k0, k0 + nbe, 1);
+    const int rest = n - k0 - nbe;
+    if (rest > 0) {
+      zc* A12 = A + k0 + (long long)(k0 + nbe) * lda;
+      hipLaunchKernelGGL(laswp_kernel, dim3((rest + 255) / 256, batch), dim3(256), 0, s, rest,
+                         A + (long long)(k0 + nbe) * lda, lda, strideA, ipiv, strideP, k0,
+                         k0 + nbe, 1);
+      int rc = ztrsm_launch('L', 'L', 'N', 'U', nbe, rest, zmake(1.0, 0.0),
+                            A + k0 + (long long)k0 * lda, lda, strideA, A12, lda, strideA, batch,
+                            s);
+      if (rc) return rc;
+      rc = gemm_sub(OP_N, rest, rest, nbe, A + (k0 + nbe) + (long long)k0 * lda, lda, strideA,
+                    A12, lda, strideA, OP_N, A + (k0 + nbe) + (long long)(k0 + nbe) * lda, lda,
+                    strideA, batch, s);
+      if (rc) return rc;
+    }
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+}  // namespace
+
+int zgetrf_launch(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
+                  int* info, int batch, hipStream_t s) {
+  if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || batch == 0) return Status::OK;
+  if (n > 2048) return Status::TOO_LARGE;
+  if (n <= 256) return getrf_run<1, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
+  if (n <= 512) return getrf_run<2, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
+  if (n <= 1024) return getrf_run<4, 8>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
+  return getrf_run<8, 4>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
+}
+
+int zlaswp_launch(int ncols, zc* B, long long ldb, long long strideB, const int* ipiv,
+                  long long strideP, int k1, int k2, int incx, int batch, hipStream_t s) {
+  if (ncols <= 0 || batch <= 0 || k2 <= k1) return Status::OK;
+  hipLaunchKernelGGL(laswp_kernel, dim3((ncols + 255) / 256, batch), dim3(256), 0, s, ncols, B, ldb,
+                     strideB, ipiv, strideP, k1, k2, incx);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+int ztrsm_launch(char side, char uplo, char trans, char diag, int m, int n, zc alpha,
+                 const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                 long long strideB, int batch, hipStream_t s) {
+  const int op = op_code(trans);
+  const bool left = (side == 'L' || side == 'l');
+  const bool lo = (uplo == 'L' || uplo == 'l');
+  const bool unit = (diag == 'U' || diag == 'u');
+  if (op < 0 || (!left && side != 'R' && side != 'r') || (!lo && uplo != 'U' && uplo != 'u') ||
+      (!unit && diag != 'N' && diag != 'n'))
+    return Status::BAD_ARG;
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  const int na = left ? m : n;
+  if (lda < (na > 1 ? na : 1) || ldb < (m > 1 ? m : 1)) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (alpha.x != 1.0 || alpha.y != 0.0) {
+    int rc = zscale_launch(m, n, alpha, nullptr, 0, B, ldb, strideB, B, ldb, strideB, batch, s);
+    if (rc) return rc;
+  }
+  // effective lower triangle of op(A)?
+  const bool eff_lower = (lo != (op != OP_N));
+  // op(A)[r0:r0+mr, c0:c0+mc] as a GEMM operand
+  auto sub = [&](int r0, int c0) -> const zc* {
+    return op == OP_N ? A + r0 + (long long)c0 * lda : A + c0 + (long long)r0 * lda;
+  };
+  int rc = 0;
+  if (left) {
+    const dim3 grid((n + 255) / 256, batch);
+    if (eff_lower) {
+      for (int r0 = 0; r0 < m; r0 += TB) {
+        const int r1 = (r0 + TB < m) ? r0 + TB : m;
+        hipLaunchKernelGGL(trsm_left_diag_kernel, grid, dim3(256), 0, s, r0, r1 - r0, n, A, lda,
+                           strideA, op, true, unit, B, ldb, strideB);
+        if (r1 < m)
+          rc = gemm_sub(op, m - r1, n, r1 - r0, sub(r1, r0), lda, strideA, B + r0, ldb, strideB,
+                        OP_N, B + r1, ldb, strideB, batch, s);
+        if (rc) return rc;
+      }
+    } else {
+      const int nblk = (m + TB - 1) / TB;
+      for (int bI = nblk - 1; bI >= 0; --bI) {
+        const int r0 = bI * TB;
+        const int r1 = (r0 + TB < m) ? r0 + TB : m;
+        hipLaunchKernelGGL(trsm_left_diag_kernel, grid, dim3(256), 0, s, r0, r1 - r0, n, A, lda,
+                           strideA, op, false, unit, B, ldb, strideB);
+        if (r0 > 0)
+          rc = gemm_sub(op, r0, n, r1 - r0, sub(0, r0), lda, strideA, B + r0, ldb, strideB, OP_N,
+                        B, ldb, strideB, batch, s);
+        if (rc) return rc;
+      }
+    }
+  } else {
+    const dim3 grid((m + 255) / 256, batch);
+    const bool eff_upper = !eff_lower;
+    if (eff_upper) {
+      for (int c0 = 0; c0 < n; c0 += TB) {
+        const int c1 = (c0 + TB < n) ? c0 + TB : n;
+        hipLaunchKernelGGL(trsm_right_diag_kernel, grid, dim3(256), 0, s, c0, c1 - c0, m, A, lda,
+                           strideA, op, true, unit, B, ldb, strideB);
+        if (c1 < n) {
+          // B[:, c1:] -= X[:, c0:c1] op(A)[c0:c1, c1:]
+          GemmArgs g{};
+          g.opA = OP_N; g.opB = op;
+          g.m = m; g.n = n - c1; g.k = c1 - c0;
+          g.alpha = zmake(-1.0, 0.0); g.beta = zmake(1.0, 0.0);
+          g.A = B + (long long)c0 * ldb; g.lda = ldb; g.strideA = strideB;
+          g.B = op == OP_N ? A + c0 + (long long)c1 * lda : A + c1 + (long long)c0 * lda;
+          g.ldb = lda; g.strideB = strideA;
+          g.C = B + (long long)c1 * ldb; g.ldc = ldb; g.strideC = strideB;
+          g.D = B + (long long)c1 * ldb; g.ldd = ldb; g.strideD = strideB;
+          g.batch = batch;
+          rc = zgemm_launch(g, s);
+          if (rc) return rc;
+        }
+      }
+    } else {
+      const int nblk = (n + TB - 1) / TB;
+      for (int bI = nblk - 1; bI >= 0; --bI) {
+        const int c0 = bI * TB;
+        const int c1 = (c0 + TB < n) ? c0 + TB : n;
+        hipLaunchKernelGGL(trsm_right_diag_kernel, grid, dim3(256), 0, s, c0, c1 - c0, m, A, lda,
+                           strideA, op, false, unit, B, ldb, strideB);
+        if (c0 > 0) {
+          // B[:, :c0] -= X[:, c0:c1] op(A)[c0:c1, :c0]
+          GemmArgs g{};
+          g.opA = OP_N; g.opB = op;
+          g.m = m; g.n = c0; g.k = c1 - c0;
+          g.alpha = zmake(-1.0, 0.0); g.beta = zmake(1.0, 0.0);
+          g.A = B + (long long)c0 * ldb; g.lda = ldb; g.strideA = strideB;
+          g.B = op == OP_N ? A + c0 : A + (long long)c0 * lda;
+          g.ldb = lda; g.strideB = strideA;
+          g.C = B; g.ldc = ldb; g.strideC = strideB;
+          g.D = B; g.ldd = ldb; g.strideD = strideB;
+          g.batch = batch;
+          rc = zgemm_launch(g, s);
+          if (rc) return rc;
+        }
+      }
+    }
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+int zgetrs_launch(char trans, int n, int nrhs, const zc* A, long long lda, long long strideA,
+                  const int* ipiv, long long strideP, zc* B, long long ldb, long long strideB,
+                  int batch, hipStream_t s) {
+  const int op = op_code(trans);
+  if (op < 0 || n < 0 || nrhs < 0 || batch < 0) return Status::BAD_ARG;
+  if (lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || nrhs == 0 || batch == 0) return Status::OK;
+  const zc one = zmake(1.0, 0.0);
+  int rc;
+  if (op == OP_N) {
+    rc = zlaswp_launch(nrhs, B, ldb, strideB, ipiv, strideP, 0, n, 1, batch, s);
+    if (rc) return rc;
+    rc = ztrsm_launch('L', 'L', 'N', 'U', n, nrhs, one, A, lda, strideA, B, ldb, strideB, batch, s);
+    if (rc) return rc;
+    return ztrsm_launch('L', 'U', 'N', 'N', n, nrhs, one, A, lda, strideA, B, ldb, strideB, batch,
+                        s);
+  }
+  const char t = op == OP_T ? 'T' : 'C';
+  rc = ztrsm_launch('L', 'U', t, 'N', n, nrhs, one, A, lda, strideA, B, ldb, strideB, batch, s);
+  if (rc) return rc;
+  rc = ztrsm_launch('L', 'L', t, 'U', n, nrhs, one, A, lda, strideA, B, ldb, strideB, batch, s);
+  if (rc) return rc;
+  return zlaswp_launch(nrhs, B, ldb, strideB, ipiv, strideP, 0, n, -1, batch, s);
+}
+
+int zgetri_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
+                  long long strideP, int batch, zc* work, hipStream_t s) {
+  if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || batch == 0) return Status::OK;
+  const long long ws = (long long)n * n;
+  int rc = zcopy2d_launch(n, n, A, lda, strideA, work, n, ws, batch, s);
+  if (rc) return rc;
+  rc = zfill_launch(n, n, zmake(0.0, 0.0), zmake(1.0, 0.0), A, lda, strideA, batch, s);
+  if (rc) return rc;
+  return zgetrs_launch('N', n, n, work, n, ws, ipiv, strideP, A, lda, strideA, batch, s);
+}
+
+}  // namespace negf

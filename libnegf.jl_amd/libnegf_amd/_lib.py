@@ -1,0 +1,105 @@
+"""ctypes binding of libnegf_hip.so (the C ABI in include/negf_hip.h).
+
+The library is the only compute path: when it is missing, or no GPU is
+visible, every entry point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = pathlib.Path(os.environ.get("LIBNEGF_HIP_LIB", _HERE / "libnegf_hip.so"))
+
+c_int32_p = C.POINTER(C.c_int32)
+c_double_p = C.POINTER(C.c_double)
+vp = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+ch = C.c_char
+d = C.c_double
+
+# name -> (restype, argtypes); mirrors include/negf_hip.h
+SIGNATURES = {
+    "negf_version": (C.c_int, []),
+    "negf_status_string": (C.c_char_p, [C.c_int]),
+    "negf_workspace_release": (C.c_int, []),
+    "negf_zgemm": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, vp, i64, c_double_p, vp,
+                             i64, vp]),
+    "negf_zgemm_strided_batched": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, i64, vp,
+                                             i64, i64, c_double_p, vp, i64, i64, i32, vp]),
+    "negf_zgetrf": (C.c_int, [i32, vp, i64, vp, vp, vp]),
+    "negf_zgetrf_strided_batched": (C.c_int, [i32, vp, i64, i64, vp, i64, vp, i32, vp]),
+    "negf_zgetrs": (C.c_int, [ch, i32, i32, vp, i64, vp, vp, i64, vp]),
+    "negf_zgetrs_strided_batched": (C.c_int, [ch, i32, i32, vp, i64, i64, vp, i64, vp, i64, i64,
+                                              i32, vp]),
+    "negf_ztrsm": (C.c_int, [ch, ch, ch, ch, i32, i32, c_double_p, vp, i64, vp, i64, vp]),
+    "negf_ztrsm_strided_batched": (C.c_int, [ch, ch, ch, ch, i32, i32, c_double_p, vp, i64, i64,
+                                             vp, i64, i64, i32, vp]),
+    "negf_zgetri": (C.c_int, [i32, vp, i64, vp, vp, vp]),
+    "negf_zgetri_strided_batched": (C.c_int, [i32, vp, i64, i64, vp, i64, vp, i32, vp]),
+    "negf_zinv_strided_batched": (C.c_int, [i32, vp, i64, i64, vp, i64, i64, i32, vp, vp]),
+    "negf_zadjoint_strided_batched": (C.c_int, [i32, i32, vp, i64, i64, vp, i64, i64, i32, vp]),
+    "negf_zcopy_strided_batched": (C.c_int, [i32, i32, vp, i64, i64, vp, i64, i64, i32, vp]),
+    "negf_zfill_strided_batched": (C.c_int, [i32, i32, c_double_p, c_double_p, vp, i64, i64, i32,
+                                             vp]),
+    "negf_rgf": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32, vp, vp]),
+    "negf_rgf_rkd": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, vp,
+                               vp, vp, vp, vp, i64, vp, i64, vp, i32, vp, vp]),
+    "negf_rgf_keldysh_energies": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d, d, vp, vp,
+                                            vp, vp, i32, vp]),
+    "negf_rgf_keldysh_energies_workspace": (C.c_int64, [i32, i32, i32, i32]),
+}
+
+_lib = None
+
+
+class NegfError(RuntimeError):
+    """A negative status from the library (bad argument, HIP error, ...)."""
+
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f"{fn} failed with status {status}: {msg}")
+        self.status = status
+
+
+class SingularError(RuntimeError):
+    """A zero pivot / singular Schur complement (LAPACK info > 0).  The
+    reference prints and exits there (src/backend_cpu.jl:151-155)."""
+
+    def __init__(self, where: str, info):
+        super().__init__(f"{where}: singular pivot, info = {info}")
+        self.info = info
+
+
+def load(path: pathlib.Path | None = None) -> C.CDLL:
+    """Load the shared library (once) and declare every signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(
+            f"{p} is not built; run `python libnegf.jl_amd/build_lib.py` (no fallback path exists)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> int:
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if name.endswith("_workspace"):
+        return rc
+    if rc < 0:
+        raise NegfError(name, rc, lib.negf_status_string(rc).decode())
+    return rc
+
+
+def cplx(z) -> C.Array:
+    z = complex(z)
+    return (C.c_double * 2)(z.real, z.imag)

@@ -1,0 +1,329 @@
+"""Host-side mirror of LibNEGF.jl's interface for the NEGF hot path.
+
+Names, argument meaning and error behaviour follow the reference so the
+parity tests read like its own tests; every call lands in libnegf_hip.so.
+
+Sweeps (``src/matinvertndiag_rgf.jl``, ``src/keldyshndiag.jl``):
+  allocate_aux_data_RGF, bndiag_of_inv_rgf_local, allocate_aux_data_Keldysh,
+  keldyshndiag, rgf_keldysh_transmission, rgf_keldysh_energies.
+Backend ops (``src/backend_cpu.jl``) on column-major complex128 device
+tensors: be_gemm, be_lu, be_mldivide, be_getrf, be_getri, be_trsm,
+be_ctranspose, be_copy_in_hw, be_fill, be_identity, be_zero_array, be_inv.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, cplx, SingularError
+from .btd import DeviceBlockTridiag, Layout
+
+__all__ = [
+    "AuxDataRGF", "AuxDataKeldysh", "allocate_aux_data_RGF", "allocate_aux_data_Keldysh",
+    "bndiag_of_inv_rgf_local", "keldyshndiag", "rgf_keldysh_transmission",
+    "rgf_keldysh_energies", "GpuLU", "be_gemm", "be_lu", "be_mldivide", "be_getrf", "be_getri",
+    "be_trsm", "be_ctranspose", "be_copy_in_hw", "be_fill", "be_identity", "be_zero_array",
+    "be_inv", "col_major", "release_workspace",
+]
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("libnegf_amd needs a visible GPU; there is no CPU fallback")
+    _lib.load()
+
+
+def release_workspace():
+    call("negf_workspace_release")
+
+
+def _check_info(where, info: torch.Tensor):
+    bad = info.cpu()
+    if bool((bad != 0).any()):
+        raise SingularError(where, bad.tolist())
+
+
+# ----------------------------------------------------------------------------- sweeps
+@dataclasses.dataclass
+class AuxDataRGF:
+    """``AuxDataRGF`` (``src/matinvertndiag_rgf.jl:10-14``).  The device
+    sweep keeps its buffers in the library's cached workspace; the holder
+    carries the per-instance info array and the ``buildFullInv`` flag."""
+
+    info: torch.Tensor
+    buildFullInv: bool = False
+
+
+def allocate_aux_data_RGF(M: DeviceBlockTridiag) -> AuxDataRGF:
+    """``src/matinvertndiag_rgf.jl:80-97``."""
+    return AuxDataRGF(torch.zeros(M.batch, dtype=torch.int32, device=M.diag.device))
+
+
+def bndiag_of_inv_rgf_local(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag,
+                            auxData: AuxDataRGF | None = None):
+    """band(Min^{-1}) into Mout (``src/matinvertndiag_rgf.jl:114-200``).
+
+    Min may hold a batch of instances of one block structure; Mout must match.
+    Raises SingularError where the reference would print and exit()."""
+    _require_gpu()
+    if auxData is not None and auxData.buildFullInv:
+        raise NotImplementedError("buildFullInv: full inverse blocks are produced by the DD-RGF "
+                                  "subdomain solver, not by this entry point")
+    if Mout.layout.b != Min.layout.b or Mout.batch != Min.batch or Mout.lower is None:
+        raise ValueError("Mout must be a non-Hermitian BlockMatrix shaped like Min")
+    L = Min.layout
+    info = auxData.info if auxData is not None else torch.zeros(Min.batch, dtype=torch.int32,
+                                                                 device=Min.diag.device)
+    call("negf_rgf", L.N, L.sizes_c(), _ptr(Min.diag), L.nd, _ptr(Min.upper), max(L.no, 1),
+         _ptr(Min.lower) if Min.lower is not None else None, max(L.no, 1),
+         _ptr(Mout.diag), _ptr(Mout.upper), _ptr(Mout.lower), Min.batch, _ptr(info), _stream())
+    _check_info("bndiag_of_inv_rgf_local", info)
+    return Mout
+
+
+@dataclasses.dataclass
+class AuxDataKeldysh:
+    """``AuxDataKeldysh`` (``src/keldyshndiag.jl:7-12``): ``buffS`` receives
+    the result (diagonal and upper blocks of G^n, Hermitian storage)."""
+
+    auxDataRGF: AuxDataRGF
+    buffS: DeviceBlockTridiag
+
+
+def allocate_aux_data_Keldysh(M: DeviceBlockTridiag, S: DeviceBlockTridiag) -> AuxDataKeldysh:
+    """``src/keldyshndiag.jl:79-107``."""
+    return AuxDataKeldysh(allocate_aux_data_RGF(M), S.similar(hermitian=True))
+
+
+def keldyshndiag(M: DeviceBlockTridiag, S: DeviceBlockTridiag, auxData: AuxDataKeldysh,
+                 Gr: DeviceBlockTridiag | None = None):
+    """band(M^{-1} S M^{-H}) into ``auxData.buffS`` (``src/keldyshndiag.jl:120-131``,
+    ``keldyshndiag_v3!`` ``:166-318``).  S is Hermitian block tridiagonal in
+    Hermitian storage.  With ``Gr`` given, the band of M^{-1} comes out of the
+    same shared upward pass (what the reference gets from a separate
+    ``bndiag_of_inv_rgf_local!`` call)."""
+    _require_gpu()
+    if not S.hermitian:
+        raise ValueError("the central operator S must use Hermitian storage")
+    L = M.layout
+    out = auxData.buffS
+    info = auxData.auxDataRGF.info
+    call("negf_rgf_rkd", L.N, L.sizes_c(), _ptr(M.diag), L.nd, _ptr(M.upper), max(L.no, 1),
+         _ptr(M.lower) if M.lower is not None else None, max(L.no, 1),
+         _ptr(S.diag), L.nd if S.batch > 1 else 0, _ptr(S.upper), max(L.no, 1) if S.batch > 1 else 0,
+         _ptr(Gr.diag) if Gr is not None else None, _ptr(Gr.upper) if Gr is not None else None,
+         _ptr(Gr.lower) if Gr is not None else None,
+         _ptr(out.diag), _ptr(out.upper), None, 0, None, 0, None, M.batch, _ptr(info), _stream())
+    _check_info("keldyshndiag", info)
+    return out
+
+
+def rgf_keldysh_transmission(M: DeviceBlockTridiag, S: DeviceBlockTridiag | None,
+                             gamma_L: torch.Tensor, gamma_R: torch.Tensor,
+                             Gr: DeviceBlockTridiag | None = None,
+                             Gn: DeviceBlockTridiag | None = None):
+    """One shared upward pass, then any of: band(G^R) into Gr, band(G^n)
+    into Gn, and T = Re Tr(Gamma_L G_{1N} Gamma_R G_{1N}^H) (returned, one per
+    instance).  gamma_L / gamma_R: (b_0, b_0) / (b_{N-1}, b_{N-1}) column-major
+    device tensors, shared by all instances."""
+    _require_gpu()
+    L = M.layout
+    info = torch.zeros(M.batch, dtype=torch.int32, device=M.diag.device)
+    T = torch.zeros(M.batch, dtype=torch.float64, device=M.diag.device)
+    gl, gr = col_major(gamma_L), col_major(gamma_R)
+    call("negf_rgf_rkd", L.N, L.sizes_c(), _ptr(M.diag), L.nd, _ptr(M.upper), max(L.no, 1),
+         _ptr(M.lower) if M.lower is not None else None, max(L.no, 1),
+         _ptr(S.diag) if S is not None else None, (L.nd if S is not None and S.batch > 1 else 0),
+         _ptr(S.upper) if S is not None else None,
+         (max(L.no, 1) if S is not None and S.batch > 1 else 0),
+         _ptr(Gr.diag) if Gr is not None else None, _ptr(Gr.upper) if Gr is not None else None,
+         _ptr(Gr.lower) if Gr is not None else None,
+         _ptr(Gn.diag) if Gn is not None else None, _ptr(Gn.upper) if Gn is not None else None,
+         _ptr(gl), 0, _ptr(gr), 0, _ptr(T), M.batch, _ptr(info), _stream())
+    _check_info("rgf_keldysh_transmission", info)
+    return T
+
+
+def _pack_blocks(blocks, device):
+    flat = np.concatenate([np.asarray(b).ravel(order="F") for b in blocks]) if blocks else \
+        np.zeros(1, dtype=np.complex128)
+    return torch.from_numpy(flat.astype(np.complex128)).to(device)
+
+
+class DeviceHamiltonianGPU:
+    """Energy-independent inputs of the energy-batched workload, resident in
+    HBM: H diagonal blocks, H_{i,i+1}, Gamma_L, Gamma_R (column-major)."""
+
+    def __init__(self, dev, device="cuda"):
+        self.N = dev.nblocks
+        self.b = dev.b
+        self.H_diag = _pack_blocks(dev.H_diag, device)
+        self.H_upper = _pack_blocks(dev.H_upper, device)
+        self.gamma_L = _pack_blocks([dev.gamma_L], device)
+        self.gamma_R = _pack_blocks([dev.gamma_R], device)
+
+
+def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float,
+                         mu_L: float, mu_R: float, kT: float, GR_diag: torch.Tensor | None = None,
+                         Gless_diag: torch.Tensor | None = None,
+                         transmission: torch.Tensor | None = None, wave: int = 0,
+                         check: bool = True):
+    """G^R and G^< diagonal blocks and T(E) for every energy (device float64
+    tensor).  GR_diag / Gless_diag: complex128 device tensors of shape
+    (nE, N*b*b) (packed diag layout per energy) or None."""
+    _require_gpu()
+    nE = int(energies.numel())
+    info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
+    call("negf_rgf_keldysh_energies", H.N, H.b, _ptr(H.H_diag), _ptr(H.H_upper),
+         _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta), float(mu_L),
+         float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag), _ptr(transmission), _ptr(info),
+         int(wave), _stream())
+    if check:
+        _check_info("rgf_keldysh_energies", info)
+    return info
+
+
+def energies_workspace_bytes(N: int, b: int, nE: int, wave: int) -> int:
+    return int(call("negf_rgf_keldysh_energies_workspace", N, b, nE, wave))
+
+
+# ----------------------------------------------------------------------------- per-op
+def col_major(x: torch.Tensor) -> torch.Tensor:
+    """Column-major (Fortran) copy of a 2-D tensor unless it already is one."""
+    if x.dim() != 2:
+        raise ValueError("expected a matrix")
+    if x.stride(0) == 1 and x.stride(1) >= max(1, x.shape[0]):
+        return x
+    return x.transpose(0, 1).contiguous().transpose(0, 1)
+
+
+def _ld(x: torch.Tensor) -> int:
+    if x.stride(0) != 1:
+        raise ValueError("matrices must be column-major (use col_major())")
+    return max(int(x.stride(1)), 1)
+
+
+def be_zero_array(shape, device="cuda") -> torch.Tensor:
+    """``src/backend_cpu.jl:50-52``."""
+    return col_major(torch.zeros(shape, dtype=torch.complex128, device=device))
+
+
+def be_identity(n: int, device="cuda") -> torch.Tensor:
+    """``src/backend_cpu.jl:54-56``."""
+    out = be_zero_array((n, n), device)
+    call("negf_zfill_strided_batched", n, n, cplx(0), cplx(1), _ptr(out), _ld(out), 0, 1,
+         _stream())
+    return out
+
+
+def be_fill(M: torch.Tensor, x) -> torch.Tensor:
+    """``src/backend_cpu.jl:86-88``."""
+    call("negf_zfill_strided_batched", M.shape[0], M.shape[1], cplx(x), cplx(x), _ptr(M), _ld(M),
+         0, 1, _stream())
+    return M
+
+
+def be_copy_in_hw(Mout: torch.Tensor, Min: torch.Tensor) -> torch.Tensor:
+    """``src/backend_cpu.jl:42-44``."""
+    call("negf_zcopy_strided_batched", Min.shape[0], Min.shape[1], _ptr(Min), _ld(Min), 0,
+         _ptr(Mout), _ld(Mout), 0, 1, _stream())
+    return Mout
+
+
+def be_ctranspose(Mout: torch.Tensor, Min: torch.Tensor) -> torch.Tensor:
+    """``adjoint!`` (``src/backend_cpu.jl:58-68``)."""
+    call("negf_zadjoint_strided_batched", Min.shape[0], Min.shape[1], _ptr(Min), _ld(Min), 0,
+         _ptr(Mout), _ld(Mout), 0, 1, _stream())
+    return Mout
+
+
+def be_gemm(tA: str, tB: str, alpha, A: torch.Tensor, B: torch.Tensor, beta,
+            C: torch.Tensor) -> torch.Tensor:
+    """zgemm (``src/backend_cpu.jl:208-219``); C is updated in place."""
+    m, n = C.shape
+    k = A.shape[1] if tA == "N" else A.shape[0]
+    call("negf_zgemm", tA.encode(), tB.encode(), m, n, k, cplx(alpha), _ptr(A), _ld(A), _ptr(B),
+         _ld(B), cplx(beta), _ptr(C), _ld(C), _stream())
+    return C
+
+
+class GpuLU:
+    """LU holder of the AMD backend (the counterpart of ``CpuLU``,
+    ``src/backend_cpu.jl:7-10``): LAPACK-layout factors and 1-based pivots on
+    the device."""
+
+    def __init__(self, A: torch.Tensor, piv: torch.Tensor):
+        self.A = A
+        self.piv = piv
+
+
+def be_lu(Mout: GpuLU | None, Min: torch.Tensor) -> GpuLU:
+    """copy + zgetrf (``src/backend_cpu.jl:147-169``); SingularError where the
+    reference exits."""
+    n = Min.shape[0]
+    if Mout is None:
+        Mout = GpuLU(be_zero_array((n, n), Min.device),
+                     torch.zeros(n, dtype=torch.int32, device=Min.device))
+    be_copy_in_hw(Mout.A, Min)
+    info = torch.zeros(1, dtype=torch.int32, device=Min.device)
+    call("negf_zgetrf", n, _ptr(Mout.A), _ld(Mout.A), _ptr(Mout.piv), _ptr(info), _stream())
+    _check_info("be_lu", info)
+    return Mout
+
+
+def be_mldivide(trans: str, Mout: torch.Tensor, Min: torch.Tensor, Mlu: GpuLU) -> torch.Tensor:
+    """Mout = op(LU)^{-1} Min via copy + zgetrs (``src/backend_cpu.jl:193-206``)."""
+    be_copy_in_hw(Mout, Min)
+    n, nrhs = Mout.shape
+    call("negf_zgetrs", trans.encode(), n, nrhs, _ptr(Mlu.A), _ld(Mlu.A), _ptr(Mlu.piv),
+         _ptr(Mout), _ld(Mout), _stream())
+    return Mout
+
+
+def be_getrf(A: torch.Tensor) -> torch.Tensor:
+    """zgetrf in place with the pivots discarded (``src/backend_cpu.jl:255-265``)."""
+    n = A.shape[0]
+    piv = torch.zeros(n, dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("negf_zgetrf", n, _ptr(A), _ld(A), _ptr(piv), _ptr(info), _stream())
+    return A
+
+
+def be_getri(A: torch.Tensor) -> torch.Tensor:
+    """zgetri with identity pivots (``src/backend_cpu.jl:229-239``)."""
+    n = A.shape[0]
+    work = torch.empty(n * n, dtype=torch.complex128, device=A.device)
+    call("negf_zgetri", n, _ptr(A), _ld(A), None, _ptr(work), _stream())
+    return A
+
+
+def be_trsm(side: str, ul: str, tA: str, dA: str, alpha, A: torch.Tensor,
+            B: torch.Tensor) -> torch.Tensor:
+    """ztrsm in place on B (``src/backend_cpu.jl:241-253``)."""
+    m, n = B.shape
+    call("negf_ztrsm", side.encode(), ul.encode(), tA.encode(), dA.encode(), m, n, cplx(alpha),
+         _ptr(A), _ld(A), _ptr(B), _ld(B), _stream())
+    return B
+
+
+def be_inv(M: torch.Tensor) -> torch.Tensor:
+    """Explicit inverse (``src/backend_cpu.jl:143-145``) by Gauss-Jordan."""
+    n = M.shape[0]
+    out = be_zero_array((n, n), M.device)
+    info = torch.zeros(1, dtype=torch.int32, device=M.device)
+    call("negf_zinv_strided_batched", n, _ptr(M), _ld(M), 0, _ptr(out), _ld(out), 0, 1,
+         _ptr(info), _stream())
+    _check_info("be_inv", info)
+    return out

@@ -1,0 +1,36 @@
+"""Algorithmic flop counts of the NEGF path (LAPACK nominal real flops).
+
+zgemm(m,n,k) = 8mnk, zgetrf(n) = 8n^3/3, zgetrs(n, nrhs=n) = 8n^3,
+zgetri(n) = 16n^3/3, each following the reference's call sequence:
+
+  F_rgf    = (58 2/3 (N-1) + 10 2/3) b^3   bndiag_of_inv_rgf_local!
+                                            (src/matinvertndiag_rgf.jl:114-200)
+  F_rkd    = (74 2/3 (N-1) + 48 (N-2) + 16 N + 18 2/3) b^3
+                                            keldyshndiag_v3! (src/keldyshndiag.jl:166-318)
+  F_upward = (26 2/3 (N-1) + 2 2/3) b^3     the upward pass both share
+  F_T      = (8 (N-2) + 16) b^3             first block row + Tr(Gamma_L G Gamma_R G^H)
+
+One "RGF+Keldysh" energy = F_rgf + F_rkd - F_upward + F_T (the shared upward
+pass counted once): 1.511e12 flop at b = 512, N = 64.  Throughput figures
+credit these algorithmic counts, whatever the kernels actually execute.
+"""
+
+
+def flops_rgf(N: int, b: int) -> float:
+    return ((58 + 2 / 3) * (N - 1) + (10 + 2 / 3)) * b ** 3
+
+
+def flops_rkd(N: int, b: int) -> float:
+    return ((74 + 2 / 3) * (N - 1) + 48 * (N - 2) + 16 * N + (18 + 2 / 3)) * b ** 3
+
+
+def flops_upward(N: int, b: int) -> float:
+    return ((26 + 2 / 3) * (N - 1) + (2 + 2 / 3)) * b ** 3
+
+
+def flops_transmission(N: int, b: int) -> float:
+    return (8 * (N - 2) + 16) * b ** 3
+
+
+def flops_rgf_keldysh_energy(N: int, b: int) -> float:
+    return flops_rgf(N, b) + flops_rkd(N, b) - flops_upward(N, b) + flops_transmission(N, b)

@@ -1,0 +1,150 @@
+"""Per-op kernels of libnegf_hip.so vs numpy/scipy (FP64 references).
+
+Covers every op the sweeps and the selected inverse call (src/backend_cpu.jl):
+zgemm (all nine N/T/C combinations, ragged sizes, broadcast strides),
+Gauss-Jordan inverse, getrf (pivots equal LAPACK's), getrs, trsm (all
+side/uplo/trans/diag variants), getri, adjoint, fill.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import relfro
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from libnegf_amd import api  # noqa: E402
+from libnegf_amd._lib import call, cplx  # noqa: E402
+import ctypes as C  # noqa: E402
+
+
+def _rand(rng, m, n):
+    return np.asfortranarray(rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n)))
+
+
+def _dev(x):
+    return api.col_major(torch.from_numpy(np.ascontiguousarray(x)).to("cuda"))
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _op(x, o):
+    return x if o == "N" else (x.T if o == "T" else x.conj().T)
+
+
+@pytest.mark.parametrize("tA", ["N", "T", "C"])
+@pytest.mark.parametrize("tB", ["N", "T", "C"])
+@pytest.mark.parametrize("mnk", [(64, 64, 64), (37, 70, 19), (130, 5, 67), (1, 1, 1)])
+def test_zgemm(tA, tB, mnk):
+    m, n, k = mnk
+    print(f"zgemm {tA}{tB} {mnk}", flush=True)
+    rng = np.random.default_rng(m * 7 + n)
+    A = _rand(rng, *((m, k) if tA == "N" else (k, m)))
+    B = _rand(rng, *((k, n) if tB == "N" else (n, k)))
+    Cm = _rand(rng, m, n)
+    alpha, beta = 0.7 - 0.2j, -1.3 + 0.5j
+    dC = _dev(Cm)
+    api.be_gemm(tA, tB, alpha, _dev(A), _dev(B), beta, dC)
+    ref = alpha * _op(A, tA) @ _op(B, tB) + beta * Cm
+    assert relfro(_host(dC), ref) < 1e-14
+
+
+def test_zgemm_beta_zero_ignores_nan_and_batched_broadcast():
+    print("zgemm batched broadcast", flush=True)
+    rng = np.random.default_rng(3)
+    b, batch = 96, 5
+    A = _rand(rng, b, b)
+    Bs = [_rand(rng, b, b) for _ in range(batch)]
+    dA = _dev(A)
+    dB = torch.from_numpy(np.stack([x.ravel(order="F") for x in Bs])).to("cuda")
+    dC = torch.full((batch, b * b), float("nan"), dtype=torch.complex128, device="cuda")
+    call("negf_zgemm_strided_batched", b"N", b"C", b, b, b, cplx(1), C.c_void_p(dA.data_ptr()), b, 0,
+         C.c_void_p(dB.data_ptr()), b, b * b, cplx(0), C.c_void_p(dC.data_ptr()), b, b * b, batch,
+         None)
+    torch.cuda.synchronize()
+    out = dC.cpu().numpy()
+    for i in range(batch):
+        assert relfro(out[i].reshape(b, b).T, A @ Bs[i].conj().T) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 100, 256, 300, 512, 600])
+def test_zinv(n):
+    print(f"zinv n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = _rand(rng, n, n) + 2 * np.sqrt(n) * np.eye(n) * (rng.random() > 0.5)
+    inv = _host(api.be_inv(_dev(A)))
+    ref = np.linalg.inv(A)
+    cond = np.linalg.cond(A)
+    assert relfro(inv, ref) < 1e-15 * cond * max(n, 10)
+
+
+def test_zinv_singular_reports_info():
+    print("zinv singular", flush=True)
+    A = np.zeros((8, 8), complex, order="F")
+    A[:4, :4] = np.eye(4)
+    with pytest.raises(api.SingularError):
+        api.be_inv(_dev(A))
+
+
+@pytest.mark.parametrize("n", [5, 64, 200, 512])
+def test_zgetrf_matches_lapack(n):
+    print(f"zgetrf n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    A = _rand(rng, n, n)
+    lu = api.be_lu(None, _dev(A))
+    lu_ref, piv_ref, info = sla.lapack.zgetrf(A)
+    assert np.array_equal(lu.piv.cpu().numpy(), piv_ref + 1)
+    assert relfro(_host(lu.A), lu_ref) < 1e-12
+
+
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_zgetrs(trans):
+    print(f"zgetrs {trans}", flush=True)
+    rng = np.random.default_rng(7)
+    n, nrhs = 150, 33
+    A = _rand(rng, n, n)
+    B = _rand(rng, n, nrhs)
+    lu = api.be_lu(None, _dev(A))
+    X = api.be_mldivide(trans, api.be_zero_array((n, nrhs)), _dev(B), lu)
+    ref = np.linalg.solve(_op(A, trans), B)
+    assert relfro(_host(X), ref) < 1e-11
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("tA", ["N", "T", "C"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_ztrsm(side, uplo, tA, diag):
+    print(f"ztrsm {side}{uplo}{tA}{diag}", flush=True)
+    rng = np.random.default_rng(11)
+    m, n = 70, 45
+    na = m if side == "L" else n
+    A = _rand(rng, na, na) + 4 * np.eye(na)
+    B = _rand(rng, m, n)
+    alpha = 1.5 - 0.5j
+    X = api.be_trsm(side, uplo, tA, diag, alpha, _dev(A), _dev(B))
+    ref = sla.blas.ztrsm(alpha, A, B, side=0 if side == "L" else 1, lower=uplo == "L",
+                         trans_a={"N": 0, "T": 1, "C": 2}[tA], diag=diag == "U")
+    assert relfro(_host(X), ref) < 1e-12
+
+
+def test_zgetri_identity_pivots_and_adjoint_fill():
+    print("zgetri / adjoint / fill", flush=True)
+    rng = np.random.default_rng(5)
+    n = 90
+    A = _rand(rng, n, n) + 3 * n * np.eye(n)        # diagonally dominant: no row swaps
+    dA = _dev(A)
+    api.be_getrf(dA)
+    api.be_getri(dA)
+    assert relfro(_host(dA), np.linalg.inv(A)) < 1e-13
+    B = _rand(rng, 33, 71)
+    out = api.be_zero_array((71, 33))
+    api.be_ctranspose(out, _dev(B))
+    assert np.array_equal(_host(out), B.conj().T)
+    I = api.be_identity(17)
+    assert np.array_equal(_host(I), np.eye(17))
+    api.be_fill(I, 2 - 1j)
+    assert np.all(_host(I) == 2 - 1j)
