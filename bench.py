@@ -43,7 +43,25 @@ def parse():
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gemm-reps", type=int, default=20)
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the dominant-kernel timing plus a plain copy of known size "
+                         "(the counter-calibration run for rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def copy_calibration(torch, nbytes=1 << 30):
+    """One plain device copy of `nbytes` through negf_zcopy (the HBM-traffic
+    counter calibration: FETCH_SIZE / WRITE_SIZE of this dispatch over
+    nbytes gives the counters' scale)."""
+    from libnegf_amd._lib import call
+    n = nbytes // 16
+    src = torch.ones(n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    s = torch.cuda.current_stream()
+    call("negf_zcopy_strided_batched", n, 1, C.c_void_p(src.data_ptr()), n, 0,
+         C.c_void_p(dst.data_ptr()), n, 0, 1, C.c_void_p(s.cuda_stream))
+    torch.cuda.synchronize()
+    return nbytes
 
 
 def gemm_roofline(torch, api, b, batch, reps):
@@ -130,6 +148,12 @@ def main():
         sys.exit(1)
     cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
     N, b, nE = a.N, a.b, a.energies
+    if a.gemm_only:
+        w = a.wave or 64
+        nbytes = copy_calibration(torch)
+        roof = gemm_roofline(torch, api, b, w, a.gemm_reps)
+        print(json.dumps({"gemm_only": True, "copy_bytes": nbytes, "roofline": roof}))
+        return
     eta, kT, muL, muR = cfg["eta"], cfg["kT"], cfg["mu_L"], cfg["mu_R"]
     dev = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
     Es = synthetic.energy_grid(nE, cfg["lo"], cfg["hi"])

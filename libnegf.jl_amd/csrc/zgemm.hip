@@ -44,7 +44,10 @@ struct GemmGroup {
   long long grid_padded;          // multiple of 8
 };
 
-template <int OPA, int OPB>
+// SMALLK = 1 tags the instantiation used for k <= 32 (the rank-NB updates of
+// the Gauss-Jordan / LU panels and k = 0 shifts): the same code, a separate
+// symbol, so profiles separate the memory-bound updates from the full GEMMs.
+template <int OPA, int OPB, int SMALLK>
 __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
   // XCD-aware remap of the flat workgroup index.
   const long long bid = blockIdx.x;
@@ -163,8 +166,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
     }
   }
 
-  // Epilogue.  Output layout of the 16x16 f64 MFMA: lane l, element r holds
-  // row 4*(l/16) + r, column l % 16.
+  // Epilogue.  Output layout of the 16x16 f64 MFMA on gfx950 (measured):
+  // lane l, element e holds row l/16 + 4*e, column l % 16.
   zc* __restrict__ D = g.D + inst * g.strideD;
   const zc* __restrict__ C = g.C ? g.C + inst * g.strideC : nullptr;
   const bool use_c = (g.beta.x != 0.0 || g.beta.y != 0.0) && C != nullptr;
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
       const int j = j0 + wn * 32 + c * 16 + li;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = i0 + wm * 32 + r * 16 + 4 * kq + e;
+        const int i = i0 + wm * 32 + r * 16 + kq + 4 * e;
         if (i < m && j < n) {
           const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
           zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
@@ -192,8 +195,12 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
 
 template <int OPA, int OPB>
 void launch_t(const GemmGroup& G, hipStream_t s) {
-  hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB>), dim3((unsigned)G.grid_padded), dim3(256), 0, s,
-                     G);
+  if (G.p[0].k <= 2 * TK)
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, 1>), dim3((unsigned)G.grid_padded), dim3(256),
+                       0, s, G);
+  else
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, 0>), dim3((unsigned)G.grid_padded), dim3(256),
+                       0, s, G);
 }
 
 int check_args(const GemmArgs& g) {
@@ -225,9 +232,15 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   for (int i = 0; i < count; ++i) {
     int rc = check_args(g[i]);
     if (rc) return rc;
+  }
+  // one launch per run of problems with the same shape and op codes
+  for (int i = 1; i < count; ++i) {
     if (g[i].m != g[0].m || g[i].n != g[0].n || g[i].k != g[0].k || g[i].opA != g[0].opA ||
-        g[i].opB != g[0].opB || g[i].batch != g[0].batch)
-      return Status::BAD_ARG;
+        g[i].opB != g[0].opB || g[i].batch != g[0].batch) {
+      int rc = zgemm_group_launch(g, i, s);
+      if (rc) return rc;
+      return zgemm_group_launch(g + i, count - i, s);
+    }
   }
   const GemmArgs& g0 = g[0];
   if (g0.m == 0 || g0.n == 0 || g0.batch == 0) return Status::OK;

@@ -1,0 +1,147 @@
+"""CPU-only checks: the oracle against the committed golden fixtures, the
+host-side logic (packed layout, generator properties, flop counts), and that
+libnegf_hip.so loads and exports every entry point include/negf_hip.h
+declares (no compute calls: there may be no GPU here)."""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+from conftest import relfro
+from libnegf_amd import synthetic, flops_rgf_keldysh_energy, flops_rgf, flops_rkd
+from libnegf_amd.synthetic import BlockTridiag, fermi
+from oracle.rgf import bndiag_of_inv_rgf_local, rgf_upward, first_row_chain
+from oracle.keldysh import keldyshndiag
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+GOLD = ROOT / "tests" / "golden"
+
+
+def unpack(flat, shapes):
+    out, o = [], 0
+    for m, n in shapes:
+        out.append(np.asfortranarray(flat[o:o + m * n].reshape(n, m).T))
+        o += m * n
+    return out
+
+
+def load_btd(z, prefix, sizes, hermitian=False):
+    N = len(sizes)
+    diag = unpack(z[f"{prefix}_diag"], [(s, s) for s in sizes])
+    up = unpack(z[f"{prefix}_upper"], [(sizes[i], sizes[i + 1]) for i in range(N - 1)])
+    lo = None if hermitian else unpack(z[f"{prefix}_lower"],
+                                       [(sizes[i + 1], sizes[i]) for i in range(N - 1)])
+    return BlockTridiag(diag, up, lo)
+
+
+@pytest.mark.parametrize("name", ["rgf_workload_N4_b8.npz", "rgf_uneven_N5.npz"])
+def test_oracle_rgf_golden(name):
+    z = np.load(GOLD / name)
+    sizes = [int(x) for x in z["sizes"]]
+    A = load_btd(z, "A", sizes)
+    G = bndiag_of_inv_rgf_local(A)
+    N = len(sizes)
+    gd = unpack(z["G_diag"], [(s, s) for s in sizes])
+    gu = unpack(z["G_upper"], [(sizes[i], sizes[i + 1]) for i in range(N - 1)])
+    gl = unpack(z["G_lower"], [(sizes[i + 1], sizes[i]) for i in range(N - 1)])
+    for i in range(N):
+        assert relfro(G[(i, i)], gd[i]) < 1e-13
+    for i in range(N - 1):
+        assert relfro(G[(i, i + 1)], gu[i]) < 1e-13
+        assert relfro(G[(i + 1, i)], gl[i]) < 1e-13
+
+
+def test_oracle_keldysh_golden():
+    z = np.load(GOLD / "keldysh_N5.npz")
+    sizes = [int(x) for x in z["sizes"]]
+    A = load_btd(z, "A", sizes)
+    S = load_btd(z, "S", sizes, hermitian=True)
+    nd, nu = keldyshndiag(A, S.diag, S.upper)
+    N = len(sizes)
+    rd = unpack(z["Gn_diag"], [(s, s) for s in sizes])
+    ru = unpack(z["Gn_upper"], [(sizes[i], sizes[i + 1]) for i in range(N - 1)])
+    for i in range(N):
+        assert relfro(nd[i], rd[i]) < 1e-13
+    for i in range(N - 1):
+        assert relfro(nu[i], ru[i]) < 1e-13
+
+
+def test_oracle_energies_golden():
+    z = np.load(GOLD / "energies_N3_b6.npz")
+    N, b = int(z["N"]), int(z["b"])
+    dev = synthetic.DeviceHamiltonian(unpack(z["H_diag"], [(b, b)] * N),
+                                      unpack(z["H_upper"], [(b, b)] * (N - 1)),
+                                      unpack(z["gamma_L"], [(b, b)])[0],
+                                      unpack(z["gamma_R"], [(b, b)])[0], seed=-1)
+    for e, E in enumerate(z["energies"]):
+        A = synthetic.assemble_A(dev, E, float(z["eta"]))
+        buf = rgf_upward(A)
+        G = bndiag_of_inv_rgf_local(A, upward=buf)
+        g1n = first_row_chain(A, G, buf)[-1]
+        T = np.real(np.trace(dev.gamma_L @ g1n @ dev.gamma_R @ g1n.conj().T))
+        assert abs(T - z["T"][e]) < 1e-12
+        gr = unpack(z["GR_diag"][e], [(b, b)] * N)
+        for i in range(N):
+            assert relfro(G[(i, i)], gr[i]) < 1e-13
+
+
+def test_generator_properties():
+    dev = synthetic.device_hamiltonian(5, 12, seed=3)
+    for h in dev.H_diag:
+        assert np.allclose(h, h.conj().T)
+    for g in (dev.gamma_L, dev.gamma_R):
+        assert np.allclose(g, g.conj().T)
+        assert np.linalg.eigvalsh(g).min() > -1e-12
+    A = synthetic.assemble_A(dev, 0.2, 1e-3)
+    M = A.to_dense()
+    # anti-Hermitian part of A is eta I + Gamma/2 >= eta I (so A is invertible)
+    ah = (M - M.conj().T) / 2j
+    assert np.linalg.eigvalsh(ah).min() > 1e-3 - 1e-12
+    # same seed, same matrices
+    dev2 = synthetic.device_hamiltonian(5, 12, seed=3)
+    assert all(np.array_equal(a, b) for a, b in zip(dev.H_diag, dev2.H_diag))
+    R = synthetic.bm_create_synthetic_random(50, 64, seed=1)
+    assert all(60 <= s <= 68 for s in R.block_sizes)
+    assert abs(fermi(0.1, 0.1, 0.025) - 0.5) < 1e-15
+    assert fermi(10.0, 0.0, 0.025) == 0.0 and fermi(-10.0, 0.0, 0.025) == 1.0
+
+
+def test_flop_counts_match_baseline():
+    assert abs(flops_rgf(64, 512) - 4.975e11) / 4.975e11 < 1e-3
+    assert abs(flops_rkd(64, 512) - 1.171e12) / 1.171e12 < 1e-3
+    assert abs(flops_rgf_keldysh_energy(64, 512) - 1.511e12) / 1.511e12 < 1e-3
+
+
+def test_packed_layout_roundtrip_on_cpu_tensors():
+    torch = pytest.importorskip("torch")
+    from libnegf_amd.btd import DeviceBlockTridiag, Layout
+    A = synthetic.bm_create_synthetic_random(4, 6, seed=2)
+    L = Layout(A.block_sizes)
+    assert L.nd == sum(s * s for s in A.block_sizes)
+    assert L.no == sum(A.block_sizes[i] * A.block_sizes[i + 1] for i in range(3))
+    D = DeviceBlockTridiag.from_host([A, A.copy()], device="cpu")
+    B = D.to_host(1)
+    for x, y in zip(A.diag + A.upper + A.lower, B.diag + B.upper + B.lower):
+        assert np.array_equal(x, y)
+    assert np.array_equal(D.block("upper", 2, 0).numpy(), A.upper[2])
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "negf_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(negf_\w+)\s*\(", text,
+                                 re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from libnegf_amd._lib import LIB_PATH, SIGNATURES
+    if not LIB_PATH.exists():
+        pytest.skip("libnegf_hip.so not built (python libnegf.jl_amd/build_lib.py)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    syms = _declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+        assert s in SIGNATURES, f"{s} has no ctypes signature"
+    assert lib.negf_version() == 10000
