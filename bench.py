@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--wave", type=int, default=0, help="energies per wave (0 = from free memory)")
+    ap.add_argument("--streams", type=int, default=1, help="concurrent energy groups")
     ap.add_argument("--energies", type=int, default=256)
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--b", type=int, default=512)
@@ -163,16 +164,18 @@ def main():
     GL = torch.empty_like(GR)
     T = torch.empty(nE, dtype=torch.float64, device="cuda")
     free, total = torch.cuda.mem_get_info()
-    wave = a.wave
+    wave, streams = a.wave, a.streams
     if wave == 0:
-        wave = min(nE, 64)
-        while wave > 1 and api.energies_workspace_bytes(N, b, nE, wave) > 0.85 * free:
+        wave = min(-(-nE // streams), 64)
+        while wave > 1 and api.energies_workspace_bytes(N, b, nE, wave, streams) > 0.85 * free:
             wave //= 2
-    print(f"# N={N} b={b} energies={nE} wave={wave} free={free/2**30:.1f}GiB", file=sys.stderr,
+    print(f"# N={N} b={b} energies={nE} wave={wave} streams={streams} free={free/2**30:.1f}GiB",
+          file=sys.stderr,
           flush=True)
 
     def step():
-        api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave, check=False)
+        api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
+                                 streams=streams, check=False)
 
     for w in range(a.warmup):
         step()
@@ -189,6 +192,12 @@ def main():
     eps = nE / dt
     fl = flops_rgf_keldysh_energy(N, b)
     roof = gemm_roofline(torch, api, b, wave, a.gemm_reps)
+    tfile = ROOT / "profiles" / "zgemm_512_x64_hbm_traffic.json"
+    if tfile.exists():
+        tj = json.loads(tfile.read_text())
+        if tj.get("b") == b and tj.get("batch") == wave:
+            roof["traffic"] = int(tj["traffic_bytes_per_launch"])
+            roof["traffic_source"] = tj["source"]
     roof["path_tflops"] = round(eps * fl / 1e12, 3)
     roof["path_frac_of_fp64_peak"] = round(eps * fl / 1e12 / FP64_PEAK_TFLOPS, 4)
     info = torch.zeros(1)
@@ -205,7 +214,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"rgf_keldysh_batched: N={N} b={b} energies={nE} "
                                f"linspace(-1,1) eta={eta} kT={kT} muL={muL} muR={muR}",
-                   "form": "plain FP64", "wave": wave, "flops_per_energy": fl},
+                   "form": "plain FP64", "wave": wave, "streams": streams,
+                   "flops_per_energy": fl},
         "roofline": roof,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_cpu": acc,

@@ -170,20 +170,23 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes,
  *               Sigma^< = i f_L Gamma_L (+) i f_R Gamma_R   (or NULL)
  *   transmission: T(E) = Tr(Gamma_L G^R_{1N} Gamma_R G^A_{N1})  (or NULL)
  *   f(E) = 1 / (1 + exp((E - mu) / kT)).
- * Energies run in waves of `wave` (0 = pick from free memory).  info:
- * device int32[nE] or NULL (layer of the first singular Schur complement). */
+ * The grid is split into `streams` contiguous groups (0 = auto, at most 4),
+ * each run on its own internal stream in waves of `wave` energies (0 = pick
+ * from free memory), so one group's latency-bound steps overlap another's
+ * GEMMs; `stream` waits for all of them.  info: device int32[nE] or NULL
+ * (layer of the first singular Schur complement). */
 int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
                               const void* H_diag, const void* H_upper,
                               const void* gamma_L, const void* gamma_R,
                               const double* energies, int32_t n_energies,
                               double eta, double mu_L, double mu_R, double kT,
                               void* GR_diag, void* Gless_diag, double* transmission,
-                              int32_t* info, int32_t wave, void* stream);
+                              int32_t* info, int32_t wave, int32_t streams, void* stream);
 
 /* Bytes of device workspace the energy-batched call uses for a given wave
- * (for sizing `wave` against free memory). */
+ * and stream count (for sizing `wave` against free memory). */
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
-                                            int32_t wave);
+                                            int32_t wave, int32_t streams);
 
 #ifdef __cplusplus
 }

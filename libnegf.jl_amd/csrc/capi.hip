@@ -274,39 +274,76 @@ int negf_rgf(int32_t nblocks, const int32_t* block_sizes, const void* diag, int6
 }
 
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
-                                            int32_t wave) {
-  if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0) return 0;
+                                            int32_t wave, int32_t streams) {
+  if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0 || streams <= 0) return 0;
   std::vector<int32_t> sz(nblocks, b);
   Layout L(nblocks, sz.data());
-  return (int64_t)(energies_work_bytes(L, n_energies, wave) + sizeof(zc) * (size_t)L.nd + 4096);
+  return (int64_t)(energies_work_bytes(L, n_energies, wave, streams) + sizeof(zc) * (size_t)L.nd +
+                   4096);
 }
+
+namespace {
+// Worker streams of the energy pipeline, created once per device (default
+// priority, non-blocking with respect to the null stream).
+constexpr int MAX_STREAMS = 4;
+std::mutex g_pool_mu;
+std::map<int, std::vector<hipStream_t>> g_pool;
+
+int stream_pool(int n, hipStream_t** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return NEGF_HIP_ERROR;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto& v = g_pool[dev];
+  while ((int)v.size() < n) {
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NEGF_HIP_ERROR;
+    v.push_back(s);
+  }
+  *out = v.data();
+  return NEGF_OK;
+}
+}  // namespace
 
 int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
                               const void* H_upper, const void* gamma_L, const void* gamma_R,
                               const double* energies, int32_t n_energies, double eta, double mu_L,
                               double mu_R, double kT, void* GR_diag, void* Gless_diag,
-                              double* transmission, int32_t* info, int32_t wave, void* stream) {
-  if (nblocks <= 0 || b <= 0 || b > 2048 || n_energies < 0 || wave < 0) return NEGF_BAD_ARG;
+                              double* transmission, int32_t* info, int32_t wave, int32_t streams,
+                              void* stream) {
+  if (nblocks <= 0 || b <= 0 || b > 2048 || n_energies < 0 || wave < 0 || streams < 0)
+    return NEGF_BAD_ARG;
   if (n_energies == 0) return NEGF_OK;
   if (!H_diag || (nblocks > 1 && !H_upper) || !gamma_L || !gamma_R || !energies)
     return NEGF_BAD_ARG;
   if (!(kT > 0.0)) return NEGF_BAD_ARG;
   std::vector<int32_t> sz(nblocks, b);
   Layout L(nblocks, sz.data());
+  // Measured on MI355X (N = 64, b = 512): one stream with waves of 64 beats
+  // two streams of 32 (each batched GEMM already fills the GPU, and the
+  // per-energy Gauss-Jordan panels do not get CUs while a GEMM holds them).
+  if (streams == 0) streams = 1;
+  if (streams > MAX_STREAMS) streams = MAX_STREAMS;
   if (wave == 0) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) return NEGF_HIP_ERROR;
-    wave = n_energies < 64 ? n_energies : 64;
+    const int per = (n_energies + streams - 1) / streams;
+    wave = per < 64 ? per : 64;
     while (wave > 1 &&
-           (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave) >
+           (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave, streams) >
                freeb / 10 * 9)
       wave /= 2;
   }
   if (wave > n_energies) wave = n_energies;
-  const size_t ew = energies_work_bytes(L, n_energies, wave);
+  while (streams > 1 && (long long)(streams - 1) * wave >= n_energies) --streams;
+  const size_t ew = energies_work_bytes(L, n_energies, wave, streams);
   void* w = nullptr;
   int rc = workspace(stream, ew + sizeof(zc) * (size_t)L.nd + 4096, &w);
   if (rc) return rc;
+  hipStream_t* pool = nullptr;
+  if (streams > 1) {
+    rc = stream_pool(streams, &pool);
+    if (rc) return rc;
+  }
   hipStream_t s = st(stream);
   // H' = H - (i/2) Gamma on the contact layers, behind the sweep workspace
   zc* Hp = (zc*)((char*)w + (ew + 255) / 256 * 256);
@@ -320,7 +357,7 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
   EnergyIn in{Hp, (const zc*)H_upper, (const zc*)gamma_L, (const zc*)gamma_R, energies,
               n_energies, eta, mu_L, mu_R, kT};
   EnergyOut out{(zc*)GR_diag, (zc*)Gless_diag, transmission, info};
-  return sweep_energies(L, in, out, wave, w, s);
+  return sweep_energies(L, in, out, wave, streams, pool, w, s);
 }
 
 }  // extern "C"

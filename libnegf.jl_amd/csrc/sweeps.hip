@@ -433,29 +433,20 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
   return Status::OK;
 }
 
-int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   void* work, hipStream_t s) {
+// One stream's share of the energy grid: energies [e_begin, e_end) in waves.
+static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
+                          int e_begin, int e_end, const zc* z, const zc* fL, const zc* fR,
+                          int* einfo, void* work, hipStream_t s) {
   const int N = L.N, b = L.b[0];
-  const int nE = in.nE;
-  // device scalars for all energies live at the tail of the workspace
-  const size_t sw = sweep_work_bytes(L, wave, true);
-  char* tail = (char*)work + sw;
-  zc* z = (zc*)tail;
-  zc* fL = z + nE;
-  zc* fR = fL + nE;
-  zc* Sdw = fR + nE;                         // wave * nd
-  zc* Suw = Sdw + (size_t)wave * L.nd;       // wave * no
-  int* einfo = (int*)(Suw + (size_t)wave * L.no);  // nE, when the caller passes none
-  hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
-                     in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
   SweepWork w;
   carve(work, L, wave, &w);
+  zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
+  zc* Suw = Sdw + (size_t)wave * L.nd;
   const zc one = ONE;
   if (hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
     return Status::HIP_ERROR;
-  const long long bb = (long long)b * b;
-  for (int e0 = 0; e0 < nE; e0 += wave) {
-    const int nb = (nE - e0) < wave ? (nE - e0) : wave;
+  for (int e0 = e_begin; e0 < e_end; e0 += wave) {
+    const int nb = (e_end - e0) < wave ? (e_end - e0) : wave;
     Source S{};
     S.diag = in.Hp_diag; S.diag_bs = 0;        // H' (contacts folded in), shared
     S.up = in.H_up; S.up_bs = 0;
@@ -485,9 +476,60 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   return Status::OK;
 }
 
-size_t energies_work_bytes(const Layout& L, int nE, int wave) {
-  return sweep_work_bytes(L, wave, true) + sizeof(zc) * (3 * (size_t)nE) +
-         sizeof(zc) * (size_t)wave * (L.nd + L.no) + sizeof(int) * (size_t)nE + 4096;
+size_t energies_group_bytes(const Layout& L, int wave) {
+  return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
+         sizeof(zc) * (size_t)wave * (L.nd + L.no) + 4096;
+}
+
+size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {
+  return sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
+         (size_t)streams * ((energies_group_bytes(L, wave) + 255) / 256 * 256);
+}
+
+// Energies are split into `streams` contiguous groups, each processed on its
+// own stream in waves of `wave`, so one group's latency-bound steps (the
+// Gauss-Jordan panels, one workgroup per energy) overlap another group's
+// GEMMs.  The caller's stream waits for all groups.
+int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
+                   int streams, hipStream_t* pool, void* work, hipStream_t s) {
+  const int nE = in.nE;
+  char* p = (char*)work;
+  zc* z = (zc*)p;
+  zc* fL = z + nE;
+  zc* fR = fL + nE;
+  int* einfo = (int*)(fR + nE);
+  char* groups = (char*)work + (sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
+                                255) / 256 * 256;
+  const size_t gbytes = (energies_group_bytes(L, wave) + 255) / 256 * 256;
+  hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
+                     in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
+  if (streams <= 1) return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, groups, s);
+  hipEvent_t fork;
+  if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return Status::HIP_ERROR;
+  hipEventRecord(fork, s);
+  // contiguous chunks, each a whole number of waves where possible
+  const int per = ((nE + streams - 1) / streams + wave - 1) / wave * wave;
+  int rc = Status::OK;
+  std::vector<hipEvent_t> joins;
+  for (int g = 0; g < streams && rc == Status::OK; ++g) {
+    const int e0 = g * per, e1 = (g + 1) * per < nE ? (g + 1) * per : nE;
+    if (e0 >= e1) break;
+    hipStreamWaitEvent(pool[g], fork, 0);
+    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, groups + g * gbytes, pool[g]);
+    hipEvent_t j;
+    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+      rc = Status::HIP_ERROR;
+      break;
+    }
+    hipEventRecord(j, pool[g]);
+    joins.push_back(j);
+  }
+  for (hipEvent_t j : joins) {
+    hipStreamWaitEvent(s, j, 0);
+    hipEventDestroy(j);
+  }
+  hipEventDestroy(fork);
+  return rc;
 }
 
 }  // namespace negf

@@ -50,13 +50,13 @@ struct EnergyOut {
 };
 
 size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
-size_t energies_work_bytes(const Layout& L, int nE, int wave);
+size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams);
 
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
               hipStream_t s);
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   void* work, hipStream_t s);
+                   int streams, hipStream_t* pool, void* work, hipStream_t s);
 
 int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,

@@ -23,14 +23,19 @@ namespace negf {
 
 namespace {
 
-constexpr int PT = 256;   // threads per panel workgroup
-
-template <int RPT, int NB>
+// PT threads per panel workgroup, RPT rows per thread (n <= PT * RPT).
+// One barrier per column: every wave reduces its own pivot candidate with
+// shuffles and the candidate's owner publishes the whole unscaled row in a
+// per-wave LDS slot; after the barrier each thread picks the winning slot and
+// scales on the fly.  Slots are double-buffered by column parity, so a fast
+// wave writing column j+1 never overwrites what a slow one still reads for j.
+template <int PT, int RPT, int NB>
 __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restrict__ X,
                                                       zc* __restrict__ W, zc* __restrict__ R,
                                                       int* __restrict__ piv,
                                                       int* __restrict__ used,
                                                       int* __restrict__ info) {
+  constexpr int NW = PT / 64;
   const int inst = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -40,13 +45,13 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
   int* usedi = used + (long long)inst * n;
   const int nbe = (n - k0) < NB ? (n - k0) : NB;
 
-  __shared__ double red_v[PT / 64];
-  __shared__ int red_i[PT / 64];
-  __shared__ zc prow[NB];
-  __shared__ int s_pc[NB];
+  __shared__ double red_v[2][NW];
+  __shared__ int red_i[2][NW];
+  __shared__ zc cand[2][NW][NB];
 
   zc P[RPT][NB];
   bool usd[RPT];
+  int pc[NB];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int r = tid + PT * q;
@@ -63,7 +68,9 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
 
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
+    pc[jj] = 0;
     if (jj < nbe) {
+      const int buf = jj & 1;
       // pivot search: max |re|+|im| over rows not yet pivoted, lowest index on ties
       double bv = -1.0;
       int bi = INT_MAX;
@@ -81,55 +88,48 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
         const int oi = __shfl_xor(bi, off);
         if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
       }
-      if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
-      __syncthreads();
-      bv = red_v[0]; bi = red_i[0];
+      // the wave's candidate row, published by its owner
 #pragma unroll
-      for (int w = 1; w < PT / 64; ++w) {
-        const double ov = red_v[w];
-        const int oi = red_i[w];
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-      }
-      const int p = bi;
-      if (tid == 0) {
-        s_pc[jj] = p;
-        if (bv <= 0.0 && info[inst] == 0) info[inst] = k0 + jj + 1;
-      }
-      // the owner of row p publishes the scaled pivot row
-      if ((p % PT) == tid) {
-        const int qp = p / PT;
+      for (int q = 0; q < RPT; ++q) {
+        if (tid + PT * q == bi) {
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          if (q == qp) {
-            const zc pv = P[q][jj];
-            const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
-#pragma unroll
-            for (int c = 0; c < NB; ++c) prow[c] = (c == jj) ? inv : zmul(P[q][c], inv);
-          }
+          for (int c = 0; c < NB; ++c) cand[buf][wave][c] = P[q][c];
         }
       }
+      if (lane == 0) { red_v[buf][wave] = bv; red_i[buf][wave] = bi; }
       __syncthreads();
-      const zc inv = prow[jj];
+      int ws = 0;
+      bv = red_v[buf][0]; bi = red_i[buf][0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) {
+        const double ov = red_v[buf][w];
+        const int oi = red_i[buf][w];
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; ws = w; }
+      }
+      const int p = bi;
+      pc[jj] = p;
+      if (tid == 0 && bv <= 0.0 && info[inst] == 0) info[inst] = k0 + jj + 1;
+      const zc pv = cand[buf][ws][jj];
+      const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = tid + PT * q;
         if (r < n) {
           if (r == p) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) P[q][c] = prow[c];
+            for (int c = 0; c < NB; ++c) P[q][c] = (c == jj) ? inv : zmul(P[q][c], inv);
             usd[q] = true;
           } else {
-            const zc f = P[q][jj];
+            const zc fi = zmul(P[q][jj], inv);
 #pragma unroll
             for (int c = 0; c < NB; ++c)
-              if (c != jj) P[q][c] = zsub(P[q][c], zmul(f, prow[c]));
-            P[q][jj] = zmake(-(f.x * inv.x - f.y * inv.y), -(f.x * inv.y + f.y * inv.x));
+              if (c != jj) P[q][c] = zsub(P[q][c], zmul(fi, cand[buf][ws][c]));
+            P[q][jj] = zmake(-fi.x, -fi.y);
           }
         }
       }
     }
   }
-  __syncthreads();
 
   // W = P - E ; X[:, J] = E ; used flags back
 #pragma unroll
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
         if (c < nbe) {
-          const bool e = (r == s_pc[c]);
+          const bool e = (r == pc[c]);
           zc w = P[q][c];
           if (e) w.x -= 1.0;
           Wi[r + (long long)c * n] = w;
@@ -149,16 +149,33 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
       }
     }
   }
-  if (tid < nbe) piv[(long long)inst * n + k0 + tid] = s_pc[tid];
-  // R = X[p_J, :] outside the panel, identity inside (ld NB)
-  for (int idx = tid; idx < NB * n; idx += PT) {
+  if (tid == 0) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < nbe) piv[(long long)inst * n + k0 + c] = pc[c];
+  }
+  // R = X[p_J, :] outside the panel, identity inside (ld NB).  Fixed trip
+  // count so the gathers are issued back to back (NB * n <= NB * PT * RPT).
+  zc g[NB * RPT];
+#pragma unroll
+  for (int t = 0; t < NB * RPT; ++t) {
+    const int idx = tid + PT * t;
     const int c = idx % NB, col = idx / NB;
-    if (c < nbe) {
-      zc v;
-      if (col >= k0 && col < k0 + nbe) v = zmake(col - k0 == c ? 1.0 : 0.0, 0.0);
-      else v = Xi[s_pc[c] + (long long)col * n];
-      Ri[idx] = v;
+    g[t] = zmake(0.0, 0.0);
+    int prow = 0;
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc)
+      if (cc == c) prow = pc[cc];
+    if (col < n && c < nbe) {
+      if (col >= k0 && col < k0 + nbe) g[t] = zmake(col - k0 == c ? 1.0 : 0.0, 0.0);
+      else g[t] = Xi[prow + (long long)col * n];
     }
+  }
+#pragma unroll
+  for (int t = 0; t < NB * RPT; ++t) {
+    const int idx = tid + PT * t;
+    const int c = idx % NB, col = idx / NB;
+    if (col < n && c < nbe) Ri[idx] = g[t];
   }
 }
 
@@ -195,7 +212,7 @@ __global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __r
   }
 }
 
-template <int RPT, int NB>
+template <int PT, int RPT, int NB>
 int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
         long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
   if (A != w.X || lda != n || strideA != (long long)n * n) {
@@ -210,7 +227,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
   }
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nbe = (n - k0) < NB ? (n - k0) : NB;
-    hipLaunchKernelGGL((gj_panel_kernel<RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, w.X, w.W,
+    hipLaunchKernelGGL((gj_panel_kernel<PT, RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, w.X, w.W,
                        w.R, w.piv, w.used, info);
     GemmArgs g{};
     g.opA = OP_N; g.opB = OP_N;
@@ -229,7 +246,9 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
-int nb_for(int n) { return n <= 512 ? 16 : (n <= 1024 ? 8 : 4); }
+// Panel width: the n x NB panel lives in registers (RPT * NB complex per
+// thread <= 32), so wider panels need more threads per row block.
+int nb_for(int n) { return n <= 1024 ? 16 : 8; }
 
 }  // namespace
 
@@ -263,10 +282,10 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  if (n <= 256) return run<1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 512) return run<2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 1024) return run<4, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  return run<8, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 256) return run<256, 1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 512) return run<256, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 1024) return run<512, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  return run<512, 4, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
 }
 
 }  // namespace negf

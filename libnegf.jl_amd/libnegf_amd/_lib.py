@@ -48,8 +48,8 @@ SIGNATURES = {
     "negf_rgf_rkd": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, vp,
                                vp, vp, vp, vp, i64, vp, i64, vp, i32, vp, vp]),
     "negf_rgf_keldysh_energies": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d, d, vp, vp,
-                                            vp, vp, i32, vp]),
-    "negf_rgf_keldysh_energies_workspace": (C.c_int64, [i32, i32, i32, i32]),
+                                            vp, vp, i32, i32, vp]),
+    "negf_rgf_keldysh_energies_workspace": (C.c_int64, [i32, i32, i32, i32, i32]),
 }
 
 _lib = None

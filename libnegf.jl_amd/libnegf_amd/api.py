@@ -179,7 +179,7 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
                          mu_L: float, mu_R: float, kT: float, GR_diag: torch.Tensor | None = None,
                          Gless_diag: torch.Tensor | None = None,
                          transmission: torch.Tensor | None = None, wave: int = 0,
-                         check: bool = True):
+                         streams: int = 0, check: bool = True):
     """G^R and G^< diagonal blocks and T(E) for every energy (device float64
     tensor).  GR_diag / Gless_diag: complex128 device tensors of shape
     (nE, N*b*b) (packed diag layout per energy) or None."""
@@ -189,14 +189,14 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
     call("negf_rgf_keldysh_energies", H.N, H.b, _ptr(H.H_diag), _ptr(H.H_upper),
          _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta), float(mu_L),
          float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag), _ptr(transmission), _ptr(info),
-         int(wave), _stream())
+         int(wave), int(streams), _stream())
     if check:
         _check_info("rgf_keldysh_energies", info)
     return info
 
 
-def energies_workspace_bytes(N: int, b: int, nE: int, wave: int) -> int:
-    return int(call("negf_rgf_keldysh_energies_workspace", N, b, nE, wave))
+def energies_workspace_bytes(N: int, b: int, nE: int, wave: int, streams: int = 1) -> int:
+    return int(call("negf_rgf_keldysh_energies_workspace", N, b, nE, wave, streams))
 
 
 # ----------------------------------------------------------------------------- per-op
