@@ -188,6 +188,35 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams);
 
+/* ------------------------------------------------------- selected inverse */
+
+/* Fill-in pattern of the block factorization (host arrays, 0-based block
+ * indices): the input pattern plus every block Block_factorization! creates
+ * (src/selected_inverse.jl:41-46).  Writes at most `capacity` entries when
+ * out_brow/out_bcol are given; returns the count (or a negative status). */
+int negf_block_fill_pattern(int32_t npl, int32_t nnzb, const int32_t* brow, const int32_t* bcol,
+                            int32_t capacity, int32_t* out_brow, int32_t* out_bcol);
+
+/* Block LU without inter-block pivoting, in place on a symmetric block
+ * pattern that already holds the fill-in: the nnzb assigned b x b blocks are
+ * given by host arrays brow/bcol (0-based) and `blocks` (host array of
+ * device pointers, leading dimension ld; instance k of a batch at
+ * blocks[j] + k*stride).  Diagonal blocks are factored without pivoting
+ * (ref_pivoting = 0) or, as the reference does, with partial pivoting whose
+ * ipiv is then discarded (ref_pivoting = 1).  info: device int32[batch],
+ * 1-based block row of the first zero pivot.
+ * Replaces Block_factorization! (src/selected_inverse.jl:9-55). */
+int negf_block_factorization(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                             const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
+                             int32_t batch, int32_t ref_pivoting, int32_t* info, void* stream);
+
+/* Selected inverse of the factored pattern, in place: every assigned block
+ * becomes the matching block of A^{-1}.  Any number of off-diagonal blocks
+ * per row.  Replaces Block_inverse! (src/selected_inverse.jl:139-206). */
+int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                       const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
+                       int32_t batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -360,4 +360,49 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
   return sweep_energies(L, in, out, wave, streams, pool, w, s);
 }
 
+// ------------------------------------------------------------ selected inverse
+int negf_block_fill_pattern(int32_t npl, int32_t nnzb, const int32_t* brow, const int32_t* bcol,
+                            int32_t capacity, int32_t* out_brow, int32_t* out_bcol) {
+  if (npl <= 0 || nnzb < 0 || (nnzb > 0 && (!brow || !bcol))) return NEGF_BAD_ARG;
+  std::vector<std::pair<int, int>> in;
+  for (int k = 0; k < nnzb; ++k) {
+    if (brow[k] < 0 || brow[k] >= npl || bcol[k] < 0 || bcol[k] >= npl) return NEGF_BAD_ARG;
+    in.push_back({brow[k], bcol[k]});
+  }
+  auto out = block_fill_pattern(npl, in);
+  if (out_brow && out_bcol) {
+    if ((int)out.size() > capacity) return NEGF_TOO_LARGE;
+    for (size_t k = 0; k < out.size(); ++k) {
+      out_brow[k] = out[k].first;
+      out_bcol[k] = out[k].second;
+    }
+  }
+  return (int)out.size();
+}
+
+int negf_block_factorization(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                             const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
+                             int32_t batch, int32_t ref_pivoting, int32_t* info, void* stream) {
+  if (!info || b <= 0 || b > 2048 || batch <= 0) return NEGF_BAD_ARG;
+  void* w = nullptr;
+  int rc = workspace(stream, selinv_work_bytes(b, batch, 1), &w);
+  if (rc) return rc;
+  return block_factorization(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch,
+                             ref_pivoting != 0, info, w, st(stream));
+}
+
+int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                       const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
+                       int32_t batch, void* stream) {
+  if (b <= 0 || b > 2048 || batch <= 0 || npl <= 0 || nnzb <= 0 || !brow || !bcol) return NEGF_BAD_ARG;
+  int maxr = 0;
+  std::vector<int> cnt(npl, 0);
+  for (int k = 0; k < nnzb; ++k)
+    if (bcol[k] > brow[k] && brow[k] >= 0 && brow[k] < npl) maxr = ++cnt[brow[k]] > maxr ? cnt[brow[k]] : maxr;
+  void* w = nullptr;
+  int rc = workspace(stream, selinv_work_bytes(b, batch, maxr), &w);
+  if (rc) return rc;
+  return block_inverse(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, w, st(stream));
+}
+
 }  // extern "C"

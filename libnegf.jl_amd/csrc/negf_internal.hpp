@@ -90,8 +90,10 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA,
 // ----------------------------------------------------------------------------
 // Batched LU with partial pivoting (LAPACK zgetrf layout, 1-based ipiv),
 // triangular solves and inversion from the factors (per-op drop-in).
+// pivot = false: no row interchanges (ipiv = identity), the LU the block
+// elimination of the selected inverse needs.
 int zgetrf_launch(int n, zc* A, long long lda, long long strideA, int* ipiv,
-                  long long strideP, int* info, int batch, hipStream_t s);
+                  long long strideP, int* info, int batch, hipStream_t s, bool pivot = true);
 int ztrsm_launch(char side, char uplo, char trans, char diag, int m, int n, zc alpha,
                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                  long long strideB, int batch, hipStream_t s);

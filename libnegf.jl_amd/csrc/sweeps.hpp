@@ -2,6 +2,7 @@
 #pragma once
 
 #include "negf_internal.hpp"
+#include <utility>
 #include <vector>
 
 namespace negf {
@@ -57,6 +58,16 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
               hipStream_t s);
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                    int streams, hipStream_t* pool, void* work, hipStream_t s);
+
+// selected inverse on a block-sparse pattern (selinv.hip)
+std::vector<std::pair<int, int>> block_fill_pattern(int npl,
+                                                    const std::vector<std::pair<int, int>>& in);
+size_t selinv_work_bytes(int b, int batch, int max_row_nnz);
+int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bcol,
+                        void* const* blocks, long long ld, long long stride, int batch,
+                        bool ref_pivoting, int* info, void* work, hipStream_t s);
+int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, void* const* blocks,
+                  long long ld, long long stride, int batch, void* work, hipStream_t s);
 
 int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,

@@ -25,7 +25,7 @@ template <int RPT, int NB>
 __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restrict__ A,
                                                       long long lda, long long strideA,
                                                       int* __restrict__ ipiv, long long strideP,
-                                                      int* __restrict__ info) {
+                                                      int* __restrict__ info, bool pivot) {
   const int inst = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -56,7 +56,8 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = k0 + tid + PT * q;
-        if (r < n && r >= cur) {
+        // without pivoting the "search" sees only the diagonal row
+        if (r < n && (pivot ? r >= cur : r == cur)) {
           const double v = zabs1(P[q][jj]);
           if (v > bv) { bv = v; bi = r; }
         }
@@ -290,22 +291,23 @@ int gemm_sub(int opA, int m, int n, int k, const zc* A, long long lda, long long
 
 template <int RPT, int NB>
 int getrf_run(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
-              int* info, int batch, hipStream_t s) {
+              int* info, int batch, bool pivot, hipStream_t s) {
   hipLaunchKernelGGL(zero_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nbe = (n - k0) < NB ? (n - k0) : NB;
     hipLaunchKernelGGL((lu_panel_kernel<RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, A, lda,
-                       strideA, ipiv, strideP, info);
+                       strideA, ipiv, strideP, info, pivot);
     // interchanges on the columns left and right of the panel
-    if (k0 > 0)
+    if (k0 > 0 && pivot)
       hipLaunchKernelGGL(laswp_kernel, dim3((k0 + 255) / 256, batch), dim3(256), 0, s, k0, A, lda,
                          strideA, ipiv, strideP, k0, k0 + nbe, 1);
     const int rest = n - k0 - nbe;
     if (rest > 0) {
       zc* A12 = A + k0 + (long long)(k0 + nbe) * lda;
-      hipLaunchKernelGGL(laswp_kernel, dim3((rest + 255) / 256, batch), dim3(256), 0, s, rest,
-                         A + (long long)(k0 + nbe) * lda, lda, strideA, ipiv, strideP, k0,
-                         k0 + nbe, 1);
+      if (pivot)
+        hipLaunchKernelGGL(laswp_kernel, dim3((rest + 255) / 256, batch), dim3(256), 0, s, rest,
+                           A + (long long)(k0 + nbe) * lda, lda, strideA, ipiv, strideP, k0,
+                           k0 + nbe, 1);
       int rc = ztrsm_launch('L', 'L', 'N', 'U', nbe, rest, zmake(1.0, 0.0),
                             A + k0 + (long long)k0 * lda, lda, strideA, A12, lda, strideA, batch,
                             s);
@@ -322,14 +324,14 @@ int getrf_run(int n, zc* A, long long lda, long long strideA, int* ipiv, long lo
 }  // namespace
 
 int zgetrf_launch(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
-                  int* info, int batch, hipStream_t s) {
+                  int* info, int batch, hipStream_t s, bool pivot) {
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  if (n <= 256) return getrf_run<1, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
-  if (n <= 512) return getrf_run<2, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
-  if (n <= 1024) return getrf_run<4, 8>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
-  return getrf_run<8, 4>(n, A, lda, strideA, ipiv, strideP, info, batch, s);
+  if (n <= 256) return getrf_run<1, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
+  if (n <= 512) return getrf_run<2, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
+  if (n <= 1024) return getrf_run<4, 8>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
+  return getrf_run<8, 4>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
 }
 
 int zlaswp_launch(int ncols, zc* B, long long ldb, long long strideB, const int* ipiv,

@@ -50,6 +50,12 @@ SIGNATURES = {
     "negf_rgf_keldysh_energies": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d, d, vp, vp,
                                             vp, vp, i32, i32, vp]),
     "negf_rgf_keldysh_energies_workspace": (C.c_int64, [i32, i32, i32, i32, i32]),
+    "negf_block_fill_pattern": (C.c_int, [i32, i32, c_int32_p, c_int32_p, i32, c_int32_p,
+                                          c_int32_p]),
+    "negf_block_factorization": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p,
+                                           C.POINTER(vp), i64, i64, i32, i32, vp, vp]),
+    "negf_block_inverse": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p, C.POINTER(vp), i64,
+                                     i64, i32, vp]),
 }
 
 _lib = None

@@ -1,0 +1,75 @@
+"""Selected inverse on the GPU vs the oracle and the reference's own checks
+(test/test_selected_inverse.jl): the factors against the no-pivot LU, the
+assigned blocks against the dense inverse, diag(A X) = 1."""
+import numpy as np
+import pytest
+
+from conftest import relfro
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from libnegf_amd import selinv as si  # noqa: E402
+from oracle import selinv as ref  # noqa: E402
+
+RIND = [1, 1, 1, 2, 3, 3, 3, 4, 5]
+CIND = [1, 3, 5, 2, 3, 4, 5, 4, 5]
+
+
+@pytest.mark.parametrize("b", [1, 100])
+def test_block_factorization_reference_case(b):
+    print(f"selinv factorization b={b}", flush=True)
+    npl = 5
+    T = si.set_sparse_Block([b] * npl, RIND, CIND, True, seed=4)
+    F = si.Block_factorization(T)
+    host_T = T.to_host()
+    lu = ref.dense_nopivot_lu(ref.full(host_T, [b] * npl))
+    FM = si.full(F)
+    assert relfro(FM, lu) <= 1e-14 * max(b, 10)
+    L = np.tril(FM, -1) + np.eye(FM.shape[0])
+    assert relfro(L @ np.triu(FM), ref.full(host_T, [b] * npl)) <= 1e-14 * max(b, 10)
+    # the oracle's restatement (pivots discarded) agrees on this dominant input
+    Fo = ref.Block_factorization(host_T, npl)
+    for k, blk in F.to_host().items():
+        assert relfro(blk, Fo[k]) < 1e-12
+
+
+@pytest.mark.parametrize("b", [1, 64])
+def test_block_inverse_reference_case(b):
+    print(f"selinv inverse b={b}", flush=True)
+    npl = 5
+    T = si.set_sparse_Block([b] * npl, RIND, CIND, True, seed=5)
+    X = si.Block_inverse(si.Block_factorization(T))
+    A = si.full(T)
+    inv = np.linalg.inv(A)
+    for (i, j), blk in X.to_host().items():
+        r = inv[i * b:(i + 1) * b, j * b:(j + 1) * b]
+        assert np.abs(blk - r).max() < 1e-13 * max(1.0, np.abs(r).max()), (i, j)
+    if b == 1:
+        d = np.diag(A @ si.full(X))
+        assert np.abs(1 - d).max() < 1e-11
+    Xo = ref.Block_inverse(ref.Block_factorization(T.to_host(), npl), npl)
+    for k, blk in X.to_host().items():
+        assert relfro(blk, Xo[k]) < 1e-12
+
+
+def test_selinv_tridiagonal_batched_matches_rgf():
+    print("selinv tridiagonal batch vs rgf", flush=True)
+    from libnegf_amd.synthetic import device_hamiltonian, assemble_A
+    from oracle.rgf import bndiag_of_inv_rgf_local
+    npl, b, batch = 6, 48, 3
+    dev = device_hamiltonian(npl, b, seed=12)
+    pat = [(i, i) for i in range(npl)] + [(i, i + 1) for i in range(npl - 1)] + \
+          [(i + 1, i) for i in range(npl - 1)]
+    T = si.BlockSparse(npl, b, pat, batch)
+    mats = [assemble_A(dev, E, 5e-2) for E in (-0.4, 0.1, 0.7)]
+    for m, A in enumerate(mats):
+        for (i, j), k in T.index.items():
+            blk = A.diag[i] if i == j else (A.upper[i] if j == i + 1 else A.lower[j])
+            T.data[m, k] = torch.from_numpy(np.asarray(blk).ravel(order="F")).cuda()
+    X = si.Block_inverse(si.Block_factorization(T))
+    for m, A in enumerate(mats):
+        G = bndiag_of_inv_rgf_local(A)
+        Xm = X.to_host(m)
+        for k in G:
+            assert relfro(Xm[k], G[k]) < 1e-10, (m, k)
