@@ -188,6 +188,28 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams);
 
+/* ----------------------------------------------------------------- DD-RGF */
+
+/* band(A^{-1}) by domain-decomposed RGF, uniform block size b, batched over
+ * instances.  Level k splits its chain into tasks [1 separator +
+ * block_sizes_d1[k] interior layers] (the last task may be shorter); the
+ * interiors of every task of every instance are solved as one batch, the
+ * separators' Schur complement by the next level or, after the last level,
+ * by RGF.  A given size that leaves an empty last task is NEGF_BAD_ARG.
+ * lower = NULL reads A_{i+1,i} as A_{i,i+1}^H.  Outputs and info as negf_rgf
+ * (info: the layer of the first singular block).
+ * Replaces bndiag_of_inv_ddrgf! (src/matinvertndiag_ddrgf_core.jl:719-758)
+ * with the levels that allocate_aux_data_DDRGF's tuner would choose
+ * (src/matinvertndiag_ddrgf_setup.jl:4-51). */
+int negf_ddrgf(int32_t nblocks, int32_t b, const void* diag, int64_t diag_bstride,
+               const void* upper, int64_t upper_bstride, const void* lower, int64_t lower_bstride,
+               void* g_diag, void* g_upper, void* g_lower, int32_t batch,
+               const int32_t* block_sizes_d1, int32_t nlevels, int32_t* info, void* stream);
+
+/* Device workspace bytes negf_ddrgf needs (0 when the sizes do not split). */
+int64_t negf_ddrgf_workspace(int32_t nblocks, int32_t b, int32_t batch,
+                             const int32_t* block_sizes_d1, int32_t nlevels);
+
 /* ------------------------------------------------------- selected inverse */
 
 /* Fill-in pattern of the block factorization (host arrays, 0-based block

@@ -360,6 +360,49 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
   return sweep_energies(L, in, out, wave, streams, pool, w, s);
 }
 
+// ---------------------------------------------------------------------- DD-RGF
+int64_t negf_ddrgf_workspace(int32_t nblocks, int32_t b, int32_t batch,
+                             const int32_t* block_sizes_d1, int32_t nlevels) {
+  if (nblocks <= 0 || b <= 0 || batch <= 0 || !block_sizes_d1 || nlevels <= 0) return 0;
+  const size_t w = ddrgf_work_bytes(nblocks, b, batch, block_sizes_d1, nlevels);
+  if (w == 0) return 0;
+  return (int64_t)(w + sizeof(zc) * (size_t)(nblocks - 1) * b * b * batch + 4096);
+}
+
+int negf_ddrgf(int32_t nblocks, int32_t b, const void* diag, int64_t diag_bstride,
+               const void* upper, int64_t upper_bstride, const void* lower, int64_t lower_bstride,
+               void* g_diag, void* g_upper, void* g_lower, int32_t batch,
+               const int32_t* block_sizes_d1, int32_t nlevels, int32_t* info, void* stream) {
+  if (nblocks <= 0 || b <= 0 || b > 2048 || batch <= 0 || !block_sizes_d1 || nlevels <= 0 ||
+      !info)
+    return NEGF_BAD_ARG;
+  if (!diag || !g_diag || (nblocks > 1 && (!upper || !g_upper || !g_lower))) return NEGF_BAD_ARG;
+  const int64_t need = negf_ddrgf_workspace(nblocks, b, batch, block_sizes_d1, nlevels);
+  if (need == 0) return NEGF_BAD_ARG;   // a level's block size does not split its chain
+  void* w = nullptr;
+  int rc = workspace(stream, (size_t)need, &w);
+  if (rc) return rc;
+  hipStream_t s = st(stream);
+  const long long bb = (long long)b * b;
+  const void* low = lower;
+  long long low_bs = lower_bstride;
+  if (!lower && nblocks > 1) {
+    // Hermitian storage: A_{i+1,i} = A_{i,i+1}^H, materialized behind the workspace
+    zc* lw = (zc*)((char*)w + ((size_t)ddrgf_work_bytes(nblocks, b, batch, block_sizes_d1,
+                                                        nlevels) + 255) / 256 * 256);
+    for (int e = 0; e < batch; ++e) {
+      rc = zadjoint_launch(b, b, (const zc*)upper + e * upper_bstride, b, bb,
+                           lw + (long long)e * (nblocks - 1) * bb, b, bb, nblocks - 1, s);
+      if (rc) return rc;
+    }
+    low = lw;
+    low_bs = (long long)(nblocks - 1) * bb;
+  }
+  BtdIn A{(const zc*)diag, diag_bstride, (const zc*)upper, upper_bstride, (const zc*)low, low_bs};
+  BtdOut G{(zc*)g_diag, (zc*)g_upper, (zc*)g_lower};
+  return ddrgf(nblocks, b, A, G, batch, block_sizes_d1, nlevels, info, w, s);
+}
+
 // ------------------------------------------------------------ selected inverse
 int negf_block_fill_pattern(int32_t npl, int32_t nnzb, const int32_t* brow, const int32_t* bcol,
                             int32_t capacity, int32_t* out_brow, int32_t* out_bcol) {

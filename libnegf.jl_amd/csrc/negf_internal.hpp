@@ -51,6 +51,9 @@ struct Status {
 // `shift` (optional, may be null) adds shift[idx * shift_stride] to every
 // diagonal element of instance idx, which builds A(E) = zI - ... in the
 // epilogue of the Schur-complement GEMM.
+// Instances: idx in [0, batch).  With inner == 0 operand X of instance idx
+// is X + idx * strideX.  With inner > 0 the batch is 2-D: X + (idx % inner) *
+// strideX + (idx / inner) * outerX (e.g. layers within energies).
 struct GemmArgs {
   int opA, opB;
   int m, n, k;
@@ -61,6 +64,8 @@ struct GemmArgs {
   zc* D; long long ldd, strideD;
   const zc* shift; long long shift_stride;
   int batch;
+  int inner;
+  long long outerA, outerB, outerC, outerD;
 };
 int zgemm_launch(const GemmArgs& g, hipStream_t s);
 
@@ -118,6 +123,11 @@ int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP,
                     long long ldq, long long strideQ, double* out, int batch, hipStream_t s);
 // B = s * A with s = scale_dev[idx * scale_stride] when scale_dev is given
 // (device array, stride 0 broadcasts), else the host value `value`.
+// Contiguous blocks of `count` elements over a 3-D index (i1 < n1, i2 < n2,
+// i3 < n3): dst[i1 d1 + i2 d2 + i3 d3] = alpha * src[i1 s1 + i2 s2 + i3 s3].
+int zcopy3d_launch(long long count, int n1, int n2, int n3, zc alpha, const zc* src,
+                   long long s1, long long s2, long long s3, zc* dst, long long d1, long long d2,
+                   long long d3, hipStream_t s);
 int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                   const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                   long long strideB, int batch, hipStream_t s);

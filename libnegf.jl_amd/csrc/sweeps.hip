@@ -532,4 +532,428 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// DD-RGF (src/matinvertndiag_ddrgf_core.jl:719-758), uniform block size b.
+//
+// Chunk c = [separator s_c, L_c interior layers] (utils.jl:278-312, 478-575).
+// The interiors of all chunks of all instances are solved as one batch:
+// chunks 0..T-2 (length bs, a separator on each side) form the "inner"
+// group, chunk T-1 (the leftover length, no right separator) the "last"
+// group.  Per chunk instance, from one RGF pass over its interior:
+//   band of g = T11^{-1}, first / last block column zF / zL and first / last
+//   block row rF / rL of g (column j of g = L^{-1} D^{-1} U^{-1} e_j and row j =
+//   e_j^T L^{-1} D^{-1} U^{-1} in the UDL factors of the upward pass), then
+//   YF = zF A[f,s_c], YL = zL A[e,s_{c+1}] (T11^{-1} T12, core:48-114),
+//   ZF = A[s_c,f] rF,  ZL = A[s_{c+1},e] rL (T21 T11^{-1}, core:203-266).
+// Schur complement over separators (core:500-654), band of its inverse by
+// RGF or by the next DD-RGF level (core:656-680), then
+//   G21 = -S^{-1} [ZF; ZL] (core:268-344), G12 band blocks, and the band of
+//   G11 = g - [YF YL] G21 (core:346-435), scattered to the original order.
+namespace {
+
+struct Bump {
+  char* base;
+  size_t off = 0;
+  zc* take(size_t elems) {
+    char* r = base ? base + off : nullptr;
+    off += (elems * sizeof(zc) + 255) / 256 * 256;
+    return (zc*)r;
+  }
+  int* take_int(size_t n) {
+    char* r = base ? base + off : nullptr;
+    off += (n * sizeof(int) + 255) / 256 * 256;
+    return (int*)r;
+  }
+  void* take_bytes(size_t n) {
+    char* r = base ? base + off : nullptr;
+    off += (n + 255) / 256 * 256;
+    return r;
+  }
+};
+
+bool dd_plan(int N, int bs, int* T, int* lastL) {
+  if (bs < 1) return false;
+  const int per = bs + 1;
+  const int nr = N / per + 1;
+  const int left = N - (nr - 1) * per;
+  if (left == 0) { *T = nr - 1; *lastL = bs; return true; }
+  if (left - 1 == 0) return false;
+  *T = nr; *lastL = left - 1;
+  return true;
+}
+
+// GEMM over a 2-D instance grid (inner index i < inner, outer o): operand X
+// at X + i * sX + o * oX.
+int gemm2(int opA, int opB, int b, zc alpha, const zc* A, long long sA, long long oA, const zc* B,
+          long long sB, long long oB, zc beta, const zc* C, long long sC, long long oC, zc* D,
+          long long sD, long long oD, int inner, int outer, hipStream_t s) {
+  GemmArgs g{};
+  g.opA = opA; g.opB = opB; g.m = g.n = g.k = b;
+  g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = b; g.strideA = sA; g.outerA = oA;
+  g.B = B; g.ldb = b; g.strideB = sB; g.outerB = oB;
+  g.C = C; g.ldc = b; g.strideC = sC; g.outerC = oC;
+  g.D = D; g.ldd = b; g.strideD = sD; g.outerD = oD;
+  g.batch = inner * outer;
+  g.inner = inner;
+  return zgemm_launch(g, s);
+}
+
+__global__ void dd_fold_info_kernel(const int* inst_info, int nC, int c0, int bs, int batch,
+                                    int* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= batch) return;
+  for (int ci = 0; ci < nC; ++ci) {
+    const int v = inst_info[e * nC + ci];
+    if (v != 0 && out[e] == 0) out[e] = (c0 + ci) * (bs + 1) + 1 + v;
+  }
+}
+
+__global__ void dd_fold_sep_info_kernel(const int* sep_info, int bs, int batch, int* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < batch && sep_info[e] != 0 && out[e] == 0) out[e] = (sep_info[e] - 1) * (bs + 1) + 1;
+}
+
+struct Group {
+  int c0, nC, L;
+  bool right;
+};
+
+}  // namespace
+
+// A: packed BTD (diag / up / low, all given, tight per-block ld = b, batch
+// strides as in BtdIn); G: outputs, tight strides.  `bw` null measures only.
+int ddrgf_run(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const int* levels,
+              int nlev, int* info, Bump& bw, hipStream_t s) {
+  const bool run = bw.base != nullptr;
+  const long long bb = (long long)b * b;
+  const long long nd = (long long)N * bb, no = (long long)(N - 1) * bb;
+  int T = 0, lastL = 0;
+  if (nlev <= 0 || !dd_plan(N, levels[0], &T, &lastL)) return Status::BAD_ARG;
+  const int bs = levels[0];
+  std::vector<int> sizes(N, b);
+  if (T <= 1) {
+    // one task: sequential RGF (core:724-729)
+    Layout L(N, sizes.data());
+    SweepWork w;
+    void* wp = bw.take_bytes(sweep_work_bytes(L, batch, false));
+    if (!run) return Status::OK;
+    carve(wp, L, batch, &w);
+    const zc one = ONE;
+    if (hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
+      return Status::HIP_ERROR;
+    Source S{};
+    S.diag = A.diag; S.diag_bs = A.diag_bs; S.up = A.up; S.up_bs = A.up_bs;
+    S.low = A.low; S.low_bs = A.low_bs; S.sLow = ONE; S.sUp = ONE;
+    RC(upward(L, S, w, batch, info, s));
+    return downward(L, w, batch, G.diag, nd, G.up, G.low, no, s);
+  }
+  std::vector<Group> groups;
+  if (T >= 2) groups.push_back(Group{0, T - 1, bs, true});
+  groups.push_back(Group{T - 1, 1, lastL, false});
+  if (run)
+    hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+
+  // reduced system buffers (T layers per instance)
+  zc* Sd = bw.take((size_t)T * bb * batch);
+  zc* Su = bw.take((size_t)(T - 1) * bb * batch);
+  zc* Sl = bw.take((size_t)(T - 1) * bb * batch);
+  zc* SId = bw.take((size_t)T * bb * batch);
+  zc* SIu = bw.take((size_t)(T - 1) * bb * batch);
+  zc* SIl = bw.take((size_t)(T - 1) * bb * batch);
+  int* sep_info = bw.take_int(batch);
+
+  struct GBuf {
+    zc *Cd, *Cu, *Cl, *Gd, *Gu, *Gl, *zF, *rF, *zL, *rL, *Xs, *YF, *ZF, *YL, *ZL, *G21F, *G21L;
+    zc *Kls, *Kus, *Kur, *Klr, *tmp;
+    int* ginfo;
+    SweepWork w;
+  };
+  std::vector<GBuf> gb(groups.size());
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& g = groups[gi];
+    const size_t NI = (size_t)batch * g.nC, L = g.L;
+    GBuf& q = gb[gi];
+    q.Cd = bw.take(NI * L * bb);
+    q.Cu = bw.take(NI * (L > 1 ? L - 1 : 1) * bb);
+    q.Cl = bw.take(NI * (L > 1 ? L - 1 : 1) * bb);
+    q.Gd = bw.take(NI * L * bb);
+    q.Gu = bw.take(NI * (L > 1 ? L - 1 : 1) * bb);
+    q.Gl = bw.take(NI * (L > 1 ? L - 1 : 1) * bb);
+    q.zF = bw.take(NI * L * bb);
+    q.rF = bw.take(NI * L * bb);
+    q.YF = bw.take(NI * L * bb);
+    q.ZF = bw.take(NI * L * bb);
+    q.G21F = bw.take(NI * L * bb);
+    q.Kls = bw.take(NI * bb);
+    q.Kus = bw.take(NI * bb);
+    q.tmp = bw.take(NI * bb);
+    if (g.right) {
+      q.zL = bw.take(NI * L * bb);
+      q.rL = bw.take(NI * L * bb);
+      q.Xs = bw.take(NI * L * bb);
+      q.YL = bw.take(NI * L * bb);
+      q.ZL = bw.take(NI * L * bb);
+      q.G21L = bw.take(NI * L * bb);
+      q.Kur = bw.take(NI * bb);
+      q.Klr = bw.take(NI * bb);
+    } else {
+      q.zL = q.rL = q.Xs = q.YL = q.ZL = q.G21L = q.Kur = q.Klr = nullptr;
+    }
+    q.ginfo = bw.take_int(NI);
+    std::vector<int> lsz(g.L, b);
+    Layout Lg(g.L, lsz.data());
+    void* wp = bw.take_bytes(sweep_work_bytes(Lg, (int)NI, false));
+    if (run) carve(wp, Lg, (int)NI, &q.w);
+  }
+  // reduced-system solver workspace (RGF at the last level, next level otherwise)
+  std::vector<int> tsz(T, b);
+  Layout LT(T, tsz.data());
+  SweepWork wT;
+  Bump sub_bw{nullptr};
+  void* wTp = nullptr;
+  if (nlev > 1) {
+    // measure the next level, then take that much
+    Bump meas{nullptr};
+    BtdIn SA{nullptr, 0, nullptr, 0, nullptr, 0};
+    BtdOut SG{nullptr, nullptr, nullptr};
+    int rc = ddrgf_run(T, b, SA, SG, batch, levels + 1, nlev - 1, nullptr, meas, s);
+    if (rc) return rc;
+    sub_bw.base = (char*)bw.take_bytes(meas.off);
+  } else {
+    wTp = bw.take_bytes(sweep_work_bytes(LT, batch, false));
+  }
+  if (!run) return Status::OK;
+
+  const zc one = ONE;
+  for (auto& q : gb)
+    if (hipMemcpyAsync(q.w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
+      return Status::HIP_ERROR;
+
+  // ---- per group: chunk RGF, columns / rows of T11^{-1}, couplings
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& g = groups[gi];
+    GBuf& q = gb[gi];
+    const int NI = batch * g.nC, L = g.L;
+    const long long Lb = (long long)L * bb, L1b = (long long)(L > 1 ? L - 1 : 1) * bb;
+    const long long cstride = (long long)(bs + 1) * bb;   // chunk to chunk in the packed arrays
+    const int f0 = g.c0 * (bs + 1) + 1, s0 = g.c0 * (bs + 1);
+    // gather the interiors and the coupling blocks (instance = e * nC + ci)
+    RC(zcopy3d_launch(bb, L, g.nC, batch, ONE, A.diag + f0 * bb, bb, cstride, A.diag_bs, q.Cd, bb,
+                      Lb, (long long)g.nC * Lb, s));
+    if (L > 1) {
+      RC(zcopy3d_launch(bb, L - 1, g.nC, batch, ONE, A.up + f0 * bb, bb, cstride, A.up_bs, q.Cu, bb,
+                        L1b, (long long)g.nC * L1b, s));
+      RC(zcopy3d_launch(bb, L - 1, g.nC, batch, ONE, A.low + f0 * bb, bb, cstride, A.low_bs, q.Cl,
+                        bb, L1b, (long long)g.nC * L1b, s));
+    }
+    RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, A.low + s0 * bb, 0, cstride, A.low_bs, q.Kls, 0, bb,
+                      (long long)g.nC * bb, s));
+    RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, A.up + s0 * bb, 0, cstride, A.up_bs, q.Kus, 0, bb,
+                      (long long)g.nC * bb, s));
+    if (g.right) {
+      const int e0 = f0 + L - 1;
+      RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, A.up + e0 * bb, 0, cstride, A.up_bs, q.Kur, 0, bb,
+                        (long long)g.nC * bb, s));
+      RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, A.low + e0 * bb, 0, cstride, A.low_bs, q.Klr, 0,
+                        bb, (long long)g.nC * bb, s));
+    }
+    // chunk RGF: UDL factors and the band of T11^{-1}
+    std::vector<int> lsz(L, b);
+    Layout Lg(L, lsz.data());
+    Source S{};
+    S.diag = q.Cd; S.diag_bs = Lb; S.up = q.Cu; S.up_bs = L1b; S.low = q.Cl; S.low_bs = L1b;
+    S.sLow = ONE; S.sUp = ONE;
+    RC(upward(Lg, S, q.w, NI, q.ginfo, s));
+    hipLaunchKernelGGL(dd_fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, q.ginfo,
+                       g.nC, g.c0, bs, batch, info);
+    RC(downward(Lg, q.w, NI, q.Gd, Lb, q.Gu, q.Gl, L1b, s));
+    const zc* gg = q.w.g;   // g_j at j*bb + inst*Lb
+    const zc* tt = q.w.t;   // t_j at j*bb + inst*L1b
+    const zc* ll = q.w.l;   // l_{j+1} at j*bb + inst*L1b
+    // first column zF_0 = g_0, zF_i = -l_i zF_{i-1};  first row rF_0 = g_0, rF_k = -rF_{k-1} t_{k-1}
+    RC(zcopy3d_launch(bb, 1, 1, NI, ONE, gg, 0, 0, Lb, q.zF, 0, 0, Lb, s));
+    RC(zcopy3d_launch(bb, 1, 1, NI, ONE, gg, 0, 0, Lb, q.rF, 0, 0, Lb, s));
+    for (int i = 1; i < L; ++i) {
+      GemmArgs two[2];
+      two[0] = ga(OP_N, OP_N, b, b, b, MONE, M{ll + (i - 1) * bb, b, L1b}, M{q.zF + (i - 1) * bb, b, Lb},
+                  ZERO, M{nullptr, 1, 0}, MW{q.zF + i * bb, b, Lb}, NI);
+      two[1] = ga(OP_N, OP_N, b, b, b, MONE, M{q.rF + (i - 1) * bb, b, Lb}, M{tt + (i - 1) * bb, b, L1b},
+                  ZERO, M{nullptr, 1, 0}, MW{q.rF + i * bb, b, Lb}, NI);
+      RC(zgemm_group_launch(two, 2, s));
+    }
+    if (g.right) {
+      // last column: x_{L-1} = I, x_i = -t_i x_{i+1}; y_i = g_i x_i; z_i = y_i - l_i z_{i-1}
+      RC(zfill_launch(b, b, ZERO, ONE, q.Xs + (L - 1) * bb, b, Lb, NI, s));
+      for (int i = L - 2; i >= 0; --i)
+        RC(gemm(OP_N, OP_N, b, b, b, MONE, M{tt + i * bb, b, L1b}, M{q.Xs + (i + 1) * bb, b, Lb},
+                ZERO, M{nullptr, 1, 0}, MW{q.Xs + i * bb, b, Lb}, NI, s));
+      RC(gemm2(OP_N, OP_N, b, ONE, gg, bb, Lb, q.Xs, bb, Lb, ZERO, nullptr, 0, 0, q.zL, bb, Lb, L,
+               NI, s));
+      for (int i = 1; i < L; ++i)
+        RC(gemm(OP_N, OP_N, b, b, b, MONE, M{ll + (i - 1) * bb, b, L1b}, M{q.zL + (i - 1) * bb, b, Lb},
+                ONE, M{q.zL + i * bb, b, Lb}, MW{q.zL + i * bb, b, Lb}, NI, s));
+      // last row: w_{L-1} = I, w_k = -w_{k+1} l_{k+1}; v_k = w_k g_k; r_k = v_k - r_{k-1} t_{k-1}
+      RC(zfill_launch(b, b, ZERO, ONE, q.Xs + (L - 1) * bb, b, Lb, NI, s));
+      for (int k = L - 2; k >= 0; --k)
+        RC(gemm(OP_N, OP_N, b, b, b, MONE, M{q.Xs + (k + 1) * bb, b, Lb}, M{ll + k * bb, b, L1b},
+                ZERO, M{nullptr, 1, 0}, MW{q.Xs + k * bb, b, Lb}, NI, s));
+      RC(gemm2(OP_N, OP_N, b, ONE, q.Xs, bb, Lb, gg, bb, Lb, ZERO, nullptr, 0, 0, q.rL, bb, Lb, L,
+               NI, s));
+      for (int k = 1; k < L; ++k)
+        RC(gemm(OP_N, OP_N, b, b, b, MONE, M{q.rL + (k - 1) * bb, b, Lb}, M{tt + (k - 1) * bb, b, L1b},
+                ONE, M{q.rL + k * bb, b, Lb}, MW{q.rL + k * bb, b, Lb}, NI, s));
+    }
+    // couplings: YF = zF A[f,s_c], ZF = A[s_c,f] rF (and YL, ZL)
+    RC(gemm2(OP_N, OP_N, b, ONE, q.zF, bb, Lb, q.Kls, 0, bb, ZERO, nullptr, 0, 0, q.YF, bb, Lb, L, NI,
+             s));
+    RC(gemm2(OP_N, OP_N, b, ONE, q.Kus, 0, bb, q.rF, bb, Lb, ZERO, nullptr, 0, 0, q.ZF, bb, Lb, L, NI,
+             s));
+    if (g.right) {
+      RC(gemm2(OP_N, OP_N, b, ONE, q.zL, bb, Lb, q.Kur, 0, bb, ZERO, nullptr, 0, 0, q.YL, bb, Lb, L,
+               NI, s));
+      RC(gemm2(OP_N, OP_N, b, ONE, q.Klr, 0, bb, q.rL, bb, Lb, ZERO, nullptr, 0, 0, q.ZL, bb, Lb, L,
+               NI, s));
+    }
+  }
+
+  // ---- Schur complement over the separators (core:500-654)
+  const long long Tb = (long long)T * bb, T1b = (long long)(T - 1) * bb;
+  const long long cstride = (long long)(bs + 1) * bb;
+  RC(zcopy3d_launch(bb, T, 1, batch, ONE, A.diag, cstride, 0, A.diag_bs, Sd, bb, 0, Tb, s));
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& g = groups[gi];
+    GBuf& q = gb[gi];
+    const int L = g.L;
+    const long long Lb = (long long)L * bb;
+    const long long nCb = (long long)g.nC * bb, nCLb = (long long)g.nC * Lb;
+    // Sd_c -= A[s_c,f] YF_0
+    RC(gemm2(OP_N, OP_N, b, MONE, q.Kus, bb, nCb, q.YF, Lb, nCLb, ONE, Sd + g.c0 * bb, bb, Tb,
+             Sd + g.c0 * bb, bb, Tb, g.nC, batch, s));
+    if (g.right) {
+      // Sd_{c+1} -= A[s_{c+1},e] YL_{L-1};  Su_c = -A[s_c,f] YL_0;  Sl_c = -A[s_{c+1},e] YF_{L-1}
+      RC(gemm2(OP_N, OP_N, b, MONE, q.Klr, bb, nCb, q.YL + (L - 1) * bb, Lb, nCLb, ONE,
+               Sd + (g.c0 + 1) * bb, bb, Tb, Sd + (g.c0 + 1) * bb, bb, Tb, g.nC, batch, s));
+      RC(gemm2(OP_N, OP_N, b, MONE, q.Kus, bb, nCb, q.YL, Lb, nCLb, ZERO, nullptr, 0, 0,
+               Su + g.c0 * bb, bb, T1b, g.nC, batch, s));
+      RC(gemm2(OP_N, OP_N, b, MONE, q.Klr, bb, nCb, q.YF + (L - 1) * bb, Lb, nCLb, ZERO, nullptr,
+               0, 0, Sl + g.c0 * bb, bb, T1b, g.nC, batch, s));
+    }
+  }
+
+  // ---- band of S^{-1} (core:656-680): RGF at the last level, else the next level
+  {
+    BtdIn SA{Sd, Tb, Su, T1b, Sl, T1b};
+    BtdOut SG{SId, SIu, SIl};
+    if (nlev > 1) {
+      RC(ddrgf_run(T, b, SA, SG, batch, levels + 1, nlev - 1, sep_info, sub_bw, s));
+    } else {
+      carve(wTp, LT, batch, &wT);
+      if (hipMemcpyAsync(wT.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
+        return Status::HIP_ERROR;
+      Source S{};
+      S.diag = Sd; S.diag_bs = Tb; S.up = Su; S.up_bs = T1b; S.low = Sl; S.low_bs = T1b;
+      S.sLow = ONE; S.sUp = ONE;
+      RC(upward(LT, S, wT, batch, sep_info, s));
+      RC(downward(LT, wT, batch, SId, Tb, SIu, SIl, T1b, s));
+    }
+    hipLaunchKernelGGL(dd_fold_sep_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s,
+                       sep_info, bs, batch, info);
+  }
+  // separator diagonal blocks of G
+  RC(zcopy3d_launch(bb, T, 1, batch, ONE, SId, bb, 0, Tb, G.diag, cstride, 0, nd, s));
+
+  // ---- back-substitution per group
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const Group& g = groups[gi];
+    GBuf& q = gb[gi];
+    const int L = g.L;
+    const int Mi = batch * g.nC;
+    const long long Lb = (long long)L * bb, L1b = (long long)(L > 1 ? L - 1 : 1) * bb;
+    const long long nCb = (long long)g.nC * bb, nCLb = (long long)g.nC * Lb;
+    const zc* SIcc = SId + g.c0 * bb;            // S^{-1}_{c,c}: (ci, e) -> ci*bb + e*Tb
+    // G21F_k = -(SI_cc ZF_k + SI_{c,c+1} ZL_k), G21L_k = -(SI_{c+1,c} ZF_k + SI_{c+1,c+1} ZL_k)
+    for (int k = 0; k < L; ++k) {
+      RC(gemm2(OP_N, OP_N, b, MONE, SIcc, bb, Tb, q.ZF + k * bb, Lb, nCLb, ZERO, nullptr, 0, 0,
+               q.G21F + k * bb, Lb, nCLb, g.nC, batch, s));
+      if (g.right) {
+        RC(gemm2(OP_N, OP_N, b, MONE, SIu + g.c0 * bb, bb, T1b, q.ZL + k * bb, Lb, nCLb, ONE,
+                 q.G21F + k * bb, Lb, nCLb, q.G21F + k * bb, Lb, nCLb, g.nC, batch, s));
+        RC(gemm2(OP_N, OP_N, b, MONE, SIl + g.c0 * bb, bb, T1b, q.ZF + k * bb, Lb, nCLb, ZERO,
+                 nullptr, 0, 0, q.G21L + k * bb, Lb, nCLb, g.nC, batch, s));
+        RC(gemm2(OP_N, OP_N, b, MONE, SId + (g.c0 + 1) * bb, bb, Tb, q.ZL + k * bb, Lb, nCLb, ONE,
+                 q.G21L + k * bb, Lb, nCLb, q.G21L + k * bb, Lb, nCLb, g.nC, batch, s));
+      }
+    }
+    // boundary blocks: G[f_c, s_c] = -(YF_0 SI_cc + YL_0 SI_{c+1,c}) -> low[s_c]
+    RC(gemm2(OP_N, OP_N, b, MONE, q.YF, Lb, nCLb, SIcc, bb, Tb, ZERO, nullptr, 0, 0, q.tmp, bb, nCb,
+             g.nC, batch, s));
+    if (g.right)
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YL, Lb, nCLb, SIl + g.c0 * bb, bb, T1b, ONE, q.tmp, bb, nCb,
+               q.tmp, bb, nCb, g.nC, batch, s));
+    RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, q.tmp, 0, bb, nCb, G.low + g.c0 * cstride, 0,
+                      cstride, no, s));
+    // G[s_c, f_c] = G21F_0 -> up[s_c]
+    RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, q.G21F, 0, Lb, nCLb, G.up + g.c0 * cstride, 0,
+                      cstride, no, s));
+    if (g.right) {
+      const long long eoff = (long long)(g.c0 * (bs + 1) + L) * bb;   // layer e_c
+      // G[e_c, s_{c+1}] = -(YF_{L-1} SI_{c,c+1} + YL_{L-1} SI_{c+1,c+1}) -> up[e_c]
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YF + (L - 1) * bb, Lb, nCLb, SIu + g.c0 * bb, bb, T1b, ZERO,
+               nullptr, 0, 0, q.tmp, bb, nCb, g.nC, batch, s));
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YL + (L - 1) * bb, Lb, nCLb, SId + (g.c0 + 1) * bb, bb, Tb,
+               ONE, q.tmp, bb, nCb, q.tmp, bb, nCb, g.nC, batch, s));
+      RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, q.tmp, 0, bb, nCb, G.up + eoff, 0, cstride, no,
+                        s));
+      // G[s_{c+1}, e_c] = G21L_{L-1} -> low[e_c]
+      RC(zcopy3d_launch(bb, 1, g.nC, batch, ONE, q.G21L + (L - 1) * bb, 0, Lb, nCLb, G.low + eoff,
+                        0, cstride, no, s));
+    }
+    // 11 part on the band: G_jk = g_jk - YF_j G21F_k - YL_j G21L_k
+    RC(gemm2(OP_N, OP_N, b, MONE, q.YF, bb, Lb, q.G21F, bb, Lb, ONE, q.Gd, bb, Lb, q.Gd, bb, Lb, L,
+             Mi, s));
+    if (g.right)
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YL, bb, Lb, q.G21L, bb, Lb, ONE, q.Gd, bb, Lb, q.Gd, bb, Lb,
+               L, Mi, s));
+    if (L > 1) {
+      // upper (j, j+1) and lower (j+1, j), j < L-1
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YF, bb, Lb, q.G21F + bb, bb, Lb, ONE, q.Gu, bb, L1b, q.Gu, bb,
+               L1b, L - 1, Mi, s));
+      RC(gemm2(OP_N, OP_N, b, MONE, q.YF + bb, bb, Lb, q.G21F, bb, Lb, ONE, q.Gl, bb, L1b, q.Gl, bb,
+               L1b, L - 1, Mi, s));
+      if (g.right) {
+        RC(gemm2(OP_N, OP_N, b, MONE, q.YL, bb, Lb, q.G21L + bb, bb, Lb, ONE, q.Gu, bb, L1b, q.Gu,
+                 bb, L1b, L - 1, Mi, s));
+        RC(gemm2(OP_N, OP_N, b, MONE, q.YL + bb, bb, Lb, q.G21L, bb, Lb, ONE, q.Gl, bb, L1b, q.Gl,
+                 bb, L1b, L - 1, Mi, s));
+      }
+    }
+    // scatter the interior band to the original order
+    const long long foff = (long long)(g.c0 * (bs + 1) + 1) * bb;
+    RC(zcopy3d_launch(bb, L, g.nC, batch, ONE, q.Gd, bb, Lb, nCLb, G.diag + foff, bb, cstride, nd,
+                      s));
+    if (L > 1) {
+      RC(zcopy3d_launch(bb, L - 1, g.nC, batch, ONE, q.Gu, bb, L1b, (long long)g.nC * L1b,
+                        G.up + foff, bb, cstride, no, s));
+      RC(zcopy3d_launch(bb, L - 1, g.nC, batch, ONE, q.Gl, bb, L1b, (long long)g.nC * L1b,
+                        G.low + foff, bb, cstride, no, s));
+    }
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+size_t ddrgf_work_bytes(int N, int b, int batch, const int* levels, int nlev) {
+  Bump meas{nullptr};
+  BtdIn A{nullptr, 0, nullptr, 0, nullptr, 0};
+  BtdOut G{nullptr, nullptr, nullptr};
+  if (ddrgf_run(N, b, A, G, batch, levels, nlev, nullptr, meas, nullptr) != Status::OK) return 0;
+  return meas.off + 4096;
+}
+
+int ddrgf(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const int* levels, int nlev,
+          int* info, void* work, hipStream_t s) {
+  Bump bw{(char*)work};
+  return ddrgf_run(N, b, A, G, batch, levels, nlev, info, bw, s);
+}
+
 }  // namespace negf

@@ -59,6 +59,11 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                    int streams, hipStream_t* pool, void* work, hipStream_t s);
 
+// DD-RGF (uniform block size b; A.low must be given)
+size_t ddrgf_work_bytes(int N, int b, int batch, const int* levels, int nlev);
+int ddrgf(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const int* levels, int nlev,
+          int* info, void* work, hipStream_t s);
+
 // selected inverse on a block-sparse pattern (selinv.hip)
 std::vector<std::pair<int, int>> block_fill_pattern(int npl,
                                                     const std::vector<std::pair<int, int>>& in);

@@ -111,7 +111,34 @@ __global__ __launch_bounds__(256) void dotc_re_kernel(int m, int n, const zc* __
 
 inline int ok() { return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR; }
 
+// grid: (chunks of the block, n1 * n2, n3)
+__global__ void copy3d_kernel(long long count, int n1, zc alpha, const zc* __restrict__ src,
+                              long long s1, long long s2, long long s3, zc* __restrict__ dst,
+                              long long d1, long long d2, long long d3) {
+  const int i12 = blockIdx.y;
+  const long long i1 = i12 % n1, i2 = i12 / n1, i3 = blockIdx.z;
+  const zc* sp = src + i1 * s1 + i2 * s2 + i3 * s3;
+  zc* dp = dst + i1 * d1 + i2 * d2 + i3 * d3;
+  const bool one = (alpha.x == 1.0 && alpha.y == 0.0);
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    dp[e] = one ? sp[e] : zmul(alpha, sp[e]);
+}
+
 }  // namespace
+
+int zcopy3d_launch(long long count, int n1, int n2, int n3, zc alpha, const zc* src, long long s1,
+                   long long s2, long long s3, zc* dst, long long d1, long long d2, long long d3,
+                   hipStream_t s) {
+  if (count < 0 || n1 < 0 || n2 < 0 || n3 < 0) return Status::BAD_ARG;
+  if (count == 0 || n1 == 0 || n2 == 0 || n3 == 0) return Status::OK;
+  if ((long long)n1 * n2 > 65535 || n3 > 65535) return Status::TOO_LARGE;
+  long long bx = (count + 255) / 256;
+  if (bx > 64) bx = 64;
+  hipLaunchKernelGGL(copy3d_kernel, dim3((unsigned)bx, n1 * n2, n3), dim3(256), 0, s, count, n1,
+                     alpha, src, s1, s2, s3, dst, d1, d2, d3);
+  return ok();
+}
 
 int zadjoint_launch(int m, int n, const zc* A, long long lda, long long strideA, zc* B,
                     long long ldb, long long strideB, int batch, hipStream_t s) {

@@ -64,8 +64,10 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
   const int m = g.m, n = g.n, k = g.k;
   const int i0 = tm * TM, j0 = tn * TN;
 
-  const zc* __restrict__ A = g.A + inst * g.strideA;
-  const zc* __restrict__ B = g.B + inst * g.strideB;
+  const long long i_in = g.inner > 0 ? inst % g.inner : inst;
+  const long long i_out = g.inner > 0 ? inst / g.inner : 0;
+  const zc* __restrict__ A = g.A + i_in * g.strideA + i_out * g.outerA;
+  const zc* __restrict__ B = g.B + i_in * g.strideB + i_out * g.outerB;
   const long long lda = g.lda, ldb = g.ldb;
 
   __shared__ double As[KP][LDS_S];
@@ -168,8 +170,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
 
   // Epilogue.  Output layout of the 16x16 f64 MFMA on gfx950 (measured):
   // lane l, element e holds row l/16 + 4*e, column l % 16.
-  zc* __restrict__ D = g.D + inst * g.strideD;
-  const zc* __restrict__ C = g.C ? g.C + inst * g.strideC : nullptr;
+  zc* __restrict__ D = g.D + i_in * g.strideD + i_out * g.outerD;
+  const zc* __restrict__ C = g.C ? g.C + i_in * g.strideC + i_out * g.outerC : nullptr;
   const bool use_c = (g.beta.x != 0.0 || g.beta.y != 0.0) && C != nullptr;
   zc sh = zmake(0.0, 0.0);
   const bool use_shift = g.shift != nullptr;

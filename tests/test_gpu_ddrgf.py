@@ -1,0 +1,68 @@
+"""DD-RGF on the GPU vs the oracle (DD-RGF and sequential RGF), one and two
+levels, including a short last task, batched instances and the uneven-device
+shapes at reduced sizes."""
+import numpy as np
+import pytest
+
+from conftest import max_block_err
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from libnegf_amd import ddrgf as dd  # noqa: E402
+from libnegf_amd.btd import DeviceBlockTridiag  # noqa: E402
+from libnegf_amd.synthetic import bm_create_synthetic_random, device_hamiltonian, assemble_A  # noqa
+from oracle.rgf import bndiag_of_inv_rgf_local  # noqa: E402
+from oracle.ddrgf import bndiag_of_inv_ddrgf as dd_ref  # noqa: E402
+
+
+def _band(h):
+    G = {(i, i): h.diag[i] for i in range(h.nblocks)}
+    for i in range(h.nblocks - 1):
+        G[(i, i + 1)] = h.upper[i]
+        G[(i + 1, i)] = h.lower[i]
+    return G
+
+
+def _run(mats, levels):
+    Min = DeviceBlockTridiag.from_host(mats)
+    Mout = Min.similar()
+    dd.bndiag_of_inv_ddrgf(Mout, Min, dd.allocate_aux_data_DDRGF(Min, levels=levels))
+    return Mout
+
+
+@pytest.mark.parametrize("npl,b,levels", [(16, 32, [3]), (10, 24, [3]), (14, 20, [2]),
+                                          (12, 16, [1]), (36, 16, [3, 2]), (5, 16, [8])])
+def test_ddrgf_vs_rgf(npl, b, levels):
+    print(f"ddrgf npl={npl} b={b} levels={levels}", flush=True)
+    A = bm_create_synthetic_random(npl, b, seed=npl + b, uneven=False)
+    G = _band(_run(A, levels).to_host())
+    R = bndiag_of_inv_rgf_local(A)
+    err = max_block_err(G, R)
+    print(f"  max block relerr vs RGF {err:.2e}")
+    assert err < 1e-11
+    if len(levels) == 1 and levels[0] < npl:
+        assert max_block_err(G, dd_ref(A, levels)) < 1e-11
+
+
+def test_ddrgf_batched_energies():
+    print("ddrgf batch of energies", flush=True)
+    dev = device_hamiltonian(32, 48, seed=5)
+    mats = [assemble_A(dev, E, 1e-3) for E in (-0.6, 0.0, 0.45)]
+    out = _run(mats, dd.levels_for_tasks(32, 8))
+    for k, A in enumerate(mats):
+        assert max_block_err(_band(out.to_host(k)), bndiag_of_inv_rgf_local(A)) < 1e-11
+
+
+def test_ddrgf_tuner_and_task_split():
+    assert dd.levels_for_tasks(128, 8) == [15]
+    assert dd.levels_for_tasks(64, 8) == [7]
+    assert dd.levels_for_tasks(32, 8) == [3]
+    assert dd.bndiag_of_inv_ddrgf_get_nr_tasks(9, 3) == [-1, 0]
+    Min = DeviceBlockTridiag([16] * 40)
+    aux = dd.allocate_aux_data_DDRGF(Min)
+    n = 40
+    for bs in aux.levels:
+        nr, _ = dd.bndiag_of_inv_ddrgf_get_nr_tasks(n, bs)
+        assert nr >= 2
+        n = nr
