@@ -188,6 +188,40 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams);
 
+/* ------------------------------------------------- CSC (the reference ABI) */
+
+/* Memory mirror of the reference's CCscMatrix
+ * (src/c_interface/libnegf_c_interface.h:12-19): 0-based CSC; nzval holds
+ * 2 doubles (re, im) per entry, i.e. C99 `double complex`.  Arrays returned
+ * by the calls below are malloc'd; the caller free()s them. */
+typedef struct {
+  int32_t m;
+  int32_t n;
+  int32_t nnz;
+  int32_t* colptr;
+  int32_t* rowval;
+  double* nzval;
+} CCscMatrix;
+
+/* Drop-in for the reference's exported function
+ * (src/c_interface/libnegf_c_interface.h:22-38, libnegf_c_interface.jl:
+ * 85-142): the CSC matrix of the block band of A^{-1} for a CSC block-
+ * tridiagonal A (ndiag keys "in"/"out" must be 3), computed on the GPU by
+ * DD-RGF (uniform blocks, ~8 partitions) or RGF.  On failure it returns
+ * nnz = 0 and null arrays (the reference has no error channel). */
+CCscMatrix run_bndiag_wrapper(int32_t m, int32_t n, int32_t nnz_in, int32_t* colptr_in,
+                              int32_t* rowval_in, double* nzval_in, int32_t n_blocks,
+                              int32_t* block_sizes_in, int32_t n_dict, const char** dict_keys_in,
+                              int32_t* dict_vals_in, int32_t is_hermitian);
+
+/* The same with a status (NEGF_OK, negative error, > 0 singular layer), a
+ * method (0 auto, 1 RGF, 2 DD-RGF) and a stream. */
+int negf_run_bndiag_csc(int32_t m, int32_t n, int32_t nnz_in, const int32_t* colptr_in,
+                        const int32_t* rowval_in, const double* nzval_in, int32_t n_blocks,
+                        const int32_t* block_sizes_in, int32_t n_dict, const char** dict_keys_in,
+                        const int32_t* dict_vals_in, int32_t is_hermitian, int32_t method,
+                        CCscMatrix* out, void* stream);
+
 /* ----------------------------------------------------------------- DD-RGF */
 
 /* band(A^{-1}) by domain-decomposed RGF, uniform block size b, batched over

@@ -20,8 +20,22 @@ i64 = C.c_int64
 ch = C.c_char
 d = C.c_double
 
+class CCscMatrix(C.Structure):
+    """Mirror of CCscMatrix (include/negf_hip.h, the reference's
+    src/c_interface/libnegf_c_interface.h:12-19)."""
+
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int32),
+                ("colptr", C.POINTER(C.c_int32)), ("rowval", C.POINTER(C.c_int32)),
+                ("nzval", C.POINTER(C.c_double))]
+
+
 # name -> (restype, argtypes); mirrors include/negf_hip.h
 SIGNATURES = {
+    "run_bndiag_wrapper": (CCscMatrix, [i32, i32, i32, c_int32_p, c_int32_p, c_double_p, i32,
+                                        c_int32_p, i32, C.POINTER(C.c_char_p), c_int32_p, i32]),
+    "negf_run_bndiag_csc": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p, c_double_p, i32,
+                                      c_int32_p, i32, C.POINTER(C.c_char_p), c_int32_p, i32, i32,
+                                      C.POINTER(CCscMatrix), vp]),
     "negf_version": (C.c_int, []),
     "negf_status_string": (C.c_char_p, [C.c_int]),
     "negf_workspace_release": (C.c_int, []),
@@ -102,7 +116,7 @@ def load(path: pathlib.Path | None = None) -> C.CDLL:
 def call(name: str, *args) -> int:
     lib = load()
     rc = getattr(lib, name)(*args)
-    if name.endswith("_workspace"):
+    if name.endswith("_workspace") or name == "run_bndiag_wrapper":
         return rc
     if rc < 0:
         raise NegfError(name, rc, lib.negf_status_string(rc).decode())

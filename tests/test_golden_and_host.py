@@ -130,8 +130,8 @@ def test_packed_layout_roundtrip_on_cpu_tensors():
 
 def _declared_symbols():
     text = (ROOT / "include" / "negf_hip.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(negf_\w+)\s*\(", text,
-                                 re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*|CCscMatrix)\s+(\w+)\s*\(",
+                                 text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
