@@ -58,9 +58,12 @@ def test_run_bndiag_wrapper_uneven_and_hermitian_storage():
     assert max_block_err(got, ref) < 1e-12
     # Hermitian input given with only its upper band: the lower band is read as the adjoint
     dev = device_hamiltonian(8, 16, seed=2)
-    H = sp.csc_matrix(np.triu(assemble_A(dev, 0.3, 0.0).to_dense() * 0 + _herm(dev)))
-    out = csc.run_bndiag_wrapper(H, [16] * 8, is_hermitian=True, method=1)
     full = _herm(dev)
+    upper_only = full.copy()
+    for i in range(7):      # Hermitian storage drops the lower off-diagonal blocks only
+        upper_only[(i + 1) * 16:(i + 2) * 16, i * 16:(i + 1) * 16] = 0
+    out = csc.run_bndiag_wrapper(sp.csc_matrix(upper_only), [16] * 8, is_hermitian=True,
+                                 method=1)
     ref = band_of_dense(np.linalg.inv(full), [16] * 8, 1)
     Dh = out.toarray()
     o = np.arange(9) * 16
