@@ -52,6 +52,13 @@ int negf_version(void);
 const char* negf_status_string(int status);
 /* Frees the cached sweep workspaces (the next sweep reallocates). */
 int negf_workspace_release(void);
+/* Autotuner knob: tile variant of the MFMA ZGEMM for k > 32, process-wide
+ * (0: 64x64 tile, one LDS buffer (default); 1: 64x64 double-buffered;
+ * 2: 64x128 one buffer; 3: 64x128 double-buffered; 4: 128x64 double-buffered). */
+int negf_gemm_variant(int32_t variant);
+/* Autotuner knob: panel of the Gauss-Jordan inverse, process-wide
+ * (0: 16 columns, 256 threads; 1: 32 columns, 512 threads, for b <= 512). */
+int negf_inv_variant(int32_t variant);
 
 /* ------------------------------------------------------------------ per-op */
 

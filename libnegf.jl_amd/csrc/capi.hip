@@ -84,6 +84,9 @@ const char* negf_status_string(int status) {
   return status > 0 ? "singular pivot" : "unknown status";
 }
 
+int negf_gemm_variant(int32_t variant) { return zgemm_set_variant(variant); }
+int negf_inv_variant(int32_t variant) { return zinv_set_variant(variant); }
+
 int negf_workspace_release(void) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   (void)hipDeviceSynchronize();

@@ -68,6 +68,8 @@ struct GemmArgs {
   long long outerA, outerB, outerC, outerD;
 };
 int zgemm_launch(const GemmArgs& g, hipStream_t s);
+// Tile variant of the k > 32 GEMMs (0..4, see zgemm.hip), process-wide.
+int zgemm_set_variant(int v);
 
 // Several independent GEMMs of one shape in one launch (fills the GPU when
 // each problem alone is a handful of tiles).
@@ -85,6 +87,8 @@ struct InvWork {
   int* used;    // n*batch
 };
 size_t zinv_work_bytes(int n, int batch);
+// Panel variant of the inverse (0: 16-column panels, 1: 32-column), process-wide.
+int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);
 // A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the
 // 1-based column of the first exactly-zero pivot.

@@ -12,12 +12,17 @@
 // A- sign flip a per-lane constant.  Two MFMAs per 16x16 output block and two
 // complex K steps.
 //
-// Tiling: a 256-thread workgroup computes a 64x64 complex tile of D; its four
-// waves each own 32x32 (2x2 MFMA blocks, re and im accumulators: 32 doubles).
-// K is staged 16 complex at a time through LDS (K'-major rows, padded), the
-// next stage prefetched into registers while the MFMAs run.  op(A) / op(B)
-// transposition and conjugation happen in the global->LDS staging, so the MFMA
-// loop is identical for all nine N/T/C combinations.
+// Tiling: a 256-thread workgroup = 2 x 2 waves; each wave owns FM x FN MFMA
+// blocks of 16 x 16 (re and im accumulators), so the workgroup tile is
+// (32 FM) x (32 FN) complex.  K is staged 16 complex at a time through LDS
+// (K'-major rows), the next stage prefetched into registers while the MFMAs
+// run; with DB the LDS is double-buffered (one barrier per stage).
+// Staging is bank-conflict free: every wave-wide LDS store writes 64
+// consecutive doubles of one row (operands contiguous along K are loaded 4
+// complex per lane and scattered to 4 rows), and rows are padded to 16 mod 32
+// doubles so the MFMA operand reads of two rows fall in disjoint banks.
+// op(A) / op(B) transposition and conjugation happen in the staging, so the
+// MFMA loop is identical for all nine N/T/C combinations.
 //
 // Launch: one flat grid over (problem, instance, tile).  Workgroups are
 // dispatched round-robin over the 8 XCDs; the index is remapped so that the
@@ -30,9 +35,9 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TK = 16;            // complex K per stage
 constexpr int KP = 2 * TK;        // K'-space rows per stage
-constexpr int LDS_S = TM + 2;     // padded LDS row, doubles
+constexpr int PAD = 16;           // row padding (doubles): rows 16 mod 32 apart
 constexpr int MAXG = 4;           // problems per grouped launch
 
 struct GemmGroup {
@@ -44,11 +49,18 @@ struct GemmGroup {
   long long grid_padded;          // multiple of 8
 };
 
+// Tile configuration (FM, FN: MFMA blocks per wave; DB: double-buffered LDS).
 // SMALLK = 1 tags the instantiation used for k <= 32 (the rank-NB updates of
 // the Gauss-Jordan / LU panels and k = 0 shifts): the same code, a separate
 // symbol, so profiles separate the memory-bound updates from the full GEMMs.
-template <int OPA, int OPB, int SMALLK>
-__global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
+template <int OPA, int OPB, int FM, int FN, bool DB, int SMALLK>
+__global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma_kernel(
+    const GemmGroup G) {
+  constexpr int TM = 32 * FM, TN = 32 * FN;
+  constexpr int SA = TM + PAD, SB = TN + PAD;
+  constexpr int RA = TM / 64, RB = TN / 64;     // 64-wide row groups per thread
+  constexpr int NBUF = DB ? 2 : 1;
+
   // XCD-aware remap of the flat workgroup index.
   const long long bid = blockIdx.x;
   const long long per = G.grid_padded >> 3;
@@ -70,63 +82,70 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
   const zc* __restrict__ B = g.B + i_in * g.strideB + i_out * g.outerB;
   const long long lda = g.lda, ldb = g.ldb;
 
-  __shared__ double As[KP][LDS_S];
-  __shared__ double Bs[KP][LDS_S];
+  __shared__ double As[NBUF][KP][SA];
+  __shared__ double Bs[NBUF][KP][SB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
+  const int tl = tid & 63, tw = tid >> 6;   // loader: column/row in a 64-group, wave
 
-  dbl4 acc_re[2][2], acc_im[2][2];
+  dbl4 acc_re[FM][FN], acc_im[FM][FN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < FM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < FN; ++b) {
       acc_re[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
       acc_im[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
-  zc ra[4], rb[4];
+  // prefetch registers: per 64-group r and slot u (4 complex)
+  zc ra[RA][4], rb[RB][4];
 
+  // A: OP_N -> element (i = tl + 64r, k = tw + 4u); T/C -> (i = tl + 64r, k = 4tw + u)
   auto load_stage = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int i, kk;
-      if (OPA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * q; }
-      else             { kk = tid & 15; i = (tid >> 4) + 16 * q; }
-      const int gi = i0 + i, gk = k0 + kk;
-      zc v = zmake(0.0, 0.0);
-      if (gi < m && gk < k) v = (OPA == OP_N) ? A[gi + gk * lda] : A[gk + gi * lda];
-      ra[q] = v;
-    }
+    for (int r = 0; r < RA; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int j, kk;
-      if (OPB == OP_N) { kk = tid & 15; j = (tid >> 4) + 16 * q; }
-      else             { j = tid & 63; kk = (tid >> 6) + 4 * q; }
-      const int gj = j0 + j, gk = k0 + kk;
-      zc v = zmake(0.0, 0.0);
-      if (gj < n && gk < k) v = (OPB == OP_N) ? B[gk + gj * ldb] : B[gj + gk * ldb];
-      rb[q] = v;
-    }
+      for (int u = 0; u < 4; ++u) {
+        const int i = tl + 64 * r;
+        const int kk = (OPA == OP_N) ? (tw + 4 * u) : (4 * tw + u);
+        const int gi = i0 + i, gk = k0 + kk;
+        zc v = zmake(0.0, 0.0);
+        if (gi < m && gk < k) v = (OPA == OP_N) ? A[gi + gk * lda] : A[gk + gi * lda];
+        ra[r][u] = v;
+      }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = tl + 64 * r;
+        const int kk = (OPB == OP_N) ? (4 * tw + u) : (tw + 4 * u);
+        const int gj = j0 + j, gk = k0 + kk;
+        zc v = zmake(0.0, 0.0);
+        if (gj < n && gk < k) v = (OPB == OP_N) ? B[gk + gj * ldb] : B[gj + gk * ldb];
+        rb[r][u] = v;
+      }
   };
-  auto store_stage = [&]() {
+  auto store_stage = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int i, kk;
-      if (OPA == OP_N) { i = tid & 63; kk = (tid >> 6) + 4 * q; }
-      else             { kk = tid & 15; i = (tid >> 4) + 16 * q; }
-      As[2 * kk][i] = ra[q].x;
-      As[2 * kk + 1][i] = (OPA == OP_C) ? -ra[q].y : ra[q].y;
-    }
+    for (int r = 0; r < RA; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int j, kk;
-      if (OPB == OP_N) { kk = tid & 15; j = (tid >> 4) + 16 * q; }
-      else             { j = tid & 63; kk = (tid >> 6) + 4 * q; }
-      Bs[2 * kk][j] = rb[q].x;
-      Bs[2 * kk + 1][j] = (OPB == OP_C) ? -rb[q].y : rb[q].y;
-    }
+      for (int u = 0; u < 4; ++u) {
+        const int i = tl + 64 * r;
+        const int kk = (OPA == OP_N) ? (tw + 4 * u) : (4 * tw + u);
+        As[buf][2 * kk][i] = ra[r][u].x;
+        As[buf][2 * kk + 1][i] = (OPA == OP_C) ? -ra[r][u].y : ra[r][u].y;
+      }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = tl + 64 * r;
+        const int kk = (OPB == OP_N) ? (4 * tw + u) : (tw + 4 * u);
+        Bs[buf][2 * kk][j] = rb[r][u].x;
+        Bs[buf][2 * kk + 1][j] = (OPB == OP_C) ? -rb[r][u].y : rb[r][u].y;
+      }
   };
 
   const int nk = (k + TK - 1) / TK;
@@ -134,36 +153,45 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
   const double asign = (kq & 1) ? -1.0 : 1.0;
   const int li = lane & 15;
 
-  load_stage(0);
-  store_stage();
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load_stage((kt + 1) * TK);
+  auto compute_stage = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < TK / 2; ++ks) {
       const int kr = 4 * ks + kq;
-      double a[2], am[2], b1[2], b2[2];
+      double a[FM], am[FM], b1[FN], b2[FN];
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        a[r] = As[kr][wm * 32 + r * 16 + li];
+      for (int r = 0; r < FM; ++r) {
+        a[r] = As[buf][kr][wm * 16 * FM + r * 16 + li];
         am[r] = asign * a[r];
       }
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        b1[c] = Bs[kr][wn * 32 + c * 16 + li];
-        b2[c] = Bs[kr ^ 1][wn * 32 + c * 16 + li];
+      for (int c = 0; c < FN; ++c) {
+        b1[c] = Bs[buf][kr][wn * 16 * FN + c * 16 + li];
+        b2[c] = Bs[buf][kr ^ 1][wn * 16 * FN + c * 16 + li];
       }
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < FM; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < FN; ++c) {
           acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[r], b1[c], acc_re[r][c], 0, 0, 0);
           acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b2[c], acc_im[r][c], 0, 0, 0);
         }
     }
-    if (kt + 1 < nk) {
+  };
+
+  load_stage(0);
+  store_stage(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = DB ? (kt & 1) : 0;
+    if (kt + 1 < nk) load_stage((kt + 1) * TK);
+    compute_stage(cur);
+    if (DB) {
+      // the other buffer was last read in stage kt-1, before the barrier below it
+      if (kt + 1 < nk) store_stage(cur ^ 1);
       __syncthreads();
-      store_stage();
+    } else if (kt + 1 < nk) {
+      __syncthreads();
+      store_stage(0);
       __syncthreads();
     }
   }
@@ -177,13 +205,13 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
   const bool use_shift = g.shift != nullptr;
   if (use_shift) sh = g.shift[inst * g.shift_stride];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < FM; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int j = j0 + wn * 32 + c * 16 + li;
+    for (int c = 0; c < FN; ++c) {
+      const int j = j0 + wn * 16 * FN + c * 16 + li;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = i0 + wm * 32 + r * 16 + kq + 4 * e;
+        const int i = i0 + wm * 16 * FM + r * 16 + kq + 4 * e;
         if (i < m && j < n) {
           const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
           zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
@@ -195,14 +223,41 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(const GemmGroup G) {
     }
 }
 
+// Tile variants (autotuner choice, negf_gemm_variant).  0: 64x64 single
+// buffer, 1: 64x64 double buffer, 2: 64x128 single, 3: 64x128 double,
+// 4: 128x64 double.  Measured on MI355X at 512^3 x 64 (TFLOP/s, NN/NC/CN/CC):
+// 0: 53.7/59.5/61.2/61.2, 1: 51.7/55.9/53.1/54.2, 2: 48.9/51.2/46.1/49.6,
+// 3: 48.4/51.6/45.3/49.6, 4: 49.7/50.7/45.2/44.9 -- occupancy (3 waves per
+// SIMD for variant 0) matters more than tile reuse at this FP64 MFMA rate.
+int g_variant = 0;
+constexpr int NVAR = 5;
+const int VAR_FM[NVAR] = {2, 2, 2, 2, 4};
+const int VAR_FN[NVAR] = {2, 2, 4, 4, 2};
+
+template <int OPA, int OPB, int FM, int FN, bool DB, int SK>
+void launch_cfg(GemmGroup G, hipStream_t s) {
+  G.tiles_m = (G.p[0].m + 32 * FM - 1) / (32 * FM);
+  G.tiles_n = (G.p[0].n + 32 * FN - 1) / (32 * FN);
+  G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * G.p[0].batch;
+  G.total = G.tiles_per_problem * G.count;
+  G.grid_padded = (G.total + 7) / 8 * 8;
+  hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, FM, FN, DB, SK>), dim3((unsigned)G.grid_padded),
+                     dim3(256), 0, s, G);
+}
+
 template <int OPA, int OPB>
 void launch_t(const GemmGroup& G, hipStream_t s) {
-  if (G.p[0].k <= 2 * TK)
-    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, 1>), dim3((unsigned)G.grid_padded), dim3(256),
-                       0, s, G);
-  else
-    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, 0>), dim3((unsigned)G.grid_padded), dim3(256),
-                       0, s, G);
+  if (G.p[0].k <= 2 * TK) {
+    launch_cfg<OPA, OPB, 2, 2, false, 1>(G, s);
+    return;
+  }
+  switch (g_variant) {
+    case 0: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
+    case 1: launch_cfg<OPA, OPB, 2, 2, true, 0>(G, s); break;
+    case 2: launch_cfg<OPA, OPB, 2, 4, false, 0>(G, s); break;
+    case 3: launch_cfg<OPA, OPB, 2, 4, true, 0>(G, s); break;
+    default: launch_cfg<OPA, OPB, 4, 2, true, 0>(G, s); break;
+  }
 }
 
 int check_args(const GemmArgs& g) {
@@ -219,10 +274,17 @@ int check_args(const GemmArgs& g) {
     if ((g.beta.x != 0.0 || g.beta.y != 0.0) && (!g.C || g.ldc < (g.m > 1 ? g.m : 1)))
       return Status::BAD_ARG;
   }
+  if (g.inner < 0) return Status::BAD_ARG;
   return Status::OK;
 }
 
 }  // namespace
+
+int zgemm_set_variant(int v) {
+  if (v < 0 || v >= NVAR) return Status::BAD_ARG;
+  g_variant = v;
+  return Status::OK;
+}
 
 int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   if (count <= 0) return Status::OK;
@@ -246,16 +308,13 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   }
   const GemmArgs& g0 = g[0];
   if (g0.m == 0 || g0.n == 0 || g0.batch == 0) return Status::OK;
-  GemmGroup G;
+  GemmGroup G{};
   for (int i = 0; i < count; ++i) G.p[i] = g[i];
   G.count = count;
+  // grid size bound with the smallest tile (64 x 64)
+  const long long tiles = (long long)((g0.m + 63) / 64) * ((g0.n + 63) / 64) * g0.batch * count;
+  if (tiles + 8 > 0x7fffffffLL) return Status::TOO_LARGE;
   // k == 0: D = beta C (+shift); the kernel handles it (no K stages, zero acc).
-  G.tiles_m = (g0.m + TM - 1) / TM;
-  G.tiles_n = (g0.n + TN - 1) / TN;
-  G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * g0.batch;
-  G.total = G.tiles_per_problem * count;
-  G.grid_padded = (G.total + 7) / 8 * 8;
-  if (G.grid_padded > 0x7fffffffLL) return Status::TOO_LARGE;
   const int key = g0.opA * 3 + g0.opB;
   switch (key) {
     case 0: launch_t<OP_N, OP_N>(G, s); break;

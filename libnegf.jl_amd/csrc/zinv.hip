@@ -247,10 +247,22 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
 }
 
 // Panel width: the n x NB panel lives in registers (RPT * NB complex per
-// thread <= 32), so wider panels need more threads per row block.
-int nb_for(int n) { return n <= 1024 ? 16 : 8; }
+// thread <= 32), so wider panels need more threads per row block.  The
+// workspace is sized for the widest panel of any variant.
+int nb_for(int n) { return n <= 1024 ? 32 : 8; }
+
+// Panel variant (autotuner choice, negf_inv_variant): 0 = 16-column panels
+// of 256 threads, 1 = 32-column panels of 512 threads (half the rank-NB
+// update passes over the work matrix, a wider barrier per column).
+int g_inv_variant = 0;
 
 }  // namespace
+
+int zinv_set_variant(int v) {
+  if (v < 0 || v > 1) return Status::BAD_ARG;
+  g_inv_variant = v;
+  return Status::OK;
+}
 
 size_t zinv_work_bytes(int n, int batch) {
   const size_t nb = nb_for(n);
@@ -282,6 +294,10 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
+  if (g_inv_variant == 1) {
+    if (n <= 256) return run<256, 1, 32>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (n <= 512) return run<512, 1, 32>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  }
   if (n <= 256) return run<256, 1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
   if (n <= 512) return run<256, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
   if (n <= 1024) return run<512, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);

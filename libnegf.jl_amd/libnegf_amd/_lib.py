@@ -39,6 +39,8 @@ SIGNATURES = {
     "negf_version": (C.c_int, []),
     "negf_status_string": (C.c_char_p, [C.c_int]),
     "negf_workspace_release": (C.c_int, []),
+    "negf_gemm_variant": (C.c_int, [i32]),
+    "negf_inv_variant": (C.c_int, [i32]),
     "negf_zgemm": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, vp, i64, c_double_p, vp,
                              i64, vp]),
     "negf_zgemm_strided_batched": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, i64, vp,
