@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture the energy job once into a HIP graph and time its replays")
     ap.add_argument("--gemm-reps", type=int, default=20)
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant-kernel timing plus a plain copy of known size "
@@ -173,9 +175,20 @@ def main():
           file=sys.stderr,
           flush=True)
 
+    graph = None
+    if a.graph:
+        t_c = time.perf_counter()
+        graph = api.EnergiesGraph(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
+                                  streams=streams)
+        print(f"# graph captured and instantiated in {time.perf_counter() - t_c:.2f}s",
+              file=sys.stderr, flush=True)
+
     def step():
-        api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
-                                 streams=streams, check=False)
+        if graph is not None:
+            graph.launch(check=False)
+        else:
+            api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
+                                     streams=streams, check=False)
 
     for w in range(a.warmup):
         step()
@@ -214,7 +227,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"rgf_keldysh_batched: N={N} b={b} energies={nE} "
                                f"linspace(-1,1) eta={eta} kT={kT} muL={muL} muR={muR}",
-                   "form": "plain FP64", "wave": wave, "streams": streams,
+                   "form": "plain FP64" + (", one hipGraph launch per step" if a.graph else ""),
+                   "wave": wave, "streams": streams,
                    "flops_per_energy": fl},
         "roofline": roof,
         "cpu_baseline": cpu,

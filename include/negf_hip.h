@@ -195,6 +195,26 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams);
 
+/* The same energy job captured once into a HIP graph (no reference
+ * counterpart).  _create records every launch of negf_rgf_keldysh_energies
+ * with these exact pointers and scalars into a graph with its own workspace
+ * (held until _destroy) and instantiates it; nothing runs until _launch.
+ * Each _launch replays the whole job on `stream` as one submission: new
+ * energies, H or Gamma are taken by writing new values into the same device
+ * arrays between launches (eta, mu_L, mu_R, kT are fixed at capture).
+ * Launches of one graph must not overlap in time (they share the workspace). */
+typedef struct negf_energies_graph negf_energies_graph;
+int negf_rgf_keldysh_energies_graph_create(int32_t nblocks, int32_t b,
+                                           const void* H_diag, const void* H_upper,
+                                           const void* gamma_L, const void* gamma_R,
+                                           const double* energies, int32_t n_energies,
+                                           double eta, double mu_L, double mu_R, double kT,
+                                           void* GR_diag, void* Gless_diag, double* transmission,
+                                           int32_t* info, int32_t wave, int32_t streams,
+                                           negf_energies_graph** graph);
+int negf_energies_graph_launch(negf_energies_graph* graph, void* stream);
+int negf_energies_graph_destroy(negf_energies_graph* graph);
+
 /* ------------------------------------------------- CSC (the reference ABI) */
 
 /* Memory mirror of the reference's CCscMatrix

@@ -307,50 +307,57 @@ int stream_pool(int n, hipStream_t** out) {
 }
 }  // namespace
 
-int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
-                              const void* H_upper, const void* gamma_L, const void* gamma_R,
-                              const double* energies, int32_t n_energies, double eta, double mu_L,
-                              double mu_R, double kT, void* GR_diag, void* Gless_diag,
-                              double* transmission, int32_t* info, int32_t wave, int32_t streams,
-                              void* stream) {
-  if (nblocks <= 0 || b <= 0 || b > 2048 || n_energies < 0 || wave < 0 || streams < 0)
+}  // extern "C"
+
+namespace {
+// Argument checks, wave size and stream count of the energy pipeline.
+// Measured on MI355X (N = 64, b = 512): one stream with waves of 64 beats
+// two streams of 32 (each batched GEMM already fills the GPU, and the
+// per-energy Gauss-Jordan panels do not get CUs while a GEMM holds them).
+int energies_plan(int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper,
+                  const void* gamma_L, const void* gamma_R, const double* energies,
+                  int32_t n_energies, double kT, int32_t* wave, int32_t* streams) {
+  if (nblocks <= 0 || b <= 0 || b > 2048 || n_energies < 0 || *wave < 0 || *streams < 0)
     return NEGF_BAD_ARG;
   if (n_energies == 0) return NEGF_OK;
   if (!H_diag || (nblocks > 1 && !H_upper) || !gamma_L || !gamma_R || !energies)
     return NEGF_BAD_ARG;
   if (!(kT > 0.0)) return NEGF_BAD_ARG;
-  std::vector<int32_t> sz(nblocks, b);
-  Layout L(nblocks, sz.data());
-  // Measured on MI355X (N = 64, b = 512): one stream with waves of 64 beats
-  // two streams of 32 (each batched GEMM already fills the GPU, and the
-  // per-energy Gauss-Jordan panels do not get CUs while a GEMM holds them).
-  if (streams == 0) streams = 1;
-  if (streams > MAX_STREAMS) streams = MAX_STREAMS;
-  if (wave == 0) {
+  if (*streams == 0) *streams = 1;
+  if (*streams > MAX_STREAMS) *streams = MAX_STREAMS;
+  if (*wave == 0) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) return NEGF_HIP_ERROR;
-    const int per = (n_energies + streams - 1) / streams;
-    wave = per < 64 ? per : 64;
-    while (wave > 1 &&
-           (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave, streams) >
+    const int per = (n_energies + *streams - 1) / *streams;
+    *wave = per < 64 ? per : 64;
+    while (*wave > 1 &&
+           (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, *wave, *streams) >
                freeb / 10 * 9)
-      wave /= 2;
+      *wave /= 2;
   }
-  if (wave > n_energies) wave = n_energies;
-  while (streams > 1 && (long long)(streams - 1) * wave >= n_energies) --streams;
+  if (*wave > n_energies) *wave = n_energies;
+  while (*streams > 1 && (long long)(*streams - 1) * *wave >= n_energies) --*streams;
+  return NEGF_OK;
+}
+
+// Everything the energy job puts on the stream: H' = H - (i/2) Gamma on the
+// contact layers (behind the sweep workspace), then the waves.  Kernels and
+// event fork/join only, so it can be captured into a graph.
+int energies_enqueue(int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper,
+                     const void* gamma_L, const void* gamma_R, const double* energies,
+                     int32_t n_energies, double eta, double mu_L, double mu_R, double kT,
+                     void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
+                     int32_t wave, int32_t streams, void* w, hipStream_t s) {
+  std::vector<int32_t> sz(nblocks, b);
+  Layout L(nblocks, sz.data());
   const size_t ew = energies_work_bytes(L, n_energies, wave, streams);
-  void* w = nullptr;
-  int rc = workspace(stream, ew + sizeof(zc) * (size_t)L.nd + 4096, &w);
-  if (rc) return rc;
   hipStream_t* pool = nullptr;
   if (streams > 1) {
-    rc = stream_pool(streams, &pool);
+    int rc = stream_pool(streams, &pool);
     if (rc) return rc;
   }
-  hipStream_t s = st(stream);
-  // H' = H - (i/2) Gamma on the contact layers, behind the sweep workspace
   zc* Hp = (zc*)((char*)w + (ew + 255) / 256 * 256);
-  rc = zcopy2d_launch((int)L.nd, 1, (const zc*)H_diag, L.nd, 0, Hp, L.nd, 0, 1, s);
+  int rc = zcopy2d_launch((int)L.nd, 1, (const zc*)H_diag, L.nd, 0, Hp, L.nd, 0, 1, s);
   if (rc) return rc;
   rc = zaxpy_launch(b, b, zmake(0.0, -0.5), nullptr, 0, (const zc*)gamma_L, b, 0, Hp, b, 0, 1, s);
   if (rc) return rc;
@@ -361,6 +368,100 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
               n_energies, eta, mu_L, mu_R, kT};
   EnergyOut out{(zc*)GR_diag, (zc*)Gless_diag, transmission, info};
   return sweep_energies(L, in, out, wave, streams, pool, w, s);
+}
+}  // namespace
+
+// A captured energy job: its own workspace (held until destroy) and the
+// instantiated graph.
+struct negf_energies_graph {
+  void* work = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
+                              const void* H_upper, const void* gamma_L, const void* gamma_R,
+                              const double* energies, int32_t n_energies, double eta, double mu_L,
+                              double mu_R, double kT, void* GR_diag, void* Gless_diag,
+                              double* transmission, int32_t* info, int32_t wave, int32_t streams,
+                              void* stream) {
+  int rc = energies_plan(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, kT,
+                         &wave, &streams);
+  if (rc || n_energies == 0) return rc;
+  void* w = nullptr;
+  rc = workspace(stream, (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave,
+                                                                      streams), &w);
+  if (rc) return rc;
+  return energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies,
+                          eta, mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave,
+                          streams, w, st(stream));
+}
+
+int negf_rgf_keldysh_energies_graph_create(
+    int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper, const void* gamma_L,
+    const void* gamma_R, const double* energies, int32_t n_energies, double eta, double mu_L,
+    double mu_R, double kT, void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
+    int32_t wave, int32_t streams, negf_energies_graph** out) {
+  if (!out) return NEGF_BAD_ARG;
+  *out = nullptr;
+  if (n_energies <= 0) return NEGF_BAD_ARG;
+  int rc = energies_plan(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, kT,
+                         &wave, &streams);
+  if (rc) return rc;
+  const size_t bytes =
+      (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave, streams);
+  negf_energies_graph* g = new negf_energies_graph();
+  if (hipMalloc(&g->work, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete g;
+    return NEGF_NO_MEMORY;
+  }
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipFree(g->work);
+    delete g;
+    return NEGF_HIP_ERROR;
+  }
+  rc = NEGF_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NEGF_HIP_ERROR;
+  if (rc == NEGF_OK) {
+    rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, eta,
+                          mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave, streams,
+                          g->work, cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (rc == NEGF_OK && (e != hipSuccess || !graph)) rc = NEGF_HIP_ERROR;
+    g->graph = graph;
+    if (rc == NEGF_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = NEGF_HIP_ERROR;
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc != NEGF_OK) {
+    (void)hipGetLastError();
+    negf_energies_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return NEGF_OK;
+}
+
+int negf_energies_graph_launch(negf_energies_graph* g, void* stream) {
+  if (!g || !g->exec) return NEGF_BAD_ARG;
+  return hipGraphLaunch(g->exec, st(stream)) == hipSuccess ? NEGF_OK : NEGF_HIP_ERROR;
+}
+
+int negf_energies_graph_destroy(negf_energies_graph* g) {
+  if (!g) return NEGF_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->work) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(g->work);
+  }
+  delete g;
+  return NEGF_OK;
 }
 
 // ---------------------------------------------------------------------- DD-RGF

@@ -74,6 +74,10 @@ GemmArgs ga(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, 
     if (_rc) return _rc;      \
   } while (0)
 
+// The device constant 1 the shift GEMMs read (a kernel store, not a host copy,
+// so the sweeps contain only kernels and can be captured into a graph).
+static int set_one(zc* p, hipStream_t s) { return zfill_launch(1, 1, ONE, ONE, p, 1, 1, 1, s); }
+
 __global__ void fold_info_kernel(const int* layer_info, int* sweep_info, int layer, int batch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < batch && layer_info[i] != 0 && sweep_info[i] == 0) sweep_info[i] = layer + 1;
@@ -408,10 +412,7 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
               hipStream_t s) {
   SweepWork w;
   carve(work, L, batch, &w);
-  const zc one = ONE;
-  RC((int)hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) == 0
-         ? Status::OK
-         : Status::HIP_ERROR);
+  RC(set_one(w.ones, s));
   Source S{};
   S.diag = A.diag; S.diag_bs = A.diag_bs;
   S.up = A.up; S.up_bs = A.up_bs;
@@ -442,9 +443,7 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
   carve(work, L, wave, &w);
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
   zc* Suw = Sdw + (size_t)wave * L.nd;
-  const zc one = ONE;
-  if (hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
-    return Status::HIP_ERROR;
+  RC(set_one(w.ones, s));
   for (int e0 = e_begin; e0 < e_end; e0 += wave) {
     const int nb = (e_end - e0) < wave ? (e_end - e0) : wave;
     Source S{};
@@ -639,9 +638,7 @@ int ddrgf_run(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const in
     void* wp = bw.take_bytes(sweep_work_bytes(L, batch, false));
     if (!run) return Status::OK;
     carve(wp, L, batch, &w);
-    const zc one = ONE;
-    if (hipMemcpyAsync(w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
-      return Status::HIP_ERROR;
+    RC(set_one(w.ones, s));
     Source S{};
     S.diag = A.diag; S.diag_bs = A.diag_bs; S.up = A.up; S.up_bs = A.up_bs;
     S.low = A.low; S.low_bs = A.low_bs; S.sLow = ONE; S.sUp = ONE;
@@ -725,10 +722,7 @@ int ddrgf_run(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const in
   }
   if (!run) return Status::OK;
 
-  const zc one = ONE;
-  for (auto& q : gb)
-    if (hipMemcpyAsync(q.w.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
-      return Status::HIP_ERROR;
+  for (auto& q : gb) RC(set_one(q.w.ones, s));
 
   // ---- per group: chunk RGF, columns / rows of T11^{-1}, couplings
   for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -849,8 +843,7 @@ int ddrgf_run(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const in
       RC(ddrgf_run(T, b, SA, SG, batch, levels + 1, nlev - 1, sep_info, sub_bw, s));
     } else {
       carve(wTp, LT, batch, &wT);
-      if (hipMemcpyAsync(wT.ones, &one, sizeof(zc), hipMemcpyHostToDevice, s) != hipSuccess)
-        return Status::HIP_ERROR;
+      RC(set_one(wT.ones, s));
       Source S{};
       S.diag = Sd; S.diag_bs = Tb; S.up = Su; S.up_bs = T1b; S.low = Sl; S.low_bs = T1b;
       S.sLow = ONE; S.sUp = ONE;

@@ -195,6 +195,49 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
     return info
 
 
+class EnergiesGraph:
+    """The energy job of ``rgf_keldysh_energies`` captured once into a HIP
+    graph (negf_rgf_keldysh_energies_graph_create) and replayed by
+    ``launch()`` as one submission.  The tensors passed here are bound for the
+    graph's lifetime: change energies / H / Gamma by writing into them in
+    place between launches; eta, mu and kT are fixed."""
+
+    def __init__(self, H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float, mu_L: float,
+                 mu_R: float, kT: float, GR_diag: torch.Tensor | None = None,
+                 Gless_diag: torch.Tensor | None = None,
+                 transmission: torch.Tensor | None = None, wave: int = 0, streams: int = 0):
+        _require_gpu()
+        nE = int(energies.numel())
+        self.info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
+        # keep every bound tensor alive as long as the graph
+        self._bound = (H, energies, GR_diag, Gless_diag, transmission)
+        h = C.c_void_p()
+        call("negf_rgf_keldysh_energies_graph_create", H.N, H.b, _ptr(H.H_diag), _ptr(H.H_upper),
+             _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta), float(mu_L),
+             float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag), _ptr(transmission),
+             _ptr(self.info), int(wave), int(streams), C.byref(h))
+        self._h = h
+
+    def launch(self, check: bool = True):
+        if self._h is None:
+            raise RuntimeError("graph destroyed")
+        call("negf_energies_graph_launch", self._h, _stream())
+        if check:
+            _check_info("rgf_keldysh_energies (graph)", self.info)
+        return self.info
+
+    def close(self):
+        if self._h is not None:
+            call("negf_energies_graph_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def energies_workspace_bytes(N: int, b: int, nE: int, wave: int, streams: int = 1) -> int:
     return int(call("negf_rgf_keldysh_energies_workspace", N, b, nE, wave, streams))
 

@@ -131,6 +131,51 @@ def test_energies_workload_vs_oracle():
         assert abs(T[e] - Tref) < 1e-11 * max(1.0, abs(Tref))
 
 
+@pytest.mark.parametrize("streams", [1, 2])
+def test_energies_graph_replays_match_eager(streams):
+    """The graph-captured job replays bit for bit what the eager call
+    computes, for new energies written in place between replays, with eager
+    work and synchronisations between the replays."""
+    N, b, eta, kT, muL, muR = 5, 48, 1e-3, 0.025, 0.1, -0.1
+    print(f"energies graph N={N} b={b} streams={streams}", flush=True)
+    dev = device_hamiltonian(N, b, seed=11)
+    H = api.DeviceHamiltonianGPU(dev)
+    nE = 6
+    E_graph = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    outs = [torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda") for _ in range(2)]
+    T = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    graph = api.EnergiesGraph(H, E_graph, eta, muL, muR, kT, outs[0], outs[1], T, wave=2,
+                              streams=streams)
+    try:
+        rng = np.random.default_rng(3)
+        for rep in range(3):
+            Es = torch.tensor(np.sort(rng.uniform(-1, 1, nE)), device="cuda")
+            GR = torch.zeros_like(outs[0])
+            GL = torch.zeros_like(outs[0])
+            Te = torch.zeros_like(T)
+            api.rgf_keldysh_energies(H, Es, eta, muL, muR, kT, GR, GL, Te, wave=2,
+                                     streams=streams)
+            torch.cuda.synchronize()
+            E_graph.copy_(Es)
+            for o in outs:
+                o.zero_()
+            T.zero_()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], GR), f"replay {rep}: G^R differs"
+            assert torch.equal(outs[1], GL), f"replay {rep}: G^< differs"
+            assert torch.equal(T, Te), f"replay {rep}: T differs"
+        # and the last replay against the oracle for one energy
+        E = float(Es[2])
+        A = assemble_A(dev, E, eta)
+        G = rgf_ref(A)
+        for i in range(N):
+            blk = outs[0][2, i * b * b:(i + 1) * b * b].cpu().numpy().reshape(b, b).T
+            assert relfro(blk, G[(i, i)]) < 1e-12
+    finally:
+        graph.close()
+
+
 def test_ward_identity_on_gpu_outputs():
     """i (G - G^H) = G (2 eta I + Gamma_L + Gamma_R) G^H on the band, both
     sides from the GPU (RKD fed with the broadening, RGF for G)."""
