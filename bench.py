@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="rgf_keldysh", choices=["rgf_keldysh", "ddrgf_uneven"],
+                    help="rgf_keldysh: the headline workload; ddrgf_uneven: three uneven devices "
+                         "by DD-RGF (8 partitions each) vs RGF")
+    ap.add_argument("--dd-energies", type=int, default=0, help="energies per device (ddrgf_uneven)")
     ap.add_argument("--graph", action="store_true",
                     help="capture the energy job once into a HIP graph and time its replays")
     ap.add_argument("--gemm-reps", type=int, default=20)
@@ -137,6 +141,107 @@ def cpu_baseline(dev, E, eta, kT, muL, muR, gpu_out):
             {"max_block_relerr": float(err), "transmission_relerr": float(terr), "energy": E})
 
 
+def ddrgf_uneven(a, torch, api):
+    """Uneven batched DD-RGF: devices (N, b) = (128, 256), (64, 512),
+    (32, 1024), each split into 8 partitions (blockSizeD1 = 15, 7, 3), 128
+    energies on linspace(-1, 1) each, eta = 1e-4; band(A(E)^{-1}) per energy.
+    A(E)'s diagonal blocks are assembled per wave on the device inside the
+    timed region (zI - H'); the coupling blocks are shared by all energies
+    (batch stride 0).  Outputs of a wave overwrite the previous wave's (the
+    full band of 128 energies of the largest device is 192 GiB).  Reported
+    per device: DD-RGF and plain RGF energies/s, and all three devices at once
+    on three streams."""
+    import numpy as np
+    from libnegf_amd import synthetic, ddrgf as dd
+    from libnegf_amd.flops import flops_rgf as F_rgf
+    from libnegf_amd._lib import call
+    cfg = synthetic.CONFIGS["ddrgf_uneven"]
+    nE, eta, wave = a.dd_energies or cfg["n_energies"], cfg["eta"], a.wave or 8
+    Es = np.linspace(-1.0, 1.0, nE)
+    p = lambda t: C.c_void_p(t.data_ptr())
+
+    class Dev:
+        def __init__(self, N, b):
+            self.N, self.b = N, b
+            h = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
+            H = api.DeviceHamiltonianGPU(h)
+            bb = b * b
+            Hp = H.H_diag.reshape(-1).clone()
+            Hp[:bb] -= 0.5j * H.gamma_L.reshape(-1)
+            Hp[(N - 1) * bb:] -= 0.5j * H.gamma_R.reshape(-1)
+            self.minus_Hp = (-Hp).reshape(1, -1)
+            up = H.H_upper.reshape(N - 1, b, b)               # column-major blocks, stored as (col, row)
+            self.up = (-up).reshape(1, -1).contiguous()
+            self.low = (-up.transpose(1, 2).conj()).reshape(1, -1).contiguous()
+            r = torch.arange(b, device="cuda")
+            self.didx = (torch.arange(N, device="cuda")[:, None] * bb + r[None, :] * (b + 1)).reshape(-1)
+            self.Ad = torch.empty((wave, N * bb), dtype=torch.complex128, device="cuda")
+            self.Gd = torch.empty_like(self.Ad)
+            self.Gu = torch.empty((wave, (N - 1) * bb), dtype=torch.complex128, device="cuda")
+            self.Gl = torch.empty_like(self.Gu)
+            self.info = torch.zeros(wave, dtype=torch.int32, device="cuda")
+            self.levels = dd.levels_for_tasks(N, cfg["parts"])
+            self.z = torch.tensor(Es + 1j * eta, dtype=torch.complex128, device="cuda")
+
+        def assemble(self, e0, n):
+            self.Ad[:n].copy_(self.minus_Hp.expand(n, -1))
+            self.Ad[:n, self.didx] += self.z[e0:e0 + n, None]
+
+        def run_wave(self, e0, method, stream):
+            n = min(wave, nE - e0)
+            self.assemble(e0, n)
+            bb = self.b * self.b
+            sz = (C.c_int32 * self.N)(*([self.b] * self.N))
+            if method == "ddrgf":
+                lv = (C.c_int32 * len(self.levels))(*self.levels)
+                call("negf_ddrgf", self.N, self.b, p(self.Ad), self.N * bb, p(self.up), 0,
+                     p(self.low), 0, p(self.Gd), p(self.Gu), p(self.Gl), n, lv, len(self.levels),
+                     p(self.info), C.c_void_p(stream.cuda_stream))
+            else:
+                call("negf_rgf", self.N, sz, p(self.Ad), self.N * bb, p(self.up), 0, p(self.low), 0,
+                     p(self.Gd), p(self.Gu), p(self.Gl), n, p(self.info),
+                     C.c_void_p(stream.cuda_stream))
+
+    devs = [Dev(N, b) for N, b in cfg["devices"]]
+    streams = [torch.cuda.Stream() for _ in devs]
+    res = []
+    for d, st in zip(devs, streams):
+        row = {"N": d.N, "b": d.b, "blockSizeD1": d.levels, "partitions": cfg["parts"],
+               "energies": nE, "wave": wave}
+        for method in ("ddrgf", "rgf"):
+            with torch.cuda.stream(st):
+                d.run_wave(0, method, st)                      # warmup
+                st.synchronize()
+                t0 = time.perf_counter()
+                for e0 in range(0, nE, wave):
+                    d.run_wave(e0, method, st)
+                st.synchronize()
+                dt = time.perf_counter() - t0
+            if bool((d.info != 0).any()):
+                raise RuntimeError(f"singular Schur complement in {method} N={d.N} b={d.b}")
+            row[f"{method}_energies_per_s"] = round(nE / dt, 3)
+            row[f"{method}_rgf_equiv_tflops"] = round(nE * F_rgf(d.N, d.b) / dt / 1e12, 2)
+            print(f"# {method} N={d.N} b={d.b}: {nE / dt:.2f} energies/s", file=sys.stderr,
+                  flush=True)
+        res.append(row)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0 in range(0, nE, wave):
+        for d, st in zip(devs, streams):
+            with torch.cuda.stream(st):
+                d.run_wave(e0, "ddrgf", st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"workload": "ddrgf_uneven: devices (128,256),(64,512),(32,1024), 8 partitions each, "
+                       f"{nE} energies on linspace(-1,1) each, eta={eta}",
+           "form": "plain FP64", "devices": res,
+           "concurrent_3_devices": {"seconds": round(dt, 3),
+                                    "device_energies_per_s": round(3 * nE / dt, 3),
+                                    "sequential_ddrgf_seconds": round(
+                                        sum(nE / r["ddrgf_energies_per_s"] for r in res), 3)}}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -149,6 +254,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.workload == "ddrgf_uneven":
+        ddrgf_uneven(a, torch, api)
+        return
     cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
     N, b, nE = a.N, a.b, a.energies
     if a.gemm_only:
