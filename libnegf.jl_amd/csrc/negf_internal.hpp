@@ -81,13 +81,12 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 // one per instance) and may be the same buffer as A when A may be destroyed.
 struct InvWork {
   zc* X;        // n*n*batch
-  zc* W;        // n*nb*batch
-  zc* R;        // nb*n*batch
+  zc* W;        // 2*n*nb*batch (W of panel J and of panel J+1)
   int* piv;     // n*batch
   int* used;    // n*batch
 };
 size_t zinv_work_bytes(int n, int batch);
-// Panel variant of the inverse (0: 16-column panels, 1: 32-column), process-wide.
+// Strip variant of the inverse (0: 16-column strips, 1: 32-column), process-wide.
 int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);
 // A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the

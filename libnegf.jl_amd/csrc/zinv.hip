@@ -1,17 +1,20 @@
 // Batched explicit inverse of complex128 blocks by blocked Gauss-Jordan
 // elimination with partial pivoting (the diagonal-block inversion of RGF).
 //
-// Algorithm (per instance, n x n work matrix X, column panels of NB):
-//   panel kernel (one workgroup per instance, the n x NB panel in registers,
-//   RPT rows per thread):  NB exchange steps; step j picks the pivot row p_j
-//   among rows not yet pivoted by max |re| + |im| (LAPACK's izamax rule, so
-//   the pivot sequence is getrf's), scales it, eliminates column j from every
-//   other row (Gauss-Jordan: rows above too).  The composite row operation T
+// Algorithm (per instance, n x n work matrix X, column strips of NB):
+//   factoring a panel (the n x NB strip in registers of one workgroup, RPT
+//   rows per thread): NB exchange steps; step j picks the pivot row p_j among
+//   rows not yet pivoted by max |re| + |im| (LAPACK's izamax rule, so the
+//   pivot sequence is getrf's), scales it, eliminates column j from every
+//   other row (Gauss-Jordan: rows above too).  The composite row operation
 //   of the NB steps acts on every other column c as
 //       X[:,c] <- X[:,c] + (P - E) X[p_J, c],
-//   P = the finished panel, E = unit columns at the pivot rows.  The kernel
-//   writes W = P - E, gathers R = X[p_J, :] (identity in the panel columns)
-//   and sets X[:,J] = E, so one MFMA GEMM  X <- X + W R  finishes the step.
+//   P = the finished panel, E = unit columns at the pivot rows.
+//   One launch per panel J applies that update to all other strips (each
+//   strip in one workgroup: gather its NB x NB block of X[p_J, :], rank-NB
+//   update in registers, store) and lets the strip of panel J+1 factor itself
+//   right after its update, so the latency-bound elimination runs under the
+//   HBM-bound updates of the same launch.  One pass over X per panel.
 //   No physical row swaps: after all panels X[p_j, k] = inv(A)[j, p_k], and
 //   the final unscramble kernel writes inv(A)[r, c] = X[p_r, q_c] with
 //   q = p^{-1}.
@@ -23,55 +26,28 @@ namespace negf {
 
 namespace {
 
-// PT threads per panel workgroup, RPT rows per thread (n <= PT * RPT).
+// Gauss-Jordan elimination of the NB columns held in registers: P[q][c] is
+// row r = tid + PT * q of the strip's column c (k0 + c of the matrix).
 // One barrier per column: every wave reduces its own pivot candidate with
 // shuffles and the candidate's owner publishes the whole unscaled row in a
 // per-wave LDS slot; after the barrier each thread picks the winning slot and
 // scales on the fly.  Slots are double-buffered by column parity, so a fast
 // wave writing column j+1 never overwrites what a slow one still reads for j.
+// Pivot rule: max |re| + |im| over rows not yet pivoted, lowest row on ties
+// (LAPACK's izamax, so the pivot sequence is getrf's).
 template <int PT, int RPT, int NB>
-__global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restrict__ X,
-                                                      zc* __restrict__ W, zc* __restrict__ R,
-                                                      int* __restrict__ piv,
-                                                      int* __restrict__ used,
-                                                      int* __restrict__ info) {
+__device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT][NB],
+                                             bool (&usd)[RPT], int (&pc)[NB],
+                                             double (&red_v)[2][PT / 64], int (&red_i)[2][PT / 64],
+                                             zc (&cand)[2][PT / 64][NB], int* info_inst) {
   constexpr int NW = PT / 64;
-  const int inst = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  zc* Xi = X + (long long)inst * n * n;
-  zc* Wi = W + (long long)inst * n * NB;
-  zc* Ri = R + (long long)inst * NB * n;
-  int* usedi = used + (long long)inst * n;
-  const int nbe = (n - k0) < NB ? (n - k0) : NB;
-
-  __shared__ double red_v[2][NW];
-  __shared__ int red_i[2][NW];
-  __shared__ zc cand[2][NW][NB];
-
-  zc P[RPT][NB];
-  bool usd[RPT];
-  int pc[NB];
-#pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int r = tid + PT * q;
-    usd[q] = true;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) P[q][c] = zmake(0.0, 0.0);
-    if (r < n) {
-      usd[q] = usedi[r] != 0;
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c < nbe) P[q][c] = Xi[r + (long long)(k0 + c) * n];
-    }
-  }
-
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
     pc[jj] = 0;
     if (jj < nbe) {
       const int buf = jj & 1;
-      // pivot search: max |re|+|im| over rows not yet pivoted, lowest index on ties
       double bv = -1.0;
       int bi = INT_MAX;
 #pragma unroll
@@ -88,7 +64,6 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
         const int oi = __shfl_xor(bi, off);
         if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
       }
-      // the wave's candidate row, published by its owner
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         if (tid + PT * q == bi) {
@@ -108,7 +83,7 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
       }
       const int p = bi;
       pc[jj] = p;
-      if (tid == 0 && bv <= 0.0 && info[inst] == 0) info[inst] = k0 + jj + 1;
+      if (tid == 0 && bv <= 0.0 && *info_inst == 0) *info_inst = k0 + jj + 1;
       const zc pv = cand[buf][ws][jj];
       const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
 #pragma unroll
@@ -130,8 +105,145 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
       }
     }
   }
+}
 
-  // W = P - E ; X[:, J] = E ; used flags back
+// One step J of the blocked Gauss-Jordan inverse, over all instances.  The
+// matrix is cut into column strips of NB (strip s = columns s*NB ..).  Panel J
+// (strip J) was factored by the previous launch, which left W_J = P_J - E_J
+// (P_J the eliminated panel, E_J the unit columns at its pivot rows p_J) and
+// the pivots.  Every other strip s gets the composite row operation of panel
+// J:  X[:, s] <- X[:, s] + W_J X[p_J, s]  (its NB x NB block of pivot rows is
+// gathered first, all inside this workgroup, so the in-place update has no
+// race), and the strip of panel J+1, right after its own update, factors
+// itself (look-ahead): its elimination overlaps the other strips' updates in
+// the same launch.  The factoring workgroups come first in the grid (blocks
+// 0 .. batch-1), the updates after them.  J = -1 only factors strip 0.
+template <int PT, int RPT, int NB>
+__global__ __launch_bounds__(PT) void gj_step_kernel(int n, int J, int nstrips, int batch,
+                                                     zc* __restrict__ X,
+                                                     const zc* __restrict__ Wcur,
+                                                     zc* __restrict__ Wnext,
+                                                     int* __restrict__ piv,
+                                                     int* __restrict__ used,
+                                                     int* __restrict__ info) {
+  constexpr int NW = PT / 64;
+  const int tid = threadIdx.x;
+  const bool has_next = J + 1 < nstrips;
+  // Work item k of the launch: k < batch factors instance k's next panel,
+  // the rest update (instance, strip) pairs.  Workgroups are dealt to the 8
+  // XCDs round-robin by block index; when batch is a multiple of 8 the items
+  // are permuted so that all of one instance's strips land on one XCD (its
+  // W_J is read into one L2) and every XCD starts with its factoring items.
+  int k = blockIdx.x;
+  if (J >= 0 && (batch & 7) == 0) {
+    const int x = k & 7, kk = k >> 3, per = batch >> 3;
+    if (kk < per) {
+      k = x + 8 * kk;                                   // factoring item of instance x + 8 kk
+    } else if (has_next) {
+      const int u = kk - per;                           // update items of this XCD's instances
+      k = batch + (u / per) * batch + x + 8 * (u % per);
+    } else {
+      const int u = kk;                                 // no factoring items in the last step
+      k = (u / per) * batch + x + 8 * (u % per);
+    }
+  }
+  int inst, s;
+  bool factor;
+  if (has_next && k < batch) {
+    inst = k;
+    s = J + 1;
+    factor = true;
+  } else {
+    const int r = has_next ? k - batch : k;
+    inst = r % batch;
+    const int slot = r / batch;  // strips other than J (and J+1 when it exists)
+    s = slot < J ? slot : slot + (has_next ? 2 : 1);
+    factor = false;
+  }
+  const int k0 = s * NB;
+  const int ncols = (n - k0) < NB ? (n - k0) : NB;
+  zc* Xi = X + (long long)inst * n * n;
+
+  __shared__ zc Rs[NB][NB];
+  __shared__ double red_v[2][NW];
+  __shared__ int red_i[2][NW];
+  __shared__ zc cand[2][NW][NB];
+
+  zc P[RPT][NB];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + PT * q;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      P[q][c] = (r < n && c < ncols) ? Xi[r + (long long)(k0 + c) * n] : zmake(0.0, 0.0);
+  }
+
+  if (J >= 0) {
+    const int kJ = J * NB;
+    const int nbJ = (n - kJ) < NB ? (n - kJ) : NB;
+    // the rows' W_J first (its latency overlaps the gather), then the NB x NB
+    // block of pivot rows of this strip, then P += W_J R with R one column at
+    // a time from LDS
+    const zc* Wi = Wcur + (long long)inst * n * NB;
+    zc w[RPT][NB];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + PT * q;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        w[q][j] = (r < n && j < nbJ) ? Wi[r + (long long)j * n] : zmake(0.0, 0.0);
+    }
+    const int* pJ = piv + (long long)inst * n + kJ;
+    for (int t = tid; t < NB * NB; t += PT) {
+      const int j = t / NB, c = t % NB;
+      Rs[j][c] = (j < nbJ && c < ncols) ? Xi[pJ[j] + (long long)(k0 + c) * n] : zmake(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      zc rc[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) rc[j] = Rs[j][c];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        double re = P[q][c].x, im = P[q][c].y;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          re = fma(w[q][j].x, rc[j].x, re);
+          re = fma(-w[q][j].y, rc[j].y, re);
+          im = fma(w[q][j].x, rc[j].y, im);
+          im = fma(w[q][j].y, rc[j].x, im);
+        }
+        P[q][c] = zmake(re, im);
+      }
+    }
+  }
+
+  if (!factor) {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + PT * q;
+      if (r < n) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < ncols) Xi[r + (long long)(k0 + c) * n] = P[q][c];
+      }
+    }
+    return;
+  }
+
+  int* usedi = used + (long long)inst * n;
+  bool usd[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + PT * q;
+    usd[q] = r < n ? usedi[r] != 0 : true;
+  }
+  int pc[NB];
+  gj_eliminate<PT, RPT, NB>(n, ncols, k0, P, usd, pc, red_v, red_i, cand, info + inst);
+
+  // X[:, strip] = P (its final value for this step); W_next = P - E; pivots
+  zc* Wn = Wnext + (long long)inst * n * NB;
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int r = tid + PT * q;
@@ -139,12 +251,11 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
       usedi[r] = usd[q] ? 1 : 0;
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
-        if (c < nbe) {
-          const bool e = (r == pc[c]);
-          zc w = P[q][c];
-          if (e) w.x -= 1.0;
-          Wi[r + (long long)c * n] = w;
-          Xi[r + (long long)(k0 + c) * n] = zmake(e ? 1.0 : 0.0, 0.0);
+        if (c < ncols) {
+          zc wv = P[q][c];
+          if (r == pc[c]) wv.x -= 1.0;
+          Wn[r + (long long)c * n] = wv;
+          Xi[r + (long long)(k0 + c) * n] = P[q][c];
         }
       }
     }
@@ -152,30 +263,7 @@ __global__ __launch_bounds__(PT) void gj_panel_kernel(int n, int k0, zc* __restr
   if (tid == 0) {
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-      if (c < nbe) piv[(long long)inst * n + k0 + c] = pc[c];
-  }
-  // R = X[p_J, :] outside the panel, identity inside (ld NB).  Fixed trip
-  // count so the gathers are issued back to back (NB * n <= NB * PT * RPT).
-  zc g[NB * RPT];
-#pragma unroll
-  for (int t = 0; t < NB * RPT; ++t) {
-    const int idx = tid + PT * t;
-    const int c = idx % NB, col = idx / NB;
-    g[t] = zmake(0.0, 0.0);
-    int prow = 0;
-#pragma unroll
-    for (int cc = 0; cc < NB; ++cc)
-      if (cc == c) prow = pc[cc];
-    if (col < n && c < nbe) {
-      if (col >= k0 && col < k0 + nbe) g[t] = zmake(col - k0 == c ? 1.0 : 0.0, 0.0);
-      else g[t] = Xi[prow + (long long)col * n];
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < NB * RPT; ++t) {
-    const int idx = tid + PT * t;
-    const int c = idx % NB, col = idx / NB;
-    if (col < n && c < nbe) Ri[idx] = g[t];
+      if (c < ncols) piv[(long long)inst * n + k0 + c] = pc[c];
   }
 }
 
@@ -225,35 +313,30 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(gj_init_kernel, dim3(blocks), dim3(256), 0, s, n, w.used, info, batch);
   }
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int nbe = (n - k0) < NB ? (n - k0) : NB;
-    hipLaunchKernelGGL((gj_panel_kernel<PT, RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, w.X, w.W,
-                       w.R, w.piv, w.used, info);
-    GemmArgs g{};
-    g.opA = OP_N; g.opB = OP_N;
-    g.m = n; g.n = n; g.k = nbe;
-    g.alpha = zmake(1.0, 0.0); g.beta = zmake(1.0, 0.0);
-    g.A = w.W; g.lda = n; g.strideA = (long long)n * NB;
-    g.B = w.R; g.ldb = NB; g.strideB = (long long)NB * n;
-    g.C = w.X; g.ldc = n; g.strideC = (long long)n * n;
-    g.D = w.X; g.ldd = n; g.strideD = (long long)n * n;
-    g.batch = batch;
-    int rc = zgemm_launch(g, s);
-    if (rc) return rc;
+  const int nstrips = (n + NB - 1) / NB;
+  const long long wst = (long long)n * NB * batch;   // one W buffer
+  hipLaunchKernelGGL((gj_step_kernel<PT, RPT, NB>), dim3(batch), dim3(PT), 0, s, n, -1, nstrips,
+                     batch, w.X, w.W, w.W, w.piv, w.used, info);
+  for (int J = 0; J < nstrips; ++J) {
+    const long long blocks = (long long)batch * (nstrips - 1);
+    if (blocks == 0) break;
+    const zc* Wc = w.W + (J & 1) * wst;
+    zc* Wn = w.W + ((J + 1) & 1) * wst;
+    hipLaunchKernelGGL((gj_step_kernel<PT, RPT, NB>), dim3((unsigned)blocks), dim3(PT), 0, s, n, J,
+                       nstrips, batch, w.X, Wc, Wn, w.piv, w.used, info);
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
                      2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
-// Panel width: the n x NB panel lives in registers (RPT * NB complex per
-// thread <= 32), so wider panels need more threads per row block.  The
-// workspace is sized for the widest panel of any variant.
-int nb_for(int n) { return n <= 1024 ? 32 : 8; }
+// Widest strip of any variant (the double-buffered W is sized for it).
+int nb_for(int n) { return n <= 512 ? 16 : (n <= 1024 ? 8 : 4); }
 
-// Panel variant (autotuner choice, negf_inv_variant): 0 = 16-column panels
-// of 256 threads, 1 = 32-column panels of 512 threads (half the rank-NB
-// update passes over the work matrix, a wider barrier per column).
+// Strip variant (autotuner choice, negf_inv_variant): 0 = 16-column strips
+// up to n = 512, 1 = 8-column strips (twice the passes over X, half the
+// elimination work per column).  One matrix row per thread, so the strip,
+// its W rows and one column of R stay in registers.
 int g_inv_variant = 0;
 
 }  // namespace
@@ -268,8 +351,7 @@ size_t zinv_work_bytes(int n, int batch) {
   const size_t nb = nb_for(n);
   size_t b = 0;
   b += sizeof(zc) * (size_t)n * n * batch;   // X
-  b += sizeof(zc) * (size_t)n * nb * batch;  // W
-  b += sizeof(zc) * nb * (size_t)n * batch;  // R
+  b += 2 * sizeof(zc) * (size_t)n * nb * batch;  // W (two buffers: panel J and J+1)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
   return b + 1024;
 }
@@ -283,8 +365,7 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
     return r;
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
-  w->W = (zc*)take(sizeof(zc) * (size_t)n * nb * batch);
-  w->R = (zc*)take(sizeof(zc) * nb * (size_t)n * batch);
+  w->W = (zc*)take(2 * sizeof(zc) * (size_t)n * nb * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
 }
@@ -294,14 +375,12 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  if (g_inv_variant == 1) {
-    if (n <= 256) return run<256, 1, 32>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-    if (n <= 512) return run<512, 1, 32>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  }
+  if (g_inv_variant == 1 && n <= 512)
+    return run<512, 1, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
   if (n <= 256) return run<256, 1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 512) return run<256, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 1024) return run<512, 2, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  return run<512, 4, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 512) return run<512, 1, 16>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 1024) return run<1024, 1, 8>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  return run<1024, 2, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
 }
 
 }  // namespace negf
