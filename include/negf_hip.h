@@ -300,6 +300,30 @@ int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow
                        const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
                        int32_t batch, void* stream);
 
+/* --------------------------------------------------------- input assembly */
+
+/* M = E S - H - Se for a batch of energies, on packed block-tridiagonal
+ * arrays (replaces build_M_from_HS, src/matloader.jl:118-122, which builds
+ * it on the sparse host matrices before bm_convert).  energies: device
+ * complex128 array [batch] (E, or E + i eta).  H, S: block tridiagonal with
+ * both off-diagonal bands (non-Hermitian storage); Se: the contact
+ * self-energy in the same layout, or NULL arrays.  Each input array has its
+ * own batch stride (0 = shared by all energies).  Outputs M_* are packed per
+ * energy (strides nd, no).  Evaluated element by element as
+ * ((E * S) - H) - Se with a rounded complex product, as Julia evaluates it. */
+int negf_build_M_from_HS(int32_t nblocks, const int32_t* block_sizes, const double* energies,
+                         int32_t batch,
+                         const void* H_diag, int64_t H_diag_bstride,
+                         const void* H_upper, int64_t H_upper_bstride,
+                         const void* H_lower, int64_t H_lower_bstride,
+                         const void* S_diag, int64_t S_diag_bstride,
+                         const void* S_upper, int64_t S_upper_bstride,
+                         const void* S_lower, int64_t S_lower_bstride,
+                         const void* Se_diag, int64_t Se_diag_bstride,
+                         const void* Se_upper, int64_t Se_upper_bstride,
+                         const void* Se_lower, int64_t Se_lower_bstride,
+                         void* M_diag, void* M_upper, void* M_lower, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -552,4 +552,38 @@ int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow
   return block_inverse(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, w, st(stream));
 }
 
+// ------------------------------------------------------------ input assembly
+int negf_build_M_from_HS(int32_t nblocks, const int32_t* block_sizes, const double* energies,
+                         int32_t batch, const void* H_diag, int64_t H_diag_bstride,
+                         const void* H_upper, int64_t H_upper_bstride, const void* H_lower,
+                         int64_t H_lower_bstride, const void* S_diag, int64_t S_diag_bstride,
+                         const void* S_upper, int64_t S_upper_bstride, const void* S_lower,
+                         int64_t S_lower_bstride, const void* Se_diag, int64_t Se_diag_bstride,
+                         const void* Se_upper, int64_t Se_upper_bstride, const void* Se_lower,
+                         int64_t Se_lower_bstride, void* M_diag, void* M_upper, void* M_lower,
+                         void* stream) {
+  if (!valid_sizes(nblocks, block_sizes) || batch <= 0 || !energies) return NEGF_BAD_ARG;
+  if (!H_diag || !S_diag || !M_diag) return NEGF_BAD_ARG;
+  if (nblocks > 1 && (!H_upper || !H_lower || !S_upper || !S_lower || !M_upper || !M_lower))
+    return NEGF_BAD_ARG;
+  if (H_diag_bstride < 0 || H_upper_bstride < 0 || H_lower_bstride < 0 || S_diag_bstride < 0 ||
+      S_upper_bstride < 0 || S_lower_bstride < 0 || Se_diag_bstride < 0 || Se_upper_bstride < 0 ||
+      Se_lower_bstride < 0)
+    return NEGF_BAD_ARG;
+  Layout L(nblocks, block_sizes);
+  hipStream_t s = st(stream);
+  const zc* z = (const zc*)energies;
+  int rc = zassemble_launch(L.nd, z, (const zc*)S_diag, S_diag_bstride, (const zc*)H_diag,
+                            H_diag_bstride, (const zc*)Se_diag, Se_diag_bstride, (zc*)M_diag, L.nd,
+                            batch, s);
+  if (rc || nblocks == 1) return rc;
+  rc = zassemble_launch(L.no, z, (const zc*)S_upper, S_upper_bstride, (const zc*)H_upper,
+                        H_upper_bstride, (const zc*)Se_upper, Se_upper_bstride, (zc*)M_upper, L.no,
+                        batch, s);
+  if (rc) return rc;
+  return zassemble_launch(L.no, z, (const zc*)S_lower, S_lower_bstride, (const zc*)H_lower,
+                          H_lower_bstride, (const zc*)Se_lower, Se_lower_bstride, (zc*)M_lower,
+                          L.no, batch, s);
+}
+
 }  // extern "C"

@@ -131,6 +131,11 @@ int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP,
 int zcopy3d_launch(long long count, int n1, int n2, int n3, zc alpha, const zc* src,
                    long long s1, long long s2, long long s3, zc* dst, long long d1, long long d2,
                    long long d3, hipStream_t s);
+// M[idx][i] = z[idx] S[idx][i] - H[idx][i] - E[idx][i] over `count` elements
+// per instance (batch strides may be 0; E may be null).
+int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, const zc* H,
+                     long long hbs, const zc* E, long long ebs, zc* M, long long mbs, int batch,
+                     hipStream_t s);
 int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                   const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                   long long strideB, int batch, hipStream_t s);

@@ -192,6 +192,49 @@ int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_st
   return ok();
 }
 
+// M = z S - H - E over `count` contiguous elements per instance (the packed
+// diag, upper or lower array of a block-tridiagonal batch), z = zv[instance];
+// S, H, E may be shared (batch stride 0), E may be null.  Evaluated as
+// ((z * S) - H) - E with the complex product rounded term by term (no FMA
+// contraction), the order of `energVal * S - H - Se`.
+__global__ __launch_bounds__(256) void assemble_kernel(long long count, const zc* __restrict__ zv,
+                                                       const zc* __restrict__ S, long long sbs,
+                                                       const zc* __restrict__ H, long long hbs,
+                                                       const zc* __restrict__ E, long long ebs,
+                                                       zc* __restrict__ M, long long mbs) {
+#pragma clang fp contract(off)
+  const long long inst = blockIdx.y;
+  const zc z = zv[inst];
+  const zc* Si = S + inst * sbs;
+  const zc* Hi = H + inst * hbs;
+  const zc* Ei = E ? E + inst * ebs : nullptr;
+  zc* Mi = M + inst * mbs;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (long long)gridDim.x * 256) {
+    const zc sv = Si[i], hv = Hi[i];
+    const double pr = z.x * sv.x, pi = z.y * sv.y, qr = z.x * sv.y, qi = z.y * sv.x;
+    zc m = zmake((pr - pi) - hv.x, (qr + qi) - hv.y);
+    if (Ei) {
+      const zc ev = Ei[i];
+      m.x = m.x - ev.x;
+      m.y = m.y - ev.y;
+    }
+    Mi[i] = m;
+  }
+}
+
+int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, const zc* H,
+                     long long hbs, const zc* E, long long ebs, zc* M, long long mbs, int batch,
+                     hipStream_t s) {
+  if (count < 0 || batch < 0) return Status::BAD_ARG;
+  if (count == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  long long bx = (count + 255) / 256;
+  if (bx > 4096) bx = 4096;
+  hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)bx, batch), dim3(256), 0, s, count, z, S, sbs, H,
+                     hbs, E, ebs, M, mbs);
+  return ok();
+}
+
 int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP, const zc* Q,
                     long long ldq, long long strideQ, double* out, int batch, hipStream_t s) {
   if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;

@@ -80,6 +80,7 @@ SIGNATURES = {
                                            C.POINTER(vp), i64, i64, i32, i32, vp, vp]),
     "negf_block_inverse": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p, C.POINTER(vp), i64,
                                      i64, i32, vp]),
+    "negf_build_M_from_HS": (C.c_int, [i32, c_int32_p, vp, i32] + [vp, i64] * 9 + [vp, vp, vp, vp]),
 }
 
 _lib = None

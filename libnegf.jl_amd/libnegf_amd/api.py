@@ -27,7 +27,7 @@ __all__ = [
     "bndiag_of_inv_rgf_local", "keldyshndiag", "rgf_keldysh_transmission",
     "rgf_keldysh_energies", "GpuLU", "be_gemm", "be_lu", "be_mldivide", "be_getrf", "be_getri",
     "be_trsm", "be_ctranspose", "be_copy_in_hw", "be_fill", "be_identity", "be_zero_array",
-    "be_inv", "col_major", "release_workspace",
+    "be_inv", "col_major", "release_workspace", "build_M_from_HS", "EnergiesGraph",
 ]
 
 
@@ -91,6 +91,58 @@ def bndiag_of_inv_rgf_local(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag,
          _ptr(Mout.diag), _ptr(Mout.upper), _ptr(Mout.lower), Min.batch, _ptr(info), _stream())
     _check_info("bndiag_of_inv_rgf_local", info)
     return Mout
+
+
+def _full_storage(M: DeviceBlockTridiag) -> DeviceBlockTridiag:
+    """Non-Hermitian storage of M (lower blocks materialised as adjoints of the
+    upper ones when M is in Hermitian storage)."""
+    if M.lower is not None:
+        return M
+    out = DeviceBlockTridiag(M.layout.b, M.batch, hermitian=False, device=M.diag.device, zero=False)
+    out.diag.copy_(M.diag)
+    out.upper.copy_(M.upper)
+    L = M.layout
+    for i in range(L.N - 1):
+        seg_u, seg_l = slice(int(L.oo[i]), int(L.oo[i + 1])), slice(int(L.oo[i]), int(L.oo[i + 1]))
+        call("negf_zadjoint_strided_batched", L.b[i], L.b[i + 1], _ptr(M.upper[:, seg_u]), L.b[i],
+             L.no, _ptr(out.lower[:, seg_l]), L.b[i + 1], L.no, M.batch, _stream())
+    return out
+
+
+def build_M_from_HS(H: DeviceBlockTridiag, S: DeviceBlockTridiag, Se: DeviceBlockTridiag | None,
+                    energies) -> DeviceBlockTridiag:
+    """M(E) = E S - H - Se for every energy (``build_M_from_HS``,
+    ``src/matloader.jl:118-122``), on the device: a batch of len(energies)
+    instances.  H, S, Se hold one instance (shared by all energies) or one per
+    energy; energies are real (E) or complex (E + i eta)."""
+    _require_gpu()
+    z = torch.as_tensor(energies, dtype=torch.complex128, device=H.diag.device).reshape(-1)
+    nE = int(z.numel())
+    L = H.layout
+    for X in (S, Se):
+        if X is not None and X.layout.b != L.b:
+            raise ValueError("H, S and Se must share the block structure")
+    ins = []
+    for X in (H, S, Se):
+        if X is None:
+            ins.append(None)
+            continue
+        if X.batch not in (1, nE):
+            raise ValueError("inputs hold one instance or one per energy")
+        ins.append(_full_storage(X))
+    Hf, Sf, Ef = ins
+    bs = lambda X, per: 0 if X.batch == 1 else per
+    M = DeviceBlockTridiag(L.b, nE, hermitian=False, device=H.diag.device, zero=False)
+    no = max(L.no, 1)
+    args = []
+    for X in (Hf, Sf, Ef):
+        if X is None:
+            args += [None, 0, None, 0, None, 0]
+        else:
+            args += [_ptr(X.diag), bs(X, L.nd), _ptr(X.upper), bs(X, no), _ptr(X.lower), bs(X, no)]
+    call("negf_build_M_from_HS", L.N, L.sizes_c(), _ptr(z), nE, *args, _ptr(M.diag),
+         _ptr(M.upper), _ptr(M.lower), _stream())
+    return M
 
 
 @dataclasses.dataclass
