@@ -266,7 +266,9 @@ static int downward(const Layout& L, const SweepWork& w, int batch, zc* gd, long
     GemmArgs two[2];
     two[0] = ga(OP_N, OP_N, bp, bi, bp, MONE, Gpp, t, ZERO, M{nullptr, 1, 0}, Gup, batch);
     two[1] = ga(OP_N, OP_N, bi, bp, bp, MONE, l, Gpp, ZERO, M{nullptr, 1, 0}, Glo, batch);
-    if (bp == bi) {
+    if (!gl) {
+      RC(zgemm_launch(two[0], s));       // G_{i,i-1} not wanted: only G_{i-1,i} feeds G_ii
+    } else if (bp == bi) {
       RC(zgemm_group_launch(two, 2, s));
     } else {
       RC(zgemm_launch(two[0], s));
@@ -314,8 +316,11 @@ static int transmission(const Layout& L, const SweepWork& w, int batch, M gammaL
 // RKD passes on Sd (packed diag, batch stride sd_bs) and Su (packed offdiag,
 // stride su_bs), in place: input Sigma (Hermitian storage), output the band of
 // A^{-1} Sigma A^{-H}.  lowSrc/sLow give A_{i+1,i} for the X recursion.
+// su_zero: the off-diagonal blocks of Sigma are zero on entry (the energy
+// driver's Sigma^n lives on the contact layers only), so the first product
+// of each upward step, t_i Su_i^H, is skipped and Su_i is written, not updated.
 static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs, zc* Su,
-               long long su_bs, const Source& S, hipStream_t s) {
+               long long su_bs, const Source& S, hipStream_t s, bool su_zero = false) {
   const int N = L.N;
   const long long gbs = L.nd, obs = L.no;
   const long long xbs = (long long)L.bmax * L.bmax;
@@ -327,15 +332,14 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
   // upward RKD (keldyshndiag.jl:230-234)
   for (int i = N - 2; i >= 0; --i) {
     const int bi = L.b[i], bj = L.b[i + 1];
-    GemmArgs two[2];
-    // Sd_i -= t_i Su_i^H ; Su_i -= t_i Sd_{i+1}
-    two[0] = ga(OP_N, OP_C, bi, bi, bj, MONE, t(i), ro(su(i)), ONE, ro(sd(i)), sd(i), batch);
-    two[1] = ga(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ONE, ro(su(i)), su(i), batch);
-    if (bi == bj) {
-      RC(zgemm_group_launch(two, 2, s));
+    // Sd_i -= t_i Su_i^H ; Su_i -= t_i Sd_{i+1}  (two launches: the second
+    // overwrites Su_i, which the first reads)
+    if (su_zero) {
+      RC(gemm(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ZERO, M{nullptr, 1, 0}, su(i),
+              batch, s));
     } else {
-      RC(zgemm_launch(two[0], s));
-      RC(zgemm_launch(two[1], s));
+      RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, t(i), ro(su(i)), ONE, ro(sd(i)), sd(i), batch, s));
+      RC(gemm(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ONE, ro(su(i)), su(i), batch, s));
     }
     // Sd_i -= Su_i t_i^H
     RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, ro(su(i)), t(i), ONE, ro(sd(i)), sd(i), batch, s));
@@ -347,18 +351,15 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
     RC(gemm(OP_N, OP_N, b0, b1, b0, ONE, g(0), ro(su(0)), ZERO, M{nullptr, 1, 0}, T1, batch, s));
     RC(gemm(OP_N, OP_C, b0, b1, b1, ONE, ro(T1), g(1), ZERO, M{nullptr, 1, 0}, su(0), batch, s));
   }
-  {
-    const int b0 = L.b[0];
-    RC(zfill_launch(b0, b0, ZERO, ONE, w.X, b0, xbs, batch, s));
-  }
+  // X_i = I + A_{i,i-1} g_{i-1} X_{i-1} t_{i-1} (X_0 = I).  Y_i = g_i X_i is
+  // formed once per layer (it rescales Su_i) and kept in T1 for the next
+  // step, whose recursion needs g_{i-1} X_{i-1} t_{i-1} = Y_{i-1} t_{i-1}.
   for (int i = 1; i + 1 < N; ++i) {
     const int bp = L.b[i - 1], bi = L.b[i], bn = L.b[i + 1];
-    MW T1{w.T1, bp, xbs}, T2{w.T2, bp, xbs};
-    // T1 = X_{i-1} t_{i-1}   (bp x bi)
-    RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, M{w.X, bp, xbs}, t(i - 1), ZERO, M{nullptr, 1, 0}, T1,
-            batch, s));
-    // T2 = g_{i-1} T1        (bp x bi)
-    RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, g(i - 1), ro(T1), ZERO, M{nullptr, 1, 0}, T2, batch, s));
+    MW T2{w.T2, bp, xbs};
+    // T2 = Y_{i-1} t_{i-1}     (bp x bi), Y_0 = g_0
+    const M Y = (i == 1) ? g(0) : M{w.T1, bp, xbs};
+    RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, Y, t(i - 1), ZERO, M{nullptr, 1, 0}, T2, batch, s));
     // X_i = A_{i,i-1} T2 + I (bi x bi)
     MW Xn{w.X, bi, xbs};
     if (S.low) {
@@ -368,7 +369,7 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
       RC(gemm(OP_C, OP_N, bi, bi, bp, S.sLow, M{S.up + L.oo[i - 1], bp, S.up_bs}, ro(T2), ZERO,
               M{nullptr, 1, 0}, Xn, batch, s, w.ones, 0));
     }
-    // T1 = g_i X_i ; T2 = T1 Su_i ; Su_i = T2 g_{i+1}^H
+    // Y_i = T1 = g_i X_i ; T2 = T1 Su_i ; Su_i = T2 g_{i+1}^H
     MW T1b{w.T1, bi, xbs}, T2b{w.T2, bi, xbs};
     RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(Xn), ZERO, M{nullptr, 1, 0}, T1b, batch, s));
     RC(gemm(OP_N, OP_N, bi, bn, bi, ONE, ro(T1b), ro(su(i)), ZERO, M{nullptr, 1, 0}, T2b, batch,
@@ -460,13 +461,13 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
     if (out.T) RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
     if (out.Gless_diag) {
-      // Sigma^n = fL Gamma_L (+) fR Gamma_R, zero elsewhere
+      // Sigma^n = fL Gamma_L (+) fR Gamma_R, zero elsewhere (the off-diagonal
+      // blocks are not stored: rkd's upward step writes them, su_zero)
       RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
-      if (N > 1) RC(zfill_launch((int)L.no, 1, ZERO, ZERO, Suw, L.no, L.no, nb, s));
       RC(zscale_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
       // (a single layer carries both contacts: accumulate)
       RC(zaxpy_launch(b, b, ZERO, fR + e0, 1, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
-      RC(rkd(L, w, nb, Sdw, L.nd, Suw, L.no, S, s));
+      RC(rkd(L, w, nb, Sdw, L.nd, Suw, L.no, S, s, true));
       // G^< = i G^n
       RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,
                        out.Gless_diag + (size_t)e0 * L.nd, L.nd, L.nd, nb, s));

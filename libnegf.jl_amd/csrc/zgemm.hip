@@ -47,6 +47,7 @@ struct GemmGroup {
   long long tiles_per_problem;    // tiles_m * tiles_n * batch
   long long total;                // tiles_per_problem * count
   long long grid_padded;          // multiple of 8
+  int m_slow;                     // tile order: 0 = row tiles fastest, 1 = column tiles fastest
 };
 
 // Tile configuration (FM, FN: MFMA blocks per wave; DB: double-buffered LDS).
@@ -71,7 +72,8 @@ __global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma
   const int tiles_inst = G.tiles_m * G.tiles_n;
   const int inst = (int)(rem / tiles_inst);
   const int tile = (int)(rem - (long long)inst * tiles_inst);
-  const int tm = tile % G.tiles_m, tn = tile / G.tiles_m;
+  const int tm = G.m_slow ? tile / G.tiles_n : tile % G.tiles_m;
+  const int tn = G.m_slow ? tile % G.tiles_n : tile / G.tiles_m;
   const GemmArgs& g = G.p[pidx];
   const int m = g.m, n = g.n, k = g.k;
   const int i0 = tm * TM, j0 = tn * TN;
@@ -230,6 +232,7 @@ __global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma
 // 3: 48.4/51.6/45.3/49.6, 4: 49.7/50.7/45.2/44.9 -- occupancy (3 waves per
 // SIMD for variant 0) matters more than tile reuse at this FP64 MFMA rate.
 int g_variant = 0;
+int g_order_nn = 1;
 constexpr int NVAR = 5;
 const int VAR_FM[NVAR] = {2, 2, 2, 2, 4};
 const int VAR_FN[NVAR] = {2, 2, 4, 4, 2};
@@ -281,6 +284,10 @@ int check_args(const GemmArgs& g) {
 }  // namespace
 
 int zgemm_set_variant(int v) {
+  if (v >= 100) {            // 100 + o: (N, N) tile order o
+    g_order_nn = v - 100;
+    return Status::OK;
+  }
   if (v < 0 || v >= NVAR) return Status::BAD_ARG;
   g_variant = v;
   return Status::OK;
@@ -311,6 +318,7 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   GemmGroup G{};
   for (int i = 0; i < count; ++i) G.p[i] = g[i];
   G.count = count;
+  G.m_slow = g_order_nn && g0.opA == OP_N && g0.opB == OP_N;
   // grid size bound with the smallest tile (64 x 64)
   const long long tiles = (long long)((g0.m + 63) / 64) * ((g0.n + 63) / 64) * g0.batch * count;
   if (tiles + 8 > 0x7fffffffLL) return Status::TOO_LARGE;

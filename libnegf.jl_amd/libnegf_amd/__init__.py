@@ -9,7 +9,7 @@ from .synthetic import (BlockTridiag, DeviceHamiltonian, device_hamiltonian, ass
                         band_edge_energies, near_edge_grid, CONFIGS)
 from ._lib import LIB_PATH, NegfError, SingularError, load as load_library
 from .flops import (flops_rgf, flops_rkd, flops_upward, flops_transmission,
-                    flops_rgf_keldysh_energy)
+                    flops_rgf_keldysh_energy, flops_energy_executed)
 
 __version__ = "1.0.0"
 

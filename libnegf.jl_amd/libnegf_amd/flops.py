@@ -11,8 +11,17 @@ zgetri(n) = 16n^3/3, each following the reference's call sequence:
   F_T      = (8 (N-2) + 16) b^3             first block row + Tr(Gamma_L G Gamma_R G^H)
 
 One "RGF+Keldysh" energy = F_rgf + F_rkd - F_upward + F_T (the shared upward
-pass counted once): 1.511e12 flop at b = 512, N = 64.  Throughput figures
-credit these algorithmic counts, whatever the kernels actually execute.
+pass counted once): 1.511e12 flop at b = 512, N = 64 -- the work of the
+reference's call sequence (credited).
+
+``flops_energy_executed`` is what the GPU energy driver actually runs per
+energy: every S_i^{-1} solve replaced by one explicit inverse (8 b^3) and
+GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), the
+central RKD recursion reusing g_i X_i, and the first upward RKD product
+skipped because Sigma^n has no off-diagonal blocks:
+  GEMMs 11 (N-1) + 5 (N-2) + 2 N + 4, inverses N
+  -> (88 (N-1) + 40 (N-2) + 24 N + 32) b^3 = 1.287e12 at b = 512, N = 64.
+The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
 
@@ -34,3 +43,7 @@ def flops_transmission(N: int, b: int) -> float:
 
 def flops_rgf_keldysh_energy(N: int, b: int) -> float:
     return flops_rgf(N, b) + flops_rkd(N, b) - flops_upward(N, b) + flops_transmission(N, b)
+
+
+def flops_energy_executed(N: int, b: int) -> float:
+    return (88 * (N - 1) + 40 * (N - 2) + 24 * N + 32) * b ** 3
