@@ -47,15 +47,15 @@ struct GemmGroup {
   long long tiles_per_problem;    // tiles_m * tiles_n * batch
   long long total;                // tiles_per_problem * count
   long long grid_padded;          // multiple of 8
-  int m_slow;                     // tile order: 0 = row tiles fastest, 1 = column tiles fastest
 };
 
 // Tile configuration (FM, FN: MFMA blocks per wave; DB: double-buffered LDS).
 // SMALLK = 1 tags the instantiation used for k <= 32 (the rank-NB updates of
 // the Gauss-Jordan / LU panels and k = 0 shifts): the same code, a separate
 // symbol, so profiles separate the memory-bound updates from the full GEMMs.
-template <int OPA, int OPB, int FM, int FN, bool DB, int SMALLK>
-__global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma_kernel(
+template <int OPA, int OPB, int FM, int FN, bool DB, int SMALLK,
+          int OCC = (FM * FN <= 4 && !DB) ? 2 : 1>
+__global__ __launch_bounds__(256, OCC) void zgemm_mfma_kernel(
     const GemmGroup G) {
   constexpr int TM = 32 * FM, TN = 32 * FN;
   constexpr int SA = TM + PAD, SB = TN + PAD;
@@ -72,8 +72,7 @@ __global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma
   const int tiles_inst = G.tiles_m * G.tiles_n;
   const int inst = (int)(rem / tiles_inst);
   const int tile = (int)(rem - (long long)inst * tiles_inst);
-  const int tm = G.m_slow ? tile / G.tiles_n : tile % G.tiles_m;
-  const int tn = G.m_slow ? tile % G.tiles_n : tile / G.tiles_m;
+  const int tm = tile % G.tiles_m, tn = tile / G.tiles_m;
   const GemmArgs& g = G.p[pidx];
   const int m = g.m, n = g.n, k = g.k;
   const int i0 = tm * TM, j0 = tn * TN;
@@ -227,24 +226,23 @@ __global__ __launch_bounds__(256, (FM * FN <= 4 && !DB) ? 2 : 1) void zgemm_mfma
 
 // Tile variants (autotuner choice, negf_gemm_variant).  0: 64x64 single
 // buffer, 1: 64x64 double buffer, 2: 64x128 single, 3: 64x128 double,
-// 4: 128x64 double.  Measured on MI355X at 512^3 x 64 (TFLOP/s, NN/NC/CN/CC):
-// 0: 53.7/59.5/61.2/61.2, 1: 51.7/55.9/53.1/54.2, 2: 48.9/51.2/46.1/49.6,
-// 3: 48.4/51.6/45.3/49.6, 4: 49.7/50.7/45.2/44.9 -- occupancy (3 waves per
-// SIMD for variant 0) matters more than tile reuse at this FP64 MFMA rate.
+// 4: 128x64 double, 5: variant 0 held to 128 VGPRs (4 workgroups per CU, a
+// 512^3 x 64 launch then runs in exactly 4 rounds).  Measured on MI355X at
+// 512^3 x 64, per-instance operands, after warm-up (TFLOP/s, NN/NC/CN/CC):
+// 0: 62.0/62.8/61.2/61.0, 5: 62.8/60.0/63.0/56.3; variants 1-4 were 4-20 %
+// below variant 0 -- occupancy matters more than tile reuse at this FP64
+// MFMA rate.
 int g_variant = 0;
-int g_order_nn = 1;
-constexpr int NVAR = 5;
-const int VAR_FM[NVAR] = {2, 2, 2, 2, 4};
-const int VAR_FN[NVAR] = {2, 2, 4, 4, 2};
+constexpr int NVAR = 6;
 
-template <int OPA, int OPB, int FM, int FN, bool DB, int SK>
+template <int OPA, int OPB, int FM, int FN, bool DB, int SK, int OCC = (FM * FN <= 4 && !DB) ? 2 : 1>
 void launch_cfg(GemmGroup G, hipStream_t s) {
   G.tiles_m = (G.p[0].m + 32 * FM - 1) / (32 * FM);
   G.tiles_n = (G.p[0].n + 32 * FN - 1) / (32 * FN);
   G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * G.p[0].batch;
   G.total = G.tiles_per_problem * G.count;
   G.grid_padded = (G.total + 7) / 8 * 8;
-  hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, FM, FN, DB, SK>), dim3((unsigned)G.grid_padded),
+  hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, FM, FN, DB, SK, OCC>), dim3((unsigned)G.grid_padded),
                      dim3(256), 0, s, G);
 }
 
@@ -259,7 +257,8 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
     case 1: launch_cfg<OPA, OPB, 2, 2, true, 0>(G, s); break;
     case 2: launch_cfg<OPA, OPB, 2, 4, false, 0>(G, s); break;
     case 3: launch_cfg<OPA, OPB, 2, 4, true, 0>(G, s); break;
-    default: launch_cfg<OPA, OPB, 4, 2, true, 0>(G, s); break;
+    case 4: launch_cfg<OPA, OPB, 4, 2, true, 0>(G, s); break;
+    default: launch_cfg<OPA, OPB, 2, 2, false, 0, 4>(G, s); break;
   }
 }
 
@@ -284,10 +283,6 @@ int check_args(const GemmArgs& g) {
 }  // namespace
 
 int zgemm_set_variant(int v) {
-  if (v >= 100) {            // 100 + o: (N, N) tile order o
-    g_order_nn = v - 100;
-    return Status::OK;
-  }
   if (v < 0 || v >= NVAR) return Status::BAD_ARG;
   g_variant = v;
   return Status::OK;
@@ -318,7 +313,6 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   GemmGroup G{};
   for (int i = 0; i < count; ++i) G.p[i] = g[i];
   G.count = count;
-  G.m_slow = g_order_nn && g0.opA == OP_N && g0.opB == OP_N;
   // grid size bound with the smallest tile (64 x 64)
   const long long tiles = (long long)((g0.m + 63) / 64) * ((g0.n + 63) / 64) * g0.batch * count;
   if (tiles + 8 > 0x7fffffffLL) return Status::TOO_LARGE;
