@@ -26,15 +26,48 @@ namespace negf {
 
 namespace {
 
+// Maximum of a double over the 64 lanes of a wave, returned to every lane:
+// DPP row shifts by 1, 2, 4, 8 within rows of 16 lanes, then row broadcasts
+// of lanes 15 and 31 (lanes without a source keep their own value), and the
+// total read from lane 63.  No LDS traffic, a few cycles per step.
+__device__ __forceinline__ double wave_max(double x) {
+  union {
+    double d;
+    int i[2];
+  } u, v;
+#define NEGF_DPP_MAX(CTRL, RMASK)                                                  \
+  u.d = x;                                                                         \
+  v.i[0] = __builtin_amdgcn_update_dpp(u.i[0], u.i[0], CTRL, RMASK, 0xf, false);  \
+  v.i[1] = __builtin_amdgcn_update_dpp(u.i[1], u.i[1], CTRL, RMASK, 0xf, false);  \
+  x = fmax(x, v.d);
+  NEGF_DPP_MAX(0x111, 0xf)   // row_shr:1
+  NEGF_DPP_MAX(0x112, 0xf)   // row_shr:2
+  NEGF_DPP_MAX(0x114, 0xf)   // row_shr:4
+  NEGF_DPP_MAX(0x118, 0xf)   // row_shr:8
+  NEGF_DPP_MAX(0x142, 0xa)   // row_bcast:15 into rows 1, 3
+  NEGF_DPP_MAX(0x143, 0xc)   // row_bcast:31 into rows 2, 3
+#undef NEGF_DPP_MAX
+  u.d = x;
+  u.i[0] = __builtin_amdgcn_readlane(u.i[0], 63);
+  u.i[1] = __builtin_amdgcn_readlane(u.i[1], 63);
+  return u.d;
+}
+
 // Gauss-Jordan elimination of the NB columns held in registers: P[q][c] is
 // row r = tid + PT * q of the strip's column c (k0 + c of the matrix).
-// One barrier per column: every wave reduces its own pivot candidate with
-// shuffles and the candidate's owner publishes the whole unscaled row in a
-// per-wave LDS slot; after the barrier each thread picks the winning slot and
-// scales on the fly.  Slots are double-buffered by column parity, so a fast
-// wave writing column j+1 never overwrites what a slow one still reads for j.
 // Pivot rule: max |re| + |im| over rows not yet pivoted, lowest row on ties
-// (LAPACK's izamax, so the pivot sequence is getrf's).
+// (LAPACK's izamax, so the pivot sequence is getrf's).  Per column: each wave
+// takes its maximum with DPP and finds the lowest row holding it by ballot
+// (a lane's rows increase with q, so the smallest q wins, then the lowest
+// lane); the winning lane publishes its whole unscaled row in the wave's LDS
+// slot; one barrier; every thread picks the winning slot and eliminates.
+// Slots are double-buffered by column parity, so a fast wave writing column
+// j+1 never overwrites what a slow one still reads for j.
+// The pivot row is scaled lazily: it keeps its unscaled values with a 1 in
+// the pivot column and remembers s = 1 / pivot (its true values are s times
+// the stored ones, and every later elimination step is linear in the row);
+// so every row runs the same branch-free update (the pivot row with a zero
+// multiplier), and the at most NB rows pivoted here are scaled once at the end.
 template <int PT, int RPT, int NB>
 __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT][NB],
                                              bool (&usd)[RPT], int (&pc)[NB],
@@ -43,68 +76,95 @@ __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT
   constexpr int NW = PT / 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  zc sc[RPT];
+  bool piv_here[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    sc[q] = zmake(1.0, 0.0);
+    piv_here[q] = false;
+  }
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
     pc[jj] = 0;
     if (jj < nbe) {
       const int buf = jj & 1;
       double bv = -1.0;
-      int bi = INT_MAX;
+      int bq = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = tid + PT * q;
         if (r < n && !usd[q]) {
           const double v = zabs1(P[q][jj]);
-          if (v > bv) { bv = v; bi = r; }
+          if (v > bv) { bv = v; bq = q; }
         }
       }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-      }
+      const double wmax = wave_max(bv);
+      int win_lane = -1, win_q = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        if (tid + PT * q == bi) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) cand[buf][wave][c] = P[q][c];
+        const unsigned long long m = __ballot(bv >= 0.0 && bv == wmax && bq == q);
+        if (win_lane < 0 && m != 0ull) {
+          win_lane = __ffsll(m) - 1;
+          win_q = q;
         }
       }
-      if (lane == 0) { red_v[buf][wave] = bv; red_i[buf][wave] = bi; }
+      if (lane == win_lane) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if (q == win_q) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) cand[buf][wave][c] = P[q][c];
+          }
+      }
+      if (lane == 0) {
+        red_v[buf][wave] = wmax;
+        red_i[buf][wave] = win_lane >= 0 ? win_lane + 64 * wave + PT * win_q : INT_MAX;
+      }
       __syncthreads();
       int ws = 0;
-      bv = red_v[buf][0]; bi = red_i[buf][0];
+      double gv = red_v[buf][0];
+      int bi = red_i[buf][0];
 #pragma unroll
       for (int w = 1; w < NW; ++w) {
         const double ov = red_v[buf][w];
         const int oi = red_i[buf][w];
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; ws = w; }
+        if (ov > gv || (ov == gv && oi < bi)) { gv = ov; bi = oi; ws = w; }
       }
       const int p = bi;
       pc[jj] = p;
-      if (tid == 0 && bv <= 0.0 && *info_inst == 0) *info_inst = k0 + jj + 1;
+      if (tid == 0 && gv <= 0.0 && *info_inst == 0) *info_inst = k0 + jj + 1;
       const zc pv = cand[buf][ws][jj];
       const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
+      zc cr[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) cr[c] = cand[buf][ws][c];
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = tid + PT * q;
-        if (r < n) {
-          if (r == p) {
+        const bool isp = (r == p);
+        const zc f = zmul(P[q][jj], inv);
+        const zc fi = isp ? zmake(0.0, 0.0) : f;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) P[q][c] = (c == jj) ? inv : zmul(P[q][c], inv);
-            usd[q] = true;
-          } else {
-            const zc fi = zmul(P[q][jj], inv);
-#pragma unroll
-            for (int c = 0; c < NB; ++c)
-              if (c != jj) P[q][c] = zsub(P[q][c], zmul(fi, cand[buf][ws][c]));
-            P[q][jj] = zmake(-fi.x, -fi.y);
+        for (int c = 0; c < NB; ++c)
+          if (c != jj) {
+            P[q][c].x = fma(-fi.x, cr[c].x, fma(fi.y, cr[c].y, P[q][c].x));
+            P[q][c].y = fma(-fi.x, cr[c].y, fma(-fi.y, cr[c].x, P[q][c].y));
           }
+        P[q][jj] = isp ? zmake(1.0, 0.0) : zmake(-fi.x, -fi.y);
+        if (isp) {
+          sc[q] = inv;
+          usd[q] = true;
+          piv_here[q] = true;
         }
       }
     }
   }
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    if (piv_here[q]) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) P[q][c] = zmul(P[q][c], sc[q]);
+    }
 }
 
 // One step J of the blocked Gauss-Jordan inverse, over all instances.  The

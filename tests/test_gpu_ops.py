@@ -81,6 +81,20 @@ def test_zinv(n):
     assert relfro(inv, ref) < 1e-15 * cond * max(n, 10)
 
 
+@pytest.mark.parametrize("n", [40, 512, 1000, 1500])
+def test_zinv_needs_pivoting(n):
+    """A row-permuted near-identity: every column has one entry ~1 and the
+    rest ~1e-8, so any pivot other than the column maximum divides by 1e-8
+    and the inverse loses ~8 digits; with the maximum chosen the result is
+    exact to rounding (cond(A) ~ 1)."""
+    print(f"zinv pivoting n={n}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    perm = rng.permutation(n)
+    A = np.asfortranarray((np.eye(n) + 1e-8 * _rand(rng, n, n))[perm])
+    inv = _host(api.be_inv(_dev(A)))
+    assert relfro(inv, np.linalg.inv(A)) < 1e-13
+
+
 def test_zinv_singular_reports_info():
     print("zinv singular", flush=True)
     A = np.zeros((8, 8), complex, order="F")
