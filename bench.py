@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="capture the energy job once into a HIP graph and time its replays")
     ap.add_argument("--gemm-reps", type=int, default=20)
+    ap.add_argument("--inv-only", action="store_true",
+                    help="only batched Gauss-Jordan inverses (--gemm-reps of them, b x b x wave) plus "
+                         "the copy of known size (counter calibration for the inverse's HBM traffic)")
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant-kernel timing plus a plain copy of known size "
                          "(the counter-calibration run for rocprofv3 --pmc)")
@@ -264,6 +267,23 @@ def main():
         nbytes = copy_calibration(torch)
         roof = gemm_roofline(torch, api, b, w, a.gemm_reps)
         print(json.dumps({"gemm_only": True, "copy_bytes": nbytes, "roofline": roof}))
+        return
+    if a.inv_only:
+        # the counter-calibration copy, then batched inverses at the sweep's shape
+        from libnegf_amd._lib import call
+        w = a.wave or 64
+        nbytes = copy_calibration(torch)
+        X = torch.randn((w, b * b), dtype=torch.complex128, device="cuda")
+        Xi = torch.empty_like(X)
+        info = torch.zeros(w, dtype=torch.int32, device="cuda")
+        s = torch.cuda.current_stream()
+        for _ in range(a.gemm_reps):
+            call("negf_zinv_strided_batched", b, C.c_void_p(X.data_ptr()), b, b * b,
+                 C.c_void_p(Xi.data_ptr()), b, b * b, w, C.c_void_p(info.data_ptr()),
+                 C.c_void_p(s.cuda_stream))
+        torch.cuda.synchronize()
+        print(json.dumps({"inv_only": True, "copy_bytes": nbytes, "inverses": a.gemm_reps,
+                          "batch": w, "n": b}))
         return
     eta, kT, muL, muR = cfg["eta"], cfg["kT"], cfg["mu_L"], cfg["mu_R"]
     dev = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
