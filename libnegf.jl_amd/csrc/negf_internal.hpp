@@ -72,7 +72,9 @@ int zgemm_launch(const GemmArgs& g, hipStream_t s);
 int zgemm_set_variant(int v);
 
 // Several independent GEMMs of one shape in one launch (fills the GPU when
-// each problem alone is a handful of tiles).
+// each problem alone is a handful of tiles).  Problems run concurrently; a
+// problem whose D is another's A, B, C or D (same base pointer) is launched
+// after it instead.
 int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 
 // ----------------------------------------------------------------------------

@@ -299,10 +299,22 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
     int rc = check_args(g[i]);
     if (rc) return rc;
   }
-  // one launch per run of problems with the same shape and op codes
+  // one launch per run of problems with the same shape and op codes; a
+  // problem whose output another problem of the run reads (or writes) starts
+  // a new run, since the problems of one launch run concurrently
+  auto conflicts = [&](int i) {
+    for (int j = 0; j < i; ++j) {
+      const void* Di = g[i].D;
+      const void* Dj = g[j].D;
+      if (Di == Dj || Di == g[j].A || Di == g[j].B || Di == g[j].C || Dj == g[i].A ||
+          Dj == g[i].B || Dj == g[i].C)
+        return true;
+    }
+    return false;
+  };
   for (int i = 1; i < count; ++i) {
     if (g[i].m != g[0].m || g[i].n != g[0].n || g[i].k != g[0].k || g[i].opA != g[0].opA ||
-        g[i].opB != g[0].opB || g[i].batch != g[0].batch) {
+        g[i].opB != g[0].opB || g[i].batch != g[0].batch || conflicts(i)) {
       int rc = zgemm_group_launch(g, i, s);
       if (rc) return rc;
       return zgemm_group_launch(g + i, count - i, s);
