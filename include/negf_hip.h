@@ -54,10 +54,13 @@ const char* negf_status_string(int status);
 int negf_workspace_release(void);
 /* Autotuner knob: tile variant of the MFMA ZGEMM for k > 32, process-wide
  * (0: 64x64 tile, one LDS buffer (default); 1: 64x64 double-buffered;
- * 2: 64x128 one buffer; 3: 64x128 double-buffered; 4: 128x64 double-buffered). */
+ * 2: 64x128 one buffer; 3: 64x128 double-buffered; 4: 128x64 double-buffered;
+ * 5: 64x64 at four workgroups per CU). */
 int negf_gemm_variant(int32_t variant);
-/* Autotuner knob: panel of the Gauss-Jordan inverse, process-wide
- * (0: 16 columns, 256 threads; 1: 32 columns, 512 threads, for b <= 512). */
+/* Autotuner knob: panel of the batched Gauss-Jordan inverse, process-wide
+ * (0: by batch size (default); 1: one column strip per panel; 2: two strips
+ * per panel -- half the passes over the matrix, twice the latency-bound
+ * factoring per pass). */
 int negf_inv_variant(int32_t variant);
 
 /* ------------------------------------------------------------------ per-op */

@@ -88,7 +88,8 @@ struct InvWork {
   int* used;    // n*batch
 };
 size_t zinv_work_bytes(int n, int batch);
-// Strip variant of the inverse (0: 16-column strips, 1: 32-column), process-wide.
+// Panel variant of the inverse (0: by batch size, 1: one strip per panel,
+// 2: two strips per panel), process-wide.
 int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);
 // A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the
