@@ -351,27 +351,19 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
     RC(gemm(OP_N, OP_N, b0, b1, b0, ONE, g(0), ro(su(0)), ZERO, M{nullptr, 1, 0}, T1, batch, s));
     RC(gemm(OP_N, OP_C, b0, b1, b1, ONE, ro(T1), g(1), ZERO, M{nullptr, 1, 0}, su(0), batch, s));
   }
-  // X_i = I + A_{i,i-1} g_{i-1} X_{i-1} t_{i-1} (X_0 = I).  Y_i = g_i X_i is
-  // formed once per layer (it rescales Su_i) and kept in T1 for the next
-  // step, whose recursion needs g_{i-1} X_{i-1} t_{i-1} = Y_{i-1} t_{i-1}.
+  // X_i = I + A_{i,i-1} g_{i-1} X_{i-1} t_{i-1} (X_0 = I) enters only as
+  // Y_i = g_i X_i = g_i + l_i (Y_{i-1} t_{i-1})  (l_i = g_i A_{i,i-1} from the
+  // upward pass), which rescales Su_i and is kept in T1 for the next step.
   for (int i = 1; i + 1 < N; ++i) {
     const int bp = L.b[i - 1], bi = L.b[i], bn = L.b[i + 1];
     MW T2{w.T2, bp, xbs};
     // T2 = Y_{i-1} t_{i-1}     (bp x bi), Y_0 = g_0
     const M Y = (i == 1) ? g(0) : M{w.T1, bp, xbs};
     RC(gemm(OP_N, OP_N, bp, bi, bp, ONE, Y, t(i - 1), ZERO, M{nullptr, 1, 0}, T2, batch, s));
-    // X_i = A_{i,i-1} T2 + I (bi x bi)
-    MW Xn{w.X, bi, xbs};
-    if (S.low) {
-      RC(gemm(OP_N, OP_N, bi, bi, bp, ONE, M{S.low + L.oo[i - 1], bi, S.low_bs}, ro(T2), ZERO,
-              M{nullptr, 1, 0}, Xn, batch, s, w.ones, 0));
-    } else {
-      RC(gemm(OP_C, OP_N, bi, bi, bp, S.sLow, M{S.up + L.oo[i - 1], bp, S.up_bs}, ro(T2), ZERO,
-              M{nullptr, 1, 0}, Xn, batch, s, w.ones, 0));
-    }
-    // Y_i = T1 = g_i X_i ; T2 = T1 Su_i ; Su_i = T2 g_{i+1}^H
+    // Y_i = T1 = g_i + l_i T2  (bi x bi)
     MW T1b{w.T1, bi, xbs}, T2b{w.T2, bi, xbs};
-    RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(Xn), ZERO, M{nullptr, 1, 0}, T1b, batch, s));
+    RC(gemm(OP_N, OP_N, bi, bi, bp, ONE, l(i), ro(T2), ONE, g(i), T1b, batch, s));
+    // T2 = T1 Su_i ; Su_i = T2 g_{i+1}^H
     RC(gemm(OP_N, OP_N, bi, bn, bi, ONE, ro(T1b), ro(su(i)), ZERO, M{nullptr, 1, 0}, T2b, batch,
             s));
     RC(gemm(OP_N, OP_C, bi, bn, bn, ONE, ro(T2b), g(i + 1), ZERO, M{nullptr, 1, 0}, su(i), batch,
