@@ -316,11 +316,8 @@ static int transmission(const Layout& L, const SweepWork& w, int batch, M gammaL
 // RKD passes on Sd (packed diag, batch stride sd_bs) and Su (packed offdiag,
 // stride su_bs), in place: input Sigma (Hermitian storage), output the band of
 // A^{-1} Sigma A^{-H}.  lowSrc/sLow give A_{i+1,i} for the X recursion.
-// su_zero: the off-diagonal blocks of Sigma are zero on entry (the energy
-// driver's Sigma^n lives on the contact layers only), so the first product
-// of each upward step, t_i Su_i^H, is skipped and Su_i is written, not updated.
 static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs, zc* Su,
-               long long su_bs, const Source& S, hipStream_t s, bool su_zero = false) {
+               long long su_bs, hipStream_t s) {
   const int N = L.N;
   const long long gbs = L.nd, obs = L.no;
   const long long xbs = (long long)L.bmax * L.bmax;
@@ -334,13 +331,8 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
     const int bi = L.b[i], bj = L.b[i + 1];
     // Sd_i -= t_i Su_i^H ; Su_i -= t_i Sd_{i+1}  (two launches: the second
     // overwrites Su_i, which the first reads)
-    if (su_zero) {
-      RC(gemm(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ZERO, M{nullptr, 1, 0}, su(i),
-              batch, s));
-    } else {
-      RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, t(i), ro(su(i)), ONE, ro(sd(i)), sd(i), batch, s));
-      RC(gemm(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ONE, ro(su(i)), su(i), batch, s));
-    }
+    RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, t(i), ro(su(i)), ONE, ro(sd(i)), sd(i), batch, s));
+    RC(gemm(OP_N, OP_N, bi, bj, bj, MONE, t(i), ro(sd(i + 1)), ONE, ro(su(i)), su(i), batch, s));
     // Sd_i -= Su_i t_i^H
     RC(gemm(OP_N, OP_C, bi, bi, bj, MONE, ro(su(i)), t(i), ONE, ro(sd(i)), sd(i), batch, s));
   }
@@ -397,6 +389,60 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
   return Status::OK;
 }
 
+// Keldysh pass of the energy driver: Sigma^n lives on the contact layers only
+// (its off-diagonal blocks are zero on entry) and the RGF pass has left the
+// G_{i,i+1} blocks (Gu).  The reference's passes (src/keldyshndiag.jl:221-318)
+// with three exact identities:
+//  * upward: with Su_i = 0 on entry, Su_i = -t_i Sd_{i+1} only feeds
+//    Sd_i -= Su_i t_i^H, i.e. Sd_i += t_i Sd_{i+1} t_i^H;
+//  * central: its recursion g_i X_i = g_i + l_i (g_{i-1} X_{i-1}) t_{i-1},
+//    g_0 X_0 = g_0, is the RGF downward recursion of G_ii, so
+//    Su_i <- g_i X_i Su_i g_{i+1}^H = G_ii (-t_i Sd_{i+1}) g_{i+1}^H
+//          = G_{i,i+1} E_{i+1},  E_j = Sd_j g_j^H  (G_{i,i+1} = -G_ii t_i);
+//  * diagonal re-scaling: Sd_j <- g_j Sd_j g_j^H = g_j E_j, sharing E_j;
+//  * downward: unchanged.
+// 14 GEMMs per layer for RGF + Keldysh + T instead of 21.
+static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs,
+                        zc* Su, long long su_bs, const zc* Gu, long long gu_bs, hipStream_t s) {
+  const int N = L.N;
+  const long long gbs = L.nd, obs = L.no;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
+  auto su = [&](int i) { return MW{Su + L.oo[i], L.b[i], su_bs}; };
+  auto gu = [&](int i) { return M{Gu + L.oo[i], L.b[i], gu_bs}; };
+  auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
+  auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], obs}; };
+  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], obs}; };  // l_i = S_i^{-1} A_{i,i-1}
+  const M none{nullptr, 1, 0};
+  // upward (keldyshndiag.jl:230-234 with Su = 0 on entry)
+  for (int i = N - 2; i >= 0; --i) {
+    const int bi = L.b[i], bj = L.b[i + 1];
+    MW T1{w.T1, bi, xbs};
+    RC(gemm(OP_N, OP_N, bi, bj, bj, ONE, t(i), ro(sd(i + 1)), ZERO, none, T1, batch, s));
+    RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ONE, ro(sd(i)), sd(i), batch, s));
+  }
+  // central + diagonal re-scaling (keldyshndiag.jl:256-301)
+  for (int i = 0; i < N; ++i) {
+    const int bi = L.b[i];
+    MW E{w.T1, bi, xbs};
+    RC(gemm(OP_N, OP_C, bi, bi, bi, ONE, ro(sd(i)), g(i), ZERO, none, E, batch, s));
+    if (i > 0)
+      RC(gemm(OP_N, OP_N, L.b[i - 1], bi, bi, ONE, gu(i - 1), ro(E), ZERO, none, su(i - 1), batch,
+              s));
+    RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(E), ZERO, none, sd(i), batch, s));
+  }
+  // downward (keldyshndiag.jl:313-317), l = l_{i+1}
+  for (int i = 0; i + 1 < N; ++i) {
+    const int bi = L.b[i], bj = L.b[i + 1];
+    RC(gemm(OP_C, OP_C, bj, bj, bi, MONE, ro(su(i)), l(i + 1), ONE, ro(sd(i + 1)), sd(i + 1), batch,
+            s));
+    RC(gemm(OP_N, OP_C, bi, bj, bi, MONE, ro(sd(i)), l(i + 1), ONE, ro(su(i)), su(i), batch, s));
+    RC(gemm(OP_N, OP_N, bj, bj, bi, MONE, l(i + 1), ro(su(i)), ONE, ro(sd(i + 1)), sd(i + 1), batch,
+            s));
+  }
+  return Status::OK;
+}
+
 // ---------------------------------------------------------------------------
 // Public sweep drivers (called by the C-ABI layer with validated arguments).
 
@@ -422,7 +468,7 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
                       s));
     if (L.N > 1)
       RC(zcopy2d_launch((int)L.no, 1, sigma->up, L.no, sigma->up_bs, gn->up, L.no, L.no, batch, s));
-    RC(rkd(L, w, batch, gn->diag, L.nd, gn->up, L.no, S, s));
+    RC(rkd(L, w, batch, gn->diag, L.nd, gn->up, L.no, s));
   }
   return Status::OK;
 }
@@ -436,6 +482,7 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
   carve(work, L, wave, &w);
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
   zc* Suw = Sdw + (size_t)wave * L.nd;
+  zc* Guw = Suw + (size_t)wave * L.no;
   RC(set_one(w.ones, s));
   for (int e0 = e_begin; e0 < e_end; e0 += wave) {
     const int nb = (e_end - e0) < wave ? (e_end - e0) : wave;
@@ -448,18 +495,18 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     S.shift = z + e0; S.shift_stride = 1;
     int* info = out.info ? out.info + e0 : einfo + e0;
     RC(upward(L, S, w, nb, info, s));
-    // G^R diagonal (off-diagonal blocks through scratch)
+    // G^R diagonal blocks, and G_{i,i+1} kept for the Keldysh pass
     zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Sdw;
-    RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
+    RC(downward(L, w, nb, grd, L.nd, Guw, nullptr, L.no, s));
     if (out.T) RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
     if (out.Gless_diag) {
       // Sigma^n = fL Gamma_L (+) fR Gamma_R, zero elsewhere (the off-diagonal
-      // blocks are not stored: rkd's upward step writes them, su_zero)
+      // blocks are never read before rkd_contacts writes them)
       RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
       RC(zscale_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
       // (a single layer carries both contacts: accumulate)
       RC(zaxpy_launch(b, b, ZERO, fR + e0, 1, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
-      RC(rkd(L, w, nb, Sdw, L.nd, Suw, L.no, S, s, true));
+      RC(rkd_contacts(L, w, nb, Sdw, L.nd, Suw, L.no, Guw, L.no, s));
       // G^< = i G^n
       RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,
                        out.Gless_diag + (size_t)e0 * L.nd, L.nd, L.nd, nb, s));
@@ -470,7 +517,7 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
 
 size_t energies_group_bytes(const Layout& L, int wave) {
   return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
-         sizeof(zc) * (size_t)wave * (L.nd + L.no) + 4096;
+         sizeof(zc) * (size_t)wave * (L.nd + 2 * L.no) + 4096;
 }
 
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {

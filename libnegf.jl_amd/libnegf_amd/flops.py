@@ -16,12 +16,13 @@ reference's call sequence (credited).
 
 ``flops_energy_executed`` is what the GPU energy driver actually runs per
 energy: every S_i^{-1} solve replaced by one explicit inverse (8 b^3) and
-GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), the
-central RKD recursion carried as Y_i = g_i X_i = g_i + l_i Y_{i-1} t_{i-1}
-(two GEMMs per layer instead of four), and the first upward RKD product
-skipped because Sigma^n has no off-diagonal blocks:
-  GEMMs 11 (N-1) + 4 (N-2) + 2 N + 4, inverses N
-  -> (88 (N-1) + 32 (N-2) + 24 N + 32) b^3 = 1.221e12 at b = 512, N = 64.
+GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), and the
+Keldysh pass specialised to a Sigma^n on the contact layers with exact
+identities (the central recursion reproduces G_ii, so Sigma's off-diagonal
+rescaling becomes G_{i,i+1} E_{i+1} and the diagonal one g_i E_i with the
+shared E_i = Sigma_ii g_i^H):
+  GEMMs 11 (N-1) + 3 N + 1, inverses N
+  -> (88 (N-1) + 32 N + 8) b^3 = 1.020e12 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
@@ -47,4 +48,4 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
 
 
 def flops_energy_executed(N: int, b: int) -> float:
-    return (88 * (N - 1) + 32 * (N - 2) + 24 * N + 32) * b ** 3
+    return (88 * (N - 1) + 32 * N + 8) * b ** 3
