@@ -391,25 +391,26 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
 
 // Keldysh pass of the energy driver: Sigma^n lives on the contact layers only
 // (its off-diagonal blocks are zero on entry) and the RGF pass has left the
-// G_{i,i+1} blocks (Gu).  The reference's passes (src/keldyshndiag.jl:221-318)
-// with three exact identities:
+// diagonal blocks G_jj.  The reference's passes (src/keldyshndiag.jl:221-318)
+// rewritten with exact identities of that case:
 //  * upward: with Su_i = 0 on entry, Su_i = -t_i Sd_{i+1} only feeds
 //    Sd_i -= Su_i t_i^H, i.e. Sd_i += t_i Sd_{i+1} t_i^H;
 //  * central: its recursion g_i X_i = g_i + l_i (g_{i-1} X_{i-1}) t_{i-1},
-//    g_0 X_0 = g_0, is the RGF downward recursion of G_ii, so
-//    Su_i <- g_i X_i Su_i g_{i+1}^H = G_ii (-t_i Sd_{i+1}) g_{i+1}^H
-//          = G_{i,i+1} E_{i+1},  E_j = Sd_j g_j^H  (G_{i,i+1} = -G_ii t_i);
-//  * diagonal re-scaling: Sd_j <- g_j Sd_j g_j^H = g_j E_j, sharing E_j;
-//  * downward: unchanged.
-// 14 GEMMs per layer for RGF + Keldysh + T instead of 21.
+//    g_0 X_0 = g_0, is the RGF downward recursion of G_ii, so the rescaled
+//    Su_i = G_ii (-t_i Sd_{i+1}) g_{i+1}^H = G_{i,i+1} E_{i+1} with
+//    E_j = Sd_j g_j^H, and the diagonal rescaling is D_j = g_j E_j;
+//  * downward: with l_{i+1} G_{i,i+1} = g_{i+1} - G_{i+1,i+1} (RGF), its three
+//    products collapse to
+//      G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1,j-1} l_j^H,  P_j = G_jj E_j,
+//    and G^n_00 = D_0.
+// 13 GEMMs per layer for RGF + Keldysh + T instead of 21.
 static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs,
-                        zc* Su, long long su_bs, const zc* Gu, long long gu_bs, hipStream_t s) {
+                        const zc* Gd, long long gd_bs, hipStream_t s) {
   const int N = L.N;
   const long long gbs = L.nd, obs = L.no;
   const long long xbs = (long long)L.bmax * L.bmax;
   auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
-  auto su = [&](int i) { return MW{Su + L.oo[i], L.b[i], su_bs}; };
-  auto gu = [&](int i) { return M{Gu + L.oo[i], L.b[i], gu_bs}; };
+  auto G = [&](int i) { return M{Gd + L.od[i], L.b[i], gd_bs}; };
   auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
   auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], obs}; };
   auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], obs}; };  // l_i = S_i^{-1} A_{i,i-1}
@@ -421,24 +422,24 @@ static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, 
     RC(gemm(OP_N, OP_N, bi, bj, bj, ONE, t(i), ro(sd(i + 1)), ZERO, none, T1, batch, s));
     RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ONE, ro(sd(i)), sd(i), batch, s));
   }
-  // central + diagonal re-scaling (keldyshndiag.jl:256-301)
-  for (int i = 0; i < N; ++i) {
-    const int bi = L.b[i];
-    MW E{w.T1, bi, xbs};
-    RC(gemm(OP_N, OP_C, bi, bi, bi, ONE, ro(sd(i)), g(i), ZERO, none, E, batch, s));
-    if (i > 0)
-      RC(gemm(OP_N, OP_N, L.b[i - 1], bi, bi, ONE, gu(i - 1), ro(E), ZERO, none, su(i - 1), batch,
-              s));
-    RC(gemm(OP_N, OP_N, bi, bi, bi, ONE, g(i), ro(E), ZERO, none, sd(i), batch, s));
-  }
-  // downward (keldyshndiag.jl:313-317), l = l_{i+1}
-  for (int i = 0; i + 1 < N; ++i) {
-    const int bi = L.b[i], bj = L.b[i + 1];
-    RC(gemm(OP_C, OP_C, bj, bj, bi, MONE, ro(su(i)), l(i + 1), ONE, ro(sd(i + 1)), sd(i + 1), batch,
-            s));
-    RC(gemm(OP_N, OP_C, bi, bj, bi, MONE, ro(sd(i)), l(i + 1), ONE, ro(su(i)), su(i), batch, s));
-    RC(gemm(OP_N, OP_N, bj, bj, bi, MONE, l(i + 1), ro(su(i)), ONE, ro(sd(i + 1)), sd(i + 1), batch,
-            s));
+  // central, diagonal re-scaling and downward (keldyshndiag.jl:256-317), layer by layer
+  for (int j = 0; j < N; ++j) {
+    const int bj = L.b[j];
+    MW E{w.T1, bj, xbs};
+    RC(gemm(OP_N, OP_C, bj, bj, bj, ONE, ro(sd(j)), g(j), ZERO, none, E, batch, s));
+    if (j == 0) {
+      RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, g(0), ro(E), ZERO, none, sd(0), batch, s));
+      continue;
+    }
+    const int bp = L.b[j - 1];
+    MW D{w.T2, bj, xbs}, P{w.X, bj, xbs}, Q{w.T1, bj, xbs};
+    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, g(j), ro(E), ZERO, none, D, batch, s));
+    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), ro(E), ZERO, none, P, batch, s));
+    // Q = l_j G^n_{j-1}; D <- Q l_j^H - D (in place)
+    RC(gemm(OP_N, OP_N, bj, bp, bp, ONE, l(j), ro(sd(j - 1)), ZERO, none, Q, batch, s));
+    RC(gemm(OP_N, OP_C, bj, bj, bp, ONE, ro(Q), l(j), MONE, ro(D), D, batch, s));
+    // G^n_jj = P + P^H + D
+    RC(zherm_add_launch(bj, w.X, bj, xbs, w.T2, bj, xbs, Sd + L.od[j], bj, sd_bs, batch, s));
   }
   return Status::OK;
 }
@@ -481,8 +482,7 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
   SweepWork w;
   carve(work, L, wave, &w);
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
-  zc* Suw = Sdw + (size_t)wave * L.nd;
-  zc* Guw = Suw + (size_t)wave * L.no;
+  zc* Gdw = Sdw + (size_t)wave * L.nd;   // G^R diagonal when the caller does not keep it
   RC(set_one(w.ones, s));
   for (int e0 = e_begin; e0 < e_end; e0 += wave) {
     const int nb = (e_end - e0) < wave ? (e_end - e0) : wave;
@@ -495,18 +495,18 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     S.shift = z + e0; S.shift_stride = 1;
     int* info = out.info ? out.info + e0 : einfo + e0;
     RC(upward(L, S, w, nb, info, s));
-    // G^R diagonal blocks, and G_{i,i+1} kept for the Keldysh pass
-    zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Sdw;
-    RC(downward(L, w, nb, grd, L.nd, Guw, nullptr, L.no, s));
+    // G^R diagonal blocks (the Keldysh pass reads them)
+    zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Gdw;
+    RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
     if (out.T) RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
     if (out.Gless_diag) {
-      // Sigma^n = fL Gamma_L (+) fR Gamma_R, zero elsewhere (the off-diagonal
-      // blocks are never read before rkd_contacts writes them)
+      // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere (its
+      // off-diagonal blocks are zero and never stored)
       RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
       RC(zscale_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
       // (a single layer carries both contacts: accumulate)
       RC(zaxpy_launch(b, b, ZERO, fR + e0, 1, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
-      RC(rkd_contacts(L, w, nb, Sdw, L.nd, Suw, L.no, Guw, L.no, s));
+      RC(rkd_contacts(L, w, nb, Sdw, L.nd, grd, L.nd, s));
       // G^< = i G^n
       RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,
                        out.Gless_diag + (size_t)e0 * L.nd, L.nd, L.nd, nb, s));
@@ -517,7 +517,7 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
 
 size_t energies_group_bytes(const Layout& L, int wave) {
   return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
-         sizeof(zc) * (size_t)wave * (L.nd + 2 * L.no) + 4096;
+         sizeof(zc) * (size_t)wave * (2 * L.nd) + 4096;
 }
 
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {

@@ -18,11 +18,12 @@ reference's call sequence (credited).
 energy: every S_i^{-1} solve replaced by one explicit inverse (8 b^3) and
 GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), and the
 Keldysh pass specialised to a Sigma^n on the contact layers with exact
-identities (the central recursion reproduces G_ii, so Sigma's off-diagonal
-rescaling becomes G_{i,i+1} E_{i+1} and the diagonal one g_i E_i with the
-shared E_i = Sigma_ii g_i^H):
-  GEMMs 11 (N-1) + 3 N + 1, inverses N
-  -> (88 (N-1) + 32 N + 8) b^3 = 1.020e12 at b = 512, N = 64.
+identities (the central recursion reproduces G_ii; with the RGF relation
+l_{i+1} G_{i,i+1} = g_{i+1} - G_{i+1,i+1} the diagonal rescaling and the
+downward pass become G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1} l_j^H with
+E_j = Sigma_jj g_j^H, D_j = g_j E_j, P_j = G_jj E_j):
+  GEMMs 13 (N-1) + 4, inverses N
+  -> (104 (N-1) + 8 N + 32) b^3 = 9.52e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
@@ -48,4 +49,4 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
 
 
 def flops_energy_executed(N: int, b: int) -> float:
-    return (88 * (N - 1) + 32 * N + 8) * b ** 3
+    return (104 * (N - 1) + 8 * N + 32) * b ** 3
