@@ -403,17 +403,19 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
 //    products collapse to
 //      G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1,j-1} l_j^H,  P_j = G_jj E_j,
 //    and G^n_00 = D_0.
-// 13 GEMMs per layer for RGF + Keldysh + T instead of 21.
-static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs,
-                        const zc* Gd, long long gd_bs, hipStream_t s) {
+// The upward step is linear in Sigma, so the driver runs it on Gamma_R alone
+// (unscaled): its result B_0 at layer 0 gives the transmission
+// T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R G_{0,N-1}^H),
+// G_{0,N-1} = G_00 prod(-t)), after which the blocks are scaled by f_R and
+// f_L Gamma_L is added on layer 0.
+// 12 GEMMs per layer for RGF + Keldysh + T instead of 21.
+static int rkd_contacts_up(const Layout& L, const SweepWork& w, int batch, zc* Sd,
+                           long long sd_bs, hipStream_t s) {
   const int N = L.N;
-  const long long gbs = L.nd, obs = L.no;
+  const long long obs = L.no;
   const long long xbs = (long long)L.bmax * L.bmax;
   auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
-  auto G = [&](int i) { return M{Gd + L.od[i], L.b[i], gd_bs}; };
-  auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
   auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], obs}; };
-  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], obs}; };  // l_i = S_i^{-1} A_{i,i-1}
   const M none{nullptr, 1, 0};
   // upward (keldyshndiag.jl:230-234 with Su = 0 on entry)
   for (int i = N - 2; i >= 0; --i) {
@@ -422,6 +424,31 @@ static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, 
     RC(gemm(OP_N, OP_N, bi, bj, bj, ONE, t(i), ro(sd(i + 1)), ZERO, none, T1, batch, s));
     RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ONE, ro(sd(i)), sd(i), batch, s));
   }
+  return Status::OK;
+}
+
+// T = Re Tr(Gamma_L G_00 B_0 G_00^H) = Re sum (Gamma_L G_00 B_0) .* conj(G_00).
+static int transmission_from_b0(const Layout& L, const SweepWork& w, int batch, M G00, M B0,
+                                M gammaL, double* T, hipStream_t s) {
+  const long long xbs = (long long)L.bmax * L.bmax;
+  const int b0 = L.b[0];
+  const M none{nullptr, 1, 0};
+  MW V{w.T1, b0, xbs}, P{w.T2, b0, xbs};
+  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, G00, B0, ZERO, none, V, batch, s));
+  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, gammaL, ro(V), ZERO, none, P, batch, s));
+  return zdotc_re_launch(b0, b0, P.p, b0, xbs, G00.p, G00.ld, G00.bs, T, batch, s);
+}
+
+static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs,
+                        const zc* Gd, long long gd_bs, hipStream_t s) {
+  const int N = L.N;
+  const long long gbs = L.nd;
+  const long long xbs = (long long)L.bmax * L.bmax;
+  auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
+  auto G = [&](int i) { return M{Gd + L.od[i], L.b[i], gd_bs}; };
+  auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
+  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], L.no}; };  // l_i = S_i^{-1} A_{i,i-1}
+  const M none{nullptr, 1, 0};
   // central, diagonal re-scaling and downward (keldyshndiag.jl:256-317), layer by layer
   for (int j = 0; j < N; ++j) {
     const int bj = L.b[j];
@@ -498,14 +525,20 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     // G^R diagonal blocks (the Keldysh pass reads them)
     zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Gdw;
     RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
-    if (out.T) RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
+    if (out.T && !out.Gless_diag)
+      RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
     if (out.Gless_diag) {
       // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere (its
-      // off-diagonal blocks are zero and never stored)
+      // off-diagonal blocks are zero and never stored).  The upward step runs
+      // on Gamma_R alone, which also yields T; then the f_R scale and f_L Gamma_L.
       RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
-      RC(zscale_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
-      // (a single layer carries both contacts: accumulate)
-      RC(zaxpy_launch(b, b, ZERO, fR + e0, 1, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
+      RC(zaxpy_launch(b, b, ONE, nullptr, 0, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
+      RC(rkd_contacts_up(L, w, nb, Sdw, L.nd, s));
+      if (out.T)
+        RC(transmission_from_b0(L, w, nb, M{grd, b, L.nd}, M{Sdw, b, L.nd}, M{in.gammaL, b, 0},
+                                out.T + e0, s));
+      RC(zscale_launch((int)L.nd, 1, ZERO, fR + e0, 1, Sdw, L.nd, L.nd, Sdw, L.nd, L.nd, nb, s));
+      RC(zaxpy_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
       RC(rkd_contacts(L, w, nb, Sdw, L.nd, grd, L.nd, s));
       // G^< = i G^n
       RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,

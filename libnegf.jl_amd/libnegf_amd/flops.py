@@ -22,8 +22,10 @@ identities (the central recursion reproduces G_ii; with the RGF relation
 l_{i+1} G_{i,i+1} = g_{i+1} - G_{i+1,i+1} the diagonal rescaling and the
 downward pass become G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1} l_j^H with
 E_j = Sigma_jj g_j^H, D_j = g_j E_j, P_j = G_jj E_j):
-  GEMMs 13 (N-1) + 4, inverses N
-  -> (104 (N-1) + 8 N + 32) b^3 = 9.52e11 at b = 512, N = 64.
+  The upward Keldysh step runs on Gamma_R alone first, whose propagated
+  B_0 gives T = Re Tr(Gamma_L G_00 B_0 G_00^H) without the first-row chain:
+  GEMMs 12 (N-1) + 4, inverses N
+  -> (96 (N-1) + 8 N + 32) b^3 = 8.79e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
@@ -49,4 +51,4 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
 
 
 def flops_energy_executed(N: int, b: int) -> float:
-    return (104 * (N - 1) + 8 * N + 32) * b ** 3
+    return (96 * (N - 1) + 8 * N + 32) * b ** 3

@@ -113,8 +113,8 @@ def test_flop_counts_match_baseline():
     assert abs(flops_rkd(64, 512) - 1.171e12) / 1.171e12 < 1e-3
     assert abs(flops_rgf_keldysh_energy(64, 512) - 1.511e12) / 1.511e12 < 1e-3
     from libnegf_amd import flops_energy_executed
-    # what the energy driver launches: 823 GEMMs of 512^3 and 64 inverses
-    assert flops_energy_executed(64, 512) == (8 * 823 + 8 * 64) * 512 ** 3
+    # what the energy driver launches: 760 GEMMs of 512^3 and 64 inverses
+    assert flops_energy_executed(64, 512) == (8 * 760 + 8 * 64) * 512 ** 3
 
 
 def test_packed_layout_roundtrip_on_cpu_tensors():
