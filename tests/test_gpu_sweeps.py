@@ -98,11 +98,26 @@ def test_keldysh_vs_oracle(N, b):
     assert max_block_err(_band(Gr.to_host()), rgf_ref(A)) < 1e-12
 
 
-def test_energies_workload_vs_oracle():
-    N, b, eta, kT, muL, muR = 6, 64, 1e-3, 0.025, 0.1, -0.1
-    print(f"energies workload N={N} b={b}", flush=True)
-    dev = device_hamiltonian(N, b, seed=4)
-    Es = np.array([-0.4, -0.05, 0.1, 0.33, 0.8])
+_ENERGIES = [-0.4, -0.05, 0.1, 0.33, 0.8]
+
+
+@pytest.mark.parametrize("N,b,eta,Es,seed,tol", [
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12),
+    (4, 160, 1e-3, _ENERGIES, 4, 1e-12),
+    (3, 256, 1e-3, _ENERGIES, 4, 1e-12),
+    (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11)])
+def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol):
+    """b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  The
+    N = 16, b = 512, eta = 1e-4 case is the production block size and
+    broadening at a depth where the Gamma_R propagation B_i = t_i B_{i+1}
+    t_i^H is strongly contracting inside an expanding map (|t_i| ~ 40, |B_i|
+    ~ 100): a forward perturbation of the computed B_i that is not of the
+    backward-stable form (e.g. forcing exact Hermitian symmetry elementwise)
+    grows ~|t|^2 per layer and fails this case by many orders."""
+    kT, muL, muR = 0.025, 0.1, -0.1
+    print(f"energies workload N={N} b={b} eta={eta}", flush=True)
+    dev = device_hamiltonian(N, b, seed=seed)
+    Es = np.array(Es)
     H = api.DeviceHamiltonianGPU(dev)
     nE = len(Es)
     GR = torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda")
@@ -125,10 +140,10 @@ def test_energies_workload_vs_oracle():
         Tref = np.real(np.trace(dev.gamma_L @ g1n @ dev.gamma_R @ g1n.conj().T))
         for i in range(N):
             blk = GR[e, i * b * b:(i + 1) * b * b].reshape(b, b).T
-            assert relfro(blk, G[(i, i)]) < 1e-12
+            assert relfro(blk, G[(i, i)]) < tol
             blk = GL[e, i * b * b:(i + 1) * b * b].reshape(b, b).T
-            assert relfro(blk, 1j * gd[i]) < 1e-12
-        assert abs(T[e] - Tref) < 1e-11 * max(1.0, abs(Tref))
+            assert relfro(blk, 1j * gd[i]) < tol
+        assert abs(T[e] - Tref) < 10 * tol * max(1.0, abs(Tref))
 
 
 @pytest.mark.parametrize("streams", [1, 2])
