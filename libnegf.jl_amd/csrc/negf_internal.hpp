@@ -134,9 +134,6 @@ int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP,
 int zcopy3d_launch(long long count, int n1, int n2, int n3, zc alpha, const zc* src,
                    long long s1, long long s2, long long s3, zc* dst, long long d1, long long d2,
                    long long d3, hipStream_t s);
-// O = P + P^H + R for square n x n blocks (O must not alias P).
-int zherm_add_launch(int n, const zc* P, long long ldp, long long sp, const zc* R, long long ldr,
-                     long long sr, zc* O, long long ldo, long long so, int batch, hipStream_t s);
 // M[idx][i] = z[idx] S[idx][i] - H[idx][i] - E[idx][i] over `count` elements
 // per instance (batch strides may be 0; E may be null).
 int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, const zc* H,

@@ -389,26 +389,15 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
   return Status::OK;
 }
 
-// Keldysh pass of the energy driver: Sigma^n lives on the contact layers only
-// (its off-diagonal blocks are zero on entry) and the RGF pass has left the
-// diagonal blocks G_jj.  The reference's passes (src/keldyshndiag.jl:221-318)
-// rewritten with exact identities of that case:
-//  * upward: with Su_i = 0 on entry, Su_i = -t_i Sd_{i+1} only feeds
-//    Sd_i -= Su_i t_i^H, i.e. Sd_i += t_i Sd_{i+1} t_i^H;
-//  * central: its recursion g_i X_i = g_i + l_i (g_{i-1} X_{i-1}) t_{i-1},
-//    g_0 X_0 = g_0, is the RGF downward recursion of G_ii, so the rescaled
-//    Su_i = G_ii (-t_i Sd_{i+1}) g_{i+1}^H = G_{i,i+1} E_{i+1} with
-//    E_j = Sd_j g_j^H, and the diagonal rescaling is D_j = g_j E_j;
-//  * downward: with l_{i+1} G_{i,i+1} = g_{i+1} - G_{i+1,i+1} (RGF), its three
-//    products collapse to
-//      G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1,j-1} l_j^H,  P_j = G_jj E_j,
-//    and G^n_00 = D_0.
-// The upward step is linear in Sigma, so the driver runs it on Gamma_R alone
-// (unscaled): its result B_0 at layer 0 gives the transmission
-// T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R G_{0,N-1}^H),
-// G_{0,N-1} = G_00 prod(-t)), after which the blocks are scaled by f_R and
-// f_L Gamma_L is added on layer 0.
-// 12 GEMMs per layer for RGF + Keldysh + T instead of 21.
+// Upward Keldysh step of the energy driver.  Sigma^n lives on the contact
+// layers only (its off-diagonal blocks are zero on entry), so the reference's
+// upward step (src/keldyshndiag.jl:230-234) reduces to Su_i = -t_i Sd_{i+1}
+// feeding Sd_i -= Su_i t_i^H only, i.e. Sd_i += t_i Sd_{i+1} t_i^H.  The step
+// is linear in Sigma; the driver runs it on Gamma_R alone (unscaled):
+// B_j = t_j B_{j+1} t_j^H, B_{N-1} = Gamma_R.  Its B_0 gives the transmission
+// T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R
+// G_{0,N-1}^H), G_{0,N-1} = G_00 prod(-t)), and f_R B_j is the right-injected
+// part of G^n (rkd_inject).
 static int rkd_contacts_up(const Layout& L, const SweepWork& w, int batch, zc* Sd,
                            long long sd_bs, hipStream_t s) {
   const int N = L.N;
@@ -439,34 +428,50 @@ static int transmission_from_b0(const Layout& L, const SweepWork& w, int batch, 
   return zdotc_re_launch(b0, b0, P.p, b0, xbs, G00.p, G00.ld, G00.bs, T, batch, s);
 }
 
-static int rkd_contacts(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long sd_bs,
-                        const zc* Gd, long long gd_bs, hipStream_t s) {
+// Keldysh diagonal of the energy driver in injection form.  With Sigma^n =
+// f_L Gamma_L on layer 0 and f_R Gamma_R on layer N-1 only,
+//   G^n_jj = f_L G_j0 Gamma_L G_j0^H + f_R G_j,N-1 Gamma_R G_j,N-1^H.
+// RGF gives both columns through its own recursions: G_j0 = -l_j G_j-1,0, so
+// C_j = G_j0 (f_L Gamma_L) G_j0^H = l_j C_j-1 l_j^H, C_0 = G_00 (f_L Gamma_L)
+// G_00^H; and G_j,N-1 = G_jj prod_k>=j (-t_k), so the right term is
+// G_jj B_j G_jj^H with B_j = t_j B_j+1 t_j^H, B_N-1 = Gamma_R (the upward
+// step, already in Sd, scaled by f_R).  Two sums of Hermitian positive
+// semidefinite terms, no cancellation.  Per layer: Q = l_j C_j-1, C_j =
+// Q l_j^H, V = G_jj B_j, G^<_jj = i V G_jj^H + i C_j (the last GEMM writes the
+// output).  The same G^n as the reference's passes (src/keldyshndiag.jl:
+// 221-318) for this Sigma^n, in 4 GEMMs per layer instead of their 5 after
+// the upward step; with RGF (5) and the upward step (2), 11 GEMMs and one
+// inverse per layer for G^R, G^< and T (the reference's sequence: 21 and
+// three LU solves).
+static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* Sd,
+                      long long sd_bs, const zc* Gd, long long gd_bs, const zc* gammaL,
+                      const zc* fL, zc* Gless, long long gl_bs, hipStream_t s) {
   const int N = L.N;
-  const long long gbs = L.nd;
   const long long xbs = (long long)L.bmax * L.bmax;
-  auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
+  const zc I1 = zmake(0.0, 1.0);
   auto G = [&](int i) { return M{Gd + L.od[i], L.b[i], gd_bs}; };
-  auto g = [&](int i) { return M{w.g + L.od[i], L.b[i], gbs}; };
   auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], L.no}; };  // l_i = S_i^{-1} A_{i,i-1}
   const M none{nullptr, 1, 0};
-  // central, diagonal re-scaling and downward (keldyshndiag.jl:256-317), layer by layer
+  const int b0 = L.b[0];
+  // C_0: T2 = f_L Gamma_L, T1 = G_00 T2, T2 = T1 G_00^H
+  RC(zscale_launch(b0, b0, ZERO, fL, 1, gammaL, b0, 0, w.T2, b0, xbs, batch, s));
+  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, G(0), M{w.T2, b0, xbs}, ZERO, none, MW{w.T1, b0, xbs},
+          batch, s));
+  RC(gemm(OP_N, OP_C, b0, b0, b0, ONE, M{w.T1, b0, xbs}, G(0), ZERO, none, MW{w.T2, b0, xbs},
+          batch, s));
   for (int j = 0; j < N; ++j) {
     const int bj = L.b[j];
-    MW E{w.T1, bj, xbs};
-    RC(gemm(OP_N, OP_C, bj, bj, bj, ONE, ro(sd(j)), g(j), ZERO, none, E, batch, s));
-    if (j == 0) {
-      RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, g(0), ro(E), ZERO, none, sd(0), batch, s));
-      continue;
+    if (j > 0) {
+      const int bp = L.b[j - 1];
+      RC(gemm(OP_N, OP_N, bj, bp, bp, ONE, l(j), M{w.T2, bp, xbs}, ZERO, none, MW{w.X, bj, xbs},
+              batch, s));
+      RC(gemm(OP_N, OP_C, bj, bj, bp, ONE, M{w.X, bj, xbs}, l(j), ZERO, none, MW{w.T2, bj, xbs},
+              batch, s));
     }
-    const int bp = L.b[j - 1];
-    MW D{w.T2, bj, xbs}, P{w.X, bj, xbs}, Q{w.T1, bj, xbs};
-    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, g(j), ro(E), ZERO, none, D, batch, s));
-    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), ro(E), ZERO, none, P, batch, s));
-    // Q = l_j G^n_{j-1}; D <- Q l_j^H - D (in place)
-    RC(gemm(OP_N, OP_N, bj, bp, bp, ONE, l(j), ro(sd(j - 1)), ZERO, none, Q, batch, s));
-    RC(gemm(OP_N, OP_C, bj, bj, bp, ONE, ro(Q), l(j), MONE, ro(D), D, batch, s));
-    // G^n_jj = P + P^H + D
-    RC(zherm_add_launch(bj, w.X, bj, xbs, w.T2, bj, xbs, Sd + L.od[j], bj, sd_bs, batch, s));
+    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), M{Sd + L.od[j], bj, sd_bs}, ZERO, none,
+            MW{w.T1, bj, xbs}, batch, s));
+    RC(gemm(OP_N, OP_C, bj, bj, bj, I1, M{w.T1, bj, xbs}, G(j), I1, M{w.T2, bj, xbs},
+            MW{Gless + L.od[j], bj, gl_bs}, batch, s));
   }
   return Status::OK;
 }
@@ -530,7 +535,8 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     if (out.Gless_diag) {
       // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere (its
       // off-diagonal blocks are zero and never stored).  The upward step runs
-      // on Gamma_R alone, which also yields T; then the f_R scale and f_L Gamma_L.
+      // on Gamma_R alone (B_j), which also yields T; then B_j is scaled by f_R
+      // and G^< = i G^n written layer by layer in injection form.
       RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
       RC(zaxpy_launch(b, b, ONE, nullptr, 0, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
       RC(rkd_contacts_up(L, w, nb, Sdw, L.nd, s));
@@ -538,11 +544,8 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
         RC(transmission_from_b0(L, w, nb, M{grd, b, L.nd}, M{Sdw, b, L.nd}, M{in.gammaL, b, 0},
                                 out.T + e0, s));
       RC(zscale_launch((int)L.nd, 1, ZERO, fR + e0, 1, Sdw, L.nd, L.nd, Sdw, L.nd, L.nd, nb, s));
-      RC(zaxpy_launch(b, b, ZERO, fL + e0, 1, in.gammaL, b, 0, Sdw, b, L.nd, nb, s));
-      RC(rkd_contacts(L, w, nb, Sdw, L.nd, grd, L.nd, s));
-      // G^< = i G^n
-      RC(zscale_launch((int)L.nd, 1, zmake(0.0, 1.0), nullptr, 0, Sdw, L.nd, L.nd,
-                       out.Gless_diag + (size_t)e0 * L.nd, L.nd, L.nd, nb, s));
+      RC(rkd_inject(L, w, nb, Sdw, L.nd, grd, L.nd, in.gammaL, fL + e0,
+                    out.Gless_diag + (size_t)e0 * L.nd, L.nd, s));
     }
   }
   return Status::OK;

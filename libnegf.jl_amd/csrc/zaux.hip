@@ -32,33 +32,6 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int m, int n, const zc* __
   }
 }
 
-// O = P + P^H + R for square n x n blocks; P^H through a 32x32 LDS tile.
-__global__ __launch_bounds__(256) void herm_add_kernel(int n, const zc* __restrict__ P,
-                                                       long long ldp, long long sp,
-                                                       const zc* __restrict__ R, long long ldr,
-                                                       long long sr, zc* __restrict__ O,
-                                                       long long ldo, long long so) {
-  __shared__ zc tile[32][33];
-  const long long inst = blockIdx.z;
-  const zc* Pi = P + inst * sp;
-  const zc* Ri = R + inst * sr;
-  zc* Oi = O + inst * so;
-  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  // tile[a][b] = P[j0 + b, i0 + a]
-  for (int a = ty; a < 32; a += 8) {
-    const int r = j0 + tx, c = i0 + a;
-    if (r < n && c < n) tile[a][tx] = Pi[r + (long long)c * ldp];
-  }
-  __syncthreads();
-  for (int jj = ty; jj < 32; jj += 8) {
-    const int i = i0 + tx, j = j0 + jj;
-    if (i < n && j < n)
-      Oi[i + (long long)j * ldo] =
-          zadd(zadd(Pi[i + (long long)j * ldp], zconj(tile[tx][jj])), Ri[i + (long long)j * ldr]);
-  }
-}
-
 __global__ void copy2d_kernel(int m, int n, const zc* __restrict__ A, long long lda,
                               long long strideA, zc* __restrict__ B, long long ldb,
                               long long strideB) {
@@ -259,17 +232,6 @@ int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, c
   if (bx > 4096) bx = 4096;
   hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)bx, batch), dim3(256), 0, s, count, z, S, sbs, H,
                      hbs, E, ebs, M, mbs);
-  return ok();
-}
-
-int zherm_add_launch(int n, const zc* P, long long ldp, long long sp, const zc* R, long long ldr,
-                     long long sr, zc* O, long long ldo, long long so, int batch, hipStream_t s) {
-  if (n < 0 || batch < 0) return Status::BAD_ARG;
-  if (n == 0 || batch == 0) return Status::OK;
-  if (batch > 65535) return Status::TOO_LARGE;
-  const int t = (n + 31) / 32;
-  hipLaunchKernelGGL(herm_add_kernel, dim3(t, t, batch), dim3(256), 0, s, n, P, ldp, sp, R, ldr, sr,
-                     O, ldo, so);
   return ok();
 }
 

@@ -206,9 +206,14 @@ __device__ __forceinline__ void apply_transform(int n, zc (&P)[RPT][S], const zc
                                                 const int* __restrict__ pt, int wt,
                                                 zc (&Rs)[S][S]) {
   const int tid = threadIdx.x;
+  // all wt pivots loaded at once (unconditional loads of in-range indices, so
+  // the compiler issues them together instead of one wait per pivot)
   int pj[S];
 #pragma unroll
-  for (int j = 0; j < S; ++j) pj[j] = j < wt ? pt[j] : -1;
+  for (int j = 0; j < S; ++j) {
+    const int v = pt[j < wt ? j : 0];
+    pj[j] = j < wt ? v : -1;
+  }
   zc w[RPT][S];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
@@ -250,6 +255,113 @@ __device__ __forceinline__ void apply_transform(int n, zc (&P)[RPT][S], const zc
   }
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Update item on the FP64 matrix cores (strips of 16 columns, one matrix row
+// per thread: each wave owns 64 rows as four 16-row blocks).  The strip is
+// held transposed in the accumulator layout of v_mfma_f64_16x16x4_f64: lane
+// l, element e of block b is X[r, k0 + c] with r = 64 wave + 16 b + l % 16,
+// c = l / 16 + 4 e, so each 16 lanes load and store 256 contiguous bytes of a
+// column.  Strip t's row operation X[:, s] += W_t X[p_t, s] is computed as
+// its transpose, X[:, s]^T += R^T W_t^T with R = X[p_t, s] (16 x 16), in the
+// complex-as-real K' = 2K form of the GEMM kernel: R^T is the A operand (16
+// doubles per lane per transform, from LDS), W_t^T the B operand (read from
+// memory, 256 contiguous bytes per 16 lanes).  Compared with the row-per-
+// thread form, the S x S block is read from LDS once per transform instead of
+// once per 4 FMAs.  Transforms t_lo .. t_hi - 1 in order; W_t at
+// Wp + (t - t_base) n S, pivots at pv + t S.  All threads call it.
+template <int PT>
+__device__ __forceinline__ void update_strip_mfma(int n, zc* __restrict__ Xi, int k0, int ncols,
+                                                  const zc* __restrict__ Wp,
+                                                  const int* __restrict__ pv, int t_lo, int t_hi,
+                                                  int t_base, zc (&Rs)[16][16]) {
+  constexpr int S = 16;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int part = lg & 1;               // K' parity of this lane's operand
+  const int jh = lane >> 5;              // complex k = 2 ks + jh
+  const double asign = part ? -1.0 : 1.0;
+  dbl4 are[4], aim[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int r = wave * 64 + b * 16 + li;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = lg + 4 * e;
+      const zc v = (r < n && c < ncols) ? Xi[r + (k0 + c) * n] : zmake(0.0, 0.0);
+      are[b][e] = v.x;
+      aim[b][e] = v.y;
+    }
+  }
+  for (int t = t_lo; t < t_hi; ++t) {
+    const int wt = (n - t * S) < S ? (n - t * S) : S;
+    const int* __restrict__ pt = pv + t * S;
+    const zc* __restrict__ Wt = Wp + (t - t_base) * n * S;   // int offsets: n <= 2048
+    // pivots first (one wait for all 16), then W_t's rows, in flight across the barrier
+    int pj[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int v = pt[j < wt ? j : 0];
+      pj[j] = j < wt ? v : -1;
+    }
+    zc wv[4][8];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = wave * 64 + b * 16 + li;
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const int j = 2 * ks + jh;
+        wv[b][ks] = (r < n && j < wt) ? Wt[r + j * n] : zmake(0.0, 0.0);
+      }
+    }
+    __syncthreads();   // Rs is free
+    for (int x = wt * S + tid; x < S * S; x += PT) Rs[x / S][x % S] = zmake(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      if (j < wt) {
+        const int p = pj[j];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (p == wave * 64 + b * 16 + li) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Rs[j][lg + 4 * e] = zmake(are[b][e], aim[b][e]);
+          }
+      }
+    }
+    __syncthreads();
+    double a[8], am[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const zc rv = Rs[2 * ks + jh][li];
+      a[ks] = part ? rv.y : rv.x;
+      am[ks] = asign * a[ks];
+    }
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const zc w = wv[b][ks];
+        const double b1 = part ? w.y : w.x;
+        const double b2 = part ? w.x : w.y;
+        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[ks], b1, are[b], 0, 0, 0);
+        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b2, aim[b], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int r = wave * 64 + b * 16 + li;
+    if (r < n) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = lg + 4 * e;
+        if (c < ncols) Xi[r + (k0 + c) * n] = zmake(are[b][e], aim[b][e]);
+      }
+    }
+  }
+}
+
 // One launch of the blocked Gauss-Jordan inverse, over all instances.  The
 // matrix is cut into column strips of S (strip s = columns s*S ..) and the
 // strips into panels of PS consecutive strips.  Panel K was factored by the
@@ -275,7 +387,7 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
                                                      zc* __restrict__ Wnext,
                                                      int* __restrict__ piv,
                                                      int* __restrict__ used,
-                                                     int* __restrict__ info) {
+                                                     int* __restrict__ info, int use_mfma) {
   constexpr int NW = PT / 64;
   const int tid = threadIdx.x;
   const int pk0 = PS * K, pk1 = (pk0 + PS) < nstrips ? pk0 + PS : nstrips;   // panel K
@@ -320,6 +432,13 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
       t0 = pk0;
     }
     const int k0 = s * S, ncols = width(s);
+    if constexpr (S == 16 && RPT == 1) {
+      if (use_mfma) {
+        update_strip_mfma<PT>(n, Xi, k0, ncols, W_of(Wcur, 0), piv + (long long)inst * n, t0, pk1,
+                              pk0, Rs);
+        return;
+      }
+    }
     load_strip<PT, RPT, S>(Xi, n, k0, ncols, P);
     for (int t = t0; t < pk1; ++t)
       apply_transform<PT, RPT, S>(n, P, W_of(Wcur, t - pk0), piv + (long long)inst * n + t * S,
@@ -410,7 +529,7 @@ __global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __r
 
 template <int PT, int RPT, int S, int PS>
 int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
-        long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+        long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s, int use_mfma) {
   if (A != w.X || lda != n || strideA != (long long)n * n) {
     int rc = zcopy2d_launch(n, n, A, lda, strideA, w.X, n, (long long)n * n, batch, s);
     if (rc) return rc;
@@ -425,7 +544,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
   const int np = (ns + PS - 1) / PS;
   const long long wst = (long long)n * S * PS * batch;   // one W buffer (a panel)
   hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
-                     batch, w.X, w.W + wst, w.W, w.piv, w.used, info);
+                     batch, w.X, w.W + wst, w.W, w.piv, w.used, info, 0);
   for (int K = 0; K < np; ++K) {
     const int pk0 = PS * K, pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
     const int nk0 = PS * (K + 1), nk1 = (nk0 + PS) < ns ? nk0 + PS : ns;
@@ -438,7 +557,8 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
     const zc* Wc = w.W + (K & 1) * wst;
     zc* Wn = w.W + ((K + 1) & 1) * wst;
     hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3((unsigned)((long long)batch * m)),
-                       dim3(PT), 0, s, n, K, ns, batch, w.X, Wc, Wn, w.piv, w.used, info);
+                       dim3(PT), 0, s, n, K, ns, batch, w.X, Wc, Wn, w.piv, w.used, info,
+                       use_mfma);
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
                      2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
@@ -452,17 +572,21 @@ int nb_for(int n) { return n <= 512 ? 32 : (n <= 1024 ? 16 : 8); }
 // Panel variant (autotuner choice, negf_inv_variant): 0 = by batch size,
 // 1 = one strip per panel, 2 = panels of two strips (one pass over X per two
 // strips, but each factoring workgroup does twice the work).  Measured on
-// MI355X at n = 512: 64 instances 4.17 ms per inverse with two-strip panels
-// against 4.5 ms with one; a single instance 1.3 ms against 1.0 ms.  Auto:
+// MI355X at n = 512: 64 instances 3.98 ms per inverse with two-strip panels
+// against 5.5 ms with one; a single instance 1.49 ms against 1.76 ms.  Auto:
 // two-strip panels from 32 instances up.  Strips are 16 columns up to
 // n = 512, 8 up to 1024, 4 beyond; one matrix row per thread up to n = 1024,
 // so a strip, its W rows and one column of R stay in registers.
+// Variants 3, 4, 5 are 0, 1, 2 with the update items on the matrix cores
+// (update_strip_mfma, n <= 512): 5.0 ms at 64 x 512 -- the kernel's 256
+// VGPRs force spills that serialise its loads, and one 512-thread workgroup
+// per CU leaves its phases unoverlapped; kept selectable for measurement.
 int g_inv_variant = 0;
 
 }  // namespace
 
 int zinv_set_variant(int v) {
-  if (v < 0 || v > 2) return Status::BAD_ARG;
+  if (v < 0 || v > 5) return Status::BAD_ARG;
   g_inv_variant = v;
   return Status::OK;
 }
@@ -495,17 +619,19 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  const bool pairs = g_inv_variant == 2 || (g_inv_variant == 0 && batch >= 32);
+  const int pv = g_inv_variant % 3;
+  const int mf = g_inv_variant >= 3 ? 1 : 0;
+  const bool pairs = pv == 2 || (pv == 0 && batch >= 32);
   if (!pairs) {
-    if (n <= 256) return run<256, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-    if (n <= 512) return run<512, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-    if (n <= 1024) return run<1024, 1, 8, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-    return run<1024, 2, 4, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (n <= 256) return run<256, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+    if (n <= 512) return run<512, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+    if (n <= 1024) return run<1024, 1, 8, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+    return run<1024, 2, 4, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
   }
-  if (n <= 256) return run<256, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 512) return run<512, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  if (n <= 1024) return run<1024, 1, 8, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-  return run<1024, 2, 4, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 256) return run<256, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+  if (n <= 512) return run<512, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+  if (n <= 1024) return run<1024, 1, 8, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+  return run<1024, 2, 4, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
 }
 
 }  // namespace negf

@@ -17,15 +17,14 @@ reference's call sequence (credited).
 ``flops_energy_executed`` is what the GPU energy driver actually runs per
 energy: every S_i^{-1} solve replaced by one explicit inverse (8 b^3) and
 GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), and the
-Keldysh pass specialised to a Sigma^n on the contact layers with exact
-identities (the central recursion reproduces G_ii; with the RGF relation
-l_{i+1} G_{i,i+1} = g_{i+1} - G_{i+1,i+1} the diagonal rescaling and the
-downward pass become G^n_jj = P_j + P_j^H - D_j + l_j G^n_{j-1} l_j^H with
-E_j = Sigma_jj g_j^H, D_j = g_j E_j, P_j = G_jj E_j):
-  The upward Keldysh step runs on Gamma_R alone first, whose propagated
-  B_0 gives T = Re Tr(Gamma_L G_00 B_0 G_00^H) without the first-row chain:
-  GEMMs 12 (N-1) + 4, inverses N
-  -> (96 (N-1) + 8 N + 32) b^3 = 8.79e11 at b = 512, N = 64.
+Keldysh pass specialised to a Sigma^n on the contact layers: the upward step
+runs on Gamma_R alone (B_j = t_j B_{j+1} t_j^H; its B_0 gives
+T = Re Tr(Gamma_L G_00 B_0 G_00^H) without the first-row chain), and
+G^n_jj = f_L C_j + f_R G_jj B_j G_jj^H with C_j = l_j C_{j-1} l_j^H
+(injection form, exact for this Sigma^n):
+  GEMMs 3 (N-1) RGF upward + 2 (N-1) RGF downward + 2 (N-1) Keldysh upward
+        + 2 transmission + 4 N injection = 11 (N-1) + 6, inverses N
+  -> (88 (N-1) + 8 N + 48) b^3 = 8.19e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
@@ -51,4 +50,4 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
 
 
 def flops_energy_executed(N: int, b: int) -> float:
-    return (96 * (N - 1) + 8 * N + 32) * b ** 3
+    return (88 * (N - 1) + 8 * N + 48) * b ** 3

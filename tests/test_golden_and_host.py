@@ -113,8 +113,12 @@ def test_flop_counts_match_baseline():
     assert abs(flops_rkd(64, 512) - 1.171e12) / 1.171e12 < 1e-3
     assert abs(flops_rgf_keldysh_energy(64, 512) - 1.511e12) / 1.511e12 < 1e-3
     from libnegf_amd import flops_energy_executed
-    # what the energy driver launches: 760 GEMMs of 512^3 and 64 inverses
-    assert flops_energy_executed(64, 512) == (8 * 760 + 8 * 64) * 512 ** 3
+    # what the energy driver launches per energy at N = 64: RGF upward 3 x 63
+    # (t, l, Schur), downward 2 x 63, Keldysh upward 2 x 63, transmission 2,
+    # injection 4 x 64 = 699 GEMMs of 512^3, and 64 inverses
+    n_gemm = 3 * 63 + 2 * 63 + 2 * 63 + 2 + 4 * 64
+    assert n_gemm == 699
+    assert flops_energy_executed(64, 512) == (8 * n_gemm + 8 * 64) * 512 ** 3
 
 
 def test_packed_layout_roundtrip_on_cpu_tensors():
