@@ -57,10 +57,13 @@ int negf_workspace_release(void);
  * 2: 64x128 one buffer; 3: 64x128 double-buffered; 4: 128x64 double-buffered;
  * 5: 64x64 at four workgroups per CU). */
 int negf_gemm_variant(int32_t variant);
-/* Autotuner knob: panel of the batched Gauss-Jordan inverse, process-wide
- * (0: by batch size (default); 1: one column strip per panel; 2: two strips
- * per panel -- half the passes over the matrix, twice the latency-bound
- * factoring per pass). */
+/* Autotuner knob: schedule of the batched Gauss-Jordan inverse, process-wide
+ * (0: automatic (default); in place, one launch per panel: 1: one column strip
+ * per panel, 2: two strips per panel; split steps for n <= 512 (the factoring
+ * of the next panel on the caller's stream beside matrix-core updates of the
+ * other strips on a second stream): 3, 4, 5: panels of 1, 2, 4 strips).
+ * Wider panels: fewer passes over the matrix, more latency-bound factoring
+ * per pass. */
 int negf_inv_variant(int32_t variant);
 
 /* ------------------------------------------------------------------ per-op */

@@ -83,13 +83,16 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 // one per instance) and may be the same buffer as A when A may be destroyed.
 struct InvWork {
   zc* X;        // n*n*batch
+  zc* X2;       // n*n*batch: the other buffer of the out-of-place steps
   zc* W;        // 2*n*nb*batch (W of panel J and of panel J+1)
   int* piv;     // n*batch
   int* used;    // n*batch
+  zc* R;        // 1024*ceil(n/16)*batch: pivot blocks of the split steps' update items
 };
 size_t zinv_work_bytes(int n, int batch);
-// Panel variant of the inverse (0: by batch size, 1: one strip per panel,
-// 2: two strips per panel), process-wide.
+// Schedule of the inverse (0: automatic, 1 / 2: in place with panels of 1 /
+// 2 strips, 3 / 4 / 5: split steps with panels of 1 / 2 / 4 strips),
+// process-wide.
 int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);
 // A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the

@@ -21,6 +21,7 @@
 // Flops: 8 n^3 (the nominal getrf + getri count).
 #include "negf_internal.hpp"
 #include <climits>
+#include <mutex>
 
 namespace negf {
 
@@ -257,111 +258,6 @@ __device__ __forceinline__ void apply_transform(int n, zc (&P)[RPT][S], const zc
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// Update item on the FP64 matrix cores (strips of 16 columns, one matrix row
-// per thread: each wave owns 64 rows as four 16-row blocks).  The strip is
-// held transposed in the accumulator layout of v_mfma_f64_16x16x4_f64: lane
-// l, element e of block b is X[r, k0 + c] with r = 64 wave + 16 b + l % 16,
-// c = l / 16 + 4 e, so each 16 lanes load and store 256 contiguous bytes of a
-// column.  Strip t's row operation X[:, s] += W_t X[p_t, s] is computed as
-// its transpose, X[:, s]^T += R^T W_t^T with R = X[p_t, s] (16 x 16), in the
-// complex-as-real K' = 2K form of the GEMM kernel: R^T is the A operand (16
-// doubles per lane per transform, from LDS), W_t^T the B operand (read from
-// memory, 256 contiguous bytes per 16 lanes).  Compared with the row-per-
-// thread form, the S x S block is read from LDS once per transform instead of
-// once per 4 FMAs.  Transforms t_lo .. t_hi - 1 in order; W_t at
-// Wp + (t - t_base) n S, pivots at pv + t S.  All threads call it.
-template <int PT>
-__device__ __forceinline__ void update_strip_mfma(int n, zc* __restrict__ Xi, int k0, int ncols,
-                                                  const zc* __restrict__ Wp,
-                                                  const int* __restrict__ pv, int t_lo, int t_hi,
-                                                  int t_base, zc (&Rs)[16][16]) {
-  constexpr int S = 16;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int li = lane & 15, lg = lane >> 4;
-  const int part = lg & 1;               // K' parity of this lane's operand
-  const int jh = lane >> 5;              // complex k = 2 ks + jh
-  const double asign = part ? -1.0 : 1.0;
-  dbl4 are[4], aim[4];
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int r = wave * 64 + b * 16 + li;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int c = lg + 4 * e;
-      const zc v = (r < n && c < ncols) ? Xi[r + (k0 + c) * n] : zmake(0.0, 0.0);
-      are[b][e] = v.x;
-      aim[b][e] = v.y;
-    }
-  }
-  for (int t = t_lo; t < t_hi; ++t) {
-    const int wt = (n - t * S) < S ? (n - t * S) : S;
-    const int* __restrict__ pt = pv + t * S;
-    const zc* __restrict__ Wt = Wp + (t - t_base) * n * S;   // int offsets: n <= 2048
-    // pivots first (one wait for all 16), then W_t's rows, in flight across the barrier
-    int pj[S];
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const int v = pt[j < wt ? j : 0];
-      pj[j] = j < wt ? v : -1;
-    }
-    zc wv[4][8];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int r = wave * 64 + b * 16 + li;
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const int j = 2 * ks + jh;
-        wv[b][ks] = (r < n && j < wt) ? Wt[r + j * n] : zmake(0.0, 0.0);
-      }
-    }
-    __syncthreads();   // Rs is free
-    for (int x = wt * S + tid; x < S * S; x += PT) Rs[x / S][x % S] = zmake(0.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      if (j < wt) {
-        const int p = pj[j];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (p == wave * 64 + b * 16 + li) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) Rs[j][lg + 4 * e] = zmake(are[b][e], aim[b][e]);
-          }
-      }
-    }
-    __syncthreads();
-    double a[8], am[8];
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const zc rv = Rs[2 * ks + jh][li];
-      a[ks] = part ? rv.y : rv.x;
-      am[ks] = asign * a[ks];
-    }
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const zc w = wv[b][ks];
-        const double b1 = part ? w.y : w.x;
-        const double b2 = part ? w.x : w.y;
-        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[ks], b1, are[b], 0, 0, 0);
-        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b2, aim[b], 0, 0, 0);
-      }
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int r = wave * 64 + b * 16 + li;
-    if (r < n) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int c = lg + 4 * e;
-        if (c < ncols) Xi[r + (k0 + c) * n] = zmake(are[b][e], aim[b][e]);
-      }
-    }
-  }
-}
-
 // One launch of the blocked Gauss-Jordan inverse, over all instances.  The
 // matrix is cut into column strips of S (strip s = columns s*S ..) and the
 // strips into panels of PS consecutive strips.  Panel K was factored by the
@@ -383,11 +279,12 @@ __device__ __forceinline__ void update_strip_mfma(int n, zc* __restrict__ Xi, in
 template <int PT, int RPT, int S, int PS>
 __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, int batch,
                                                      zc* __restrict__ X,
+                                                     zc* __restrict__ Xout,
                                                      const zc* __restrict__ Wcur,
                                                      zc* __restrict__ Wnext,
                                                      int* __restrict__ piv,
                                                      int* __restrict__ used,
-                                                     int* __restrict__ info, int use_mfma) {
+                                                     int* __restrict__ info) {
   constexpr int NW = PT / 64;
   const int tid = threadIdx.x;
   const int pk0 = PS * K, pk1 = (pk0 + PS) < nstrips ? pk0 + PS : nstrips;   // panel K
@@ -409,6 +306,7 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
   }
   const int cat = k / batch, inst = k - cat * batch;
   zc* Xi = X + (long long)inst * n * n;
+  zc* Xo = Xout + (long long)inst * n * n;   // the factoring item's output (== Xi in place)
 
   __shared__ zc Rs[S][S];
   __shared__ double red_v[2][NW];
@@ -432,13 +330,6 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
       t0 = pk0;
     }
     const int k0 = s * S, ncols = width(s);
-    if constexpr (S == 16 && RPT == 1) {
-      if (use_mfma) {
-        update_strip_mfma<PT>(n, Xi, k0, ncols, W_of(Wcur, 0), piv + (long long)inst * n, t0, pk1,
-                              pk0, Rs);
-        return;
-      }
-    }
     load_strip<PT, RPT, S>(Xi, n, k0, ncols, P);
     for (int t = t0; t < pk1; ++t)
       apply_transform<PT, RPT, S>(n, P, W_of(Wcur, t - pk0), piv + (long long)inst * n + t * S,
@@ -479,7 +370,7 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
             zc wv = P[q][c];
             if (r == pc[c]) wv.x -= 1.0;
             Wn[r + (long long)c * n] = wv;
-            Xi[r + (long long)(k0 + c) * n] = P[q][c];
+            Xo[r + (long long)(k0 + c) * n] = P[q][c];
           }
         }
       }
@@ -527,9 +418,301 @@ __global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __r
   }
 }
 
+// Item decoding shared by the prep and update kernels of one split step:
+// per instance the strips outside panels K and K+1 (all of panel K's
+// operations, t from pk0), then panel K's own strips (strip pk0 + u gets the
+// operations of the later strips of its panel; none = a copy).
+struct SplitStep {
+  int pk0, pk1, hi, nupd, items;
+  __host__ __device__ SplitStep(int K, int PS, int ns) {
+    pk0 = PS * K;
+    pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
+    const int nk0 = PS * (K + 1), nk1 = (nk0 + PS) < ns ? nk0 + PS : ns;
+    hi = nk0 < ns ? nk1 : pk1;     // strips [pk0, hi) are not plain update strips
+    nupd = ns - (hi - pk0);
+    items = nupd + (pk1 - pk0);
+  }
+  __device__ void item(int it, int* s, int* t_lo) const {
+    if (it < nupd) {
+      *s = it < pk0 ? it : it + (hi - pk0);
+      *t_lo = pk0;
+    } else {
+      *s = pk0 + (it - nupd);
+      *t_lo = *s + 1;
+    }
+  }
+};
+
+// Pivot blocks of every update item of a split step (one wave per
+// (instance, item)): R_t = X'[p_t, s] for the item's operations t = t_lo ..
+// pk1 - 1, X' = Xin after the operations before t, i.e.
+//   R_t = Xin[p_t, s] + sum_{t' < t} W_t'[p_t, :] R_t',
+// zero outside the strip and the pivots; stored as Rb[(inst items + item)
+// MAXPS + u][j][c].  Xin is not written during the step.
+constexpr int MAXPS = 4;
+__global__ __launch_bounds__(64) void gj_prep_kernel(int n, int K, int PS, int ns, int batch,
+                                                     const zc* __restrict__ Xin,
+                                                     const zc* __restrict__ Wcur,
+                                                     const int* __restrict__ piv,
+                                                     zc* __restrict__ Rb) {
+  constexpr int S = 16;
+  const SplitStep st(K, PS, ns);
+  const int inst = blockIdx.x % batch, it = blockIdx.x / batch;
+  int s, t_lo;
+  st.item(it, &s, &t_lo);
+  const int nt = st.pk1 - t_lo;
+  if (nt <= 0) return;
+  const int lane = threadIdx.x;
+  const int k0 = s * S, ncols = (n - k0) < S ? (n - k0) : S;
+  const zc* __restrict__ Xi = Xin + (long long)inst * n * n;
+  const int* __restrict__ pv = piv + (long long)inst * n;
+  zc* __restrict__ Ro = Rb + ((long long)inst * st.items + it) * MAXPS * S * S;
+  // rows padded to 17 entries: 16 lanes touch 16 rows of one column
+  __shared__ zc Rs[MAXPS][S][S + 1];
+  __shared__ zc Wg[S][S + 1];
+  for (int u = 0; u < nt; ++u) {
+    const int t = t_lo + u;
+    const int wt = (n - t * S) < S ? (n - t * S) : S;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int x = lane + 64 * m, j = x & 15, c = x >> 4;
+      zc v = zmake(0.0, 0.0);
+      if (j < wt && c < ncols) v = Xi[pv[t * S + j] + (k0 + c) * n];
+      Rs[u][j][c] = v;
+    }
+  }
+  __syncthreads();
+  // chain: block u gets the operations of blocks 0 .. u-1 in order
+  for (int u = 1; u < nt; ++u) {
+    const int tu = t_lo + u, wtu = (n - tu * S) < S ? (n - tu * S) : S;
+    for (int v = 0; v < u; ++v) {
+      const int tv = t_lo + v, wtv = (n - tv * S) < S ? (n - tv * S) : S;
+      const zc* __restrict__ Wv = Wcur + ((long long)inst * PS + (tv - st.pk0)) * n * S;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int x = lane + 64 * m, j = x & 15, q = x >> 4;
+        Wg[j][q] = (j < wtu && q < wtv) ? Wv[pv[tu * S + j] + q * n] : zmake(0.0, 0.0);
+      }
+      __syncthreads();
+      zc add[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int x = lane + 64 * m, j = x & 15, c = x >> 4;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          const zc wq = Wg[j][q];
+          const zc rq = Rs[v][q][c];
+          re = fma(wq.x, rq.x, fma(-wq.y, rq.y, re));
+          im = fma(wq.x, rq.y, fma(wq.y, rq.x, im));
+        }
+        add[m] = zmake(re, im);
+      }
+      __syncthreads();   // Wg reads done before the next gather
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int x = lane + 64 * m, j = x & 15, c = x >> 4;
+        Rs[u][j][c] = zadd(Rs[u][j][c], add[m]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int u = 0; u < nt; ++u)
+    for (int x = lane; x < S * S; x += 64) Ro[u * S * S + x] = Rs[u][x / S][x % S];
+}
+
+// Update items of a split step, out of place, on the matrix cores: one wave
+// per (instance, item, 64-row chunk):
+//   Xout[rows, s] = Xin[rows, s] + sum_t W_t[rows, :] R_t
+// with the pivot blocks R_t from gj_prep_kernel, so the chunks of a strip are
+// independent and every load of a wave (its rows, W_t rows, R_t) is issued
+// before its first product.  The rows live transposed in the f64 MFMA
+// accumulator layout (lane l, element e of block b: row r0 + 16 b + l % 16,
+// column l / 16 + 4 e); each operation is X^T += R^T W^T as four real
+// products per four complex k (A = parts of R^T, B = parts of W^T).  Step
+// m covers complex k = 4 m + l / 16 for every lane, so each lane loads whole
+// complex numbers of W and R and no two lanes load the same one.  Xout gets
+// every strip but panel K+1's, which the factoring writes.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void gj_update_kernel(
+    int n, int K, int PS, int ns, int batch, int nchunk, const zc* __restrict__ Xin,
+    zc* __restrict__ Xout, const zc* __restrict__ Wcur, const zc* __restrict__ Rb) {
+  constexpr int S = 16;
+  const SplitStep st(K, PS, ns);
+  long long k = blockIdx.x;
+  if ((batch & 7) == 0) {   // all items of one instance on one XCD (its W in one L2)
+    const long long x = k & 7, kk = k >> 3, per = batch >> 3;
+    k = (kk / per) * batch + x + 8 * (kk % per);
+  }
+  const int cat = (int)(k / batch), inst = (int)(k - (long long)cat * batch);
+  const int it = cat / nchunk, chunk = cat - it * nchunk;
+  int s, t_lo;
+  st.item(it, &s, &t_lo);
+  const int nt = st.pk1 - t_lo;   // 0 .. PS
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lg = lane >> 4;
+  const int k0 = s * S, ncols = (n - k0) < S ? (n - k0) : S;
+  const int r0 = chunk * 64;
+  const zc* __restrict__ Xi = Xin + (long long)inst * n * n;
+  zc* __restrict__ Xo = Xout + (long long)inst * n * n;
+  const zc* __restrict__ Wp = Wcur + (long long)inst * PS * n * S;
+  const zc* __restrict__ Ri = Rb + ((long long)inst * st.items + it) * MAXPS * S * S;
+
+  dbl4 are[4], aim[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int r = r0 + b * 16 + li;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = lg + 4 * e;
+      const zc v = (r < n && c < ncols) ? Xi[r + (k0 + c) * n] : zmake(0.0, 0.0);
+      are[b][e] = v.x;
+      aim[b][e] = v.y;
+    }
+  }
+  for (int u = 0; u < nt; ++u) {
+    const int t = t_lo + u;
+    const int wt = (n - t * S) < S ? (n - t * S) : S;
+    const zc* __restrict__ Wt = Wp + (t - st.pk0) * n * S;
+    zc rr[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) rr[m] = Ri[u * S * S + (4 * m + lg) * S + li];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = r0 + b * 16 + li;
+      const bool rok = r < n;
+      zc wv[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int j = 4 * m + lg;
+        wv[m] = (rok && j < wt) ? Wt[r + j * n] : zmake(0.0, 0.0);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wv[m].x, are[b], 0, 0, 0);
+        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wv[m].y, aim[b], 0, 0, 0);
+        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-rr[m].y, wv[m].y, are[b], 0, 0, 0);
+        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].y, wv[m].x, aim[b], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int r = r0 + b * 16 + li;
+    if (r < n) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = lg + 4 * e;
+        if (c < ncols) Xo[r + (k0 + c) * n] = zmake(are[b][e], aim[b][e]);
+      }
+    }
+  }
+}
+
+// Second stream per calling stream for the split steps (update kernel beside
+// the factoring kernel), with its fork / join events; created on first use,
+// default priority, non-blocking.
+struct AuxSlot {
+  int device;
+  hipStream_t key;
+  hipStream_t aux;
+  hipEvent_t fork, join;
+};
+constexpr int MAX_AUX = 64;
+AuxSlot g_aux[MAX_AUX];
+int g_naux = 0;
+std::mutex g_aux_mu;
+
+int aux_for(hipStream_t s, AuxSlot** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return Status::HIP_ERROR;
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  for (int i = 0; i < g_naux; ++i)
+    if (g_aux[i].device == dev && g_aux[i].key == s) {
+      *out = &g_aux[i];
+      return Status::OK;
+    }
+  if (g_naux == MAX_AUX) {
+    // more calling streams than slots: share one of this device's (the events
+    // order the work of one call; calls are enqueued one after another)
+    for (int i = 0; i < g_naux; ++i)
+      if (g_aux[i].device == dev) {
+        *out = &g_aux[i];
+        return Status::OK;
+      }
+    return Status::HIP_ERROR;
+  }
+  AuxSlot a{dev, s, nullptr, nullptr, nullptr};
+  if (hipStreamCreateWithFlags(&a.aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return Status::HIP_ERROR;
+  }
+  g_aux[g_naux] = a;
+  *out = &g_aux[g_naux++];
+  return Status::OK;
+}
+
+// Out-of-place steps with the update items in gj_update_kernel on a second
+// stream (S = 16): per panel K the factoring of panel K+1 (gj_step_kernel,
+// one workgroup per instance, Xin -> Xout) and the updates of every other
+// strip (Xin -> Xout) run concurrently; the buffers swap after each step.
+template <int PT, int PS>
+int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
+              long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+  constexpr int S = 16;
+  AuxSlot* ax = nullptr;
+  int rc = aux_for(s, &ax);
+  if (rc) return rc;
+  if (A != w.X || lda != n || strideA != (long long)n * n) {
+    rc = zcopy2d_launch(n, n, A, lda, strideA, w.X, n, (long long)n * n, batch, s);
+    if (rc) return rc;
+  }
+  {
+    long long total = (long long)n * batch;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gj_init_kernel, dim3(blocks), dim3(256), 0, s, n, w.used, info, batch);
+  }
+  const int ns = (n + S - 1) / S;
+  const int np = (ns + PS - 1) / PS;
+  const int nchunk = (n + 63) / 64;
+  const long long wst = (long long)n * S * PS * batch;
+  hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
+                     batch, w.X, w.X, w.W + wst, w.W, w.piv, w.used, info);
+  zc* cur = w.X;
+  zc* nxt = w.X2;
+  for (int K = 0; K < np; ++K) {
+    const int pk0 = PS * K, pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
+    const bool has_next = PS * (K + 1) < ns;
+    const SplitStep st(K, PS, ns);
+    const zc* Wc = w.W + (K & 1) * wst;
+    zc* Wn = w.W + ((K + 1) & 1) * wst;
+    // fork: the update stream waits for everything before this step
+    if (hipEventRecord(ax->fork, s) != hipSuccess || hipStreamWaitEvent(ax->aux, ax->fork, 0) != hipSuccess)
+      return Status::HIP_ERROR;
+    if (has_next)
+      hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, K, ns,
+                         batch, cur, nxt, Wc, Wn, w.piv, w.used, info);
+    hipLaunchKernelGGL(gj_prep_kernel, dim3((unsigned)(batch * st.items)), dim3(64), 0, ax->aux, n,
+                       K, PS, ns, batch, cur, Wc, w.piv, w.R);
+    hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)((long long)batch * st.items * nchunk)),
+                       dim3(64), 0, ax->aux, n, K, PS, ns, batch, nchunk, cur, nxt, Wc, w.R);
+    // join
+    if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
+      return Status::HIP_ERROR;
+    zc* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
+                     2 * n * sizeof(int), s, n, cur, w.piv, Ainv, ldinv, strideInv);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
 template <int PT, int RPT, int S, int PS>
 int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
-        long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s, int use_mfma) {
+        long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
   if (A != w.X || lda != n || strideA != (long long)n * n) {
     int rc = zcopy2d_launch(n, n, A, lda, strideA, w.X, n, (long long)n * n, batch, s);
     if (rc) return rc;
@@ -544,7 +727,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
   const int np = (ns + PS - 1) / PS;
   const long long wst = (long long)n * S * PS * batch;   // one W buffer (a panel)
   hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
-                     batch, w.X, w.W + wst, w.W, w.piv, w.used, info, 0);
+                     batch, w.X, w.X, w.W + wst, w.W, w.piv, w.used, info);
   for (int K = 0; K < np; ++K) {
     const int pk0 = PS * K, pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
     const int nk0 = PS * (K + 1), nk1 = (nk0 + PS) < ns ? nk0 + PS : ns;
@@ -557,8 +740,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
     const zc* Wc = w.W + (K & 1) * wst;
     zc* Wn = w.W + ((K + 1) & 1) * wst;
     hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3((unsigned)((long long)batch * m)),
-                       dim3(PT), 0, s, n, K, ns, batch, w.X, Wc, Wn, w.piv, w.used, info,
-                       use_mfma);
+                       dim3(PT), 0, s, n, K, ns, batch, w.X, w.X, Wc, Wn, w.piv, w.used, info);
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
                      2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
@@ -567,20 +749,20 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
 
 // Widest panel (strips x strip width) of any variant: the two W buffers hold
 // one panel each.
-int nb_for(int n) { return n <= 512 ? 32 : (n <= 1024 ? 16 : 8); }
+int nb_for(int n) { return n <= 512 ? 64 : (n <= 1024 ? 16 : 8); }
 
-// Panel variant (autotuner choice, negf_inv_variant): 0 = by batch size,
-// 1 = one strip per panel, 2 = panels of two strips (one pass over X per two
-// strips, but each factoring workgroup does twice the work).  Measured on
-// MI355X at n = 512: 64 instances 3.98 ms per inverse with two-strip panels
-// against 5.5 ms with one; a single instance 1.49 ms against 1.76 ms.  Auto:
-// two-strip panels from 32 instances up.  Strips are 16 columns up to
-// n = 512, 8 up to 1024, 4 beyond; one matrix row per thread up to n = 1024,
-// so a strip, its W rows and one column of R stay in registers.
-// Variants 3, 4, 5 are 0, 1, 2 with the update items on the matrix cores
-// (update_strip_mfma, n <= 512): 5.0 ms at 64 x 512 -- the kernel's 256
-// VGPRs force spills that serialise its loads, and one 512-thread workgroup
-// per CU leaves its phases unoverlapped; kept selectable for measurement.
+// Variant (autotuner choice, negf_inv_variant).  In place, one launch per
+// panel with the update items in gj_step_kernel: 1 = one-strip panels, 2 =
+// two-strip panels.  Split steps (run_split, strips of 16, n <= 512): 3, 4,
+// 5 = panels of 1, 2, 4 strips.  0 = automatic: split with four-strip panels
+// for 256 < n <= 512 and at least 32 instances, else in place with two-strip
+// panels from 32 instances up, one-strip below.  Measured on MI355X (ms per
+// batched inverse): 64 x 512: split/4 3.18, split/2 3.85, in place/2 3.95,
+// in place/1 5.48; 1 x 512: in place/1 1.48, in place/2 1.75, split/4 2.46;
+// 64 x 256: in place/2 0.79, split/2 0.86, split/4 1.04.  Strips are 16
+// columns up to n = 512, 8 up to 1024, 4 beyond; one matrix row per thread up
+// to n = 1024 (the factoring keeps a strip, its W rows and one column of R in
+// registers).
 int g_inv_variant = 0;
 
 }  // namespace
@@ -594,10 +776,11 @@ int zinv_set_variant(int v) {
 size_t zinv_work_bytes(int n, int batch) {
   const size_t nb = nb_for(n);
   size_t b = 0;
-  b += sizeof(zc) * (size_t)n * n * batch;   // X
+  b += 2 * sizeof(zc) * (size_t)n * n * batch;   // X, X2
   b += 2 * sizeof(zc) * (size_t)n * nb * batch;  // W (two buffers: panel J and J+1)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
-  return b + 1024;
+  b += sizeof(zc) * (size_t)((n + 15) / 16) * 1024 * batch;  // R: four 16 x 16 blocks per strip
+  return b + 2048;
 }
 
 void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
@@ -609,9 +792,11 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
     return r;
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
+  w->X2 = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
   w->W = (zc*)take(2 * sizeof(zc) * (size_t)n * nb * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
+  w->R = (zc*)take(sizeof(zc) * (size_t)((n + 15) / 16) * 1024 * batch);
 }
 
 int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
@@ -619,19 +804,31 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  const int pv = g_inv_variant % 3;
-  const int mf = g_inv_variant >= 3 ? 1 : 0;
-  const bool pairs = pv == 2 || (pv == 0 && batch >= 32);
-  if (!pairs) {
-    if (n <= 256) return run<256, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-    if (n <= 512) return run<512, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-    if (n <= 1024) return run<1024, 1, 8, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-    return run<1024, 2, 4, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+  const int v = g_inv_variant;
+  const bool split = n <= 512 && (v >= 3 || (v == 0 && n > 256 && batch >= 32));
+  if (split) {
+    const int ps = v == 3 ? 1 : (v == 4 ? 2 : 4);
+    if (n <= 256) {
+      if (ps == 1) return run_split<256, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+      if (ps == 2) return run_split<256, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+      return run_split<256, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    }
+    if (ps == 1) return run_split<512, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (ps == 2) return run_split<512, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    return run_split<512, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
   }
-  if (n <= 256) return run<256, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-  if (n <= 512) return run<512, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-  if (n <= 1024) return run<1024, 1, 8, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
-  return run<1024, 2, 4, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, mf);
+  // in place (v >= 3 with n > 512 falls back to the automatic panel choice)
+  const bool pairs = v == 2 || ((v == 0 || v >= 3) && batch >= 32);
+  if (!pairs) {
+    if (n <= 256) return run<256, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (n <= 512) return run<512, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (n <= 1024) return run<1024, 1, 8, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    return run<1024, 2, 4, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  }
+  if (n <= 256) return run<256, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 512) return run<512, 1, 16, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  if (n <= 1024) return run<1024, 1, 8, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+  return run<1024, 2, 4, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
 }
 
 }  // namespace negf

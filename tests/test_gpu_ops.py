@@ -95,6 +95,33 @@ def test_zinv_needs_pivoting(n):
     assert relfro(inv, np.linalg.inv(A)) < 1e-13
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("n,batch", [(300, 6), (512, 3), (64, 2)])
+def test_zinv_every_schedule(variant, n, batch):
+    """Every schedule of the batched inverse (in place with panels of 1 / 2
+    strips, split steps with panels of 1 / 2 / 4 strips, automatic) on ragged
+    (n = 300: a 12-column last strip, a partial last 64-row chunk) and full
+    shapes, against numpy."""
+    print(f"zinv variant={variant} n={n} batch={batch}", flush=True)
+    rng = np.random.default_rng(n + batch)
+    As = [_rand(rng, n, n) + (2 * np.sqrt(n) * np.eye(n) if k % 2 else 0) for k in range(batch)]
+    dA = torch.from_numpy(np.stack([a.ravel(order="F") for a in As])).to("cuda")
+    dI = torch.empty_like(dA)
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    call("negf_inv_variant", variant)
+    try:
+        call("negf_zinv_strided_batched", n, C.c_void_p(dA.data_ptr()), n, n * n,
+             C.c_void_p(dI.data_ptr()), n, n * n, batch, C.c_void_p(info.data_ptr()), None)
+        torch.cuda.synchronize()
+    finally:
+        call("negf_inv_variant", 0)
+    out = dI.cpu().numpy()
+    assert not info.cpu().numpy().any()
+    for k, a in enumerate(As):
+        inv = out[k].reshape(n, n).T
+        assert relfro(inv, np.linalg.inv(a)) < 1e-15 * np.linalg.cond(a) * max(n, 10)
+
+
 def test_zinv_singular_reports_info():
     print("zinv singular", flush=True)
     A = np.zeros((8, 8), complex, order="F")
