@@ -83,11 +83,9 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 // one per instance) and may be the same buffer as A when A may be destroyed.
 struct InvWork {
   zc* X;        // n*n*batch
-  zc* X2;       // n*n*batch: the other buffer of the out-of-place steps
   zc* W;        // 2*n*nb*batch (W of panel J and of panel J+1)
   int* piv;     // n*batch
   int* used;    // n*batch
-  zc* R;        // 1024*ceil(n/16)*batch: pivot blocks of the split steps' update items
 };
 size_t zinv_work_bytes(int n, int batch);
 // Schedule of the inverse (0: automatic, 1 / 2: in place with panels of 1 /

@@ -279,12 +279,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 template <int PT, int RPT, int S, int PS>
 __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, int batch,
                                                      zc* __restrict__ X,
-                                                     zc* __restrict__ Xout,
                                                      const zc* __restrict__ Wcur,
                                                      zc* __restrict__ Wnext,
                                                      int* __restrict__ piv,
                                                      int* __restrict__ used,
-                                                     int* __restrict__ info) {
+                                                     int* __restrict__ info, int prev_applied) {
   constexpr int NW = PT / 64;
   const int tid = threadIdx.x;
   const int pk0 = PS * K, pk1 = (pk0 + PS) < nstrips ? pk0 + PS : nstrips;   // panel K
@@ -306,7 +305,6 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
   }
   const int cat = k / batch, inst = k - cat * batch;
   zc* Xi = X + (long long)inst * n * n;
-  zc* Xo = Xout + (long long)inst * n * n;   // the factoring item's output (== Xi in place)
 
   __shared__ zc Rs[S][S];
   __shared__ double red_v[2][NW];
@@ -343,7 +341,7 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
   for (int s = nk0; s < nk1; ++s) {
     const int k0 = s * S, ncols = width(s);
     load_strip<PT, RPT, S>(Xi, n, k0, ncols, P);
-    for (int t = pk0; K >= 0 && t < pk1; ++t)   // panel K (none for K = -1)
+    for (int t = pk0; K >= 0 && !prev_applied && t < pk1; ++t)   // panel K (none for K = -1)
       apply_transform<PT, RPT, S>(n, P, W_of(Wcur, t - pk0), piv + (long long)inst * n + t * S,
                                   width(t), Rs);
     for (int t = nk0; t < s; ++t)     // panel K+1's earlier strips
@@ -370,7 +368,7 @@ __global__ __launch_bounds__(PT) void gj_step_kernel(int n, int K, int nstrips, 
             zc wv = P[q][c];
             if (r == pc[c]) wv.x -= 1.0;
             Wn[r + (long long)c * n] = wv;
-            Xo[r + (long long)(k0 + c) * n] = P[q][c];
+            Xi[r + (long long)(k0 + c) * n] = P[q][c];
           }
         }
       }
@@ -443,99 +441,32 @@ struct SplitStep {
   }
 };
 
-// Pivot blocks of every update item of a split step (one wave per
-// (instance, item)): R_t = X'[p_t, s] for the item's operations t = t_lo ..
-// pk1 - 1, X' = Xin after the operations before t, i.e.
-//   R_t = Xin[p_t, s] + sum_{t' < t} W_t'[p_t, :] R_t',
-// zero outside the strip and the pivots; stored as Rb[(inst items + item)
-// MAXPS + u][j][c].  Xin is not written during the step.
 constexpr int MAXPS = 4;
-__global__ __launch_bounds__(64) void gj_prep_kernel(int n, int K, int PS, int ns, int batch,
-                                                     const zc* __restrict__ Xin,
-                                                     const zc* __restrict__ Wcur,
-                                                     const int* __restrict__ piv,
-                                                     zc* __restrict__ Rb) {
-  constexpr int S = 16;
-  const SplitStep st(K, PS, ns);
-  const int inst = blockIdx.x % batch, it = blockIdx.x / batch;
-  int s, t_lo;
-  st.item(it, &s, &t_lo);
-  const int nt = st.pk1 - t_lo;
-  if (nt <= 0) return;
-  const int lane = threadIdx.x;
-  const int k0 = s * S, ncols = (n - k0) < S ? (n - k0) : S;
-  const zc* __restrict__ Xi = Xin + (long long)inst * n * n;
-  const int* __restrict__ pv = piv + (long long)inst * n;
-  zc* __restrict__ Ro = Rb + ((long long)inst * st.items + it) * MAXPS * S * S;
-  // rows padded to 17 entries: 16 lanes touch 16 rows of one column
-  __shared__ zc Rs[MAXPS][S][S + 1];
-  __shared__ zc Wg[S][S + 1];
-  for (int u = 0; u < nt; ++u) {
-    const int t = t_lo + u;
-    const int wt = (n - t * S) < S ? (n - t * S) : S;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int x = lane + 64 * m, j = x & 15, c = x >> 4;
-      zc v = zmake(0.0, 0.0);
-      if (j < wt && c < ncols) v = Xi[pv[t * S + j] + (k0 + c) * n];
-      Rs[u][j][c] = v;
-    }
-  }
-  __syncthreads();
-  // chain: block u gets the operations of blocks 0 .. u-1 in order
-  for (int u = 1; u < nt; ++u) {
-    const int tu = t_lo + u, wtu = (n - tu * S) < S ? (n - tu * S) : S;
-    for (int v = 0; v < u; ++v) {
-      const int tv = t_lo + v, wtv = (n - tv * S) < S ? (n - tv * S) : S;
-      const zc* __restrict__ Wv = Wcur + ((long long)inst * PS + (tv - st.pk0)) * n * S;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int x = lane + 64 * m, j = x & 15, q = x >> 4;
-        Wg[j][q] = (j < wtu && q < wtv) ? Wv[pv[tu * S + j] + q * n] : zmake(0.0, 0.0);
-      }
-      __syncthreads();
-      zc add[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int x = lane + 64 * m, j = x & 15, c = x >> 4;
-        double re = 0.0, im = 0.0;
-#pragma unroll
-        for (int q = 0; q < S; ++q) {
-          const zc wq = Wg[j][q];
-          const zc rq = Rs[v][q][c];
-          re = fma(wq.x, rq.x, fma(-wq.y, rq.y, re));
-          im = fma(wq.x, rq.y, fma(wq.y, rq.x, im));
-        }
-        add[m] = zmake(re, im);
-      }
-      __syncthreads();   // Wg reads done before the next gather
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int x = lane + 64 * m, j = x & 15, c = x >> 4;
-        Rs[u][j][c] = zadd(Rs[u][j][c], add[m]);
-      }
-    }
-  }
-  __syncthreads();
-  for (int u = 0; u < nt; ++u)
-    for (int x = lane; x < S * S; x += 64) Ro[u * S * S + x] = Rs[u][x / S][x % S];
-}
 
-// Update items of a split step, out of place, on the matrix cores: one wave
-// per (instance, item, 64-row chunk):
-//   Xout[rows, s] = Xin[rows, s] + sum_t W_t[rows, :] R_t
-// with the pivot blocks R_t from gj_prep_kernel, so the chunks of a strip are
-// independent and every load of a wave (its rows, W_t rows, R_t) is issued
-// before its first product.  The rows live transposed in the f64 MFMA
-// accumulator layout (lane l, element e of block b: row r0 + 16 b + l % 16,
-// column l / 16 + 4 e); each operation is X^T += R^T W^T as four real
-// products per four complex k (A = parts of R^T, B = parts of W^T).  Step
-// m covers complex k = 4 m + l / 16 for every lane, so each lane loads whole
-// complex numbers of W and R and no two lanes load the same one.  Xout gets
-// every strip but panel K+1's, which the factoring writes.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void gj_update_kernel(
-    int n, int K, int PS, int ns, int batch, int nchunk, const zc* __restrict__ Xin,
-    zc* __restrict__ Xout, const zc* __restrict__ Wcur, const zc* __restrict__ Rb) {
+// Update items of a split step, in place, one workgroup per (instance,
+// strip) owning the whole strip (wave w: rows 64 w .. 64 w + 63, n <= PT).
+// Strip s gets the row operations t = t_lo .. pk1 - 1 of panel K in order,
+//   X[:, s] += sum_t W_t R_t,  R_t = X'[p_t, s] (X' = X after the operations
+//   before t) = X[p_t, s] + sum_{t' < t} W_t'[p_t, :] R_t'.
+// The pivot rows come from the registers of the waves that own them (through
+// LDS), the chain is formed in LDS, and each wave applies the operations to
+// its rows on the matrix cores: rows held transposed in the accumulator
+// layout of v_mfma_f64_16x16x4_f64 (lane l, element e of block b: row
+// 64 w + 16 b + l % 16, column l / 16 + 4 e), four real products per four
+// complex k, step m covering k = 4 m + l / 16 so that each lane loads whole
+// complex numbers of W and R and no two lanes the same one; W and the strip
+// go through buffer operations (one VGPR of offset per lane) so that 4 waves
+// per SIMD fit in 128 VGPRs.  Items per
+// instance (next_panel = 0): the strips outside panels K and K+1 (all of
+// panel K's operations), then panel K's strips but its last (the operations
+// of the later strips of the panel); this runs beside the factoring of panel
+// K+1.  With next_panel = 1: panel K+1's strips (all of panel K's
+// operations), launched just before that factoring.
+template <int PT>
+__global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(4))) void
+gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
+                       zc* __restrict__ X, const zc* __restrict__ Wcur,
+                       const int* __restrict__ piv) {
   constexpr int S = 16;
   const SplitStep st(K, PS, ns);
   long long k = blockIdx.x;
@@ -543,48 +474,128 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void gj
     const long long x = k & 7, kk = k >> 3, per = batch >> 3;
     k = (kk / per) * batch + x + 8 * (kk % per);
   }
-  const int cat = (int)(k / batch), inst = (int)(k - (long long)cat * batch);
-  const int it = cat / nchunk, chunk = cat - it * nchunk;
+  const int it = (int)(k / batch), inst = (int)(k - (long long)it * batch);
   int s, t_lo;
-  st.item(it, &s, &t_lo);
-  const int nt = st.pk1 - t_lo;   // 0 .. PS
-  const int lane = threadIdx.x;
+  if (next_panel) {   // panel K+1's strips, ahead of their factoring
+    s = st.pk1 + it;
+    t_lo = st.pk0;
+  } else {
+    st.item(it, &s, &t_lo);
+  }
+  const int nt = st.pk1 - t_lo;   // >= 1 for every launched item
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int k0 = s * S, ncols = (n - k0) < S ? (n - k0) : S;
-  const int r0 = chunk * 64;
-  const zc* __restrict__ Xi = Xin + (long long)inst * n * n;
-  zc* __restrict__ Xo = Xout + (long long)inst * n * n;
+  zc* __restrict__ Xi = X + (long long)inst * n * n;
+  const int* __restrict__ pv = piv + (long long)inst * n;
   const zc* __restrict__ Wp = Wcur + (long long)inst * PS * n * S;
-  const zc* __restrict__ Ri = Rb + ((long long)inst * st.items + it) * MAXPS * S * S;
-
+  // rows padded to 17 entries: 16 threads touch 16 rows of one column.
+  // Wg[u (u - 1) / 2 + v][j][q] = W_v[p_u[j], q] for the chain pairs v < u.
+  constexpr int NPAIR = MAXPS * (MAXPS - 1) / 2;
+  __shared__ zc Rs[MAXPS][S][S + 1];
+  __shared__ zc Wg[NPAIR][S][S + 1];
+  __shared__ int pjs[MAXPS][S];
+  for (int x = tid; x < MAXPS * S; x += PT) {
+    const int u = x / S, j = x - u * S, t = t_lo + u;
+    const int wt = (n - t * S) < S ? (n - t * S) : S;
+    pjs[u][j] = (u < nt && j < wt) ? pv[t * S + j] : -1;
+  }
+  for (int x = tid; x < MAXPS * S * S; x += PT) {
+    const int u = x / (S * S), r = x - u * S * S;
+    Rs[u][r / S][r % S] = zmake(0.0, 0.0);
+  }
+  __syncthreads();
+  // every W block of the chain, issued together with the strip loads below
+  const int npair = nt * (nt - 1) / 2;
+  for (int x = tid; x < npair * S * S; x += PT) {
+    const int pr = x / (S * S), r = x - pr * S * S, j = r & 15, q = r >> 4;
+    int u = 1;
+    while ((u + 1) * u / 2 <= pr) ++u;
+    const int v = pr - u * (u - 1) / 2;
+    const int tv = t_lo + v, wtv = (n - tv * S) < S ? (n - tv * S) : S;
+    const int row = pjs[u][j];
+    Wg[pr][j][q] = (row >= 0 && q < wtv) ? Wp[(tv - st.pk0) * n * S + row + q * n] : zmake(0.0, 0.0);
+  }
+  // the strip through buffer operations (per-lane offset of row 64 w + l % 16,
+  // column k0 + l / 16; block b in the immediate, column group 4 e scalar)
+  const __amdgpu_buffer_rsrc_t xs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)Xi, (short)0, n * n * (int)sizeof(zc), 0x00020000);
+  const int xoff = (wave * 64 + li + (k0 + lg) * n) * (int)sizeof(zc);
   dbl4 are[4], aim[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int r = r0 + b * 16 + li;
+    const bool rok = wave * 64 + b * 16 + li < n;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int c = lg + 4 * e;
-      const zc v = (r < n && c < ncols) ? Xi[r + (k0 + c) * n] : zmake(0.0, 0.0);
-      are[b][e] = v.x;
-      aim[b][e] = v.y;
+      const bool ok = rok && lg + 4 * e < ncols;
+      const auto raw = __builtin_amdgcn_raw_buffer_load_b128(
+          xs, xoff + b * 16 * (int)sizeof(zc), ok ? 4 * e * n * (int)sizeof(zc) : 0, 0);
+      zc v;
+      __builtin_memcpy(&v, &raw, sizeof(zc));
+      are[b][e] = ok ? v.x : 0.0;
+      aim[b][e] = ok ? v.y : 0.0;
     }
   }
+  __syncthreads();
+  // the owners of the pivot rows publish them (values before this step)
+  for (int u = 0; u < nt; ++u)
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int p = pjs[u][j];
+      if (p >= 0 && (p >> 6) == wave && (p & 15) == li) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (((p & 63) >> 4) == b) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Rs[u][j][lg + 4 * e] = zmake(are[b][e], aim[b][e]);
+          }
+      }
+    }
+  __syncthreads();
+  // chain: block u gets the operations of blocks 0 .. u-1 (final by then)
+  for (int u = 1; u < nt; ++u) {
+    if (tid < S * S) {
+      const int j = tid & 15, c = tid >> 4;
+      double re = Rs[u][j][c].x, im = Rs[u][j][c].y;
+      for (int v = 0; v < u; ++v) {
+        const int pr = u * (u - 1) / 2 + v;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          const zc wq = Wg[pr][j][q];
+          const zc rq = Rs[v][q][c];
+          re = fma(wq.x, rq.x, fma(-wq.y, rq.y, re));
+          im = fma(wq.x, rq.y, fma(wq.y, rq.x, im));
+        }
+      }
+      Rs[u][j][c] = zmake(re, im);
+    }
+    __syncthreads();
+  }
+  // W_t through buffer loads: one VGPR of per-lane byte offset (row
+  // 64 w + l % 16, column l / 16), the 16-row block in the immediate offset
+  // and the column group 4 m in the scalar offset, so the 16 loads of a
+  // transform need no 64-bit address each (those spilled at 128 VGPRs).
+  const int voff = (wave * 64 + li + lg * n) * (int)sizeof(zc);
   for (int u = 0; u < nt; ++u) {
     const int t = t_lo + u;
     const int wt = (n - t * S) < S ? (n - t * S) : S;
     const zc* __restrict__ Wt = Wp + (t - st.pk0) * n * S;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, n * S * (int)sizeof(zc), 0x00020000);
     zc rr[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) rr[m] = Ri[u * S * S + (4 * m + lg) * S + li];
+    for (int m = 0; m < 4; ++m) rr[m] = Rs[u][4 * m + lg][li];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int r = r0 + b * 16 + li;
-      const bool rok = r < n;
+      const bool rok = wave * 64 + b * 16 + li < n;
       zc wv[4];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const int j = 4 * m + lg;
-        wv[m] = (rok && j < wt) ? Wt[r + j * n] : zmake(0.0, 0.0);
+        const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + b * 16 * (int)sizeof(zc),
+                                                               4 * m * n * (int)sizeof(zc), 0);
+        zc w;
+        __builtin_memcpy(&w, &raw, sizeof(zc));
+        wv[m] = (rok && 4 * m + lg < wt) ? w : zmake(0.0, 0.0);
       }
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
@@ -597,12 +608,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void gj
   }
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int r = r0 + b * 16 + li;
-    if (r < n) {
+    const bool rok = wave * 64 + b * 16 + li < n;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int c = lg + 4 * e;
-        if (c < ncols) Xo[r + (k0 + c) * n] = zmake(are[b][e], aim[b][e]);
+    for (int e = 0; e < 4; ++e) {
+      if (rok && lg + 4 * e < ncols) {
+        const zc v = zmake(are[b][e], aim[b][e]);
+        __attribute__((ext_vector_type(4))) unsigned int raw;
+        __builtin_memcpy(&raw, &v, sizeof(zc));
+        __builtin_amdgcn_raw_buffer_store_b128(raw, xs, xoff + b * 16 * (int)sizeof(zc),
+                                               4 * e * n * (int)sizeof(zc), 0);
       }
     }
   }
@@ -653,13 +667,17 @@ int aux_for(hipStream_t s, AuxSlot** out) {
   return Status::OK;
 }
 
-// Out-of-place steps with the update items in gj_update_kernel on a second
-// stream (S = 16): per panel K the factoring of panel K+1 (gj_step_kernel,
-// one workgroup per instance, Xin -> Xout) and the updates of every other
-// strip (Xin -> Xout) run concurrently; the buffers swap after each step.
+// Split steps in place.  Per panel K, on the caller's stream: panel K's
+// operations applied to panel K+1's strips on the matrix cores
+// (gj_strip_update_kernel, next_panel = 1), alone on the GPU, then the
+// factoring of panel K+1 (gj_step_kernel, one workgroup per instance, with
+// panel K's operations already applied); on a second stream, forked after
+// that first launch so it does not queue ahead of it: the updates of every
+// other strip (gj_strip_update_kernel).  The two streams touch disjoint
+// strips and both finish before the next step.
 template <int PT, int PS>
 int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
-              long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+                 long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
   constexpr int S = 16;
   AuxSlot* ax = nullptr;
   int rc = aux_for(s, &ax);
@@ -676,37 +694,34 @@ int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, lo
   }
   const int ns = (n + S - 1) / S;
   const int np = (ns + PS - 1) / PS;
-  const int nchunk = (n + 63) / 64;
   const long long wst = (long long)n * S * PS * batch;
   hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
-                     batch, w.X, w.X, w.W + wst, w.W, w.piv, w.used, info);
-  zc* cur = w.X;
-  zc* nxt = w.X2;
+                     batch, w.X, w.W + wst, w.W, w.piv, w.used, info, 0);
   for (int K = 0; K < np; ++K) {
-    const int pk0 = PS * K, pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
     const bool has_next = PS * (K + 1) < ns;
     const SplitStep st(K, PS, ns);
+    const int items = st.items - 1;   // panel K's last strip has no later operation
     const zc* Wc = w.W + (K & 1) * wst;
     zc* Wn = w.W + ((K + 1) & 1) * wst;
-    // fork: the update stream waits for everything before this step
+    // panel K+1's strips first, alone on the GPU (the factoring waits for
+    // them); then the fork, so the other strips' updates do not queue ahead
+    const int nnext = has_next ? (st.pk1 + PS < ns ? st.pk1 + PS : ns) - st.pk1 : 0;
+    if (has_next)
+      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nnext)),
+                         dim3(PT), 0, s, n, K, PS, ns, batch, 1, w.X, Wc, w.piv);
     if (hipEventRecord(ax->fork, s) != hipSuccess || hipStreamWaitEvent(ax->aux, ax->fork, 0) != hipSuccess)
       return Status::HIP_ERROR;
     if (has_next)
       hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, K, ns,
-                         batch, cur, nxt, Wc, Wn, w.piv, w.used, info);
-    hipLaunchKernelGGL(gj_prep_kernel, dim3((unsigned)(batch * st.items)), dim3(64), 0, ax->aux, n,
-                       K, PS, ns, batch, cur, Wc, w.piv, w.R);
-    hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)((long long)batch * st.items * nchunk)),
-                       dim3(64), 0, ax->aux, n, K, PS, ns, batch, nchunk, cur, nxt, Wc, w.R);
-    // join
+                         batch, w.X, Wc, Wn, w.piv, w.used, info, 1);
+    if (items > 0)
+      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * items)),
+                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, w.X, Wc, w.piv);
     if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
       return Status::HIP_ERROR;
-    zc* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
-                     2 * n * sizeof(int), s, n, cur, w.piv, Ainv, ldinv, strideInv);
+                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
@@ -727,7 +742,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
   const int np = (ns + PS - 1) / PS;
   const long long wst = (long long)n * S * PS * batch;   // one W buffer (a panel)
   hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
-                     batch, w.X, w.X, w.W + wst, w.W, w.piv, w.used, info);
+                     batch, w.X, w.W + wst, w.W, w.piv, w.used, info, 0);
   for (int K = 0; K < np; ++K) {
     const int pk0 = PS * K, pk1 = (pk0 + PS) < ns ? pk0 + PS : ns;
     const int nk0 = PS * (K + 1), nk1 = (nk0 + PS) < ns ? nk0 + PS : ns;
@@ -740,7 +755,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
     const zc* Wc = w.W + (K & 1) * wst;
     zc* Wn = w.W + ((K + 1) & 1) * wst;
     hipLaunchKernelGGL((gj_step_kernel<PT, RPT, S, PS>), dim3((unsigned)((long long)batch * m)),
-                       dim3(PT), 0, s, n, K, ns, batch, w.X, w.X, Wc, Wn, w.piv, w.used, info);
+                       dim3(PT), 0, s, n, K, ns, batch, w.X, Wc, Wn, w.piv, w.used, info, 0);
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
                      2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
@@ -757,9 +772,9 @@ int nb_for(int n) { return n <= 512 ? 64 : (n <= 1024 ? 16 : 8); }
 // 5 = panels of 1, 2, 4 strips.  0 = automatic: split with four-strip panels
 // for 256 < n <= 512 and at least 32 instances, else in place with two-strip
 // panels from 32 instances up, one-strip below.  Measured on MI355X (ms per
-// batched inverse): 64 x 512: split/4 3.18, split/2 3.85, in place/2 3.95,
-// in place/1 5.48; 1 x 512: in place/1 1.48, in place/2 1.75, split/4 2.46;
-// 64 x 256: in place/2 0.79, split/2 0.86, split/4 1.04.  Strips are 16
+// batched inverse): 64 x 512: split/4 2.98, split/2 3.80, in place/2 3.90,
+// in place/1 5.39; 1 x 512: in place/1 1.48, in place/2 1.75, split/4 1.96;
+// 64 x 256: in place/2 0.79, split/2 0.84, split/4 0.85.  Strips are 16
 // columns up to n = 512, 8 up to 1024, 4 beyond; one matrix row per thread up
 // to n = 1024 (the factoring keeps a strip, its W rows and one column of R in
 // registers).
@@ -776,11 +791,10 @@ int zinv_set_variant(int v) {
 size_t zinv_work_bytes(int n, int batch) {
   const size_t nb = nb_for(n);
   size_t b = 0;
-  b += 2 * sizeof(zc) * (size_t)n * n * batch;   // X, X2
+  b += sizeof(zc) * (size_t)n * n * batch;   // X
   b += 2 * sizeof(zc) * (size_t)n * nb * batch;  // W (two buffers: panel J and J+1)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
-  b += sizeof(zc) * (size_t)((n + 15) / 16) * 1024 * batch;  // R: four 16 x 16 blocks per strip
-  return b + 2048;
+  return b + 1024;
 }
 
 void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
@@ -792,11 +806,9 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
     return r;
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
-  w->X2 = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
   w->W = (zc*)take(2 * sizeof(zc) * (size_t)n * nb * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
-  w->R = (zc*)take(sizeof(zc) * (size_t)((n + 15) / 16) * 1024 * batch);
 }
 
 int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
