@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--wave", type=int, default=0, help="energies per wave (0 = from free memory)")
     ap.add_argument("--streams", type=int, default=1, help="concurrent energy groups")
+    ap.add_argument("--gemm-variant", type=int, default=None,
+                    help="ZGEMM tile variant (negf_gemm_variant; default: the library's)")
     ap.add_argument("--energies", type=int, default=256)
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--b", type=int, default=512)
@@ -257,6 +259,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.gemm_variant is not None:
+        from libnegf_amd._lib import call
+        call("negf_gemm_variant", a.gemm_variant)
     if a.workload == "ddrgf_uneven":
         ddrgf_uneven(a, torch, api)
         return
