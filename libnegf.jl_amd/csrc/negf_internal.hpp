@@ -114,6 +114,11 @@ int zlaswp_launch(int ncols, zc* B, long long ldb, long long strideB, const int*
 int zgetrs_launch(char trans, int n, int nrhs, const zc* A, long long lda, long long strideA,
                   const int* ipiv, long long strideP, zc* B, long long ldb, long long strideB,
                   int batch, hipStream_t s);
+// X = tri(A)^{-1} (uplo 'U' / 'L', diag 'N' / 'U'), out of place, X zero
+// outside the triangle; work: ztrtri_work_bytes.
+size_t ztrtri_work_bytes(int n, int batch);
+int ztrtri_launch(char uplo, char diag, int n, const zc* A, long long lda, long long strideA,
+                  zc* X, long long ldx, long long strideX, int batch, zc* work, hipStream_t s);
 int zgetri_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
                   long long strideP, int batch, zc* work, hipStream_t s);
 

@@ -2,7 +2,9 @@
 // pivoting (Block_factorization!, src/selected_inverse.jl:9-55) and the
 // Takahashi-style inversion of the assigned blocks (Block_inverse!, :139-206),
 // both in place and batched over instances.  Every block step is one batched
-// launch (getrf / trsm / MFMA gemm / getri) over all instances.
+// launch over all instances: getrf of the diagonal block, then its explicit
+// triangular inverses U_ii^{-1}, L_ii^{-1} (ztrtri), after which every
+// triangular solve of the reference (trsm, getri) is one MFMA GEMM.
 //
 // Differences from the reference, on purpose:
 //   * the diagonal blocks are factored WITHOUT pivoting by default, which is
@@ -18,6 +20,7 @@
 #include "negf_internal.hpp"
 #include "sweeps.hpp"
 
+#include <algorithm>
 #include <map>
 #include <set>
 #include <utility>
@@ -58,11 +61,6 @@ int gemm(char ta, char tb, zc alpha, const zc* A, const zc* B, zc beta, zc* C, c
     if (_rc) return _rc;   \
   } while (0)
 
-__global__ void iota1_kernel(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = i + 1;
-}
-
 __global__ void fold_info_kernel(const int* layer_info, int* out, int layer, int batch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < batch && layer_info[i] != 0 && out[i] == 0) out[i] = layer + 1;
@@ -90,8 +88,23 @@ std::vector<std::pair<int, int>> block_fill_pattern(int npl, const std::vector<s
 
 size_t selinv_work_bytes(int b, int batch, int max_row_nnz) {
   const size_t bb = (size_t)b * b * batch * sizeof(zc);
-  return bb * (2 * (size_t)max_row_nnz + 2) + sizeof(int) * ((size_t)b * batch + 2 * batch) + 8192;
+  // (block_inverse) 2 per off-diagonal of a row and the accumulator; (both)
+  // U_ii^{-1}, L_ii^{-1}, a product temporary and the triangular-inverse
+  // scratch; (block_factorization) the pivots and info words in front
+  return bb * (2 * (size_t)max_row_nnz + 1 + 3) + ztrtri_work_bytes(b, batch) +
+         sizeof(int) * ((size_t)b * batch + 2 * batch) + 8192;
 }
+
+namespace {
+// U_ii^{-1} and L_ii^{-1} (unit) of the factored diagonal block, into Ui / Li (ld b).
+int tri_inverses(const zc* Aii, long long ld, long long stride, int b, int batch, zc* Ui, zc* Li,
+                 zc* twork, hipStream_t s) {
+  const long long tb = (long long)b * b;
+  int rc = ztrtri_launch('U', 'N', b, Aii, ld, stride, Ui, b, tb, batch, twork, s);
+  if (rc) return rc;
+  return ztrtri_launch('L', 'U', b, Aii, ld, stride, Li, b, tb, batch, twork, s);
+}
+}  // namespace
 
 static int build(int npl, int b, int nnzb, const int* brow, const int* bcol, void* const* blocks,
                  long long ld, long long stride, int batch, Pattern* P) {
@@ -123,19 +136,43 @@ int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bc
           if (P.has(i, l) && !P.has(k, l)) return Status::BAD_ARG;
   int* ipiv = (int*)work;
   int* linfo = ipiv + (size_t)b * batch;
+  const long long tb = (long long)b * b;
+  char* zp = (char*)work + ((sizeof(int) * ((size_t)b * batch + 2 * batch) + 255) / 256) * 256;
+  zc* Ui = (zc*)zp;
+  zc* Li = Ui + tb * batch;
+  zc* Tm = Li + tb * batch;
+  zc* twork = Tm + tb * batch;
+  auto prod = [&](const zc* A, long long lda, long long sA, const zc* B, long long ldb,
+                  long long sB, zc* D, long long ldd, long long sD) {
+    GemmArgs g{};
+    g.opA = OP_N; g.opB = OP_N;
+    g.m = g.n = g.k = b;
+    g.alpha = ONE; g.beta = ZERO;
+    g.A = A; g.lda = lda; g.strideA = sA;
+    g.B = B; g.ldb = ldb; g.strideB = sB;
+    g.D = D; g.ldd = ldd; g.strideD = sD;
+    g.batch = batch;
+    return zgemm_launch(g, s);
+  };
   hipLaunchKernelGGL(zero_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
   for (int i = 0; i < npl; ++i) {
     zc* Aii = P.at(i, i);
     RC(zgetrf_launch(b, Aii, ld, stride, ipiv, b, linfo, batch, s, ref_pivoting));
     hipLaunchKernelGGL(fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, linfo, info, i,
                        batch);
+    // the triangular solves of :28-34 as products with the explicit inverses
+    bool any = false;
+    for (int j = i + 1; j < npl; ++j) any = any || P.has(j, i) || P.has(i, j);
+    if (any) RC(tri_inverses(Aii, ld, stride, b, batch, Ui, Li, twork, s));
     for (int j = i + 1; j < npl; ++j) {
-      if (P.has(j, i))
-        RC(ztrsm_launch('R', 'U', 'N', 'N', b, b, ONE, Aii, ld, stride, P.at(j, i), ld, stride,
-                        batch, s));
-      if (P.has(i, j))
-        RC(ztrsm_launch('L', 'L', 'N', 'U', b, b, ONE, Aii, ld, stride, P.at(i, j), ld, stride,
-                        batch, s));
+      if (P.has(j, i)) {   // L_ji = A_ji U_ii^{-1}
+        RC(prod(P.at(j, i), ld, stride, Ui, b, tb, Tm, b, tb));
+        RC(zcopy2d_launch(b, b, Tm, b, tb, P.at(j, i), ld, stride, batch, s));
+      }
+      if (P.has(i, j)) {   // U_ij = L_ii^{-1} A_ij
+        RC(prod(Li, b, tb, P.at(i, j), ld, stride, Tm, b, tb));
+        RC(zcopy2d_launch(b, b, Tm, b, tb, P.at(i, j), ld, stride, batch, s));
+      }
     }
     for (int k = i + 1; k < npl; ++k) {
       if (!P.has(i, k)) continue;
@@ -162,10 +199,6 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
   }
   const long long tb = (long long)b * b;   // temporaries: tight, ld b
   char* p = (char*)work;
-  int* ident = (int*)p;
-  p += ((sizeof(int) * (size_t)b + 255) / 256) * 256;
-  zc* getri_work = (zc*)p;
-  p += sizeof(zc) * tb * batch;
   zc* Aii_acc = (zc*)p;
   p += sizeof(zc) * tb * batch;
   std::vector<zc*> U(maxr), Lt(maxr);
@@ -175,7 +208,13 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
     Lt[r] = (zc*)p;
     p += sizeof(zc) * tb * batch;
   }
-  hipLaunchKernelGGL(iota1_kernel, dim3((b + 255) / 256), dim3(256), 0, s, ident, b);
+  zc* Ui = (zc*)p;
+  p += sizeof(zc) * tb * batch;
+  zc* Li = (zc*)p;
+  p += sizeof(zc) * tb * batch;
+  zc* Tm = (zc*)p;
+  p += sizeof(zc) * tb * batch;
+  zc* twork = (zc*)p;
   // temporaries use ld = b; the pattern's blocks use ld = P.ld
   auto gemm_mixed = [&](char ta, char tb_, zc alpha, const zc* A, long long lda, long long sA,
                         const zc* B, long long ldb, long long sB, zc beta, zc* C, long long ldc,
@@ -192,9 +231,15 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
     return zgemm_launch(g, s);
   };
   const long long bs = stride, tbs = tb;
-  RC(zgetri_launch(b, P.at(npl - 1, npl - 1), ld, bs, ident, 0, batch, getri_work, s));   // :158
+  // X_NN = (L U)^{-1} = U^{-1} L^{-1}  (:158)
+  {
+    zc* Ann = P.at(npl - 1, npl - 1);
+    RC(tri_inverses(Ann, ld, bs, b, batch, Ui, Li, twork, s));
+    RC(gemm_mixed('N', 'N', ONE, Ui, b, tbs, Li, b, tbs, ZERO, Ann, ld, bs));
+  }
   for (int i = npl - 2; i >= 0; --i) {
     zc* Aii = P.at(i, i);
+    RC(tri_inverses(Aii, ld, bs, b, batch, Ui, Li, twork, s));
     std::vector<int> js;
     for (int j = i + 1; j < npl; ++j) {
       if (!P.has(i, j)) continue;
@@ -209,8 +254,12 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
                       Lt[v], b, tbs));
         first = false;
       }
-      RC(ztrsm_launch('L', 'U', 'N', 'N', b, b, MONE, Aii, ld, bs, U[v], b, tbs, batch, s));  // :172
-      RC(ztrsm_launch('R', 'L', 'N', 'U', b, b, MONE, Aii, ld, bs, Lt[v], b, tbs, batch, s)); // :174
+      // U_j <- -U_ii^{-1} U_j (:172), L_j <- -L_j L_ii^{-1} (:174): products into the
+      // temporary, whose buffer then takes the operand's place
+      RC(gemm_mixed('N', 'N', MONE, Ui, b, tbs, U[v], b, tbs, ZERO, Tm, b, tbs));
+      std::swap(U[v], Tm);
+      RC(gemm_mixed('N', 'N', MONE, Lt[v], b, tbs, Li, b, tbs, ZERO, Tm, b, tbs));
+      std::swap(Lt[v], Tm);
       js.push_back(j);
     }
     bool first = true;
@@ -224,9 +273,20 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
       RC(zcopy2d_launch(b, b, U[v], b, tbs, P.at(i, j), ld, bs, batch, s));
     }
     if (first) RC(zfill_launch(b, b, ZERO, ZERO, Aii_acc, b, tbs, batch, s));
-    RC(ztrsm_launch('R', 'L', 'N', 'U', b, b, ONE, Aii, ld, bs, Aii_acc, b, tbs, batch, s));   // :195
-    RC(zgetri_launch(b, Aii, ld, bs, ident, 0, batch, getri_work, s));                        // :197
-    RC(zaxpy_launch(b, b, MONE, nullptr, 0, Aii_acc, b, tbs, Aii, ld, bs, batch, s));          // :199
+    // X_ii = U^{-1} L^{-1} - (sum_j X_ij L_ji) L^{-1}  (:195-199)
+    RC(gemm_mixed('N', 'N', ONE, Aii_acc, b, tbs, Li, b, tbs, ZERO, Tm, b, tbs));
+    {
+      GemmArgs g{};
+      g.opA = OP_N; g.opB = OP_N;
+      g.m = g.n = g.k = b;
+      g.alpha = ONE; g.beta = MONE;
+      g.A = Ui; g.lda = b; g.strideA = tbs;
+      g.B = Li; g.ldb = b; g.strideB = tbs;
+      g.C = Tm; g.ldc = b; g.strideC = tbs;
+      g.D = Aii; g.ldd = ld; g.strideD = bs;
+      g.batch = batch;
+      RC(zgemm_launch(g, s));
+    }
   }
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }

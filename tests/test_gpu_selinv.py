@@ -16,7 +16,7 @@ RIND = [1, 1, 1, 2, 3, 3, 3, 4, 5]
 CIND = [1, 3, 5, 2, 3, 4, 5, 4, 5]
 
 
-@pytest.mark.parametrize("b", [1, 100])
+@pytest.mark.parametrize("b", [1, 100, 200])
 def test_block_factorization_reference_case(b):
     print(f"selinv factorization b={b}", flush=True)
     npl = 5
@@ -34,7 +34,7 @@ def test_block_factorization_reference_case(b):
         assert relfro(blk, Fo[k]) < 1e-12
 
 
-@pytest.mark.parametrize("b", [1, 64])
+@pytest.mark.parametrize("b", [1, 64, 150])
 def test_block_inverse_reference_case(b):
     print(f"selinv inverse b={b}", flush=True)
     npl = 5
@@ -73,3 +73,25 @@ def test_selinv_tridiagonal_batched_matches_rgf():
         Xm = X.to_host(m)
         for k in G:
             assert relfro(Xm[k], G[k]) < 1e-10, (m, k)
+
+
+def test_selinv_production_block_size_matches_rgf():
+    """b = 512 (the triangular inverses' full recursion: 64-leaves merged at
+    64, 128 and 256) on the sweep's own workload, against the oracle's RGF."""
+    print("selinv tridiagonal N=6 b=512 vs rgf", flush=True)
+    from libnegf_amd.synthetic import device_hamiltonian, assemble_A
+    from oracle.rgf import bndiag_of_inv_rgf_local
+    npl, b = 6, 512
+    dev = device_hamiltonian(npl, b, seed=13)
+    A = assemble_A(dev, 0.3, 1e-3)
+    pat = [(i, i) for i in range(npl)] + [(i, i + 1) for i in range(npl - 1)] + \
+          [(i + 1, i) for i in range(npl - 1)]
+    T = si.BlockSparse(npl, b, pat, 1)
+    for (i, j), k in T.index.items():
+        blk = A.diag[i] if i == j else (A.upper[i] if j == i + 1 else A.lower[j])
+        T.data[0, k] = torch.from_numpy(np.asarray(blk).ravel(order="F")).cuda()
+    X = si.Block_inverse(si.Block_factorization(T)).to_host(0)
+    G = bndiag_of_inv_rgf_local(A)
+    err = max(relfro(X[k], G[k]) for k in G)
+    print(f"  max block relerr {err:.2e}", flush=True)
+    assert err < 1e-11
