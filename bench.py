@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="rgf_keldysh", choices=["rgf_keldysh", "ddrgf_uneven"],
+    ap.add_argument("--workload", default="rgf_keldysh",
+                    choices=["rgf_keldysh", "ddrgf_uneven", "single_energy"],
                     help="rgf_keldysh: the headline workload; ddrgf_uneven: three uneven devices "
                          "by DD-RGF (8 partitions each) vs RGF")
     ap.add_argument("--dd-energies", type=int, default=0, help="energies per device (ddrgf_uneven)")
@@ -144,6 +145,104 @@ def cpu_baseline(dev, E, eta, kT, muL, muR, gpu_out):
              "sample": f"1 energy (E={E:+.4f}) of the same workload: RGF + RKD + T through "
                        f"scipy/OpenBLAS 0.3.29, {dt:.1f} s"},
             {"max_block_relerr": float(err), "transmission_relerr": float(terr), "energy": E})
+
+
+def single_energy(a, torch, api):
+    """The single-energy drop-in at production shape: A(E) of the synthetic
+    device with N=64, b=512, E=0.3, eta=1e-4, one call of negf_rgf (band of
+    A^{-1}) and one selected inverse (Block_factorization + Block_inverse on
+    the tridiagonal pattern) of the same matrix, each timed over reps after a
+    warm-up; RGF also at 4 / 16 / 64 copies per call.  Errors: RGF vs the
+    oracle's RGF on the host, and the selected inverse vs the GPU RGF; the host
+    RGF's time is the CPU baseline."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, str(ROOT))
+    from libnegf_amd import synthetic, selinv as si
+    from libnegf_amd.btd import DeviceBlockTridiag
+    from libnegf_amd.flops import flops_rgf
+    N, b, E, eta = 64, 512, 0.3, 1e-4
+    dev = synthetic.device_hamiltonian(N, b, seed=1234)
+    A = synthetic.assemble_A(dev, E, eta)
+    reps = max(a.steps, 1)
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    rgf = {}
+    G1 = None
+    M1 = DeviceBlockTridiag.from_host(A)
+    for batch in (1, 4, 16, 64):
+        Min = DeviceBlockTridiag(M1.layout.b, batch, False, "cuda", zero=False)
+        Min.diag.copy_(M1.diag.expand(batch, -1))
+        Min.upper.copy_(M1.upper.expand(batch, -1))
+        Min.lower.copy_(M1.lower.expand(batch, -1))
+        Mout = Min.similar(zero=False)
+        aux = api.allocate_aux_data_RGF(Min)
+        t = timed(lambda: api.bndiag_of_inv_rgf_local(Mout, Min, aux))
+        rgf[batch] = {"ms_per_call": round(t * 1e3, 2), "ms_per_energy": round(t / batch * 1e3, 2),
+                      "credited_tflops": round(batch * flops_rgf(N, b) / t / 1e12, 2)}
+        print(f"# rgf batch {batch}: {t * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        if batch == 1:
+            G1 = Mout.to_host(0)
+        del Min, Mout, aux
+        torch.cuda.empty_cache()
+    pat = [(i, i) for i in range(N)] + [(i, i + 1) for i in range(N - 1)] + \
+          [(i + 1, i) for i in range(N - 1)]
+    T = si.BlockSparse(N, b, pat, 1)
+    for (i, j), k in T.index.items():
+        blk = A.diag[i] if i == j else (A.upper[i] if j == i + 1 else A.lower[j])
+        T.data[0, k] = torch.from_numpy(np.asarray(blk).ravel(order="F")).cuda()
+    F = T.with_pattern(si.fill_pattern(N, T.pattern))
+    tf = ti = 0.0
+    X = None
+    for r in range(reps + 1):
+        X = F.copy()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        si.Block_factorization_(X)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        si.Block_inverse_(X)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if r > 0:
+            tf += (t1 - t0) / reps
+            ti += (t2 - t1) / reps
+    print(f"# selected inverse {1e3 * (tf + ti):.1f} ms", file=sys.stderr, flush=True)
+    Xh = X.to_host(0)
+    band = {}
+    for i in range(N):
+        band[(i, i)] = G1.diag[i]
+    for i in range(N - 1):
+        band[(i, i + 1)] = G1.upper[i]
+        band[(i + 1, i)] = G1.lower[i]
+    rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    si_err = max(rel(Xh[k], band[k]) for k in band)
+    cpu = None
+    rgf_err = None
+    if not a.no_cpu_baseline:
+        from oracle.rgf import bndiag_of_inv_rgf_local
+        cores = min(os.cpu_count() or 1, 16)
+        with threadpool_limits(limits=cores, user_api="blas"):
+            t0 = time.perf_counter()
+            ref = bndiag_of_inv_rgf_local(A)
+            dt = time.perf_counter() - t0
+        rgf_err = max(rel(band[k], ref[k]) for k in ref)
+        cpu = {"value": round(1.0 / dt, 4), "unit": "energies/s", "cores": cores, "kind": "port",
+               "sample": f"the same energy's RGF through scipy/OpenBLAS 0.3.29, {dt:.2f} s"}
+    out = {"workload": "single_energy: N=64 b=512 E=0.3 eta=1e-4 (negf_rgf, selected inverse)",
+           "rgf": rgf, "selected_inverse_ms": round(1e3 * (tf + ti), 1),
+           "selected_inverse_split_ms": {"factorization": round(1e3 * tf, 1), "inverse": round(1e3 * ti, 1)},
+           "rgf_relerr_vs_cpu_oracle": rgf_err, "selinv_relerr_vs_gpu_rgf": si_err,
+           "reps": reps, "cpu_baseline": cpu}
+    print(json.dumps(out))
 
 
 def ddrgf_uneven(a, torch, api):
@@ -264,6 +363,9 @@ def main():
         call("negf_gemm_variant", a.gemm_variant)
     if a.workload == "ddrgf_uneven":
         ddrgf_uneven(a, torch, api)
+        return
+    if a.workload == "single_energy":
+        single_energy(a, torch, api)
         return
     cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
     N, b, nE = a.N, a.b, a.energies
