@@ -191,6 +191,49 @@ def test_energies_graph_replays_match_eager(streams):
         graph.close()
 
 
+def test_energies_graph_split_inverse_replays_match_eager():
+    """Waves of 32 energies at b = 320 take the batched inverse's split steps
+    (the factoring on the job's stream, the strip updates on the library's
+    second stream, joined by events every step); captured into a graph, the
+    fork/join becomes graph edges and replays must equal the eager call bit for
+    bit, and the oracle for a sampled energy."""
+    N, b, eta, kT, muL, muR = 3, 320, 1e-3, 0.025, 0.1, -0.1
+    nE = 32
+    print(f"energies graph split inverse N={N} b={b} wave={nE}", flush=True)
+    dev = device_hamiltonian(N, b, seed=19)
+    H = api.DeviceHamiltonianGPU(dev)
+    E_graph = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    outs = [torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda") for _ in range(2)]
+    T = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    graph = api.EnergiesGraph(H, E_graph, eta, muL, muR, kT, outs[0], outs[1], T, wave=nE)
+    try:
+        rng = np.random.default_rng(5)
+        for rep in range(2):
+            Es = torch.tensor(np.sort(rng.uniform(-1, 1, nE)), device="cuda")
+            GR = torch.zeros_like(outs[0])
+            GL = torch.zeros_like(outs[0])
+            Te = torch.zeros_like(T)
+            api.rgf_keldysh_energies(H, Es, eta, muL, muR, kT, GR, GL, Te, wave=nE)
+            torch.cuda.synchronize()
+            E_graph.copy_(Es)
+            for o in outs:
+                o.zero_()
+            T.zero_()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], GR), f"replay {rep}: G^R differs"
+            assert torch.equal(outs[1], GL), f"replay {rep}: G^< differs"
+            assert torch.equal(T, Te), f"replay {rep}: T differs"
+        k = 7
+        A = assemble_A(dev, float(Es[k]), eta)
+        G = rgf_ref(A)
+        for i in range(N):
+            blk = outs[0][k, i * b * b:(i + 1) * b * b].cpu().numpy().reshape(b, b).T
+            assert relfro(blk, G[(i, i)]) < 1e-12
+    finally:
+        graph.close()
+
+
 def test_ward_identity_on_gpu_outputs():
     """i (G - G^H) = G (2 eta I + Gamma_L + Gamma_R) G^H on the band, both
     sides from the GPU (RKD fed with the broadening, RGF for G)."""
