@@ -576,34 +576,44 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
   // and the column group 4 m in the scalar offset, so the 16 loads of a
   // transform need no 64-bit address each (those spilled at 128 VGPRs).
   const int voff = (wave * 64 + li + lg * n) * (int)sizeof(zc);
-  for (int u = 0; u < nt; ++u) {
+  // W of (operation u, block b) loaded one block ahead of its products
+  auto load_w = [&](int u, int b, zc (&w)[4]) {
     const int t = t_lo + u;
     const int wt = (n - t * S) < S ? (n - t * S) : S;
     const zc* __restrict__ Wt = Wp + (t - st.pk0) * n * S;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, n * S * (int)sizeof(zc), 0x00020000);
+    const bool rok = wave * 64 + b * 16 + li < n;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + b * 16 * (int)sizeof(zc),
+                                                             4 * m * n * (int)sizeof(zc), 0);
+      zc v;
+      __builtin_memcpy(&v, &raw, sizeof(zc));
+      w[m] = (rok && 4 * m + lg < wt) ? v : zmake(0.0, 0.0);
+    }
+  };
+  zc wcur[4], wnxt[4];
+  load_w(0, 0, wcur);
+  for (int u = 0; u < nt; ++u) {
     zc rr[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) rr[m] = Rs[u][4 * m + lg][li];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const bool rok = wave * 64 + b * 16 + li < n;
-      zc wv[4];
+      if (b < 3)
+        load_w(u, b + 1, wnxt);
+      else if (u + 1 < nt)
+        load_w(u + 1, 0, wnxt);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + b * 16 * (int)sizeof(zc),
-                                                               4 * m * n * (int)sizeof(zc), 0);
-        zc w;
-        __builtin_memcpy(&w, &raw, sizeof(zc));
-        wv[m] = (rok && 4 * m + lg < wt) ? w : zmake(0.0, 0.0);
+        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wcur[m].x, are[b], 0, 0, 0);
+        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wcur[m].y, aim[b], 0, 0, 0);
+        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-rr[m].y, wcur[m].y, are[b], 0, 0, 0);
+        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].y, wcur[m].x, aim[b], 0, 0, 0);
       }
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wv[m].x, are[b], 0, 0, 0);
-        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].x, wv[m].y, aim[b], 0, 0, 0);
-        are[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-rr[m].y, wv[m].y, are[b], 0, 0, 0);
-        aim[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[m].y, wv[m].x, aim[b], 0, 0, 0);
-      }
+      for (int m = 0; m < 4; ++m) wcur[m] = wnxt[m];
     }
   }
 #pragma unroll
