@@ -495,11 +495,13 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
   __shared__ zc Rs[MAXPS][S][S + 1];
   __shared__ zc Wg[NPAIR][S][S + 1];
   __shared__ int pjs[MAXPS][S];
+  __shared__ int rowslot[PT];   // rowslot[r] = 16 u + j if row r is pivot j of operation u
   for (int x = tid; x < MAXPS * S; x += PT) {
     const int u = x / S, j = x - u * S, t = t_lo + u;
     const int wt = (n - t * S) < S ? (n - t * S) : S;
     pjs[u][j] = (u < nt && j < wt) ? pv[t * S + j] : -1;
   }
+  rowslot[tid] = -1;
   for (int x = tid; x < MAXPS * S * S; x += PT) {
     const int u = x / (S * S), r = x - u * S * S;
     Rs[u][r / S][r % S] = zmake(0.0, 0.0);
@@ -521,6 +523,10 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
   const __amdgpu_buffer_rsrc_t xs =
       __builtin_amdgcn_make_buffer_rsrc((void*)Xi, (short)0, n * n * (int)sizeof(zc), 0x00020000);
   const int xoff = (wave * 64 + li + (k0 + lg) * n) * (int)sizeof(zc);
+  if (tid < MAXPS * S) {
+    const int pr = pjs[tid / S][tid % S];
+    if (pr >= 0) rowslot[pr] = tid;
+  }
   dbl4 are[4], aim[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
@@ -538,19 +544,15 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
   }
   __syncthreads();
   // the owners of the pivot rows publish them (values before this step)
-  for (int u = 0; u < nt; ++u)
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const int p = pjs[u][j];
-      if (p >= 0 && (p >> 6) == wave && (p & 15) == li) {
+  for (int b = 0; b < 4; ++b) {
+    const int r = wave * 64 + b * 16 + li;
+    const int sl = r < n ? rowslot[r] : -1;
+    if (sl >= 0) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (((p & 63) >> 4) == b) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) Rs[u][j][lg + 4 * e] = zmake(are[b][e], aim[b][e]);
-          }
-      }
+      for (int e = 0; e < 4; ++e) Rs[sl / S][sl % S][lg + 4 * e] = zmake(are[b][e], aim[b][e]);
     }
+  }
   __syncthreads();
   // chain: block u gets the operations of blocks 0 .. u-1 (final by then)
   for (int u = 1; u < nt; ++u) {
