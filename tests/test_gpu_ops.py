@@ -52,6 +52,43 @@ def test_zgemm(tA, tB, mnk):
     assert relfro(_host(dC), ref) < 1e-14
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tA", ["N", "T", "C"])
+@pytest.mark.parametrize("tB", ["N", "T", "C"])
+def test_zgemm_padded_ld_interior_and_edge_tiles(variant, tA, tB):
+    """Leading dimensions above the row count and per-instance strides, with
+    tiles wholly inside m x n (unchecked staging) next to ragged edge tiles
+    and a k that is, then is not, a multiple of the 16-deep K stage."""
+    rng = np.random.default_rng(17 + variant)
+    batch, pad = 2, 3
+    call("negf_gemm_variant", variant)
+    try:
+        for m, n, k in ((160, 200, 48), (256, 128, 80), (100, 70, 37)):
+            print(f"zgemm padded variant={variant} {tA}{tB} {(m, n, k)}", flush=True)
+            ra, ca = (m, k) if tA == "N" else (k, m)
+            rb, cb = (k, n) if tB == "N" else (n, k)
+            lda, ldb, ldc = ra + pad, rb + pad, m + pad
+            As = [_rand(rng, lda, ca) for _ in range(batch)]
+            Bs = [_rand(rng, ldb, cb) for _ in range(batch)]
+            Cs = [_rand(rng, ldc, n) for _ in range(batch)]
+            flat = lambda xs: torch.from_numpy(np.stack([x.ravel(order="F") for x in xs])).to("cuda")
+            dA, dB, dC = flat(As), flat(Bs), flat(Cs)
+            alpha, beta = 0.4 + 1.1j, 0.3 - 0.6j
+            call("negf_zgemm_strided_batched", tA.encode(), tB.encode(), m, n, k, cplx(alpha),
+                 C.c_void_p(dA.data_ptr()), lda, lda * ca, C.c_void_p(dB.data_ptr()), ldb, ldb * cb,
+                 cplx(beta), C.c_void_p(dC.data_ptr()), ldc, ldc * n, batch, None)
+            torch.cuda.synchronize()
+            out = dC.cpu().numpy()
+            for i in range(batch):
+                got = out[i].reshape(n, ldc).T
+                ref = alpha * _op(As[i][:ra, :], tA) @ _op(Bs[i][:rb, :], tB) + beta * Cs[i][:m, :]
+                assert relfro(got[:m, :], ref) < 1e-14
+                # rows past m inside the leading dimension stay untouched
+                assert np.array_equal(got[m:, :], Cs[i][m:, :])
+    finally:
+        call("negf_gemm_variant", 0)
+
+
 def test_zgemm_beta_zero_ignores_nan_and_batched_broadcast():
     print("zgemm batched broadcast", flush=True)
     rng = np.random.default_rng(3)
