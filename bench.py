@@ -100,7 +100,8 @@ def gemm_roofline(torch, api, b, batch, reps):
     dt = e0.elapsed_time(e1) / 1e3 / reps
     flops = 8.0 * b ** 3 * batch
     ach = flops / dt / 1e12
-    return {"kernel": "zgemm_mfma_kernel<N,N> (FP64 MFMA 16x16x4)", "bound": "mfma",
+    return {"kernel": "zgemm_lds_kernel<N,N,2> (FP64 MFMA 16x16x4, operands DMA'd into LDS)",
+            "bound": "mfma",
             "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "launch": f"{b}x{b}x{b} complex128 x {batch}", "avg_launch_us": round(dt * 1e6, 1),

@@ -68,7 +68,7 @@ struct GemmArgs {
   long long outerA, outerB, outerC, outerD;
 };
 int zgemm_launch(const GemmArgs& g, hipStream_t s);
-// Tile variant of the k > 32 GEMMs (0..4, see zgemm.hip), process-wide.
+// Tile variant of the k > 32 GEMMs (0..8, see zgemm.hip), process-wide.
 int zgemm_set_variant(int v);
 
 // Several independent GEMMs of one shape in one launch (fills the GPU when

@@ -23,6 +23,10 @@
 // doubles so the MFMA operand reads of two rows fall in disjoint banks.
 // op(A) / op(B) transposition and conjugation happen in the staging, so the
 // MFMA loop is identical for all nine N/T/C combinations.
+// Launches whose tiles all lie inside the output (the sweeps' b x b blocks
+// with b a multiple of 64) take zgemm_lds_kernel instead: the same tile and
+// MFMA loop, the operands loaded straight into LDS by the buffer unit
+// (no register staging, one barrier per 8-deep stage).
 //
 // Launch: one flat grid over (problem, instance, tile).  Workgroups are
 // dispatched round-robin over the 8 XCDs; the index is remapped so that the
@@ -48,6 +52,41 @@ struct GemmGroup {
   long long total;                // tiles_per_problem * count
   long long grid_padded;          // multiple of 8
 };
+
+// Epilogue of a wave's FM x FN MFMA blocks whose top-left corner is (ib, jb):
+// D = alpha acc + beta C (+ shift on the diagonal), rows < m and columns < n.
+// Output layout of the 16x16 f64 MFMA on gfx950 (measured): lane l, element
+// e holds row l/16 + 4*e, column l % 16.
+template <int FM, int FN>
+__device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long long i_in,
+                                           long long i_out, int ib, int jb, int lane,
+                                           const dbl4 (&acc_re)[FM][FN],
+                                           const dbl4 (&acc_im)[FM][FN]) {
+  const int kq = lane >> 4, li = lane & 15;
+  zc* __restrict__ D = g.D + i_in * g.strideD + i_out * g.outerD;
+  const zc* __restrict__ C = g.C ? g.C + i_in * g.strideC + i_out * g.outerC : nullptr;
+  const bool use_c = (g.beta.x != 0.0 || g.beta.y != 0.0) && C != nullptr;
+  zc sh = zmake(0.0, 0.0);
+  const bool use_shift = g.shift != nullptr;
+  if (use_shift) sh = g.shift[inst * g.shift_stride];
+#pragma unroll
+  for (int r = 0; r < FM; ++r)
+#pragma unroll
+    for (int c = 0; c < FN; ++c) {
+      const int j = jb + c * 16 + li;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = ib + r * 16 + kq + 4 * e;
+        if (i < g.m && j < g.n) {
+          const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
+          zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
+          if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
+          if (use_shift && i == j) out = zadd(out, sh);
+          D[i + j * g.ldd] = out;
+        }
+      }
+    }
+}
 
 // Tile configuration (FM, FN: MFMA blocks per wave; DB: double-buffered LDS).
 // SMALLK = 1 tags the instantiation used for k <= 32 (the rank-NB updates of
@@ -197,43 +236,171 @@ __global__ __launch_bounds__(256, OCC) void zgemm_mfma_kernel(
     }
   }
 
-  // Epilogue.  Output layout of the 16x16 f64 MFMA on gfx950 (measured):
-  // lane l, element e holds row l/16 + 4*e, column l % 16.
-  zc* __restrict__ D = g.D + i_in * g.strideD + i_out * g.outerD;
-  const zc* __restrict__ C = g.C ? g.C + i_in * g.strideC + i_out * g.outerC : nullptr;
-  const bool use_c = (g.beta.x != 0.0 || g.beta.y != 0.0) && C != nullptr;
-  zc sh = zmake(0.0, 0.0);
-  const bool use_shift = g.shift != nullptr;
-  if (use_shift) sh = g.shift[inst * g.shift_stride];
-#pragma unroll
-  for (int r = 0; r < FM; ++r)
-#pragma unroll
-    for (int c = 0; c < FN; ++c) {
-      const int j = j0 + wn * 16 * FN + c * 16 + li;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = i0 + wm * 16 * FM + r * 16 + kq + 4 * e;
-        if (i < m && j < n) {
-          const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
-          zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
-          if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
-          if (use_shift && i == j) out = zadd(out, sh);
-          D[i + j * g.ldd] = out;
-        }
-      }
-    }
+  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 16 * FM, j0 + wn * 16 * FN, lane, acc_re, acc_im);
 }
 
-// Tile variants (autotuner choice, negf_gemm_variant).  0: 64x64 single
-// buffer, 1: 64x64 double buffer, 2: 64x128 single, 3: 64x128 double,
-// 4: 128x64 double, 5: variant 0 held to 128 VGPRs (4 workgroups per CU, a
-// 512^3 x 64 launch then runs in exactly 4 rounds).  Measured on MI355X at
-// 512^3 x 64, per-instance operands, after warm-up (TFLOP/s, NN/NC/CN/CC):
-// 0: 62.0/62.8/61.2/61.0, 5: 62.8/60.0/63.0/56.3; variants 1-4 were 4-20 %
-// below variant 0 -- occupancy matters more than tile reuse at this FP64
-// MFMA rate.
+// Launches whose tiles all lie inside m x n (m, n multiples of 64, k a
+// multiple of LTK, each operand under 2 GiB): the same 64 x 64 tile and MFMA
+// loop, with the operands going straight from memory into LDS
+// (buffer_load_dwordx4 ... lds, 16 bytes per lane), LTK complex k per stage,
+// NBUF stages in flight and one barrier per stage -- no register staging and
+// no LDS stores.  LDS per stage and operand: [k][x] complex, x the tile's row
+// (A) or column (B), one 1 KB k-row per wave-wide load (lane l writes x = l).
+// Operands contiguous along x load one cache line per 8 lanes; those
+// contiguous along k one 16-byte piece per lane.  In the MFMA loop lane
+// (l % 16, kq = l / 16) reads component kq & 1 of complex k = 2 ks + kq / 2
+// (B'' the other component); the two 32-lane halves of each read fall in 64
+// distinct banks.  Conjugation is a per-lane sign.
+constexpr int LTK = 8;
+
+template <int OPA, int OPB, int NBUF>
+__global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
+  constexpr int FM = 2, FN = 2;
+  const long long bid = blockIdx.x;
+  const long long per = G.grid_padded >> 3;
+  const long long logical = (bid & 7) * per + (bid >> 3);
+  if (logical >= G.total) return;
+  const int pidx = (int)(logical / G.tiles_per_problem);
+  const long long rem = logical - (long long)pidx * G.tiles_per_problem;
+  const int tiles_inst = G.tiles_m * G.tiles_n;
+  const int inst = (int)(rem / tiles_inst);
+  const int tile = (int)(rem - (long long)inst * tiles_inst);
+  const int tm = tile % G.tiles_m, tn = tile / G.tiles_m;
+  const GemmArgs& g = G.p[pidx];
+  const int m = g.m, n = g.n, k = g.k;
+  const int i0 = tm * 64, j0 = tn * 64;
+  const long long i_in = g.inner > 0 ? inst % g.inner : inst;
+  const long long i_out = g.inner > 0 ? inst / g.inner : 0;
+  const zc* A = g.A + i_in * g.strideA + i_out * g.outerA;
+  const zc* B = g.B + i_in * g.strideB + i_out * g.outerB;
+  const int lda = (int)g.lda, ldb = (int)g.ldb;
+
+  __shared__ zc Ls[NBUF][2][LTK][64];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+
+  // byte extents of op(A) (rowsA x colsA, ld lda) and op(B); the host
+  // guarantees they fit in 31 bits, so every offset below does too
+  const int rowsA = OPA == OP_N ? m : k, colsA = OPA == OP_N ? k : m;
+  const int rowsB = OPB == OP_N ? k : n, colsB = OPB == OP_N ? n : k;
+  const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)A, (short)0, (rowsA - 1 + (colsA - 1) * lda + 1) * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)B, (short)0, (rowsB - 1 + (colsB - 1) * ldb + 1) * 16, 0x00020000);
+  // wave w loads k-rows 2w and 2w + 1 of each operand; lane offsets in bytes
+  int voA[2], voB[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int kk = 2 * wave + q;
+    voA[q] = (OPA == OP_N ? (i0 + lane + kk * lda) : (kk + (i0 + lane) * lda)) * 16;
+    voB[q] = (OPB == OP_N ? (kk + (j0 + lane) * ldb) : (j0 + lane + kk * ldb)) * 16;
+  }
+  auto issue = [&](int kt) {
+    const int buf = kt % NBUF, k0 = kt * LTK;
+    const int sA = OPA == OP_N ? k0 * lda * 16 : k0 * 16;
+    const int sB = OPB == OP_N ? k0 * 16 : k0 * ldb * 16;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int kk = 2 * wave + q;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsA, (__attribute__((address_space(3))) void*)&Ls[buf][0][kk][0], 16, voA[q], sA, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsB, (__attribute__((address_space(3))) void*)&Ls[buf][1][kk][0], 16, voB[q], sB, 0, 0);
+    }
+  };
+
+  dbl4 acc_re[FM][FN], acc_im[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      acc_re[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+      acc_im[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const int kq = lane >> 4, li = lane & 15, ri = kq & 1;
+  const double sgn = ri ? -1.0 : 1.0;   // the A- sign of K' row kq
+  // double offsets within a stage's operand: component ri (B'': 1 - ri)
+  const int offa = (wm * 32 + li) * 2 + ri;
+  const int offb1 = (wn * 32 + li) * 2 + ri, offb2 = (wn * 32 + li) * 2 + (1 - ri);
+  // Signs, all carried by the A operand so that B goes to the MFMA as loaded:
+  // Re term  A-[i,k'] B'[k',j]  = sgn^(1 + [A conj] + [B conj]) a_ri b_ri,
+  // Im term  A'[i,k'] B''[k',j] = (A conj ? sgn : 1)(B conj ? -sgn : 1) a_ri b_(1-ri)
+  // (conjugation negates the imaginary component, ri = 1 for A and B', ri = 0
+  // for B'').
+  constexpr bool re_sgn = ((1 + (OPA == OP_C) + (OPB == OP_C)) & 1) != 0;
+  constexpr bool im_sgn = (OPA == OP_C) != (OPB == OP_C);
+  constexpr bool im_neg = OPB == OP_C;
+
+  auto compute_stage = [&](int buf) {
+    const double* __restrict__ La = reinterpret_cast<const double*>(&Ls[buf][0][0][0]);
+    const double* __restrict__ Lb = reinterpret_cast<const double*>(&Ls[buf][1][0][0]);
+#pragma unroll
+    for (int ks = 0; ks < LTK / 2; ++ks) {
+      const int kr = (2 * ks + (kq >> 1)) * 128;   // doubles per k-row: 64 complex
+      double a[FM], am[FM], b1[FN], b2[FN];
+#pragma unroll
+      for (int r = 0; r < FM; ++r) {
+        const double v = La[kr + offa + 32 * r];
+        am[r] = re_sgn ? sgn * v : v;
+        const double vi = im_sgn ? sgn * v : v;
+        a[r] = im_neg ? -vi : vi;
+      }
+#pragma unroll
+      for (int c = 0; c < FN; ++c) {
+        b1[c] = Lb[kr + offb1 + 32 * c];
+        b2[c] = Lb[kr + offb2 + 32 * c];
+      }
+#pragma unroll
+      for (int r = 0; r < FM; ++r)
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[r], b1[c], acc_re[r][c], 0, 0, 0);
+          acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b2[c], acc_im[r][c], 0, 0, 0);
+        }
+    }
+  };
+
+  // stage kt is waited for (vmcnt counts this wave's loads, in order: those
+  // of the NBUF - 2 later stages may stay in flight), then one barrier makes
+  // it visible and frees the buffer of stage kt - 1 for stage kt + NBUF - 1
+  const int nk = k / LTK;
+#pragma unroll
+  for (int p = 0; p < NBUF - 1; ++p)
+    if (p < nk) issue(p);
+  for (int kt = 0; kt < nk; ++kt) {
+    const int ahead = (nk - 1 - kt) < (NBUF - 2) ? (nk - 1 - kt) : (NBUF - 2);
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(0xF78);        // vmcnt(8)
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(0xF74);   // vmcnt(4)
+    else __builtin_amdgcn_s_waitcnt(0xF70);                   // vmcnt(0)
+    // a bare s_barrier: __syncthreads()' fence would wait for every load in
+    // flight (vmcnt(0)), the later stages included; the clobbers keep the
+    // compiler from moving LDS accesses across it
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (kt + NBUF - 1 < nk) issue(kt + NBUF - 1);
+    compute_stage(kt % NBUF);
+  }
+  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 32, j0 + wn * 32, lane, acc_re, acc_im);
+}
+
+// Tile variants (autotuner choice, negf_gemm_variant).  0: zgemm_lds_kernel
+// with two stages (double-buffered LDS) where the whole launch qualifies,
+// else 64x64 register-staged; 1: 64x64 register-staged, double buffer,
+// 2: 64x128 single, 3: 64x128 double, 4: 128x64 double, 5: 64x64 register-
+// staged held to 128 VGPRs (4 workgroups per CU); 6, 7: zgemm_lds_kernel with
+// three, four stages; 8: 64x64 register-staged, single buffer, everywhere.
+// Measured on MI355X at 512^3 x 64, per-instance operands, after warm-up
+// (TFLOP/s, NN/NC/CN/CC): 0: 69.5/69.2/69.0/68.6, 6: 69.0/68.9/68.6/68.1,
+// 8: 62.1/62.7/61.3/61.0; variants 1-5 were at or below 8 (occupancy
+// matters more than tile reuse at this FP64 MFMA rate).  MFMA issue alone
+// (register operands, no memory) reaches 70.5 here.
 int g_variant = 0;
-constexpr int NVAR = 6;
+constexpr int NVAR = 9;
 
 template <int OPA, int OPB, int FM, int FN, bool DB, int SK, int OCC = (FM * FN <= 4 && !DB) ? 2 : 1>
 void launch_cfg(GemmGroup G, hipStream_t s) {
@@ -246,14 +413,46 @@ void launch_cfg(GemmGroup G, hipStream_t s) {
                      dim3(256), 0, s, G);
 }
 
+// Every tile of the group inside m x n, whole LTK stages, and 31-bit byte
+// offsets into each operand (the buffer loads of zgemm_lds_kernel).
+bool lds_eligible(const GemmGroup& G) {
+  const GemmArgs& g0 = G.p[0];
+  if (g0.m % 64 || g0.n % 64 || g0.k % LTK) return false;
+  for (int i = 0; i < G.count; ++i) {
+    const GemmArgs& g = G.p[i];
+    const long long rowsA = g.opA == OP_N ? g.m : g.k, colsA = g.opA == OP_N ? g.k : g.m;
+    const long long rowsB = g.opB == OP_N ? g.k : g.n, colsB = g.opB == OP_N ? g.n : g.k;
+    if ((rowsA + (colsA - 1) * g.lda) * 16 >= (1LL << 31)) return false;
+    if ((rowsB + (colsB - 1) * g.ldb) * 16 >= (1LL << 31)) return false;
+  }
+  return true;
+}
+
+template <int OPA, int OPB, int NBUF>
+void launch_lds(GemmGroup G, hipStream_t s) {
+  G.tiles_m = G.p[0].m / 64;
+  G.tiles_n = G.p[0].n / 64;
+  G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * G.p[0].batch;
+  G.total = G.tiles_per_problem * G.count;
+  G.grid_padded = (G.total + 7) / 8 * 8;
+  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF>), dim3((unsigned)G.grid_padded), dim3(256), 0,
+                     s, G);
+}
+
 template <int OPA, int OPB>
 void launch_t(const GemmGroup& G, hipStream_t s) {
   if (G.p[0].k <= 2 * TK) {
     launch_cfg<OPA, OPB, 2, 2, false, 1>(G, s);
     return;
   }
+  if ((g_variant == 0 || g_variant == 6 || g_variant == 7) && lds_eligible(G)) {
+    if (g_variant == 6) launch_lds<OPA, OPB, 3>(G, s);
+    else if (g_variant == 7) launch_lds<OPA, OPB, 4>(G, s);
+    else launch_lds<OPA, OPB, 2>(G, s);
+    return;
+  }
   switch (g_variant) {
-    case 0: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
+    case 0: case 6: case 7: case 8: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
     case 1: launch_cfg<OPA, OPB, 2, 2, true, 0>(G, s); break;
     case 2: launch_cfg<OPA, OPB, 2, 4, false, 0>(G, s); break;
     case 3: launch_cfg<OPA, OPB, 2, 4, true, 0>(G, s); break;

@@ -52,7 +52,7 @@ def test_zgemm(tA, tB, mnk):
     assert relfro(_host(dC), ref) < 1e-14
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("tA", ["N", "T", "C"])
 @pytest.mark.parametrize("tB", ["N", "T", "C"])
 def test_zgemm_padded_ld_interior_and_edge_tiles(variant, tA, tB):
