@@ -80,7 +80,10 @@ def copy_calibration(torch, nbytes=1 << 30):
 def gemm_roofline(torch, api, b, batch, reps):
     """Average duration of the batched FP64-MFMA ZGEMM at the sweep's shape
     (b x b x b, `batch` instances, one operand broadcast like the shared
-    coupling blocks), timed with HIP events on the launching stream."""
+    coupling blocks), timed with HIP events on the launching stream after 40
+    untimed launches: the clocks ramp over the first ~20 launches of this size
+    (1.29 ms down to 1.0 ms each from an idle GPU), and the timed launches
+    should see them as the sweep does."""
     from libnegf_amd._lib import call, cplx
     s = torch.cuda.current_stream()
     A = torch.randn((b * b,), dtype=torch.complex128, device="cuda")
@@ -89,7 +92,7 @@ def gemm_roofline(torch, api, b, batch, reps):
     args = lambda: (b"N", b"N", b, b, b, cplx(1), C.c_void_p(A.data_ptr()), b, 0,
                     C.c_void_p(B.data_ptr()), b, b * b, cplx(0), C.c_void_p(D.data_ptr()), b,
                     b * b, batch, C.c_void_p(s.cuda_stream))
-    for _ in range(3):
+    for _ in range(40):
         call("negf_zgemm_strided_batched", *args())
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
