@@ -46,10 +46,12 @@ def parse():
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="rgf_keldysh",
-                    choices=["rgf_keldysh", "ddrgf_uneven", "single_energy"],
+                    choices=["rgf_keldysh", "ddrgf_uneven", "single_energy", "cpu_pool"],
                     help="rgf_keldysh: the headline workload; ddrgf_uneven: three uneven devices "
                          "by DD-RGF (8 partitions each) vs RGF")
     ap.add_argument("--dd-energies", type=int, default=0, help="energies per device (ddrgf_uneven)")
+    ap.add_argument("--procs", type=int, default=0,
+                    help="host processes of the cpu_pool workload (0 = min(cores, 16))")
     ap.add_argument("--graph", action="store_true",
                     help="capture the energy job once into a HIP graph and time its replays")
     ap.add_argument("--gemm-reps", type=int, default=20)
@@ -350,8 +352,83 @@ def ddrgf_uneven(a, torch, api):
     print(json.dumps(out))
 
 
+def _pool_energy(idx, E, N, b, barrier, queue):
+    """One worker of the cpu_pool workload: the oracle's RGF + RKD + T for one
+    energy with one BLAS thread, timed between a common start barrier and its
+    own end."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, str(ROOT))
+    sys.path.insert(0, str(ROOT / "libnegf.jl_amd"))
+    from oracle.rgf import bndiag_of_inv_rgf_local, rgf_upward, first_row_chain
+    from oracle.keldysh import keldyshndiag
+    from libnegf_amd import synthetic
+    cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
+    dev = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
+    with threadpool_limits(limits=1, user_api="blas"):
+        barrier.wait()
+        t0 = time.time()
+        A = synthetic.assemble_A(dev, E, cfg["eta"])
+        buf = rgf_upward(A)
+        G = bndiag_of_inv_rgf_local(A, upward=buf)
+        Sd = [np.zeros((b, b), complex, order="F") for _ in range(N)]
+        Sd[0] = np.asfortranarray(synthetic.fermi(E, cfg["mu_L"], cfg["kT"]) * dev.gamma_L)
+        Sd[-1] = np.asfortranarray(Sd[-1] + synthetic.fermi(E, cfg["mu_R"], cfg["kT"]) * dev.gamma_R)
+        Su = [np.zeros((b, b), complex, order="F") for _ in range(N - 1)]
+        keldyshndiag(A, Sd, Su, upward=buf)
+        g1n = first_row_chain(A, G, buf)[-1]
+        T = float(np.real(np.trace(dev.gamma_L @ g1n @ dev.gamma_R @ g1n.conj().T)))
+        t1 = time.time()
+    queue.put((idx, t0, t1, T))
+
+
+def cpu_pool(a):
+    """The oracle on the host with energies spread over P processes, one BLAS
+    thread each (the shape of the reference's own benchmark runs,
+    benchmrk/runbenchmrks.sh:44,47): P energies of the headline grid, one per
+    process, started together; energies/s = P / (last end - first start).
+    Runs before anything touches the GPU (the workers are spawned)."""
+    import multiprocessing as mp
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "libnegf.jl_amd"))
+    from libnegf_amd import synthetic
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
+    P = a.procs or min(os.cpu_count() or 1, 16)
+    Es = synthetic.energy_grid(a.energies, cfg["lo"], cfg["hi"])
+    pick = [Es[int(k)] for k in np.linspace(0, len(Es) - 1, P)]
+    ctx = mp.get_context("spawn")
+    barrier, queue = ctx.Barrier(P), ctx.Queue()
+    procs = [ctx.Process(target=_pool_energy, args=(i, float(E), a.N, a.b, barrier, queue))
+             for i, E in enumerate(pick)]
+    for pr in procs:
+        pr.start()
+    res = [queue.get() for _ in procs]
+    for pr in procs:
+        pr.join()
+        if pr.exitcode != 0:
+            print(f"cpu_pool worker exited with {pr.exitcode}", file=sys.stderr)
+            sys.exit(1)
+    t0, t1 = min(r[1] for r in res), max(r[2] for r in res)
+    per = sorted(r[2] - r[1] for r in res)
+    print(json.dumps({
+        "workload": f"cpu_pool: N={a.N} b={a.b}, {P} energies of linspace(-1,1) ({a.energies} points), "
+                    f"one per process",
+        "cpu_baseline": {"value": round(P / (t1 - t0), 5), "unit": "energies/s", "cores": P,
+                         "kind": "port",
+                         "sample": f"{P} energies of the same workload, RGF + RKD + T through "
+                                   f"scipy/OpenBLAS 0.3.29, one process and one BLAS thread each, "
+                                   f"{t1 - t0:.1f} s wall"},
+        "seconds_per_energy": {"min": round(per[0], 2), "median": round(per[len(per) // 2], 2),
+                               "max": round(per[-1], 2)},
+        "transmission": {f"{pick[r[0]]:+.4f}": r[3] for r in sorted(res)}}))
+
+
 def main():
     a = parse()
+    if a.workload == "cpu_pool":
+        cpu_pool(a)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); energies shard "
               "across GPUs with no communication, see the design notes", file=sys.stderr)
