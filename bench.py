@@ -8,7 +8,7 @@ diagonal blocks and T(E) for all 256 energies, outputs resident in HBM.
 Inputs (H, Gamma, energies) are generated by the seeded generator and moved
 to HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--wave 64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--wave 96]
 Prints one JSON line.  Extra fields: `roofline` for the dominant kernel (the
 FP64-MFMA batched ZGEMM, timed with HIP events on the stream it runs on, at
 the sweep's shape) and `cpu_baseline` (the scipy/OpenBLAS restatement of the
@@ -484,9 +484,15 @@ def main():
     free, total = torch.cuda.mem_get_info()
     wave, streams = a.wave, a.streams
     if wave == 0:
-        wave = min(-(-nE // streams), 64)
-        while wave > 1 and api.energies_workspace_bytes(N, b, nE, wave, streams) > 0.85 * free:
-            wave //= 2
+        # the largest wave whose workspace fits: waves of 96 beat 64 by ~1 % at
+        # b = 512 (the inverse's factoring, one workgroup per energy, is
+        # latency-bound, so more energies per step share its latency)
+        cap = -(-nE // streams)
+        wave = 1
+        for w in (96, 64, 48, 32, 16, 8, 4, 2, 1):
+            if w <= cap and api.energies_workspace_bytes(N, b, nE, w, streams) <= 0.85 * free:
+                wave = w
+                break
     print(f"# N={N} b={b} energies={nE} wave={wave} streams={streams} free={free/2**30:.1f}GiB",
           file=sys.stderr,
           flush=True)
@@ -521,7 +527,7 @@ def main():
     eps = nE / dt
     fl = flops_rgf_keldysh_energy(N, b)
     roof = gemm_roofline(torch, api, b, wave, a.gemm_reps)
-    tfile = ROOT / "profiles" / "zgemm_512_x64_hbm_traffic.json"
+    tfile = ROOT / "profiles" / f"zgemm_{b}_x{wave}_hbm_traffic.json"
     if tfile.exists():
         tj = json.loads(tfile.read_text())
         if tj.get("b") == b and tj.get("batch") == wave:

@@ -188,8 +188,9 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes,
  *   transmission: T(E) = Tr(Gamma_L G^R_{1N} Gamma_R G^A_{N1})  (or NULL)
  *   f(E) = 1 / (1 + exp((E - mu) / kT)).
  * The grid is split into `streams` contiguous groups (0 = auto, at most 4),
- * each run on its own internal stream in waves of `wave` energies (0 = pick
- * from free memory), so one group's latency-bound steps overlap another's
+ * each run on its own internal stream in waves of `wave` energies (0 = the
+ * largest of 96, 64, 48, 32, ... whose workspace fits 90 % of free memory),
+ * so one group's latency-bound steps overlap another's
  * GEMMs; `stream` waits for all of them.  info: device int32[nE] or NULL
  * (layer of the first singular Schur complement). */
 int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
