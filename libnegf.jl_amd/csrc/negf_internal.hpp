@@ -63,6 +63,8 @@ struct GemmArgs {
   const zc* C; long long ldc, strideC;
   zc* D; long long ldd, strideD;
   const zc* shift; long long shift_stride;
+  // optional per-instance factor of the product: alpha * scale[idx * scale_stride]
+  const zc* scale; long long scale_stride;
   int batch;
   int inner;
   long long outerA, outerB, outerC, outerD;

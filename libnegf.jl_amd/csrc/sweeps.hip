@@ -42,7 +42,8 @@ struct MW {
 inline M ro(const MW& w) { return M{w.p, w.ld, w.bs}; }
 
 int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D, int batch,
-         hipStream_t s, const zc* shift = nullptr, long long shift_stride = 0) {
+         hipStream_t s, const zc* shift = nullptr, long long shift_stride = 0,
+         const zc* scale = nullptr, long long scale_stride = 0) {
   GemmArgs g{};
   g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
   g.alpha = alpha; g.beta = beta;
@@ -51,6 +52,7 @@ int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C
   g.C = C.p; g.ldc = C.ld; g.strideC = C.bs;
   g.D = D.p; g.ldd = D.ld; g.strideD = D.bs;
   g.shift = shift; g.shift_stride = shift_stride;
+  g.scale = scale; g.scale_stride = scale_stride;
   g.batch = batch;
   return zgemm_launch(g, s);
 }
@@ -393,8 +395,10 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
 // layers only (its off-diagonal blocks are zero on entry), so the reference's
 // upward step (src/keldyshndiag.jl:230-234) reduces to Su_i = -t_i Sd_{i+1}
 // feeding Sd_i -= Su_i t_i^H only, i.e. Sd_i += t_i Sd_{i+1} t_i^H.  The step
-// is linear in Sigma; the driver runs it on Gamma_R alone (unscaled):
-// B_j = t_j B_{j+1} t_j^H, B_{N-1} = Gamma_R.  Its B_0 gives the transmission
+// is linear in Sigma; the driver runs it on Gamma_R alone (unscaled), so
+// Sd_i is zero before its own step for i < N-1 and the step writes
+// B_j = t_j B_{j+1} t_j^H over whatever Sd_j holds; Sd_{N-1} = Gamma_R on
+// entry.  Its B_0 gives the transmission
 // T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R
 // G_{0,N-1}^H), G_{0,N-1} = G_00 prod(-t)), and f_R B_j is the right-injected
 // part of G^n (rkd_inject).
@@ -411,7 +415,7 @@ static int rkd_contacts_up(const Layout& L, const SweepWork& w, int batch, zc* S
     const int bi = L.b[i], bj = L.b[i + 1];
     MW T1{w.T1, bi, xbs};
     RC(gemm(OP_N, OP_N, bi, bj, bj, ONE, t(i), ro(sd(i + 1)), ZERO, none, T1, batch, s));
-    RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ONE, ro(sd(i)), sd(i), batch, s));
+    RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ZERO, none, sd(i), batch, s));
   }
   return Status::OK;
 }
@@ -434,18 +438,18 @@ static int transmission_from_b0(const Layout& L, const SweepWork& w, int batch, 
 // RGF gives both columns through its own recursions: G_j0 = -l_j G_j-1,0, so
 // C_j = G_j0 (f_L Gamma_L) G_j0^H = l_j C_j-1 l_j^H, C_0 = G_00 (f_L Gamma_L)
 // G_00^H; and G_j,N-1 = G_jj prod_k>=j (-t_k), so the right term is
-// G_jj B_j G_jj^H with B_j = t_j B_j+1 t_j^H, B_N-1 = Gamma_R (the upward
-// step, already in Sd, scaled by f_R).  Two sums of Hermitian positive
+// f_R G_jj B_j G_jj^H with B_j = t_j B_j+1 t_j^H, B_N-1 = Gamma_R (the upward
+// step, already in Sd, unscaled).  Two sums of Hermitian positive
 // semidefinite terms, no cancellation.  Per layer: Q = l_j C_j-1, C_j =
-// Q l_j^H, V = G_jj B_j, G^<_jj = i V G_jj^H + i C_j (the last GEMM writes the
-// output).  The same G^n as the reference's passes (src/keldyshndiag.jl:
+// Q l_j^H, V = f_R G_jj B_j (f_R per energy in the GEMM's epilogue), G^<_jj =
+// i V G_jj^H + i C_j (the last GEMM writes the output).  The same G^n as the reference's passes (src/keldyshndiag.jl:
 // 221-318) for this Sigma^n, in 4 GEMMs per layer instead of their 5 after
 // the upward step; with RGF (5) and the upward step (2), 11 GEMMs and one
 // inverse per layer for G^R, G^< and T (the reference's sequence: 21 and
 // three LU solves).
 static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* Sd,
                       long long sd_bs, const zc* Gd, long long gd_bs, const zc* gammaL,
-                      const zc* fL, zc* Gless, long long gl_bs, hipStream_t s) {
+                      const zc* fL, const zc* fR, zc* Gless, long long gl_bs, hipStream_t s) {
   const int N = L.N;
   const long long xbs = (long long)L.bmax * L.bmax;
   const zc I1 = zmake(0.0, 1.0);
@@ -469,7 +473,7 @@ static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* 
               batch, s));
     }
     RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), M{Sd + L.od[j], bj, sd_bs}, ZERO, none,
-            MW{w.T1, bj, xbs}, batch, s));
+            MW{w.T1, bj, xbs}, batch, s, nullptr, 0, fR, 1));
     RC(gemm(OP_N, OP_C, bj, bj, bj, I1, M{w.T1, bj, xbs}, G(j), I1, M{w.T2, bj, xbs},
             MW{Gless + L.od[j], bj, gl_bs}, batch, s));
   }
@@ -535,16 +539,15 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     if (out.Gless_diag) {
       // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere (its
       // off-diagonal blocks are zero and never stored).  The upward step runs
-      // on Gamma_R alone (B_j), which also yields T; then B_j is scaled by f_R
-      // and G^< = i G^n written layer by layer in injection form.
-      RC(zfill_launch((int)L.nd, 1, ZERO, ZERO, Sdw, L.nd, L.nd, nb, s));
-      RC(zaxpy_launch(b, b, ONE, nullptr, 0, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
+      // on Gamma_R alone (B_j, written layer by layer from B_N-1 = Gamma_R, so
+      // no other block needs clearing), which also yields T; then G^< = i G^n
+      // is written layer by layer in injection form, f_R applied there.
+      RC(zcopy2d_launch(b, b, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
       RC(rkd_contacts_up(L, w, nb, Sdw, L.nd, s));
       if (out.T)
         RC(transmission_from_b0(L, w, nb, M{grd, b, L.nd}, M{Sdw, b, L.nd}, M{in.gammaL, b, 0},
                                 out.T + e0, s));
-      RC(zscale_launch((int)L.nd, 1, ZERO, fR + e0, 1, Sdw, L.nd, L.nd, Sdw, L.nd, L.nd, nb, s));
-      RC(rkd_inject(L, w, nb, Sdw, L.nd, grd, L.nd, in.gammaL, fL + e0,
+      RC(rkd_inject(L, w, nb, Sdw, L.nd, grd, L.nd, in.gammaL, fL + e0, fR + e0,
                     out.Gless_diag + (size_t)e0 * L.nd, L.nd, s));
     }
   }

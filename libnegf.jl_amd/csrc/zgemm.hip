@@ -54,7 +54,8 @@ struct GemmGroup {
 };
 
 // Epilogue of a wave's FM x FN MFMA blocks whose top-left corner is (ib, jb):
-// D = alpha acc + beta C (+ shift on the diagonal), rows < m and columns < n.
+// D = alpha (scale) acc + beta C (+ shift on the diagonal), rows < m and
+// columns < n.
 // Output layout of the 16x16 f64 MFMA on gfx950 (measured): lane l, element
 // e holds row l/16 + 4*e, column l % 16.
 template <int FM, int FN>
@@ -69,6 +70,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long lon
   zc sh = zmake(0.0, 0.0);
   const bool use_shift = g.shift != nullptr;
   if (use_shift) sh = g.shift[inst * g.shift_stride];
+  const zc alpha = g.scale ? zmul(g.alpha, g.scale[inst * g.scale_stride]) : g.alpha;
 #pragma unroll
   for (int r = 0; r < FM; ++r)
 #pragma unroll
@@ -79,7 +81,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long lon
         const int i = ib + r * 16 + kq + 4 * e;
         if (i < g.m && j < g.n) {
           const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
-          zc out = zmake(g.alpha.x * vr - g.alpha.y * vi, g.alpha.x * vi + g.alpha.y * vr);
+          zc out = zmake(alpha.x * vr - alpha.y * vi, alpha.x * vi + alpha.y * vr);
           if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
           if (use_shift && i == j) out = zadd(out, sh);
           D[i + j * g.ldd] = out;
