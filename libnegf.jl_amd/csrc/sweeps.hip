@@ -584,29 +584,36 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   if (streams <= 1) return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, groups, s);
   hipEvent_t fork;
   if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return Status::HIP_ERROR;
-  hipEventRecord(fork, s);
+  if (hipEventRecord(fork, s) != hipSuccess) {
+    (void)hipEventDestroy(fork);
+    return Status::HIP_ERROR;
+  }
   // contiguous chunks, each a whole number of waves where possible
   const int per = ((nE + streams - 1) / streams + wave - 1) / wave * wave;
   int rc = Status::OK;
-  std::vector<hipEvent_t> joins;
   for (int g = 0; g < streams && rc == Status::OK; ++g) {
     const int e0 = g * per, e1 = (g + 1) * per < nE ? (g + 1) * per : nE;
     if (e0 >= e1) break;
-    hipStreamWaitEvent(pool[g], fork, 0);
-    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, groups + g * gbytes, pool[g]);
-    hipEvent_t j;
-    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamWaitEvent(pool[g], fork, 0) != hipSuccess) {
       rc = Status::HIP_ERROR;
       break;
     }
-    hipEventRecord(j, pool[g]);
-    joins.push_back(j);
+    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, groups + g * gbytes, pool[g]);
+    // join group g back into the caller's stream; where an event cannot carry
+    // the edge, wait for the group on the host so later work on `s` still
+    // follows it
+    hipEvent_t j;
+    bool joined = false;
+    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess) {
+      joined = hipEventRecord(j, pool[g]) == hipSuccess && hipStreamWaitEvent(s, j, 0) == hipSuccess;
+      (void)hipEventDestroy(j);
+    }
+    if (!joined) {
+      (void)hipStreamSynchronize(pool[g]);
+      if (rc == Status::OK) rc = Status::HIP_ERROR;
+    }
   }
-  for (hipEvent_t j : joins) {
-    hipStreamWaitEvent(s, j, 0);
-    hipEventDestroy(j);
-  }
-  hipEventDestroy(fork);
+  (void)hipEventDestroy(fork);
   return rc;
 }
 

@@ -101,13 +101,16 @@ def test_keldysh_vs_oracle(N, b):
 _ENERGIES = [-0.4, -0.05, 0.1, 0.33, 0.8]
 
 
-@pytest.mark.parametrize("N,b,eta,Es,seed,tol", [
-    (6, 64, 1e-3, _ENERGIES, 4, 1e-12),
-    (4, 160, 1e-3, _ENERGIES, 4, 1e-12),
-    (3, 256, 1e-3, _ENERGIES, 4, 1e-12),
-    (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11)])
-def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol):
-    """b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  The
+@pytest.mark.parametrize("N,b,eta,Es,seed,tol,streams", [
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 2),
+    (4, 160, 1e-3, _ENERGIES, 4, 1e-12, 1),
+    (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1),
+    (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11, 1)])
+def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams):
+    """Two streams split the 5 energies into groups of 4 and 1 (waves of 2),
+    each group forked from and joined back into the caller's stream.
+    b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  The
     N = 16, b = 512, eta = 1e-4 case is the production block size and
     broadening at a depth where the Gamma_R propagation B_i = t_i B_{i+1}
     t_i^H is strongly contracting inside an expanding map (|t_i| ~ 40, |B_i|
@@ -115,7 +118,7 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol):
     backward-stable form (e.g. forcing exact Hermitian symmetry elementwise)
     grows ~|t|^2 per layer and fails this case by many orders."""
     kT, muL, muR = 0.025, 0.1, -0.1
-    print(f"energies workload N={N} b={b} eta={eta}", flush=True)
+    print(f"energies workload N={N} b={b} eta={eta} streams={streams}", flush=True)
     dev = device_hamiltonian(N, b, seed=seed)
     Es = np.array(Es)
     H = api.DeviceHamiltonianGPU(dev)
@@ -124,7 +127,7 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol):
     GL = torch.zeros_like(GR)
     T = torch.zeros(nE, dtype=torch.float64, device="cuda")
     api.rgf_keldysh_energies(H, torch.tensor(Es, device="cuda"), eta, muL, muR, kT, GR, GL, T,
-                             wave=2)
+                             wave=2, streams=streams)
     GR, GL, T = GR.cpu().numpy(), GL.cpu().numpy(), T.cpu().numpy()
     for e, E in enumerate(Es):
         A = assemble_A(dev, E, eta)
