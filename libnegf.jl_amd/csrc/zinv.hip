@@ -122,23 +122,34 @@ __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT
         red_i[buf][wave] = win_lane >= 0 ? win_lane + 64 * wave + PT * win_q : INT_MAX;
       }
       __syncthreads();
+      // all waves' (maximum, row) pairs in one round of LDS reads, then a
+      // branch-free pick (so the reads are not serialised behind compares)
+      double rv[NW];
+      int rix[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        rv[w] = red_v[buf][w];
+        rix[w] = red_i[buf][w];
+      }
       int ws = 0;
-      double gv = red_v[buf][0];
-      int bi = red_i[buf][0];
+      double gv = rv[0];
+      int bi = rix[0];
 #pragma unroll
       for (int w = 1; w < NW; ++w) {
-        const double ov = red_v[buf][w];
-        const int oi = red_i[buf][w];
-        if (ov > gv || (ov == gv && oi < bi)) { gv = ov; bi = oi; ws = w; }
+        const bool take = rv[w] > gv || (rv[w] == gv && rix[w] < bi);
+        gv = take ? rv[w] : gv;
+        bi = take ? rix[w] : bi;
+        ws = take ? w : ws;
       }
       const int p = bi;
       pc[jj] = p;
       if (tid == 0 && gv <= 0.0 && *info_inst == 0) *info_inst = k0 + jj + 1;
-      const zc pv = cand[buf][ws][jj];
-      const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
+      // the winning row in one round of reads, its pivot entry among them
       zc cr[NB];
 #pragma unroll
       for (int c = 0; c < NB; ++c) cr[c] = cand[buf][ws][c];
+      const zc pv = cr[jj];
+      const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = tid + PT * q;
