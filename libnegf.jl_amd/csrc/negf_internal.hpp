@@ -25,6 +25,33 @@ __host__ __device__ inline zc zmul(zc a, zc b) {
 }
 __host__ __device__ inline zc zconj(zc a) { return zmake(a.x, -a.y); }
 __host__ __device__ inline double zabs1(zc a) { return fabs(a.x) + fabs(a.y); }
+
+// Maximum of a double over the 64 lanes of a wave, returned to every lane:
+// DPP row shifts by 1, 2, 4, 8 within rows of 16 lanes, then row broadcasts
+// of lanes 15 and 31 (lanes without a source keep their own value), and the
+// total read from lane 63.  No LDS traffic, a few cycles per step.
+__device__ __forceinline__ double wave_max(double x) {
+  union {
+    double d;
+    int i[2];
+  } u, v;
+#define NEGF_DPP_MAX(CTRL, RMASK)                                                  \
+  u.d = x;                                                                         \
+  v.i[0] = __builtin_amdgcn_update_dpp(u.i[0], u.i[0], CTRL, RMASK, 0xf, false);  \
+  v.i[1] = __builtin_amdgcn_update_dpp(u.i[1], u.i[1], CTRL, RMASK, 0xf, false);  \
+  x = fmax(x, v.d);
+  NEGF_DPP_MAX(0x111, 0xf)   // row_shr:1
+  NEGF_DPP_MAX(0x112, 0xf)   // row_shr:2
+  NEGF_DPP_MAX(0x114, 0xf)   // row_shr:4
+  NEGF_DPP_MAX(0x118, 0xf)   // row_shr:8
+  NEGF_DPP_MAX(0x142, 0xa)   // row_bcast:15 into rows 1, 3
+  NEGF_DPP_MAX(0x143, 0xc)   // row_bcast:31 into rows 2, 3
+#undef NEGF_DPP_MAX
+  u.d = x;
+  u.i[0] = __builtin_amdgcn_readlane(u.i[0], 63);
+  u.i[1] = __builtin_amdgcn_readlane(u.i[1], 63);
+  return u.d;
+}
 // 1/z with Smith's scaling (R. L. Smith, "Algorithm 116: Complex division",
 // CACM 5(8), 1962), which avoids overflow/underflow of |z|^2.
 __host__ __device__ inline zc zrcp(zc z) {

@@ -27,33 +27,6 @@ namespace negf {
 
 namespace {
 
-// Maximum of a double over the 64 lanes of a wave, returned to every lane:
-// DPP row shifts by 1, 2, 4, 8 within rows of 16 lanes, then row broadcasts
-// of lanes 15 and 31 (lanes without a source keep their own value), and the
-// total read from lane 63.  No LDS traffic, a few cycles per step.
-__device__ __forceinline__ double wave_max(double x) {
-  union {
-    double d;
-    int i[2];
-  } u, v;
-#define NEGF_DPP_MAX(CTRL, RMASK)                                                  \
-  u.d = x;                                                                         \
-  v.i[0] = __builtin_amdgcn_update_dpp(u.i[0], u.i[0], CTRL, RMASK, 0xf, false);  \
-  v.i[1] = __builtin_amdgcn_update_dpp(u.i[1], u.i[1], CTRL, RMASK, 0xf, false);  \
-  x = fmax(x, v.d);
-  NEGF_DPP_MAX(0x111, 0xf)   // row_shr:1
-  NEGF_DPP_MAX(0x112, 0xf)   // row_shr:2
-  NEGF_DPP_MAX(0x114, 0xf)   // row_shr:4
-  NEGF_DPP_MAX(0x118, 0xf)   // row_shr:8
-  NEGF_DPP_MAX(0x142, 0xa)   // row_bcast:15 into rows 1, 3
-  NEGF_DPP_MAX(0x143, 0xc)   // row_bcast:31 into rows 2, 3
-#undef NEGF_DPP_MAX
-  u.d = x;
-  u.i[0] = __builtin_amdgcn_readlane(u.i[0], 63);
-  u.i[1] = __builtin_amdgcn_readlane(u.i[1], 63);
-  return u.d;
-}
-
 // Gauss-Jordan elimination of the NB columns held in registers: P[q][c] is
 // row r = tid + PT * q of the strip's column c (k0 + c of the matrix).
 // Pivot rule: max |re| + |im| over rows not yet pivoted, lowest row on ties

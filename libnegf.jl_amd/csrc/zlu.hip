@@ -51,31 +51,44 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
   for (int jj = 0; jj < NB; ++jj) {
     if (jj < nbe) {
       const int cur = k0 + jj;
+      // each lane's best row (the lowest on ties: rows grow with q), the
+      // wave's maximum by DPP (no LDS round trips), the lowest row holding it
+      // by ballot (smallest q, then lowest lane)
       double bv = -1.0;
-      int bi = INT_MAX;
+      int bq = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = k0 + tid + PT * q;
         // without pivoting the "search" sees only the diagonal row
         if (r < n && (pivot ? r >= cur : r == cur)) {
           const double v = zabs1(P[q][jj]);
-          if (v > bv) { bv = v; bi = r; }
+          if (v > bv) { bv = v; bq = q; }
         }
       }
+      const double wmax = wave_max(bv);
+      int win = INT_MAX;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      for (int q = RPT - 1; q >= 0; --q) {
+        const unsigned long long m = __ballot(bv >= 0.0 && bv == wmax && bq == q);
+        if (m != 0ull) win = k0 + 64 * wave + (__ffsll(m) - 1) + PT * q;
       }
-      if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+      if (lane == 0) { red_v[wave] = wmax; red_i[wave] = win; }
       __syncthreads();
-      bv = red_v[0]; bi = red_i[0];
+      // all waves' pairs in one round of LDS reads, then a branch-free pick
+      double rv[PT / 64];
+      int rix[PT / 64];
+#pragma unroll
+      for (int w = 0; w < PT / 64; ++w) {
+        rv[w] = red_v[w];
+        rix[w] = red_i[w];
+      }
+      bv = rv[0];
+      int bi = rix[0];
 #pragma unroll
       for (int w = 1; w < PT / 64; ++w) {
-        const double ov = red_v[w];
-        const int oi = red_i[w];
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        const bool take = rv[w] > bv || (rv[w] == bv && rix[w] < bi);
+        bv = take ? rv[w] : bv;
+        bi = take ? rix[w] : bi;
       }
       const int p = bi;
       if (tid == 0) {
@@ -95,7 +108,11 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
         }
       }
       __syncthreads();
-      const zc pv = rowpiv[jj];
+      // the pivot row in one round of reads, its pivot entry among them
+      zc rp[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) rp[c] = rowpiv[c];
+      const zc pv = rp[jj];
       const bool nz = (pv.x != 0.0 || pv.y != 0.0);
       const zc inv = nz ? zrcp(pv) : zmake(0.0, 0.0);
 #pragma unroll
@@ -104,7 +121,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
         if (r < n && r >= cur) {
           if (r == cur) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) P[q][c] = rowpiv[c];
+            for (int c = 0; c < NB; ++c) P[q][c] = rp[c];
           } else {
             if (r == p) {
 #pragma unroll
@@ -115,7 +132,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
               P[q][jj] = l;
 #pragma unroll
               for (int c = 0; c < NB; ++c)
-                if (c > jj) P[q][c] = zsub(P[q][c], zmul(l, rowpiv[c]));
+                if (c > jj) P[q][c] = zsub(P[q][c], zmul(l, rp[c]));
             }
           }
         }
