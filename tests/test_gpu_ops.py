@@ -178,6 +178,46 @@ def test_zgetrf_matches_lapack(n):
     assert relfro(_host(lu.A), lu_ref) < 1e-12
 
 
+def _hadamard_phased(n, perm_seed=None):
+    """Sylvester-Hadamard matrix with the columns multiplied by 1, i, -1, -i
+    in turn (rows optionally permuted): every entry has |re| + |im| = 1, and
+    LU with partial pivoting keeps every intermediate a small dyadic rational,
+    so it is exact in any operation order and its ties stay exact ties."""
+    H = np.array([[1.0]])
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    A = H * np.array([1, 1j, -1, -1j])[np.arange(n) % 4][None, :]
+    if perm_seed is not None:
+        A = A[np.random.default_rng(perm_seed).permutation(n)]
+    return np.asfortranarray(A)
+
+
+@pytest.mark.parametrize("n", [64, 256, 512, 1024])
+@pytest.mark.parametrize("perm_seed", [None, 5])
+def test_zgetrf_exact_ties_match_lapack(n, perm_seed):
+    """Pivot ties broken as LAPACK's izamax does (the first row among equal
+    |re| + |im|), for each rows-per-thread layout of the panel kernel: on a
+    matrix whose factorization is exact, ipiv and the factors equal
+    LAPACK's bit for bit."""
+    print(f"zgetrf ties n={n} perm={perm_seed}", flush=True)
+    A = _hadamard_phased(n, perm_seed)
+    lu = api.be_lu(None, _dev(A))
+    lu_ref, piv_ref, info = sla.lapack.zgetrf(A)
+    assert info == 0
+    assert np.array_equal(lu.piv.cpu().numpy(), piv_ref + 1)
+    assert np.array_equal(_host(lu.A), lu_ref)
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_zinv_exact_ties(n):
+    """Gauss-Jordan on the same tie-laden matrix (rows permuted): A A^H = n I,
+    so inv(A) = A^H / n."""
+    print(f"zinv ties n={n}", flush=True)
+    A = _hadamard_phased(n, 5)
+    inv = _host(api.be_inv(_dev(A)))
+    assert relfro(inv, A.conj().T / n) < 1e-15
+
+
 @pytest.mark.parametrize("trans", ["N", "T", "C"])
 def test_zgetrs(trans):
     print(f"zgetrs {trans}", flush=True)
