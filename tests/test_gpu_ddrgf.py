@@ -32,8 +32,11 @@ def _run(mats, levels):
 
 
 @pytest.mark.parametrize("npl,b,levels", [(16, 32, [3]), (10, 24, [3]), (14, 20, [2]),
-                                          (12, 16, [1]), (36, 16, [3, 2]), (5, 16, [8])])
+                                          (12, 16, [1]), (36, 16, [3, 2]), (5, 16, [8]),
+                                          (12, 64, [3])])
 def test_ddrgf_vs_rgf(npl, b, levels):
+    """b = 64 runs the chunk solves' GEMMs over the (chunk x instance) grid on
+    the interior-tile kernel (operands loaded straight into LDS)."""
     print(f"ddrgf npl={npl} b={b} levels={levels}", flush=True)
     A = bm_create_synthetic_random(npl, b, seed=npl + b, uneven=False)
     G = _band(_run(A, levels).to_host())
