@@ -102,13 +102,16 @@ _ENERGIES = [-0.4, -0.05, 0.1, 0.33, 0.8]
 
 
 @pytest.mark.parametrize("N,b,eta,Es,seed,tol,streams", [
+    (1, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
+    (2, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
     (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
     (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 2),
     (4, 160, 1e-3, _ENERGIES, 4, 1e-12, 1),
     (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1),
     (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11, 1)])
 def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams):
-    """Two streams split the 5 energies into groups of 4 and 1 (waves of 2),
+    """N = 1: both contacts on the one layer, no coupling blocks; N = 2: no
+    interior layer.  Two streams split the 5 energies into groups of 4 and 1 (waves of 2),
     each group forked from and joined back into the caller's stream.
     b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  The
     N = 16, b = 512, eta = 1e-4 case is the production block size and
