@@ -92,6 +92,10 @@ struct GemmArgs {
   const zc* shift; long long shift_stride;
   // optional per-instance factor of the product: alpha * scale[idx * scale_stride]
   const zc* scale; long long scale_stride;
+  // 1 / -1: the caller guarantees the result is Hermitian / anti-Hermitian
+  // (m == n); kernels may then compute the tiles on and below the diagonal
+  // only and write the rest as +-conj of its mirror.  0: every element.
+  int herm;
   int batch;
   int inner;
   long long outerA, outerB, outerC, outerD;

@@ -43,7 +43,7 @@ inline M ro(const MW& w) { return M{w.p, w.ld, w.bs}; }
 
 int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D, int batch,
          hipStream_t s, const zc* shift = nullptr, long long shift_stride = 0,
-         const zc* scale = nullptr, long long scale_stride = 0) {
+         const zc* scale = nullptr, long long scale_stride = 0, int herm = 0) {
   GemmArgs g{};
   g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
   g.alpha = alpha; g.beta = beta;
@@ -53,6 +53,7 @@ int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C
   g.D = D.p; g.ldd = D.ld; g.strideD = D.bs;
   g.shift = shift; g.shift_stride = shift_stride;
   g.scale = scale; g.scale_stride = scale_stride;
+  g.herm = herm;
   g.batch = batch;
   return zgemm_launch(g, s);
 }
@@ -474,8 +475,11 @@ static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* 
     }
     RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), M{Sd + L.od[j], bj, sd_bs}, ZERO, none,
             MW{w.T1, bj, xbs}, batch, s, nullptr, 0, fR, 1));
+    // G^<_jj = i (V G_jj^H + C_j) = i G^n_jj is anti-Hermitian and only
+    // written out (nothing later reads it), so its upper triangle is taken as
+    // the mirror of the lower one
     RC(gemm(OP_N, OP_C, bj, bj, bj, I1, M{w.T1, bj, xbs}, G(j), I1, M{w.T2, bj, xbs},
-            MW{Gless + L.od[j], bj, gl_bs}, batch, s));
+            MW{Gless + L.od[j], bj, gl_bs}, batch, s, nullptr, 0, nullptr, 0, -1));
   }
   return Status::OK;
 }

@@ -51,6 +51,7 @@ struct GemmGroup {
   long long tiles_per_problem;    // tiles_m * tiles_n * batch
   long long total;                // tiles_per_problem * count
   long long grid_padded;          // multiple of 8
+  int tri;                        // tiles on and below the diagonal only (herm)
 };
 
 // Epilogue of a wave's FM x FN MFMA blocks whose top-left corner is (ib, jb):
@@ -58,11 +59,13 @@ struct GemmGroup {
 // columns < n.
 // Output layout of the 16x16 f64 MFMA on gfx950 (measured): lane l, element
 // e holds row l/16 + 4*e, column l % 16.
+// With mirror (a tile strictly below the diagonal of a herm product) each
+// element (i, j) is also written at (j, i) as herm * conj.
 template <int FM, int FN>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long long i_in,
                                            long long i_out, int ib, int jb, int lane,
                                            const dbl4 (&acc_re)[FM][FN],
-                                           const dbl4 (&acc_im)[FM][FN]) {
+                                           const dbl4 (&acc_im)[FM][FN], bool mirror = false) {
   const int kq = lane >> 4, li = lane & 15;
   zc* __restrict__ D = g.D + i_in * g.strideD + i_out * g.outerD;
   const zc* __restrict__ C = g.C ? g.C + i_in * g.strideC + i_out * g.outerC : nullptr;
@@ -85,6 +88,8 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long lon
           if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
           if (use_shift && i == j) out = zadd(out, sh);
           D[i + j * g.ldd] = out;
+          if (mirror)
+            D[j + i * g.ldd] = g.herm > 0 ? zmake(out.x, -out.y) : zmake(-out.x, out.y);
         }
       }
     }
@@ -264,10 +269,19 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
   if (logical >= G.total) return;
   const int pidx = (int)(logical / G.tiles_per_problem);
   const long long rem = logical - (long long)pidx * G.tiles_per_problem;
-  const int tiles_inst = G.tiles_m * G.tiles_n;
+  const int tiles_inst = G.tri ? G.tiles_m * (G.tiles_m + 1) / 2 : G.tiles_m * G.tiles_n;
   const int inst = (int)(rem / tiles_inst);
   const int tile = (int)(rem - (long long)inst * tiles_inst);
-  const int tm = tile % G.tiles_m, tn = tile / G.tiles_m;
+  int tm, tn;
+  if (G.tri) {   // tile = tm (tm + 1) / 2 + tn, tn <= tm
+    tm = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+    while ((tm + 1) * (tm + 2) / 2 <= tile) ++tm;
+    while (tm * (tm + 1) / 2 > tile) --tm;
+    tn = tile - tm * (tm + 1) / 2;
+  } else {
+    tm = tile % G.tiles_m;
+    tn = tile / G.tiles_m;
+  }
   const GemmArgs& g = G.p[pidx];
   const int m = g.m, n = g.n, k = g.k;
   const int i0 = tm * 64, j0 = tn * 64;
@@ -387,7 +401,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     if (kt + NBUF - 1 < nk) issue(kt + NBUF - 1);
     compute_stage(kt % NBUF);
   }
-  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 32, j0 + wn * 32, lane, acc_re, acc_im);
+  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 32, j0 + wn * 32, lane, acc_re, acc_im,
+                     G.tri && tm > tn);
 }
 
 // Tile variants (autotuner choice, negf_gemm_variant).  0: zgemm_lds_kernel
@@ -434,7 +449,9 @@ template <int OPA, int OPB, int NBUF>
 void launch_lds(GemmGroup G, hipStream_t s) {
   G.tiles_m = G.p[0].m / 64;
   G.tiles_n = G.p[0].n / 64;
-  G.tiles_per_problem = (long long)G.tiles_m * G.tiles_n * G.p[0].batch;
+  G.tri = G.count == 1 && G.p[0].herm != 0 && G.p[0].m == G.p[0].n;
+  G.tiles_per_problem = (G.tri ? (long long)G.tiles_m * (G.tiles_m + 1) / 2
+                               : (long long)G.tiles_m * G.tiles_n) * G.p[0].batch;
   G.total = G.tiles_per_problem * G.count;
   G.grid_padded = (G.total + 7) / 8 * 8;
   hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF>), dim3((unsigned)G.grid_padded), dim3(256), 0,
@@ -478,6 +495,7 @@ int check_args(const GemmArgs& g) {
       return Status::BAD_ARG;
   }
   if (g.inner < 0) return Status::BAD_ARG;
+  if (g.herm != 0 && (g.m != g.n || (g.herm != 1 && g.herm != -1))) return Status::BAD_ARG;
   return Status::OK;
 }
 
@@ -515,7 +533,8 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   };
   for (int i = 1; i < count; ++i) {
     if (g[i].m != g[0].m || g[i].n != g[0].n || g[i].k != g[0].k || g[i].opA != g[0].opA ||
-        g[i].opB != g[0].opB || g[i].batch != g[0].batch || conflicts(i)) {
+        g[i].opB != g[0].opB || g[i].batch != g[0].batch || g[0].herm != 0 || g[i].herm != 0 ||
+        conflicts(i)) {
       int rc = zgemm_group_launch(g, i, s);
       if (rc) return rc;
       return zgemm_group_launch(g + i, count - i, s);

@@ -24,7 +24,11 @@ G^n_jj = f_L C_j + f_R G_jj B_j G_jj^H with C_j = l_j C_{j-1} l_j^H
 (injection form, exact for this Sigma^n):
   GEMMs 3 (N-1) RGF upward + 2 (N-1) RGF downward + 2 (N-1) Keldysh upward
         + 2 transmission + 4 N injection = 11 (N-1) + 6, inverses N
-  -> (88 (N-1) + 8 N + 48) b^3 = 8.19e11 at b = 512, N = 64.
+  -> (88 (N-1) + 8 N + 48) b^3 = 8.19e11 at b = 512, N = 64,
+less the tiles the last injection GEMM of each layer skips: G^<_jj is
+anti-Hermitian, so for b a multiple of 64 only the T (T + 1) / 2 of the
+T^2 64 x 64 tiles (T = b / 64) on and below the diagonal are computed and the
+rest mirrored: 7.89e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 """
 
@@ -49,5 +53,15 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
     return flops_rgf(N, b) + flops_rkd(N, b) - flops_upward(N, b) + flops_transmission(N, b)
 
 
+def injection_tile_fraction(b: int) -> float:
+    """Fraction of the last injection GEMM's tiles computed (the rest are
+    mirrored): T (T + 1) / 2 of T^2 for b = 64 T, else all."""
+    if b % 64:
+        return 1.0
+    t = b // 64
+    return t * (t + 1) / 2 / (t * t)
+
+
 def flops_energy_executed(N: int, b: int) -> float:
-    return (88 * (N - 1) + 8 * N + 48) * b ** 3
+    skipped = 8 * N * (1.0 - injection_tile_fraction(b))
+    return (88 * (N - 1) + 8 * N + 48 - skipped) * b ** 3
