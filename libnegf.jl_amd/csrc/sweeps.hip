@@ -396,7 +396,7 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
 // layers only (its off-diagonal blocks are zero on entry), so the reference's
 // upward step (src/keldyshndiag.jl:230-234) reduces to Su_i = -t_i Sd_{i+1}
 // feeding Sd_i -= Su_i t_i^H only, i.e. Sd_i += t_i Sd_{i+1} t_i^H.  The step
-// is linear in Sigma; the driver runs it on Gamma_R alone (unscaled), so
+// is linear in Sigma; the energy pipeline runs it on Gamma_R alone (unscaled), so
 // Sd_i is zero before its own step for i < N-1 and the step writes
 // B_j = t_j B_{j+1} t_j^H over whatever Sd_j holds; Sd_{N-1} = Gamma_R on
 // entry.  Its B_0 gives the transmission
