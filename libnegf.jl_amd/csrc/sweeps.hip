@@ -396,10 +396,10 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
 // layers only (its off-diagonal blocks are zero on entry), so the reference's
 // upward step (src/keldyshndiag.jl:230-234) reduces to Su_i = -t_i Sd_{i+1}
 // feeding Sd_i -= Su_i t_i^H only, i.e. Sd_i += t_i Sd_{i+1} t_i^H.  The step
-// is linear in Sigma; the energy pipeline runs it on Gamma_R alone (unscaled), so
-// Sd_i is zero before its own step for i < N-1 and the step writes
-// B_j = t_j B_{j+1} t_j^H over whatever Sd_j holds; Sd_{N-1} = Gamma_R on
-// entry.  Its B_0 gives the transmission
+// is linear in Sigma; the energy pipeline runs it on Gamma_R alone (unscaled):
+// Sd_{N-1} = Gamma_R on entry, and since Sigma^n_ii = 0 for i < N-1 the
+// "+=" is an assignment, B_j = t_j B_{j+1} t_j^H written over whatever Sd_j
+// held (so the blocks need no clearing).  Its B_0 gives the transmission
 // T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R
 // G_{0,N-1}^H), G_{0,N-1} = G_00 prod(-t)), and f_R B_j is the right-injected
 // part of G^n (rkd_inject).
