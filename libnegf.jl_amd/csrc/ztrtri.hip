@@ -4,14 +4,16 @@
 // triangular solves with U_ii and L_ii into single MFMA GEMMs.
 //
 // Recursive 2 x 2 blocking, bottom up:
-//   leaves: the 64 x 64 diagonal blocks, one wave each: the block in LDS,
-//           lane j computes column j of its inverse by substitution;
-//   level s (s = 64, 128, ...): every pair of neighbouring diagonal blocks
+//   leaves: the 32 x 32 diagonal blocks, one wave each: the block in LDS,
+//           lane j computes column j of its inverse by substitution (its
+//           serial chain grows with the square of the leaf: 32 beat 64 by
+//           6 % on the selected inverse at b = 512, 16 was within noise of 32);
+//   level s (s = 32, 64, 128, ...): every pair of neighbouring diagonal blocks
 //           of size s (the second possibly ragged) is merged,
 //             upper:  X12 = -X11 A12 X22     lower:  X21 = -X22 A21 X11,
 //           as two GEMMs batched over (pair, instance) with the 2-D
 //           instance grid of the GEMM kernel.
-// Flops: n^3 / 3 complex multiply-adds (the ztrtri count), in log2(n / 64)
+// Flops: n^3 / 3 complex multiply-adds (the ztrtri count), in log2(n / 32)
 // GEMM levels plus the leaves.
 #include "negf_internal.hpp"
 
@@ -19,7 +21,7 @@ namespace negf {
 
 namespace {
 
-constexpr int LEAF = 64;
+constexpr int LEAF = 32;
 
 // One wave per (leaf block, instance).  Zeroes the leaf's strictly
 // off-triangle part of X and writes the leaf inverse.
