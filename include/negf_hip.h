@@ -53,13 +53,15 @@ const char* negf_status_string(int status);
 /* Frees the cached sweep workspaces (the next sweep reallocates). */
 int negf_workspace_release(void);
 /* Autotuner knob: tile variant of the MFMA ZGEMM for k > 32, process-wide
- * (0 (default): 64x64 tile, operands loaded straight into two LDS stages when
- * every tile lies inside the output (m, n multiples of 64, k of 8), else
- * variant 8; 1: 64x64 register-staged, double-buffered; 2: 64x128 one buffer;
- * 3: 64x128 double-buffered; 4: 128x64 double-buffered; 5: 64x64 at four
- * workgroups per CU; 6, 7: as 0 with three, four LDS stages; 8: 64x64
- * register-staged, one LDS buffer, for every shape).  Results of all variants
- * agree to rounding. */
+ * (0 (default): operands loaded straight into two LDS stages when every tile
+ * lies inside the output (k a multiple of 8), with 64x64 tiles (m, n multiples
+ * of 64) for launches of at least 512 such tiles, else 32x32 tiles (m, n
+ * multiples of 32), else variant 8; 1: 64x64 register-staged,
+ * double-buffered; 2: 64x128 one buffer; 3: 64x128 double-buffered; 4: 128x64
+ * double-buffered; 5: 64x64 at four workgroups per CU; 6, 7: 64x64 with three,
+ * four LDS stages; 8: 64x64 register-staged, one LDS buffer, for every shape;
+ * 9, 10: two LDS stages with 64x64, 32x32 tiles wherever they fit).  Results
+ * of all variants agree to rounding. */
 int negf_gemm_variant(int32_t variant);
 /* Autotuner knob: schedule of the batched Gauss-Jordan inverse, process-wide
  * (0: automatic (default); in place, one launch per panel: 1: one column strip

@@ -26,7 +26,8 @@
 // Launches whose tiles all lie inside the output (the sweeps' b x b blocks
 // with b a multiple of 64) take zgemm_lds_kernel instead: the same tile and
 // MFMA loop, the operands loaded straight into LDS by the buffer unit
-// (no register staging, one barrier per 8-deep stage).
+// (no register staging, one barrier per 8-deep stage); launches too small to
+// fill the chip with 64 x 64 tiles take its 32 x 32 form.
 //
 // Launch: one flat grid over (problem, instance, tile).  Workgroups are
 // dispatched round-robin over the 8 XCDs; the index is remapped so that the
@@ -260,9 +261,11 @@ __global__ __launch_bounds__(256, OCC) void zgemm_mfma_kernel(
 // distinct banks.  Conjugation is a per-lane sign.
 constexpr int LTK = 8;
 
-template <int OPA, int OPB, int NBUF>
+template <int OPA, int OPB, int NBUF, int T>
 __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
-  constexpr int FM = 2, FN = 2;
+  static_assert(T == 64 || T == 32, "tile of 64 or 32");
+  constexpr int FM = T / 32, FN = T / 32;   // 16 x 16 MFMA blocks per wave and direction
+  constexpr int QL = T / 32;                // DMA loads per wave, operand and stage
   const long long bid = blockIdx.x;
   const long long per = G.grid_padded >> 3;
   const long long logical = (bid & 7) * per + (bid >> 3);
@@ -284,14 +287,14 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
   }
   const GemmArgs& g = G.p[pidx];
   const int m = g.m, n = g.n, k = g.k;
-  const int i0 = tm * 64, j0 = tn * 64;
+  const int i0 = tm * T, j0 = tn * T;
   const long long i_in = g.inner > 0 ? inst % g.inner : inst;
   const long long i_out = g.inner > 0 ? inst / g.inner : 0;
   const zc* A = g.A + i_in * g.strideA + i_out * g.outerA;
   const zc* B = g.B + i_in * g.strideB + i_out * g.outerB;
   const int lda = (int)g.lda, ldb = (int)g.ldb;
 
-  __shared__ zc Ls[NBUF][2][LTK][64];
+  __shared__ zc Ls[NBUF][2][LTK][T];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -306,20 +309,23 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
       (void*)A, (short)0, (rowsA - 1 + (colsA - 1) * lda + 1) * 16, 0x00020000);
   const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
       (void*)B, (short)0, (rowsB - 1 + (colsB - 1) * ldb + 1) * 16, 0x00020000);
-  // wave w loads k-rows 2w and 2w + 1 of each operand; lane offsets in bytes
-  int voA[2], voB[2];
+  // wave w loads k-rows 2w and 2w + 1 of each operand: with T = 64 one row
+  // per load, with T = 32 both rows in one (lanes 32..63 land in the second,
+  // since a row is 32 complex); lane offsets in bytes
+  const int xl = lane & (T - 1), kl = T == 32 ? (lane >> 5) : 0;
+  int voA[2], voB[2];   // QL used (a size of QL here drops the host stubs)
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int kk = 2 * wave + q;
-    voA[q] = (OPA == OP_N ? (i0 + lane + kk * lda) : (kk + (i0 + lane) * lda)) * 16;
-    voB[q] = (OPB == OP_N ? (kk + (j0 + lane) * ldb) : (j0 + lane + kk * ldb)) * 16;
+  for (int q = 0; q < QL; ++q) {
+    const int kk = 2 * wave + q + kl;
+    voA[q] = (OPA == OP_N ? (i0 + xl + kk * lda) : (kk + (i0 + xl) * lda)) * 16;
+    voB[q] = (OPB == OP_N ? (kk + (j0 + xl) * ldb) : (j0 + xl + kk * ldb)) * 16;
   }
   auto issue = [&](int kt) {
     const int buf = kt % NBUF, k0 = kt * LTK;
     const int sA = OPA == OP_N ? k0 * lda * 16 : k0 * 16;
     const int sB = OPB == OP_N ? k0 * 16 : k0 * ldb * 16;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < QL; ++q) {
       const int kk = 2 * wave + q;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rsA, (__attribute__((address_space(3))) void*)&Ls[buf][0][kk][0], 16, voA[q], sA, 0, 0);
@@ -340,8 +346,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
   const int kq = lane >> 4, li = lane & 15, ri = kq & 1;
   const double sgn = ri ? -1.0 : 1.0;   // the A- sign of K' row kq
   // double offsets within a stage's operand: component ri (B'': 1 - ri)
-  const int offa = (wm * 32 + li) * 2 + ri;
-  const int offb1 = (wn * 32 + li) * 2 + ri, offb2 = (wn * 32 + li) * 2 + (1 - ri);
+  const int offa = (wm * 16 * FM + li) * 2 + ri;
+  const int offb1 = (wn * 16 * FN + li) * 2 + ri, offb2 = (wn * 16 * FN + li) * 2 + (1 - ri);
   // Signs, all carried by the A operand so that B goes to the MFMA as loaded:
   // Re term  A-[i,k'] B'[k',j]  = sgn^(1 + [A conj] + [B conj]) a_ri b_ri,
   // Im term  A'[i,k'] B''[k',j] = (A conj ? sgn : 1)(B conj ? -sgn : 1) a_ri b_(1-ri)
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     const double* __restrict__ Lb = reinterpret_cast<const double*>(&Ls[buf][1][0][0]);
 #pragma unroll
     for (int ks = 0; ks < LTK / 2; ++ks) {
-      const int kr = (2 * ks + (kq >> 1)) * 128;   // doubles per k-row: 64 complex
+      const int kr = (2 * ks + (kq >> 1)) * 2 * T;   // doubles per k-row: T complex
       double a[FM], am[FM], b1[FN], b2[FN];
 #pragma unroll
       for (int r = 0; r < FM; ++r) {
@@ -380,18 +386,19 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     }
   };
 
-  // stage kt is waited for (vmcnt counts this wave's loads, in order: those
-  // of the NBUF - 2 later stages may stay in flight), then one barrier makes
-  // it visible and frees the buffer of stage kt - 1 for stage kt + NBUF - 1
+  // stage kt is waited for (vmcnt counts this wave's loads, in order, 2 QL a
+  // stage: those of the NBUF - 2 later stages may stay in flight), then one
+  // barrier makes it visible and frees the buffer of stage kt - 1 for stage
+  // kt + NBUF - 1
   const int nk = k / LTK;
 #pragma unroll
   for (int p = 0; p < NBUF - 1; ++p)
     if (p < nk) issue(p);
   for (int kt = 0; kt < nk; ++kt) {
     const int ahead = (nk - 1 - kt) < (NBUF - 2) ? (nk - 1 - kt) : (NBUF - 2);
-    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(0xF78);        // vmcnt(8)
-    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(0xF74);   // vmcnt(4)
-    else __builtin_amdgcn_s_waitcnt(0xF70);                   // vmcnt(0)
+    if (ahead >= 2) __builtin_amdgcn_s_waitcnt(0xF70 + 4 * QL);        // vmcnt(4 QL)
+    else if (ahead == 1) __builtin_amdgcn_s_waitcnt(0xF70 + 2 * QL);   // vmcnt(2 QL)
+    else __builtin_amdgcn_s_waitcnt(0xF70);                            // vmcnt(0)
     // a bare s_barrier: __syncthreads()' fence would wait for every load in
     // flight (vmcnt(0)), the later stages included; the clobbers keep the
     // compiler from moving LDS accesses across it
@@ -401,23 +408,26 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     if (kt + NBUF - 1 < nk) issue(kt + NBUF - 1);
     compute_stage(kt % NBUF);
   }
-  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 32, j0 + wn * 32, lane, acc_re, acc_im,
-                     G.tri && tm > tn);
+  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 16 * FM, j0 + wn * 16 * FN, lane, acc_re,
+                     acc_im, G.tri && tm > tn);
 }
 
 // Tile variants (autotuner choice, negf_gemm_variant).  0: zgemm_lds_kernel
 // with two stages (double-buffered LDS) where the whole launch qualifies,
-// else 64x64 register-staged; 1: 64x64 register-staged, double buffer,
-// 2: 64x128 single, 3: 64x128 double, 4: 128x64 double, 5: 64x64 register-
-// staged held to 128 VGPRs (4 workgroups per CU); 6, 7: zgemm_lds_kernel with
-// three, four stages; 8: 64x64 register-staged, single buffer, everywhere.
+// 64 x 64 tiles for launches of at least 512 of them, else 32 x 32 (m, n
+// multiples of 32), else 64x64 register-staged; 1: 64x64 register-staged,
+// double buffer, 2: 64x128 single, 3: 64x128 double, 4: 128x64 double, 5:
+// 64x64 register-staged held to 128 VGPRs (4 workgroups per CU); 6, 7:
+// zgemm_lds_kernel with three, four stages; 8: 64x64 register-staged, single
+// buffer, everywhere; 9, 10: zgemm_lds_kernel, two stages, 64 x 64 and
+// 32 x 32 tiles wherever they qualify.  Variants 6, 7, 9, 10 fall back to 8.
 // Measured on MI355X at 512^3 x 64, per-instance operands, after warm-up
 // (TFLOP/s, NN/NC/CN/CC): 0: 69.5/69.2/69.0/68.6, 6: 69.0/68.9/68.6/68.1,
 // 8: 62.1/62.7/61.3/61.0; variants 1-5 were at or below 8 (occupancy
 // matters more than tile reuse at this FP64 MFMA rate).  MFMA issue alone
 // (register operands, no memory) reaches 70.5 here.
 int g_variant = 0;
-constexpr int NVAR = 9;
+constexpr int NVAR = 11;
 
 template <int OPA, int OPB, int FM, int FN, bool DB, int SK, int OCC = (FM * FN <= 4 && !DB) ? 2 : 1>
 void launch_cfg(GemmGroup G, hipStream_t s) {
@@ -432,9 +442,9 @@ void launch_cfg(GemmGroup G, hipStream_t s) {
 
 // Every tile of the group inside m x n, whole LTK stages, and 31-bit byte
 // offsets into each operand (the buffer loads of zgemm_lds_kernel).
-bool lds_eligible(const GemmGroup& G) {
+bool lds_eligible(const GemmGroup& G, int T) {
   const GemmArgs& g0 = G.p[0];
-  if (g0.m % 64 || g0.n % 64 || g0.k % LTK) return false;
+  if (g0.m % T || g0.n % T || g0.k % LTK) return false;
   for (int i = 0; i < G.count; ++i) {
     const GemmArgs& g = G.p[i];
     const long long rowsA = g.opA == OP_N ? g.m : g.k, colsA = g.opA == OP_N ? g.k : g.m;
@@ -445,17 +455,17 @@ bool lds_eligible(const GemmGroup& G) {
   return true;
 }
 
-template <int OPA, int OPB, int NBUF>
+template <int OPA, int OPB, int NBUF, int T>
 void launch_lds(GemmGroup G, hipStream_t s) {
-  G.tiles_m = G.p[0].m / 64;
-  G.tiles_n = G.p[0].n / 64;
+  G.tiles_m = G.p[0].m / T;
+  G.tiles_n = G.p[0].n / T;
   G.tri = G.count == 1 && G.p[0].herm != 0 && G.p[0].m == G.p[0].n;
   G.tiles_per_problem = (G.tri ? (long long)G.tiles_m * (G.tiles_m + 1) / 2
                                : (long long)G.tiles_m * G.tiles_n) * G.p[0].batch;
   G.total = G.tiles_per_problem * G.count;
   G.grid_padded = (G.total + 7) / 8 * 8;
-  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF>), dim3((unsigned)G.grid_padded), dim3(256), 0,
-                     s, G);
+  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF, T>), dim3((unsigned)G.grid_padded), dim3(256),
+                     0, s, G);
 }
 
 template <int OPA, int OPB>
@@ -464,14 +474,23 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
     launch_cfg<OPA, OPB, 2, 2, false, 1>(G, s);
     return;
   }
-  if ((g_variant == 0 || g_variant == 6 || g_variant == 7) && lds_eligible(G)) {
-    if (g_variant == 6) launch_lds<OPA, OPB, 3>(G, s);
-    else if (g_variant == 7) launch_lds<OPA, OPB, 4>(G, s);
-    else launch_lds<OPA, OPB, 2>(G, s);
-    return;
+  const GemmArgs& g0 = G.p[0];
+  const long long tiles64 = (long long)(g0.m / 64) * (g0.n / 64) * g0.batch * G.count;
+  switch (g_variant) {
+    case 0:
+      // fewer than two 64 x 64 tiles per CU (one instance of a 512^3 product
+      // has 64), or m, n multiples of 32 only: 32 x 32 tiles
+      if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64>(G, s);
+      if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32>(G, s);
+      break;
+    case 6: if (lds_eligible(G, 64)) return launch_lds<OPA, OPB, 3, 64>(G, s); break;
+    case 7: if (lds_eligible(G, 64)) return launch_lds<OPA, OPB, 4, 64>(G, s); break;
+    case 9: if (lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64>(G, s); break;
+    case 10: if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32>(G, s); break;
+    default: break;
   }
   switch (g_variant) {
-    case 0: case 6: case 7: case 8: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
+    case 0: case 6: case 7: case 8: case 9: case 10: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
     case 1: launch_cfg<OPA, OPB, 2, 2, true, 0>(G, s); break;
     case 2: launch_cfg<OPA, OPB, 2, 4, false, 0>(G, s); break;
     case 3: launch_cfg<OPA, OPB, 2, 4, true, 0>(G, s); break;

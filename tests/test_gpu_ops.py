@@ -52,7 +52,7 @@ def test_zgemm(tA, tB, mnk):
     assert relfro(_host(dC), ref) < 1e-14
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("tA", ["N", "T", "C"])
 @pytest.mark.parametrize("tB", ["N", "T", "C"])
 def test_zgemm_padded_ld_interior_and_edge_tiles(variant, tA, tB):
@@ -63,7 +63,7 @@ def test_zgemm_padded_ld_interior_and_edge_tiles(variant, tA, tB):
     batch, pad = 2, 3
     call("negf_gemm_variant", variant)
     try:
-        for m, n, k in ((160, 200, 48), (256, 128, 80), (100, 70, 37)):
+        for m, n, k in ((160, 200, 48), (256, 128, 80), (96, 224, 40), (100, 70, 37)):
             print(f"zgemm padded variant={variant} {tA}{tB} {(m, n, k)}", flush=True)
             ra, ca = (m, k) if tA == "N" else (k, m)
             rb, cb = (k, n) if tB == "N" else (n, k)

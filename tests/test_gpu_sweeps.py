@@ -101,19 +101,25 @@ def test_keldysh_vs_oracle(N, b):
 _ENERGIES = [-0.4, -0.05, 0.1, 0.33, 0.8]
 
 
-@pytest.mark.parametrize("N,b,eta,Es,seed,tol,streams", [
-    (1, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
-    (2, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
-    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 1),
-    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 2),
-    (4, 160, 1e-3, _ENERGIES, 4, 1e-12, 1),
-    (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1),
-    (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11, 1)])
-def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams):
+@pytest.mark.parametrize("N,b,eta,Es,seed,tol,streams,gemm", [
+    (1, 64, 1e-3, _ENERGIES, 4, 1e-12, 1, 0),
+    (2, 64, 1e-3, _ENERGIES, 4, 1e-12, 1, 0),
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 1, 0),
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 2, 0),
+    (6, 64, 1e-3, _ENERGIES, 4, 1e-12, 1, 9),
+    (4, 160, 1e-3, _ENERGIES, 4, 1e-12, 1, 0),
+    (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1, 0),
+    (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1, 9),
+    (3, 256, 1e-3, _ENERGIES, 4, 1e-12, 1, 8),
+    (16, 512, 1e-4, [-0.61, 0.26984126984126977], 1234, 2e-11, 1, 0)])
+def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams, gemm):
     """N = 1: both contacts on the one layer, no coupling blocks; N = 2: no
     interior layer.  Two streams split the 5 energies into groups of 4 and 1 (waves of 2),
     each group forked from and joined back into the caller's stream.
-    b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  The
+    b = 160 (ragged last 64-tile) and 256 span several GEMM tiles.  Waves of
+    2 energies at these sizes take the 32 x 32 GEMM tiles; gemm = 9 forces
+    the 64 x 64 LDS tiles (the form of full-size waves), gemm = 8 the
+    register-staged GEMM, each including the Hermitian-mirrored output.  The
     N = 16, b = 512, eta = 1e-4 case is the production block size and
     broadening at a depth where the Gamma_R propagation B_i = t_i B_{i+1}
     t_i^H is strongly contracting inside an expanding map (|t_i| ~ 40, |B_i|
@@ -121,7 +127,7 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams):
     backward-stable form (e.g. forcing exact Hermitian symmetry elementwise)
     grows ~|t|^2 per layer and fails this case by many orders."""
     kT, muL, muR = 0.025, 0.1, -0.1
-    print(f"energies workload N={N} b={b} eta={eta} streams={streams}", flush=True)
+    print(f"energies workload N={N} b={b} eta={eta} streams={streams} gemm={gemm}", flush=True)
     dev = device_hamiltonian(N, b, seed=seed)
     Es = np.array(Es)
     H = api.DeviceHamiltonianGPU(dev)
@@ -129,8 +135,13 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams):
     GR = torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda")
     GL = torch.zeros_like(GR)
     T = torch.zeros(nE, dtype=torch.float64, device="cuda")
-    api.rgf_keldysh_energies(H, torch.tensor(Es, device="cuda"), eta, muL, muR, kT, GR, GL, T,
-                             wave=2, streams=streams)
+    api.call("negf_gemm_variant", gemm)
+    try:
+        api.rgf_keldysh_energies(H, torch.tensor(Es, device="cuda"), eta, muL, muR, kT, GR, GL, T,
+                                 wave=2, streams=streams)
+        torch.cuda.synchronize()
+    finally:
+        api.call("negf_gemm_variant", 0)
     GR, GL, T = GR.cpu().numpy(), GL.cpu().numpy(), T.cpu().numpy()
     for e, E in enumerate(Es):
         A = assemble_A(dev, E, eta)
