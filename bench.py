@@ -105,7 +105,7 @@ def gemm_roofline(torch, api, b, batch, reps):
     dt = e0.elapsed_time(e1) / 1e3 / reps
     flops = 8.0 * b ** 3 * batch
     ach = flops / dt / 1e12
-    return {"kernel": "zgemm_lds_kernel<N,N,2> (FP64 MFMA 16x16x4, operands DMA'd into LDS)",
+    return {"kernel": "zgemm_lds_kernel<N,N,2,64> (FP64 MFMA 16x16x4, 64x64 tiles, operands DMA'd into LDS)",
             "bound": "mfma",
             "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
