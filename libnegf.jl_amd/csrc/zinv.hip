@@ -445,9 +445,11 @@ constexpr int MAXPS = 4;
 // panel K's operations), then panel K's strips but its last (the operations
 // of the later strips of the panel); this runs beside the factoring of panel
 // K+1.  With next_panel = 1: panel K+1's strips (all of panel K's
-// operations), launched just before that factoring.
+// operations), launched just before that factoring.  Four waves per SIMD
+// (128 VGPRs) is the target at PT = 512, the sweep's n = 512; the 256-thread
+// form (n <= 256, selected schedules only) is left to the compiler.
 template <int PT>
-__global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(4))) void
+__global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(PT == 512 ? 4 : 1))) void
 gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
                        zc* __restrict__ X, const zc* __restrict__ Wcur,
                        const int* __restrict__ piv) {
