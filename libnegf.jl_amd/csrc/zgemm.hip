@@ -564,8 +564,8 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   GemmGroup G{};
   for (int i = 0; i < count; ++i) G.p[i] = g[i];
   G.count = count;
-  // grid size bound with the smallest tile (64 x 64)
-  const long long tiles = (long long)((g0.m + 63) / 64) * ((g0.n + 63) / 64) * g0.batch * count;
+  // grid size bound with the smallest tile (32 x 32)
+  const long long tiles = (long long)((g0.m + 31) / 32) * ((g0.n + 31) / 32) * g0.batch * count;
   if (tiles + 8 > 0x7fffffffLL) return Status::TOO_LARGE;
   // k == 0: D = beta C (+shift); the kernel handles it (no K stages, zero acc).
   const int key = g0.opA * 3 + g0.opB;
