@@ -42,6 +42,17 @@ namespace {
 // the stored ones, and every later elimination step is linear in the row);
 // so every row runs the same branch-free update (the pivot row with a zero
 // multiplier), and the at most NB rows pivoted here are scaled once at the end.
+// 1 / z = conj(z) / |z|^2 with z scaled by a power of two (exact) so that
+// |z|^2 lies in [1, 8): one division on the column's critical path (Smith's
+// form needs a branch and two dependent divisions); within 2 ulp per
+// component.  z != 0.
+__device__ __forceinline__ zc zrcp_scaled(zc z) {
+  const int e = ilogb(fmax(fabs(z.x), fabs(z.y)));
+  const double xs = ldexp(z.x, -e), ys = ldexp(z.y, -e);
+  const double r = 1.0 / fma(xs, xs, ys * ys);
+  return zmake(ldexp(xs * r, -e), ldexp(-ys * r, -e));
+}
+
 template <int PT, int RPT, int NB>
 __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT][NB],
                                              bool (&usd)[RPT], int (&pc)[NB],
@@ -104,16 +115,17 @@ __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT
         rv[w] = red_v[buf][w];
         rix[w] = red_i[buf][w];
       }
-      int ws = 0;
+      // the largest value, then the lowest row holding it (a max and a min
+      // over the slots: compares chained on the running pick compile to
+      // scalar branches, the values being uniform); the winner's slot is its
+      // row's wave
       double gv = rv[0];
-      int bi = rix[0];
 #pragma unroll
-      for (int w = 1; w < NW; ++w) {
-        const bool take = rv[w] > gv || (rv[w] == gv && rix[w] < bi);
-        gv = take ? rv[w] : gv;
-        bi = take ? rix[w] : bi;
-        ws = take ? w : ws;
-      }
+      for (int w = 1; w < NW; ++w) gv = fmax(gv, rv[w]);
+      int bi = INT_MAX;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) bi = min(bi, rv[w] == gv ? rix[w] : INT_MAX);
+      const int ws = bi == INT_MAX ? 0 : (bi % PT) >> 6;
       const int p = bi;
       pc[jj] = p;
       if (tid == 0 && gv <= 0.0 && *info_inst == 0) *info_inst = k0 + jj + 1;
@@ -122,7 +134,7 @@ __device__ __forceinline__ void gj_eliminate(int n, int nbe, int k0, zc (&P)[RPT
 #pragma unroll
       for (int c = 0; c < NB; ++c) cr[c] = cand[buf][ws][c];
       const zc pv = cr[jj];
-      const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp(pv);
+      const zc inv = (pv.x == 0.0 && pv.y == 0.0) ? zmake(0.0, 0.0) : zrcp_scaled(pv);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = tid + PT * q;
