@@ -82,14 +82,15 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
         rv[w] = red_v[w];
         rix[w] = red_i[w];
       }
+      // the largest value, then the lowest row holding it (LAPACK's first
+      // maximum), as a max and a min: compares chained on the running pick
+      // compile to scalar branches, the values being wave-uniform
       bv = rv[0];
-      int bi = rix[0];
 #pragma unroll
-      for (int w = 1; w < PT / 64; ++w) {
-        const bool take = rv[w] > bv || (rv[w] == bv && rix[w] < bi);
-        bv = take ? rv[w] : bv;
-        bi = take ? rix[w] : bi;
-      }
+      for (int w = 1; w < PT / 64; ++w) bv = fmax(bv, rv[w]);
+      int bi = INT_MAX;
+#pragma unroll
+      for (int w = 0; w < PT / 64; ++w) bi = min(bi, rv[w] == bv ? rix[w] : INT_MAX);
       const int p = bi;
       if (tid == 0) {
         ip[cur] = p + 1;
