@@ -116,7 +116,7 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 // one per instance) and may be the same buffer as A when A may be destroyed.
 struct InvWork {
   zc* X;        // n*n*batch
-  zc* W;        // 2*n*nb*batch (W of panel J and of panel J+1)
+  zc* W;        // 3*n*nb*batch (W of panels J, J+1, J+2)
   int* piv;     // n*batch
   int* used;    // n*batch
 };

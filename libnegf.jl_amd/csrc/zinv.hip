@@ -462,9 +462,9 @@ constexpr int MAXPS = 4;
 // form (n <= 256, selected schedules only) is left to the compiler.
 template <int PT>
 __global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(PT == 512 ? 4 : 1))) void
-gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
-                       zc* __restrict__ X, const zc* __restrict__ Wcur,
-                       const int* __restrict__ piv) {
+gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel, int it_base,
+                       int skip_at, int skip_len, zc* __restrict__ X,
+                       const zc* __restrict__ Wcur, const int* __restrict__ piv) {
   constexpr int S = 16;
   const SplitStep st(K, PS, ns);
   long long k = blockIdx.x;
@@ -472,13 +472,13 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel,
     const long long x = k & 7, kk = k >> 3, per = batch >> 3;
     k = (kk / per) * batch + x + 8 * (kk % per);
   }
-  const int it = (int)(k / batch), inst = (int)(k - (long long)it * batch);
+  const int itl = (int)(k / batch), inst = (int)(k - (long long)itl * batch);
   int s, t_lo;
   if (next_panel) {   // panel K+1's strips, ahead of their factoring
-    s = st.pk1 + it;
+    s = st.pk1 + itl;
     t_lo = st.pk0;
-  } else {
-    st.item(it, &s, &t_lo);
+  } else {            // items it_base + [0, skip_at) and it_base + skip_at + skip_len + ...
+    st.item(it_base + (itl < skip_at ? itl : itl + skip_len), &s, &t_lo);
   }
   const int nt = st.pk1 - t_lo;   // >= 1 for every launched item
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -679,12 +679,15 @@ int aux_for(hipStream_t s, AuxSlot** out) {
 
 // Split steps in place.  Per panel K, on the caller's stream: panel K's
 // operations applied to panel K+1's strips on the matrix cores
-// (gj_strip_update_kernel, next_panel = 1), alone on the GPU, then the
-// factoring of panel K+1 (gj_step_kernel, one workgroup per instance, with
-// panel K's operations already applied); on a second stream, forked after
-// that first launch so it does not queue ahead of it: the updates of every
-// other strip (gj_strip_update_kernel).  The two streams touch disjoint
-// strips and both finish before the next step.
+// (gj_strip_update_kernel, next_panel = 1), then the factoring of panel K+1
+// (gj_step_kernel, one workgroup per instance, with panel K's operations
+// already applied).  On a second stream, forked after that first launch so
+// it does not queue ahead of it: panel K's operations on panel K+2's strips
+// (part A), an event, then on every other strip (part B).  The next step's
+// first launch waits for part A only, so part B overlaps it and the next
+// factoring.  The two streams touch disjoint strips at any time; the
+// factoring of panel K+2 writes its W while part B of step K still reads
+// panel K's, hence three W buffers.  The second stream is joined at the end.
 template <int PT, int PS>
 int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
                  long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
@@ -707,29 +710,44 @@ int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, lo
   const long long wst = (long long)n * S * PS * batch;
   hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, -1, ns,
                      batch, w.X, w.W + wst, w.W, w.piv, w.used, info, 0);
+  bool partA_pending = false;   // the join event carries the last part A
   for (int K = 0; K < np; ++K) {
     const bool has_next = PS * (K + 1) < ns;
     const SplitStep st(K, PS, ns);
     const int items = st.items - 1;   // panel K's last strip has no later operation
-    const zc* Wc = w.W + (K & 1) * wst;
-    zc* Wn = w.W + ((K + 1) & 1) * wst;
-    // panel K+1's strips first, alone on the GPU (the factoring waits for
-    // them); then the fork, so the other strips' updates do not queue ahead
+    const zc* Wc = w.W + (K % 3) * wst;
+    zc* Wn = w.W + ((K + 1) % 3) * wst;
+    // panel K+1's strips got panel K-1's operations in the last part A
+    if (partA_pending && hipStreamWaitEvent(s, ax->join, 0) != hipSuccess) return Status::HIP_ERROR;
+    partA_pending = false;
     const int nnext = has_next ? (st.pk1 + PS < ns ? st.pk1 + PS : ns) - st.pk1 : 0;
     if (has_next)
       hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nnext)),
-                         dim3(PT), 0, s, n, K, PS, ns, batch, 1, w.X, Wc, w.piv);
+                         dim3(PT), 0, s, n, K, PS, ns, batch, 1, 0, 0, 0, w.X, Wc, w.piv);
     if (hipEventRecord(ax->fork, s) != hipSuccess || hipStreamWaitEvent(ax->aux, ax->fork, 0) != hipSuccess)
       return Status::HIP_ERROR;
     if (has_next)
       hipLaunchKernelGGL((gj_step_kernel<PT, 1, S, PS>), dim3(batch), dim3(PT), 0, s, n, K, ns,
                          batch, w.X, Wc, Wn, w.piv, w.used, info, 1);
-    if (items > 0)
-      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * items)),
-                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, w.X, Wc, w.piv);
-    if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
-      return Status::HIP_ERROR;
+    // part A: panel K+2's strips, items pk0 .. pk0 + nA - 1 (the first plain
+    // strips after panel K+1)
+    const int nA = has_next ? (ns - st.hi < PS ? ns - st.hi : PS) : 0;
+    if (nA > 0) {
+      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nA)),
+                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, st.pk0, INT_MAX, 0, w.X,
+                         Wc, w.piv);
+      if (hipEventRecord(ax->join, ax->aux) != hipSuccess) return Status::HIP_ERROR;
+      partA_pending = true;
+    }
+    // part B: every other item
+    const int nB = items - nA;
+    if (nB > 0)
+      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nB)),
+                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, 0, st.pk0, nA, w.X, Wc,
+                         w.piv);
   }
+  if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
+    return Status::HIP_ERROR;
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
                      2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
@@ -802,7 +820,7 @@ size_t zinv_work_bytes(int n, int batch) {
   const size_t nb = nb_for(n);
   size_t b = 0;
   b += sizeof(zc) * (size_t)n * n * batch;   // X
-  b += 2 * sizeof(zc) * (size_t)n * nb * batch;  // W (two buffers: panel J and J+1)
+  b += 3 * sizeof(zc) * (size_t)n * nb * batch;  // W (three buffers: panels J .. J+2)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
   return b + 1024;
 }
@@ -816,7 +834,7 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
     return r;
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
-  w->W = (zc*)take(2 * sizeof(zc) * (size_t)n * nb * batch);
+  w->W = (zc*)take(3 * sizeof(zc) * (size_t)n * nb * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
 }
