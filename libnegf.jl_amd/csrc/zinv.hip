@@ -467,12 +467,23 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel, 
                        const zc* __restrict__ Wcur, const int* __restrict__ piv) {
   constexpr int S = 16;
   const SplitStep st(K, PS, ns);
-  long long k = blockIdx.x;
-  if ((batch & 7) == 0) {   // all items of one instance on one XCD (its W in one L2)
-    const long long x = k & 7, kk = k >> 3, per = batch >> 3;
-    k = (kk / per) * batch + x + 8 * (kk % per);
+  // Workgroups are dealt to the 8 XCDs round-robin by block index.  With
+  // batch a multiple of 8, XCD x takes instances x, x + 8, ... and runs
+  // through all items of one instance before the next, so the workgroups in
+  // flight on an XCD share the W of two or three instances in its L2 (with
+  // instances fastest, all of its instances' W, 6 MB at 96 x 512, would
+  // cycle through the 4 MB L2).
+  const long long k = blockIdx.x;
+  int itl, inst;
+  if ((batch & 7) == 0) {
+    const long long nit = (long long)gridDim.x / batch;   // items per instance
+    const long long x = k & 7, kk = k >> 3;
+    itl = (int)(kk % nit);
+    inst = (int)(x + 8 * (kk / nit));
+  } else {
+    itl = (int)(k / batch);
+    inst = (int)(k - (long long)itl * batch);
   }
-  const int itl = (int)(k / batch), inst = (int)(k - (long long)itl * batch);
   int s, t_lo;
   if (next_panel) {   // panel K+1's strips, ahead of their factoring
     s = st.pk1 + itl;
