@@ -533,7 +533,7 @@ def main():
         if tj.get("b") == b and tj.get("batch") == wave:
             roof["traffic"] = int(tj["traffic_bytes_per_launch"])
             roof["traffic_source"] = tj["source"]
-    fx = flops_energy_executed(N, b)
+    fx = flops_energy_executed(N, b, wave)
     roof["path_tflops_executed"] = round(eps * fx / 1e12, 3)
     roof["path_frac_of_fp64_peak"] = round(eps * fx / 1e12 / FP64_PEAK_TFLOPS, 4)
     roof["path_tflops_credited"] = round(eps * fl / 1e12, 3)

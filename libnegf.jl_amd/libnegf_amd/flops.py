@@ -53,15 +53,21 @@ def flops_rgf_keldysh_energy(N: int, b: int) -> float:
     return flops_rgf(N, b) + flops_rkd(N, b) - flops_upward(N, b) + flops_transmission(N, b)
 
 
-def injection_tile_fraction(b: int) -> float:
+def injection_tile_fraction(b: int, wave: int | None = None) -> float:
     """Fraction of the last injection GEMM's tiles computed (the rest are
-    mirrored): T (T + 1) / 2 of T^2 for b = 64 T, else all."""
-    if b % 64:
+    mirrored): T (T + 1) / 2 of T^2 tiles, with the library's default tile:
+    64 x 64 for b = 64 T when the launch has at least 512 such tiles (wave
+    None: a large wave), else 32 x 32 for b a multiple of 32, else all
+    computed (the register-staged GEMM does not mirror)."""
+    if b % 64 == 0 and (wave is None or (b // 64) ** 2 * wave >= 512):
+        t = b // 64
+    elif b % 32 == 0:
+        t = b // 32
+    else:
         return 1.0
-    t = b // 64
     return t * (t + 1) / 2 / (t * t)
 
 
-def flops_energy_executed(N: int, b: int) -> float:
-    skipped = 8 * N * (1.0 - injection_tile_fraction(b))
+def flops_energy_executed(N: int, b: int, wave: int | None = None) -> float:
+    skipped = 8 * N * (1.0 - injection_tile_fraction(b, wave))
     return (88 * (N - 1) + 8 * N + 48 - skipped) * b ** 3

@@ -108,7 +108,7 @@ def test_generator_properties():
     assert fermi(10.0, 0.0, 0.025) == 0.0 and fermi(-10.0, 0.0, 0.025) == 1.0
 
 
-def test_flop_counts_match_baseline():
+def test_flop_counts_of_the_path():
     assert abs(flops_rgf(64, 512) - 4.975e11) / 4.975e11 < 1e-3
     assert abs(flops_rkd(64, 512) - 1.171e12) / 1.171e12 < 1e-3
     assert abs(flops_rgf_keldysh_energy(64, 512) - 1.511e12) / 1.511e12 < 1e-3
@@ -120,7 +120,13 @@ def test_flop_counts_match_baseline():
     assert n_gemm == 699
     # the 64 last injection GEMMs compute 36 of their 64 tiles (anti-Hermitian G^<)
     assert flops_energy_executed(64, 512) == (8 * n_gemm + 8 * 64 - 8 * 64 * 28 / 64) * 512 ** 3
-    assert flops_energy_executed(4, 160) == (8 * (11 * 3 + 6) + 8 * 4) * 160 ** 3
+    assert flops_energy_executed(64, 512, 96) == flops_energy_executed(64, 512)
+    # b = 160: 32 x 32 tiles, 15 of 25 computed; b = 512 at a wave of 4
+    # (256 tiles of 64 x 64): 32 x 32 tiles, 136 of 256
+    assert flops_energy_executed(4, 160) == (8 * (11 * 3 + 6) + 8 * 4 - 8 * 4 * 10 / 25) * 160 ** 3
+    assert flops_energy_executed(64, 512, 4) == (8 * n_gemm + 8 * 64 - 8 * 64 * 120 / 256) * 512 ** 3
+    # b = 100: register-staged GEMM, nothing mirrored
+    assert flops_energy_executed(4, 100) == (8 * (11 * 3 + 6) + 8 * 4) * 100 ** 3
 
 
 def test_packed_layout_roundtrip_on_cpu_tensors():
