@@ -8,9 +8,9 @@ TEST INFRASTRUCTURE ONLY (see oracle/backend.py).
 above it (multipliers ``:58-69``, w x w Schur update ``:72-89``), the
 downward pass fills the w-band (``:100-128``).  Two defects of the reference
 are not reproduced: it reads a ``nrsType`` field that ``BlockMatrix`` does
-not have (``:13, 17, 33-35``; ``src/blockmatrix.jl:10-17``), so it cannot run as
-written, and it writes the
-Schur updates back into ``Min`` (``:86``); here the input is copied first.
+not have (``:13, 17, 33-35``; ``src/blockmatrix.jl:10-17``), so it cannot
+run as written, and it writes the Schur updates back into ``Min``
+(``:86``); here the input is copied first.
 
 ``bm_fuse_to_tridiagonal`` restates ``src/blockmatrix.jl:1075-1106``: w
 adjacent layers fused into one block, through the dense matrix.
