@@ -8,9 +8,9 @@
 # whole bndiag_of_inv_rgf_local! / keldyshndiag! / bndiag_of_inv_ddrgf!
 # calls when the caller opts in.
 #
-# This file is the binding a LibNEGF.jl maintainer adds; Julia is not part
-# of this build's toolchain, so it is exercised through the same C ABI from
-# the Python host mirror (libnegf_amd) and its tests.
+# This file is the binding a LibNEGF.jl maintainer adds.  The C ABI it
+# calls is the one the Python host mirror (libnegf_amd) and its tests
+# exercise; this file itself has not been run under Julia.
 
 using AMDGPU
 import LinearAlgebra
