@@ -50,7 +50,10 @@ extern "C" {
 int negf_version(void);
 /* Static description of a status code. */
 const char* negf_status_string(int status);
-/* Frees the cached sweep workspaces (the next sweep reallocates). */
+/* Frees the cached sweep workspaces and the per-stream second streams of the
+ * split-step inverse (the next call recreates them).  At most 64 calling
+ * streams per process hold such a second stream at a time; beyond that the
+ * split-step inverse returns NEGF_HIP_ERROR until this is called. */
 int negf_workspace_release(void);
 /* Autotuner knob: tile variant of the MFMA ZGEMM for k > 32, process-wide
  * (0 (default): operands loaded straight into two LDS stages when every tile
@@ -71,6 +74,14 @@ int negf_gemm_variant(int32_t variant);
  * Wider panels: fewer passes over the matrix, more latency-bound factoring
  * per pass. */
 int negf_inv_variant(int32_t variant);
+/* The schedule the inverse runs for (n, batch, variant): out[0] = column strip
+ * width, out[1] = strips per panel, out[2] = W buffers of n * out[0] * out[1]
+ * elements per instance.  Returns 1 for split steps, 0 in place, < 0 for bad
+ * arguments.  No GPU work. */
+int negf_zinv_schedule(int32_t n, int32_t batch, int32_t variant, int32_t out[3]);
+/* Device workspace bytes of a batched n x n inverse; a workspace for n serves
+ * every smaller block size at the same batch.  No GPU work. */
+int64_t negf_zinv_workspace(int32_t n, int32_t batch);
 
 /* ------------------------------------------------------------------ per-op */
 

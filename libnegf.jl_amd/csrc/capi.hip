@@ -87,12 +87,29 @@ const char* negf_status_string(int status) {
 int negf_gemm_variant(int32_t variant) { return zgemm_set_variant(variant); }
 int negf_inv_variant(int32_t variant) { return zinv_set_variant(variant); }
 
+int negf_zinv_schedule(int32_t n, int32_t batch, int32_t variant, int32_t out[3]) {
+  if (!out) return NEGF_BAD_ARG;
+  int S = 0, PS = 0, nbuf = 0;
+  const int rc = zinv_schedule(n, batch, variant, &S, &PS, &nbuf);
+  if (rc < 0) return rc;
+  out[0] = S;
+  out[1] = PS;
+  out[2] = nbuf;
+  return rc;
+}
+
+int64_t negf_zinv_workspace(int32_t n, int32_t batch) {
+  if (n <= 0 || n > 2048 || batch <= 0) return 0;
+  return (int64_t)zinv_work_bytes(n, batch);
+}
+
 int negf_workspace_release(void) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   (void)hipDeviceSynchronize();
   for (auto& kv : g_ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
   g_ws.clear();
+  zinv_aux_release_all();
   return NEGF_OK;
 }
 
@@ -355,7 +372,8 @@ int energies_enqueue(int32_t nblocks, int32_t b, const void* H_diag, const void*
                      const void* gamma_L, const void* gamma_R, const double* energies,
                      int32_t n_energies, double eta, double mu_L, double mu_R, double kT,
                      void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
-                     int32_t wave, int32_t streams, void* w, hipStream_t s) {
+                     int32_t wave, int32_t streams, const InvAux* aux, hipEvent_t* events,
+                     void* w, hipStream_t s) {
   std::vector<int32_t> sz(nblocks, b);
   Layout L(nblocks, sz.data());
   const size_t ew = energies_work_bytes(L, n_energies, wave, streams);
@@ -375,16 +393,23 @@ int energies_enqueue(int32_t nblocks, int32_t b, const void* H_diag, const void*
   EnergyIn in{Hp, (const zc*)H_upper, (const zc*)gamma_L, (const zc*)gamma_R, energies,
               n_energies, eta, mu_L, mu_R, kT};
   EnergyOut out{(zc*)GR_diag, (zc*)Gless_diag, transmission, info};
-  return sweep_energies(L, in, out, wave, streams, pool, w, s);
+  EnergyStreams es{pool, aux, events};
+  return sweep_energies(L, in, out, wave, streams, es, w, s);
 }
 }  // namespace
 
 // A captured energy job: its own workspace (held until destroy) and the
 // instantiated graph.
+// The streams and events the capture records (the inverse's second stream
+// and its fork / join events per group, the groups' fork / join events) are
+// the graph's own, created before the capture and destroyed with the graph,
+// so nothing recorded into it is destroyed or shared while it exists.
 struct negf_energies_graph {
   void* work = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<InvAux> aux;
+  std::vector<hipEvent_t> events;
 };
 
 extern "C" {
@@ -404,7 +429,7 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b, const void* H_diag,
   if (rc) return rc;
   return energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies,
                           eta, mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave,
-                          streams, w, st(stream));
+                          streams, nullptr, nullptr, w, st(stream));
 }
 
 int negf_rgf_keldysh_energies_graph_create(
@@ -426,10 +451,22 @@ int negf_rgf_keldysh_energies_graph_create(
     delete g;
     return NEGF_NO_MEMORY;
   }
+  g->aux.assign(streams, InvAux{nullptr, nullptr, nullptr});
+  for (auto& a : g->aux)
+    if (zinv_aux_create(&a) != NEGF_OK) {
+      negf_energies_graph_destroy(g);
+      return NEGF_HIP_ERROR;
+    }
+  g->events.assign(1 + streams, nullptr);
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      negf_energies_graph_destroy(g);
+      return NEGF_HIP_ERROR;
+    }
   hipStream_t cs = nullptr;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
-    (void)hipFree(g->work);
-    delete g;
+    negf_energies_graph_destroy(g);
     return NEGF_HIP_ERROR;
   }
   rc = NEGF_OK;
@@ -437,7 +474,7 @@ int negf_rgf_keldysh_energies_graph_create(
   if (rc == NEGF_OK) {
     rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, eta,
                           mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave, streams,
-                          g->work, cs);
+                          g->aux.data(), g->events.data(), g->work, cs);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(cs, &graph);
     if (rc == NEGF_OK && (e != hipSuccess || !graph)) rc = NEGF_HIP_ERROR;
@@ -462,12 +499,13 @@ int negf_energies_graph_launch(negf_energies_graph* g, void* stream) {
 
 int negf_energies_graph_destroy(negf_energies_graph* g) {
   if (!g) return NEGF_OK;
+  (void)hipDeviceSynchronize();
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
-  if (g->work) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(g->work);
-  }
+  for (auto& a : g->aux) zinv_aux_destroy(&a);
+  for (auto& e : g->events)
+    if (e) (void)hipEventDestroy(e);
+  if (g->work) (void)hipFree(g->work);
   delete g;
   return NEGF_OK;
 }

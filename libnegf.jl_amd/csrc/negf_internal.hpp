@@ -114,18 +114,41 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s);
 // Batched explicit inverse by blocked Gauss-Jordan with partial pivoting.
 // Ainv[idx] = inv(A[idx]).  `X` is the in-place work matrix (n x n, ld n,
 // one per instance) and may be the same buffer as A when A may be destroyed.
+// Second stream of the split-step inverse (the strip updates beside the
+// factoring) with its fork / join events.  All null: the library's own slot
+// for the calling stream (created on first use, kept until
+// zinv_aux_release_all).  A caller that captures launches into a graph owns
+// one for the graph's lifetime instead (zinv_aux_create / _destroy).
+struct InvAux {
+  hipStream_t aux;
+  hipEvent_t fork, join;
+};
+int zinv_aux_create(InvAux* a);
+void zinv_aux_destroy(InvAux* a);
+// Destroys the library's per-stream slots (the device is synchronized first).
+void zinv_aux_release_all();
+
 struct InvWork {
   zc* X;        // n*n*batch
-  zc* W;        // 3*n*nb*batch (W of panels J, J+1, J+2)
+  zc* W;        // 3*zinv_w_elems(n)*batch (W of panels J, J+1, J+2)
   int* piv;     // n*batch
   int* used;    // n*batch
+  int nmax;     // the n and batch the buffers were carved for: zinv_launch
+  int batchmax; // rejects larger problems
+  InvAux aux;   // null members: the per-calling-stream slot
 };
 size_t zinv_work_bytes(int n, int batch);
+// Elements of one W buffer per instance that serve every block size <= n.
+size_t zinv_w_elems(int n);
 // Schedule of the inverse (0: automatic, 1 / 2: in place with panels of 1 /
 // 2 strips, 3 / 4 / 5: split steps with panels of 1 / 2 / 4 strips),
 // process-wide.
 int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);
+// The schedule zinv_launch runs for (n, batch, variant): strip width, strips
+// per panel and W buffers (each n * strip * panel per instance).  Returns 1
+// for split steps, 0 in place, < 0 for invalid arguments.
+int zinv_schedule(int n, int batch, int variant, int* strip, int* panel, int* nbuf);
 // A: input (not modified unless A == X), Ainv: output; info[idx] = 0 or the
 // 1-based column of the first exactly-zero pivot.
 int zinv_launch(int n, const zc* A, long long lda, long long strideA,

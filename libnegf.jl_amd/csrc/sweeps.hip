@@ -517,10 +517,11 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
 // One stream's share of the energy grid: energies [e_begin, e_end) in waves.
 static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                           int e_begin, int e_end, const zc* z, const zc* fL, const zc* fR,
-                          int* einfo, void* work, hipStream_t s) {
+                          int* einfo, const InvAux* aux, void* work, hipStream_t s) {
   const int N = L.N, b = L.b[0];
   SweepWork w;
   carve(work, L, wave, &w);
+  if (aux) w.inv.aux = *aux;
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
   zc* Gdw = Sdw + (size_t)wave * L.nd;   // G^R diagonal when the caller does not keep it
   RC(set_one(w.ones, s));
@@ -573,7 +574,7 @@ size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {
 // Gauss-Jordan panels, one workgroup per energy) overlap another group's
 // GEMMs.  The caller's stream waits for all groups.
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   int streams, hipStream_t* pool, void* work, hipStream_t s) {
+                   int streams, const EnergyStreams& es, void* work, hipStream_t s) {
   const int nE = in.nE;
   char* p = (char*)work;
   zc* z = (zc*)p;
@@ -585,39 +586,46 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   const size_t gbytes = (energies_group_bytes(L, wave) + 255) / 256 * 256;
   hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
                      in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
-  if (streams <= 1) return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, groups, s);
-  hipEvent_t fork;
-  if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return Status::HIP_ERROR;
-  if (hipEventRecord(fork, s) != hipSuccess) {
-    (void)hipEventDestroy(fork);
-    return Status::HIP_ERROR;
+  if (streams <= 1)
+    return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, es.aux, groups, s);
+  hipStream_t* pool = es.pool;
+  // fork / join events: the caller's (kept for a captured graph's lifetime),
+  // else created here and destroyed on return
+  std::vector<hipEvent_t> own;
+  hipEvent_t* ev = es.events;
+  if (!ev) {
+    own.assign(1 + streams, nullptr);
+    for (auto& e : own)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        for (auto& x : own)
+          if (x) (void)hipEventDestroy(x);
+        return Status::HIP_ERROR;
+      }
+    ev = own.data();
   }
+  int rc = hipEventRecord(ev[0], s) == hipSuccess ? Status::OK : Status::HIP_ERROR;
   // contiguous chunks, each a whole number of waves where possible
   const int per = ((nE + streams - 1) / streams + wave - 1) / wave * wave;
-  int rc = Status::OK;
   for (int g = 0; g < streams && rc == Status::OK; ++g) {
     const int e0 = g * per, e1 = (g + 1) * per < nE ? (g + 1) * per : nE;
     if (e0 >= e1) break;
-    if (hipStreamWaitEvent(pool[g], fork, 0) != hipSuccess) {
+    if (hipStreamWaitEvent(pool[g], ev[0], 0) != hipSuccess) {
       rc = Status::HIP_ERROR;
       break;
     }
-    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, groups + g * gbytes, pool[g]);
-    // join group g back into the caller's stream; where an event cannot carry
-    // the edge, wait for the group on the host so later work on `s` still
-    // follows it
-    hipEvent_t j;
-    bool joined = false;
-    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess) {
-      joined = hipEventRecord(j, pool[g]) == hipSuccess && hipStreamWaitEvent(s, j, 0) == hipSuccess;
-      (void)hipEventDestroy(j);
-    }
+    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, es.aux ? es.aux + g : nullptr,
+                        groups + g * gbytes, pool[g]);
+    // join group g back into the caller's stream; where the event cannot
+    // carry the edge, wait for the group on the host so later work on `s`
+    // still follows it
+    const bool joined = hipEventRecord(ev[1 + g], pool[g]) == hipSuccess &&
+                        hipStreamWaitEvent(s, ev[1 + g], 0) == hipSuccess;
     if (!joined) {
       (void)hipStreamSynchronize(pool[g]);
       if (rc == Status::OK) rc = Status::HIP_ERROR;
     }
   }
-  (void)hipEventDestroy(fork);
+  for (auto& x : own) (void)hipEventDestroy(x);
   return rc;
 }
 

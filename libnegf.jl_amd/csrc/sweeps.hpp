@@ -56,8 +56,19 @@ size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams);
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
               hipStream_t s);
+// Streams and events of the energy job.  `pool`: the group streams (streams
+// > 1).  `aux`: one inverse aux set per group, or null for the library's
+// per-stream slots.  `events`: 1 + streams events (fork, then one join per
+// group), or null to create and destroy them within the call.  A caller that
+// captures the job into a graph passes its own aux sets and events and keeps
+// them until the graph is destroyed.
+struct EnergyStreams {
+  hipStream_t* pool;
+  const InvAux* aux;
+  hipEvent_t* events;
+};
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   int streams, hipStream_t* pool, void* work, hipStream_t s);
+                   int streams, const EnergyStreams& es, void* work, hipStream_t s);
 
 // DD-RGF (uniform block size b; A.low must be given)
 size_t ddrgf_work_bytes(int N, int b, int batch, const int* levels, int nlev);

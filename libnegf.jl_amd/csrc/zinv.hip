@@ -645,46 +645,36 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel, 
 
 // Second stream per calling stream for the split steps (update kernel beside
 // the factoring kernel), with its fork / join events; created on first use,
-// default priority, non-blocking.
+// default priority, non-blocking.  A slot lives until zinv_aux_release_all:
+// keyed by the stream's handle, it is looked up again on every call from that
+// stream.  When the table is full a new calling stream gets an error, never
+// another stream's slot (two callers recording and waiting on one fork event
+// from different host threads could pair the wrong fork with a launch).
 struct AuxSlot {
   int device;
   hipStream_t key;
-  hipStream_t aux;
-  hipEvent_t fork, join;
+  InvAux a;
 };
 constexpr int MAX_AUX = 64;
 AuxSlot g_aux[MAX_AUX];
 int g_naux = 0;
 std::mutex g_aux_mu;
 
-int aux_for(hipStream_t s, AuxSlot** out) {
+int aux_for(hipStream_t s, InvAux* out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return Status::HIP_ERROR;
   std::lock_guard<std::mutex> lk(g_aux_mu);
   for (int i = 0; i < g_naux; ++i)
     if (g_aux[i].device == dev && g_aux[i].key == s) {
-      *out = &g_aux[i];
+      *out = g_aux[i].a;
       return Status::OK;
     }
-  if (g_naux == MAX_AUX) {
-    // more calling streams than slots: share one of this device's (the events
-    // order the work of one call; calls are enqueued one after another)
-    for (int i = 0; i < g_naux; ++i)
-      if (g_aux[i].device == dev) {
-        *out = &g_aux[i];
-        return Status::OK;
-      }
-    return Status::HIP_ERROR;
-  }
-  AuxSlot a{dev, s, nullptr, nullptr, nullptr};
-  if (hipStreamCreateWithFlags(&a.aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) {
-    (void)hipGetLastError();
-    return Status::HIP_ERROR;
-  }
-  g_aux[g_naux] = a;
-  *out = &g_aux[g_naux++];
+  if (g_naux == MAX_AUX) return Status::HIP_ERROR;
+  AuxSlot a{dev, s, {nullptr, nullptr, nullptr}};
+  const int rc = zinv_aux_create(&a.a);
+  if (rc) return rc;
+  g_aux[g_naux++] = a;
+  *out = a.a;
   return Status::OK;
 }
 
@@ -703,9 +693,11 @@ template <int PT, int PS>
 int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
                  long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
   constexpr int S = 16;
-  AuxSlot* ax = nullptr;
-  int rc = aux_for(s, &ax);
+  InvAux axv = w.aux;
+  int rc = Status::OK;
+  if (!axv.aux || !axv.fork || !axv.join) rc = aux_for(s, &axv);
   if (rc) return rc;
+  const InvAux* ax = &axv;
   if (A != w.X || lda != n || strideA != (long long)n * n) {
     rc = zcopy2d_launch(n, n, A, lda, strideA, w.X, n, (long long)n * n, batch, s);
     if (rc) return rc;
@@ -821,23 +813,66 @@ int g_inv_variant = 0;
 
 }  // namespace
 
+int zinv_aux_create(InvAux* a) {
+  a->aux = nullptr;
+  a->fork = a->join = nullptr;
+  if (hipStreamCreateWithFlags(&a->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&a->join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    zinv_aux_destroy(a);
+    return Status::HIP_ERROR;
+  }
+  return Status::OK;
+}
+
+void zinv_aux_destroy(InvAux* a) {
+  if (a->aux) (void)hipStreamSynchronize(a->aux);
+  if (a->fork) (void)hipEventDestroy(a->fork);
+  if (a->join) (void)hipEventDestroy(a->join);
+  if (a->aux) (void)hipStreamDestroy(a->aux);
+  a->aux = nullptr;
+  a->fork = a->join = nullptr;
+}
+
+void zinv_aux_release_all() {
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  int cur = 0;
+  const bool have_dev = hipGetDevice(&cur) == hipSuccess;
+  for (int i = 0; i < g_naux; ++i) {
+    if (have_dev) (void)hipSetDevice(g_aux[i].device);
+    zinv_aux_destroy(&g_aux[i].a);
+  }
+  if (have_dev) (void)hipSetDevice(cur);
+  g_naux = 0;
+}
+
 int zinv_set_variant(int v) {
   if (v < 0 || v > 5) return Status::BAD_ARG;
   g_inv_variant = v;
   return Status::OK;
 }
 
+size_t zinv_w_elems(int n) {
+  // one W buffer per instance: the widest panel any schedule uses at this n
+  // is nb_for(n) columns.  nb_for falls as n grows (64 -> 16 -> 8), so a
+  // smaller block can need more than a larger one: a workspace carved for n
+  // holds the largest need over every n' <= n.
+  size_t c = (size_t)n * nb_for(n);
+  if (n > 512) c = c > (size_t)512 * nb_for(512) ? c : (size_t)512 * nb_for(512);
+  if (n > 1024) c = c > (size_t)1024 * nb_for(1024) ? c : (size_t)1024 * nb_for(1024);
+  return c;
+}
+
 size_t zinv_work_bytes(int n, int batch) {
-  const size_t nb = nb_for(n);
   size_t b = 0;
   b += sizeof(zc) * (size_t)n * n * batch;   // X
-  b += 3 * sizeof(zc) * (size_t)n * nb * batch;  // W (three buffers: panels J .. J+2)
+  b += 3 * sizeof(zc) * zinv_w_elems(n) * batch;  // W (three buffers: panels J .. J+2)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
   return b + 1024;
 }
 
 void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
-  const size_t nb = nb_for(n);
   char* p = (char*)base;
   auto take = [&](size_t bytes) {
     char* r = p;
@@ -845,9 +880,29 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
     return r;
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
-  w->W = (zc*)take(3 * sizeof(zc) * (size_t)n * nb * batch);
+  w->W = (zc*)take(3 * sizeof(zc) * zinv_w_elems(n) * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
+  w->nmax = n;
+  w->batchmax = batch;
+  w->aux = InvAux{nullptr, nullptr, nullptr};
+}
+
+int zinv_schedule(int n, int batch, int variant, int* strip, int* panel, int* nbuf) {
+  if (n <= 0 || n > 2048 || batch <= 0 || variant < 0 || variant > 5) return Status::BAD_ARG;
+  const int v = variant;
+  const bool split = n <= 512 && (v >= 3 || (v == 0 && n > 256 && batch >= 32));
+  *strip = n <= 512 ? 16 : (n <= 1024 ? 8 : 4);
+  if (split) {
+    *panel = v == 3 ? 1 : (v == 4 ? 2 : 4);
+    *nbuf = 3;
+  } else {
+    // in place (v >= 3 with n > 512 falls back to the automatic panel choice)
+    const bool pairs = v == 2 || ((v == 0 || v >= 3) && batch >= 32);
+    *panel = pairs ? 2 : 1;
+    *nbuf = 2;
+  }
+  return split ? 1 : 0;
 }
 
 int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
@@ -855,22 +910,24 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1) || ldinv < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
   if (n > 2048) return Status::TOO_LARGE;
-  const int v = g_inv_variant;
-  const bool split = n <= 512 && (v >= 3 || (v == 0 && n > 256 && batch >= 32));
+  // the workspace was carved for at most nmax x nmax blocks and batchmax
+  // instances (zinv_work_carve); the schedule's W buffers must fit in it
+  if (n > w.nmax || batch > w.batchmax) return Status::BAD_ARG;
+  int S = 0, PS = 0, nbuf = 0;
+  const int split = zinv_schedule(n, batch, g_inv_variant, &S, &PS, &nbuf);
+  if (split < 0) return split;
+  if ((size_t)nbuf * n * S * PS > 3 * zinv_w_elems(w.nmax)) return Status::BAD_ARG;
   if (split) {
-    const int ps = v == 3 ? 1 : (v == 4 ? 2 : 4);
     if (n <= 256) {
-      if (ps == 1) return run_split<256, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-      if (ps == 2) return run_split<256, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+      if (PS == 1) return run_split<256, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+      if (PS == 2) return run_split<256, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
       return run_split<256, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
     }
-    if (ps == 1) return run_split<512, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
-    if (ps == 2) return run_split<512, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (PS == 1) return run_split<512, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
+    if (PS == 2) return run_split<512, 2>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
     return run_split<512, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
   }
-  // in place (v >= 3 with n > 512 falls back to the automatic panel choice)
-  const bool pairs = v == 2 || ((v == 0 || v >= 3) && batch >= 32);
-  if (!pairs) {
+  if (PS == 1) {
     if (n <= 256) return run<256, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
     if (n <= 512) return run<512, 1, 16, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);
     if (n <= 1024) return run<1024, 1, 8, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);

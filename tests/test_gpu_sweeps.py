@@ -287,3 +287,49 @@ def test_rgf_production_shape_vs_oracle():
     err = max_block_err(G, ref)
     print(f"  max block relerr {err:.2e}", flush=True)
     assert err < 2e-11
+
+
+def _uneven_instances(sizes, shifts, seed):
+    """One block-tridiagonal matrix with the given block sizes, shifted by
+    z_k I per instance: A_k = z_k I - H, H a random Hermitian-coupled band."""
+    rng = np.random.default_rng(seed)
+    N = len(sizes)
+    cn = lambda m, n: (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))) / np.sqrt(2 * n)
+    H_d = []
+    for s in sizes:
+        X = cn(s, s)
+        H_d.append((X + X.conj().T) / 2 + np.diag(rng.uniform(-2, 2, s)))
+    H_u = [cn(sizes[i], sizes[i + 1]) for i in range(N - 1)]
+    from libnegf_amd.synthetic import BlockTridiag
+    out = []
+    for z in shifts:
+        out.append(BlockTridiag(
+            [np.asfortranarray(z * np.eye(s) - h) for s, h in zip(sizes, H_d)],
+            [np.asfortranarray(-u) for u in H_u],
+            [np.asfortranarray(-u.conj().T) for u in H_u]))
+    return out
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("sizes,batch,inv_variant", [
+    ([600, 512, 600], 32, 0),    # the 512 layer takes split steps (64-column panels)
+    ([600, 512, 600], 1, 4),     # forced split steps at one instance
+    ([1100, 1024], 32, 0),       # the 1024 layer: 8-column strips, two per panel
+])
+def test_rgf_uneven_layers_smaller_block_wider_panel(sizes, batch, inv_variant):
+    """Regression: the inverse workspace of a sweep is carved for the largest
+    block, but a smaller layer can take a schedule with wider W panels.
+    Every instance is checked against the oracle on a sample."""
+    print(f"rgf uneven sizes={sizes} batch={batch} inv_variant={inv_variant}", flush=True)
+    shifts = [complex(E, 1e-2) for E in np.linspace(-0.9, 0.9, batch)]
+    mats = _uneven_instances(sizes, shifts, seed=5)
+    api.call("negf_inv_variant", inv_variant)
+    try:
+        out = _gpu_rgf(mats)
+        torch.cuda.synchronize()
+    finally:
+        api.call("negf_inv_variant", 0)
+    for k in sorted({0, batch // 2, batch - 1}):
+        err = max_block_err(_band(out.to_host(k)), rgf_ref(mats[k]))
+        print(f"  instance {k}: max block relerr {err:.2e}")
+        assert err < 1e-12
