@@ -169,6 +169,24 @@ int negf_rgf(int32_t nblocks, const int32_t* block_sizes,
              void* g_diag, void* g_upper, void* g_lower,
              int32_t batch, int32_t* info, void* stream);
 
+/* negf_rgf without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as negf_rgf; the workspace is a static device array of
+ * NEGF_RGF_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.
+ * Returns NEGF_TOO_LARGE when the sweep's workspace (about
+ * 16 * batch * (sum b_i^2 + 2 sum b_i b_{i+1} + 4 bmax^2 + 3 * 64 * bmax)
+ * bytes) does not fit.  Calls that overlap in time on one GPU share the
+ * array and race: the caller orders them.  The diagonal-block inverses
+ * always take the in-place schedule (no second stream). */
+#define NEGF_RGF_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int negf_rgf_noalloc(int32_t nblocks, const int32_t* block_sizes,
+                     const void* diag, int64_t diag_bstride,
+                     const void* upper, int64_t upper_bstride,
+                     const void* lower, int64_t lower_bstride,
+                     void* g_diag, void* g_upper, void* g_lower,
+                     int32_t batch, int32_t* info, void* stream);
+
 /* Recursive Keldysh: band(A^{-1} Sigma A^{-H}) for a Hermitian
  * block-tridiagonal Sigma given in Hermitian storage (diag + upper); output
  * gn_diag / gn_upper in the same storage.  The optional RGF outputs reuse

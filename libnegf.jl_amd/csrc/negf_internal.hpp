@@ -136,6 +136,7 @@ struct InvWork {
   int nmax;     // the n and batch the buffers were carved for: zinv_launch
   int batchmax; // rejects larger problems
   InvAux aux;   // null members: the per-calling-stream slot
+  int inplace_only;  // 1: never the split steps (no second stream is used)
 };
 size_t zinv_work_bytes(int n, int batch);
 // Elements of one W buffer per instance that serve every block size <= n.

@@ -489,9 +489,10 @@ static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* 
 
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
-              hipStream_t s) {
+              hipStream_t s, bool inplace_inverse) {
   SweepWork w;
   carve(work, L, batch, &w);
+  w.inv.inplace_only = inplace_inverse ? 1 : 0;
   RC(set_one(w.ones, s));
   Source S{};
   S.diag = A.diag; S.diag_bs = A.diag_bs;

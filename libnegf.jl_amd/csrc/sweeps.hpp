@@ -53,9 +53,11 @@ struct EnergyOut {
 size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams);
 
+// inplace_inverse: the diagonal-block inverses never take the split steps
+// (which run on a second stream the library creates on first use).
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
-              hipStream_t s);
+              hipStream_t s, bool inplace_inverse = false);
 // Streams and events of the energy job.  `pool`: the group streams (streams
 // > 1).  `aux`: one inverse aux set per group, or null for the library's
 // per-stream slots.  `events`: 1 + streams events (fork, then one join per

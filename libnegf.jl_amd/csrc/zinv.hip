@@ -886,6 +886,7 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
   w->nmax = n;
   w->batchmax = batch;
   w->aux = InvAux{nullptr, nullptr, nullptr};
+  w->inplace_only = 0;
 }
 
 int zinv_schedule(int n, int batch, int variant, int* strip, int* panel, int* nbuf) {
@@ -914,8 +915,10 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   // instances (zinv_work_carve); the schedule's W buffers must fit in it
   if (n > w.nmax || batch > w.batchmax) return Status::BAD_ARG;
   int S = 0, PS = 0, nbuf = 0;
-  const int split = zinv_schedule(n, batch, g_inv_variant, &S, &PS, &nbuf);
+  int split = zinv_schedule(n, batch, g_inv_variant, &S, &PS, &nbuf);
   if (split < 0) return split;
+  if (split && w.inplace_only)   // the automatic in-place panel choice instead
+    split = zinv_schedule(n, batch, batch >= 32 ? 2 : 1, &S, &PS, &nbuf);
   if ((size_t)nbuf * n * S * PS > 3 * zinv_w_elems(w.nmax)) return Status::BAD_ARG;
   if (split) {
     if (n <= 256) {

@@ -194,3 +194,28 @@ def test_inverse_workspace_serves_every_smaller_block():
     assert lib.negf_zinv_schedule(512, 32, 0, out) == 1 and out[0] * out[1] == 64
     assert lib.negf_zinv_schedule(0, 1, 0, out) < 0
     assert lib.negf_zinv_schedule(16, 1, 6, out) < 0
+
+
+def test_rgf_noalloc_refuses_beyond_its_scratch():
+    """negf_rgf_noalloc checks the sweep's workspace against its static
+    device array (64 MiB per GPU) before any HIP call: N = 64 blocks of 512
+    (about 1 GiB) is refused with NEGF_TOO_LARGE; bad sizes with
+    NEGF_BAD_ARG.  The pointers are never dereferenced."""
+    from libnegf_amd._lib import LIB_PATH, load
+    if not LIB_PATH.exists():
+        pytest.skip("libnegf_hip.so not built")
+    lib = load()
+    dummy = ctypes.c_void_p(16)
+    sizes = (ctypes.c_int32 * 64)(*([512] * 64))
+    rc = lib.negf_rgf_noalloc(64, sizes, dummy, 0, dummy, 0, dummy, 0, dummy, None, None, 1, None,
+                              None)
+    assert rc == -2 and b"beyond" in lib.negf_status_string(rc)
+    # 64 MiB: 16 * (2 * 48 * 400^2 ...) just beyond at one instance of N = 48, b = 400
+    sizes = (ctypes.c_int32 * 48)(*([400] * 48))
+    assert lib.negf_rgf_noalloc(48, sizes, dummy, 0, dummy, 0, dummy, 0, dummy, None, None, 1,
+                                None, None) == -2
+    bad = (ctypes.c_int32 * 2)(64, 0)
+    assert lib.negf_rgf_noalloc(2, bad, dummy, 0, dummy, 0, dummy, 0, dummy, None, None, 1, None,
+                                None) == -1
+    assert lib.negf_rgf_noalloc(2, None, dummy, 0, dummy, 0, dummy, 0, dummy, None, None, 1, None,
+                                None) == -1
