@@ -97,6 +97,23 @@ int negf_zgemm_strided_batched(char transA, char transB, int32_t m, int32_t n, i
                                const double beta[2], void* C, int64_t ldc, int64_t strideC,
                                int32_t batch, void* stream);
 
+/* negf_zgemm_strided_batched in a chosen arithmetic (operands and results
+ * stay complex128 in memory): precision 0 = FP64 MFMA (the same call);
+ * 1 = FP32 MFMA with FP32 accumulation (operands rounded to FP32; the
+ * 4-real-product complex form) where the LDS-DMA kernel applies, FP64
+ * otherwise; 2 = FP64 emulated on the int8 matrix cores (Ozaki scheme II:
+ * per row of op(A) / column of op(B) a power-of-two scale keeping beta bits,
+ * residues modulo n_moduli coprime moduli <= 256, 0 = 13, beta = 45 at
+ * k = 512; error at most 2^-beta of the row / column maxima per term).
+ * No counterpart in the reference: the reduced-precision products of the
+ * mixed-precision energy path. */
+int negf_zgemm_prec_strided_batched(char transA, char transB, int32_t m, int32_t n, int32_t k,
+                                    const double alpha[2], const void* A, int64_t lda,
+                                    int64_t strideA, const void* B, int64_t ldb, int64_t strideB,
+                                    const double beta[2], void* C, int64_t ldc, int64_t strideC,
+                                    int32_t batch, int32_t precision, int32_t n_moduli,
+                                    void* stream);
+
 /* LU with partial pivoting in place, LAPACK layout; ipiv (device, n int32,
  * 1-based) and info (device, 1 int32).  Replaces the getrf inside be_lu!
  * (src/backend_cpu.jl:147-169) and be_getrf! (:255-265, which discards ipiv). */
@@ -236,6 +253,33 @@ int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
  * and stream count (for sizing `wave` against free memory). */
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams);
+
+/* The energy job of negf_rgf_keldysh_energies with its complex products in
+ * a reduced or emulated precision (the diagonal-block inverses stay FP64):
+ * precision 1 = FP32 MFMA (4 real products, FP32 accumulation), 2 = FP64
+ * emulated on the int8 matrix cores (Ozaki scheme II, n_moduli moduli, 0 =
+ * 13), 0 = FP64.  After the pass, each energy's error estimate
+ * est = 2 u sum_i ||g_i||_F (u = 2^-24 for FP32, 2^-beta for the emulation,
+ * g_i the upward pass's diagonal-block inverses; calibrated against the
+ * oracle, see the tests) decides: energies with est > tol (or a non-finite
+ * est) are computed again in FP64 and their outputs replace the first ones.
+ * est_host[n_energies], fallback_host[n_energies] (1 = re-run) and
+ * *n_fallback are host outputs (each may be NULL); the call synchronizes the
+ * stream.  Other arguments as negf_rgf_keldysh_energies.  No counterpart in
+ * the reference (its reduced-precision intent: FieldType, test/common_to_test.jl:20-26). */
+int negf_rgf_keldysh_energies_mixed(int32_t nblocks, int32_t b, const void* H_diag,
+                                    const void* H_upper, const void* gamma_L,
+                                    const void* gamma_R, const double* energies,
+                                    int32_t n_energies, double eta, double mu_L, double mu_R,
+                                    double kT, void* GR_diag, void* Gless_diag,
+                                    double* transmission, int32_t* info, int32_t wave,
+                                    int32_t streams, int32_t precision, int32_t n_moduli,
+                                    double tol, double* est_host, int32_t* fallback_host,
+                                    int32_t* n_fallback, void* stream);
+/* Device workspace bytes negf_rgf_keldysh_energies_mixed uses for a wave. */
+int64_t negf_rgf_keldysh_energies_mixed_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
+                                                  int32_t wave, int32_t streams, int32_t precision,
+                                                  int32_t n_moduli);
 
 /* The same energy job captured once into a HIP graph (no reference
  * counterpart).  _create records every launch of negf_rgf_keldysh_energies

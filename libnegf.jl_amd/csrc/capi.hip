@@ -132,6 +132,41 @@ int negf_zgemm_strided_batched(char transA, char transB, int32_t m, int32_t n, i
   return zgemm_launch(g, st(stream));
 }
 
+int negf_zgemm_prec_strided_batched(char transA, char transB, int32_t m, int32_t n, int32_t k,
+                                    const double alpha[2], const void* A, int64_t lda,
+                                    int64_t strideA, const void* B, int64_t ldb, int64_t strideB,
+                                    const double beta[2], void* C, int64_t ldc, int64_t strideC,
+                                    int32_t batch, int32_t precision, int32_t n_moduli,
+                                    void* stream) {
+  if (!valid_op(transA) || !valid_op(transB) || !alpha || !beta) return NEGF_BAD_ARG;
+  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16)
+    return NEGF_BAD_ARG;
+  GemmArgs g{};
+  g.opA = opc(transA); g.opB = opc(transB);
+  g.m = m; g.n = n; g.k = k;
+  g.alpha = zz(alpha); g.beta = zz(beta);
+  g.A = (const zc*)A; g.lda = lda; g.strideA = strideA;
+  g.B = (const zc*)B; g.ldb = ldb; g.strideB = strideB;
+  g.C = (const zc*)C; g.ldc = ldc; g.strideC = strideC;
+  g.D = (zc*)C; g.ldd = ldc; g.strideD = strideC;
+  g.batch = batch;
+  g.prec = precision;
+  g.nmod = n_moduli;
+  if (precision == 2 && m > 0 && n > 0 && k > 0 && batch > 0) {
+    // the residue planes live in the per-stream cached workspace; C is read
+    // in the epilogue, after the operands' conversion, so C may not alias
+    // the workspace (it cannot: the workspace is the library's own)
+    const size_t bytes = zgemm_emu_work_bytes(m, n, k, strideA == 0 ? 1 : batch,
+                                              strideB == 0 ? 1 : batch, n_moduli);
+    void* w = nullptr;
+    int rc = workspace(stream, bytes, &w, 2);
+    if (rc) return rc;
+    g.work = w;
+    g.work_bytes = (long long)bytes;
+  }
+  return zgemm_launch(g, st(stream));
+}
+
 int negf_zgemm(char transA, char transB, int32_t m, int32_t n, int32_t k, const double alpha[2],
                const void* A, int64_t lda, const void* B, int64_t ldb, const double beta[2],
                void* C, int64_t ldc, void* stream) {
@@ -373,10 +408,10 @@ int energies_enqueue(int32_t nblocks, int32_t b, const void* H_diag, const void*
                      int32_t n_energies, double eta, double mu_L, double mu_R, double kT,
                      void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
                      int32_t wave, int32_t streams, const InvAux* aux, hipEvent_t* events,
-                     void* w, hipStream_t s) {
+                     void* w, hipStream_t s, const EnergyPrec* pr = nullptr) {
   std::vector<int32_t> sz(nblocks, b);
   Layout L(nblocks, sz.data());
-  const size_t ew = energies_work_bytes(L, n_energies, wave, streams);
+  const size_t ew = energies_work_bytes(L, n_energies, wave, streams, pr);
   hipStream_t* pool = nullptr;
   if (streams > 1) {
     int rc = stream_pool(streams, &pool);
@@ -394,7 +429,7 @@ int energies_enqueue(int32_t nblocks, int32_t b, const void* H_diag, const void*
               n_energies, eta, mu_L, mu_R, kT};
   EnergyOut out{(zc*)GR_diag, (zc*)Gless_diag, transmission, info};
   EnergyStreams es{pool, aux, events};
-  return sweep_energies(L, in, out, wave, streams, es, w, s);
+  return sweep_energies(L, in, out, wave, streams, es, w, s, pr);
 }
 }  // namespace
 
@@ -509,6 +544,167 @@ int negf_energies_graph_destroy(negf_energies_graph* g) {
   delete g;
   return NEGF_OK;
 }
+
+// ------------------------------------------------------------- mixed precision
+namespace {
+
+// The fallback energies' outputs, computed in FP64 into temporaries, copied
+// to their places in the caller's outputs (idx: their positions).
+__global__ void scatter_energy_outputs_kernel(long long nd, const int* idx, const zc* tGR,
+                                              const zc* tGL, const double* tT, const int* tI,
+                                              zc* GR, zc* GL, double* T, int* I) {
+  const int k = blockIdx.y;
+  const long long dst = (long long)idx[k];
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nd; i += 256LL * gridDim.x) {
+    if (GR) GR[dst * nd + i] = tGR[k * nd + i];
+    if (GL) GL[dst * nd + i] = tGL[k * nd + i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (T) T[dst] = tT[k];
+    if (I) I[dst] = tI[k];
+  }
+}
+
+constexpr int kFallbackWave = 32;
+
+struct MixedLayout {
+  size_t main;       // the reduced-precision job (and, later, the FP64 re-runs)
+  size_t est;        // per-energy estimates
+  size_t fb;         // fallback: energies, indices, temporary outputs
+  size_t total;
+};
+
+MixedLayout mixed_layout(int32_t nblocks, int32_t b, int32_t nE, int32_t wave, int32_t streams,
+                         const EnergyPrec& pr) {
+  std::vector<int32_t> sz(nblocks, b);
+  Layout L(nblocks, sz.data());
+  MixedLayout m;
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  m.main = up(energies_work_bytes(L, nE, wave, streams, &pr) + sizeof(zc) * (size_t)L.nd + 4096);
+  m.est = up(sizeof(double) * (size_t)nE);
+  const int fw = wave < kFallbackWave ? wave : kFallbackWave;
+  m.fb = up(sizeof(double) * nE) + up(sizeof(int) * nE) +
+         up(sizeof(zc) * (size_t)fw * 2 * L.nd) + up(sizeof(double) * fw) + up(sizeof(int) * fw);
+  m.total = m.main + m.est + m.fb;
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t negf_rgf_keldysh_energies_mixed_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
+                                                  int32_t wave, int32_t streams, int32_t precision,
+                                                  int32_t n_moduli) {
+  if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0 || streams <= 0) return 0;
+  if (precision < 0 || precision > 2) return 0;
+  double dummy = 0.0;
+  EnergyPrec pr{precision, n_moduli, &dummy};
+  return (int64_t)mixed_layout(nblocks, b, n_energies, wave, streams, pr).total;
+}
+
+int negf_rgf_keldysh_energies_mixed(int32_t nblocks, int32_t b, const void* H_diag,
+                                    const void* H_upper, const void* gamma_L,
+                                    const void* gamma_R, const double* energies,
+                                    int32_t n_energies, double eta, double mu_L, double mu_R,
+                                    double kT, void* GR_diag, void* Gless_diag,
+                                    double* transmission, int32_t* info, int32_t wave,
+                                    int32_t streams, int32_t precision, int32_t n_moduli,
+                                    double tol, double* est_host, int32_t* fallback_host,
+                                    int32_t* n_fallback, void* stream) {
+  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
+      !(tol >= 0.0))
+    return NEGF_BAD_ARG;
+  int32_t w0 = wave, s0 = streams;
+  int rc = energies_plan(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, kT,
+                         &w0, &s0);
+  if (rc || n_energies == 0) return rc;
+  double dummy = 0.0;
+  EnergyPrec pr{precision, n_moduli, &dummy};
+  if (wave == 0) {
+    // the largest wave whose workspace (with the emulation's planes and the
+    // fallback's temporaries) fits 90 % of free memory
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return NEGF_HIP_ERROR;
+    static const int sizes[] = {96, 64, 48, 32, 16, 8, 4, 2, 1};
+    w0 = 1;
+    for (int c : sizes) {
+      const int cand = c < n_energies ? c : n_energies;
+      if (mixed_layout(nblocks, b, n_energies, cand, s0, pr).total <= freeb / 10 * 9) {
+        w0 = cand;
+        break;
+      }
+    }
+  }
+  const MixedLayout ml = mixed_layout(nblocks, b, n_energies, w0, s0, pr);
+  void* w = nullptr;
+  rc = workspace(stream, ml.total, &w);
+  if (rc) return rc;
+  hipStream_t s = st(stream);
+  char* base = (char*)w;
+  double* est_dev = (double*)(base + ml.main);
+  pr.est = est_dev;
+  // 1. the whole grid in the reduced-precision form
+  rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, eta,
+                        mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, w0, s0, nullptr,
+                        nullptr, w, s, precision == 0 ? nullptr : &pr);
+  if (rc) return rc;
+  std::vector<double> est(n_energies, 0.0);
+  if (precision != 0) {
+    if (hipMemcpyAsync(est.data(), est_dev, sizeof(double) * n_energies, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return NEGF_HIP_ERROR;
+  }
+  // 2. energies whose estimate exceeds tol (or is not finite) again in FP64
+  std::vector<double> hE(n_energies);
+  if (hipMemcpy(hE.data(), energies, sizeof(double) * n_energies, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return NEGF_HIP_ERROR;
+  std::vector<double> fbE;
+  std::vector<int32_t> fbI;
+  for (int e = 0; e < n_energies; ++e) {
+    const bool fb = precision != 0 && !(est[e] <= tol);
+    if (fallback_host) fallback_host[e] = fb ? 1 : 0;
+    if (est_host) est_host[e] = est[e];
+    if (fb) {
+      fbE.push_back(hE[e]);
+      fbI.push_back(e);
+    }
+  }
+  if (n_fallback) *n_fallback = (int32_t)fbE.size();
+  if (fbE.empty()) return NEGF_OK;
+  std::vector<int32_t> sz(nblocks, b);
+  Layout L(nblocks, sz.data());
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  char* fb = base + ml.main + ml.est;
+  double* dE = (double*)fb;
+  int32_t* dI = (int32_t*)(fb + up(sizeof(double) * n_energies));
+  zc* tG = (zc*)(fb + up(sizeof(double) * n_energies) + up(sizeof(int) * n_energies));
+  const int fw = w0 < kFallbackWave ? w0 : kFallbackWave;
+  zc* tGL = tG + (size_t)fw * L.nd;
+  double* tT = (double*)((char*)tG + up(sizeof(zc) * (size_t)fw * 2 * L.nd));
+  int32_t* tI = (int32_t*)((char*)tT + up(sizeof(double) * fw));
+  const int nfb = (int)fbE.size();
+  if (hipMemcpyAsync(dE, fbE.data(), sizeof(double) * nfb, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dI, fbI.data(), sizeof(int32_t) * nfb, hipMemcpyHostToDevice, s) != hipSuccess)
+    return NEGF_HIP_ERROR;
+  for (int c0 = 0; c0 < nfb; c0 += fw) {
+    const int cnt = nfb - c0 < fw ? nfb - c0 : fw;
+    rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, dE + c0, cnt, eta, mu_L,
+                          mu_R, kT, GR_diag ? (void*)tG : nullptr, Gless_diag ? (void*)tGL : nullptr,
+                          transmission ? tT : nullptr, info ? tI : nullptr, cnt, 1, nullptr,
+                          nullptr, w, s, nullptr);
+    if (rc) return rc;
+    const int gx = (int)((L.nd + 255) / 256 < 64 ? (L.nd + 255) / 256 : 64);
+    hipLaunchKernelGGL(scatter_energy_outputs_kernel, dim3(gx, cnt), dim3(256), 0, s, L.nd,
+                       dI + c0, tG, tGL, tT, tI, (zc*)GR_diag, (zc*)Gless_diag, transmission, info);
+    if (hipGetLastError() != hipSuccess) return NEGF_HIP_ERROR;
+  }
+  return hipStreamSynchronize(s) == hipSuccess ? NEGF_OK : NEGF_HIP_ERROR;
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------------- DD-RGF
 int64_t negf_ddrgf_workspace(int32_t nblocks, int32_t b, int32_t batch,

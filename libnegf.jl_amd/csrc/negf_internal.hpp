@@ -99,8 +99,27 @@ struct GemmArgs {
   int batch;
   int inner;
   long long outerA, outerB, outerC, outerD;
+  // arithmetic of the product (operands and results stay complex128 in
+  // memory): 0 = FP64 MFMA; 1 = FP32 MFMA with FP32 accumulation (operands
+  // rounded to FP32, the 4-real-product complex form) where the launch takes
+  // the LDS-DMA kernel, FP64 otherwise; 2 = integer-emulated FP64 (Ozaki
+  // scheme II on int8 MFMA, zgemm_emu.hip)
+  int prec;
+  // prec = 2: device workspace for the residue planes (zgemm_emu_work_bytes)
+  // and the number of moduli (0: 13)
+  void* work;
+  long long work_bytes;
+  int nmod;
 };
 int zgemm_launch(const GemmArgs& g, hipStream_t s);
+// The integer-emulated product (prec = 2, k > 0), one problem.
+int zgemm_emu_launch(const GemmArgs& g, hipStream_t s);
+// Workspace of zgemm_emu_launch for an m x n x k product whose A / B hold
+// batchA / batchB distinct instances (1 for a broadcast operand).
+size_t zgemm_emu_work_bytes(int m, int n, int k, int batchA, int batchB, int nmod);
+int zgemm_emu_default_moduli();
+// beta (bits kept per row / column scale) for nmod moduli at this k.
+int zgemm_emu_params(int nmod, int k, int* beta);
 // Tile variant of the k > 32 GEMMs (0..8, see zgemm.hip), process-wide.
 int zgemm_set_variant(int v);
 

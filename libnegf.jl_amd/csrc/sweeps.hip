@@ -41,10 +41,32 @@ struct MW {
 };
 inline M ro(const MW& w) { return M{w.p, w.ld, w.bs}; }
 
+// Arithmetic of the products issued by this host thread (GemmArgs::prec and
+// the emulation's workspace), set for the duration of an energy job by
+// PrecScope; FP64 otherwise.
+struct PrecCtx {
+  int prec, nmod;
+  void* work;
+  long long bytes;
+};
+thread_local PrecCtx t_prec{0, 0, nullptr, 0};
+struct PrecScope {
+  PrecCtx saved;
+  explicit PrecScope(const PrecCtx& c) : saved(t_prec) { t_prec = c; }
+  ~PrecScope() { t_prec = saved; }
+};
+inline void apply_prec(GemmArgs& g) {
+  g.prec = t_prec.prec;
+  g.nmod = t_prec.nmod;
+  g.work = t_prec.work;
+  g.work_bytes = t_prec.bytes;
+}
+
 int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D, int batch,
          hipStream_t s, const zc* shift = nullptr, long long shift_stride = 0,
          const zc* scale = nullptr, long long scale_stride = 0, int herm = 0) {
   GemmArgs g{};
+  apply_prec(g);
   g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
   g.alpha = alpha; g.beta = beta;
   g.A = A.p; g.lda = A.ld; g.strideA = A.bs;
@@ -61,6 +83,7 @@ int gemm(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C
 GemmArgs ga(int opA, int opB, int m, int n, int k, zc alpha, M A, M B, zc beta, M C, MW D,
             int batch) {
   GemmArgs g{};
+  apply_prec(g);
   g.opA = opA; g.opB = opB; g.m = m; g.n = n; g.k = k;
   g.alpha = alpha; g.beta = beta;
   g.A = A.p; g.lda = A.ld; g.strideA = A.bs;
@@ -103,7 +126,45 @@ __global__ void energy_scalars_kernel(int n, const double* E, double eta, double
   fR[i] = zmake(0.5 * (1.0 - tanh(0.5 * (e - muR) / kT)), 0.0);
 }
 
+// nrm[e * N + i] = ||g_i||_F for the b x b blocks g_i of instance e (one
+// workgroup per (layer, instance); fixed summation order).
+__global__ __launch_bounds__(256) void block_fro_kernel(const zc* g, long long gbs, int b, int N,
+                                                         double* nrm) {
+  const int i = blockIdx.x, e = blockIdx.y;
+  const zc* x = g + (long long)e * gbs + (long long)i * b * b;
+  const long long n = (long long)b * b;
+  double acc = 0.0;
+  for (long long k = threadIdx.x; k < n; k += 256) acc = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, acc));
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nrm[(long long)e * N + i] = sqrt(red[0]);
+}
+
+// est[e] = c sum_i nrm[e * N + i]
+__global__ void energy_estimate_kernel(const double* nrm, int N, int nb, double c, double* est) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nb) return;
+  double s = 0.0;
+  for (int i = 0; i < N; ++i) s += nrm[(long long)e * N + i];
+  est[e] = c * s;
+}
+
 }  // namespace
+
+double energy_prec_unit(int prec, int nmod, int k) {
+  if (prec == 1) return 0x1p-24;
+  if (prec == 2) {
+    int beta = 0;
+    if (zgemm_emu_params(nmod, k, &beta)) return 1.0;
+    return ldexp(1.0, -beta);
+  }
+  return 0.0;
+}
 
 // ---------------------------------------------------------------------------
 Layout::Layout(int N_, const int* sizes) : N(N_), b(sizes, sizes + N_) {
@@ -516,15 +577,31 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
 }
 
 // One stream's share of the energy grid: energies [e_begin, e_end) in waves.
+static size_t energies_group_fp64_bytes(const Layout& L, int wave) {
+  return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
+         sizeof(zc) * (size_t)wave * (2 * L.nd) + 4096;
+}
+
 static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                           int e_begin, int e_end, const zc* z, const zc* fL, const zc* fR,
-                          int* einfo, const InvAux* aux, void* work, hipStream_t s) {
+                          int* einfo, const InvAux* aux, void* work, hipStream_t s,
+                          const EnergyPrec* pr) {
   const int N = L.N, b = L.b[0];
   SweepWork w;
   carve(work, L, wave, &w);
   if (aux) w.inv.aux = *aux;
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
   zc* Gdw = Sdw + (size_t)wave * L.nd;   // G^R diagonal when the caller does not keep it
+  // reduced-precision products: the emulation's residue workspace and the
+  // per-layer norms of the estimate behind the FP64 group workspace
+  const int prec = pr ? pr->prec : 0;
+  char* extra = (char*)work + (energies_group_fp64_bytes(L, wave) + 255) / 256 * 256;
+  size_t emu_bytes = 0;
+  if (prec == 2) emu_bytes = zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, wave, wave, pr->nmod);
+  double* nrm = (double*)(extra + (emu_bytes + 255) / 256 * 256);
+  PrecScope scope(PrecCtx{prec, pr ? pr->nmod : 0, prec == 2 ? (void*)extra : nullptr,
+                          (long long)emu_bytes});
+  const double est_c = pr && pr->est ? 2.0 * energy_prec_unit(prec, pr->nmod, L.bmax) : 0.0;
   RC(set_one(w.ones, s));
   for (int e0 = e_begin; e0 < e_end; e0 += wave) {
     const int nb = (e_end - e0) < wave ? (e_end - e0) : wave;
@@ -537,6 +614,11 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     S.shift = z + e0; S.shift_stride = 1;
     int* info = out.info ? out.info + e0 : einfo + e0;
     RC(upward(L, S, w, nb, info, s));
+    if (pr && pr->est) {
+      hipLaunchKernelGGL(block_fro_kernel, dim3(N, nb), dim3(256), 0, s, w.g, L.nd, b, N, nrm);
+      hipLaunchKernelGGL(energy_estimate_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, nrm, N, nb,
+                         est_c, pr->est + e0);
+    }
     // G^R diagonal blocks (the Keldysh pass reads them)
     zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Gdw;
     RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
@@ -560,14 +642,17 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
   return Status::OK;
 }
 
-size_t energies_group_bytes(const Layout& L, int wave) {
-  return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
-         sizeof(zc) * (size_t)wave * (2 * L.nd) + 4096;
+size_t energies_group_bytes(const Layout& L, int wave, const EnergyPrec* pr) {
+  size_t b = (energies_group_fp64_bytes(L, wave) + 255) / 256 * 256;
+  if (pr && pr->prec == 2)
+    b += (zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, wave, wave, pr->nmod) + 255) / 256 * 256;
+  if (pr && pr->est) b += sizeof(double) * (size_t)wave * L.N + 256;
+  return b;
 }
 
-size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {
+size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams, const EnergyPrec* pr) {
   return sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
-         (size_t)streams * ((energies_group_bytes(L, wave) + 255) / 256 * 256);
+         (size_t)streams * ((energies_group_bytes(L, wave, pr) + 255) / 256 * 256);
 }
 
 // Energies are split into `streams` contiguous groups, each processed on its
@@ -575,7 +660,8 @@ size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams) {
 // Gauss-Jordan panels, one workgroup per energy) overlap another group's
 // GEMMs.  The caller's stream waits for all groups.
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   int streams, const EnergyStreams& es, void* work, hipStream_t s) {
+                   int streams, const EnergyStreams& es, void* work, hipStream_t s,
+                   const EnergyPrec* pr) {
   const int nE = in.nE;
   char* p = (char*)work;
   zc* z = (zc*)p;
@@ -584,11 +670,11 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   int* einfo = (int*)(fR + nE);
   char* groups = (char*)work + (sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
                                 255) / 256 * 256;
-  const size_t gbytes = (energies_group_bytes(L, wave) + 255) / 256 * 256;
+  const size_t gbytes = (energies_group_bytes(L, wave, pr) + 255) / 256 * 256;
   hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
                      in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
   if (streams <= 1)
-    return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, es.aux, groups, s);
+    return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, es.aux, groups, s, pr);
   hipStream_t* pool = es.pool;
   // fork / join events: the caller's (kept for a captured graph's lifetime),
   // else created here and destroyed on return
@@ -615,7 +701,7 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
       break;
     }
     rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, es.aux ? es.aux + g : nullptr,
-                        groups + g * gbytes, pool[g]);
+                        groups + g * gbytes, pool[g], pr);
     // join group g back into the caller's stream; where the event cannot
     // carry the edge, wait for the group on the host so later work on `s`
     // still follows it

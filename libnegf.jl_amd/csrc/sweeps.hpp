@@ -50,8 +50,21 @@ struct EnergyOut {
   int* info;                   // nE or null
 };
 
+// Arithmetic of the energy job's products (GemmArgs::prec; the diagonal-block
+// inverses stay FP64) and the per-energy error estimate of a reduced-precision
+// run: est[e] = 2 u sum_i ||g_i||_F, u the unit of the products (2^-24 for
+// FP32, 2^-beta for the emulation), g_i = S_i^{-1} the upward pass's inverses.
+struct EnergyPrec {
+  int prec;        // 0 = FP64, 1 = FP32 MFMA, 2 = int8-emulated FP64
+  int nmod;        // moduli of the emulation (0: 13)
+  double* est;     // device [nE] or null
+};
+// u of a precision form at inner dimension k (0 for FP64).
+double energy_prec_unit(int prec, int nmod, int k);
+
 size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
-size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams);
+size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams,
+                           const EnergyPrec* pr = nullptr);
 
 // inplace_inverse: the diagonal-block inverses never take the split steps
 // (which run on a second stream the library creates on first use).
@@ -70,7 +83,8 @@ struct EnergyStreams {
   hipEvent_t* events;
 };
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                   int streams, const EnergyStreams& es, void* work, hipStream_t s);
+                   int streams, const EnergyStreams& es, void* work, hipStream_t s,
+                   const EnergyPrec* pr = nullptr);
 
 // DD-RGF (uniform block size b; A.low must be given)
 size_t ddrgf_work_bytes(int N, int b, int batch, const int* levels, int nlev);

@@ -34,9 +34,12 @@
 // tiles of one instance run on one XCD and share its L2.
 #include "negf_internal.hpp"
 
+#include <type_traits>
+
 namespace negf {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -62,11 +65,13 @@ struct GemmGroup {
 // e holds row l/16 + 4*e, column l % 16.
 // With mirror (a tile strictly below the diagonal of a herm product) each
 // element (i, j) is also written at (j, i) as herm * conj.
-template <int FM, int FN>
+// The 16x16 f32 MFMA (the FP32 form) holds row 4 * (l / 16) + e instead
+// (F32ROWS).
+template <int FM, int FN, typename V = dbl4, bool F32ROWS = false>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long long i_in,
                                            long long i_out, int ib, int jb, int lane,
-                                           const dbl4 (&acc_re)[FM][FN],
-                                           const dbl4 (&acc_im)[FM][FN], bool mirror = false) {
+                                           const V (&acc_re)[FM][FN],
+                                           const V (&acc_im)[FM][FN], bool mirror = false) {
   const int kq = lane >> 4, li = lane & 15;
   zc* __restrict__ D = g.D + i_in * g.strideD + i_out * g.outerD;
   const zc* __restrict__ C = g.C ? g.C + i_in * g.strideC + i_out * g.outerC : nullptr;
@@ -82,9 +87,9 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int inst, long lon
       const int j = jb + c * 16 + li;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = ib + r * 16 + kq + 4 * e;
+        const int i = ib + r * 16 + (F32ROWS ? 4 * kq + e : kq + 4 * e);
         if (i < g.m && j < g.n) {
-          const double vr = acc_re[r][c][e], vi = acc_im[r][c][e];
+          const double vr = (double)acc_re[r][c][e], vi = (double)acc_im[r][c][e];
           zc out = zmake(alpha.x * vr - alpha.y * vi, alpha.x * vi + alpha.y * vr);
           if (use_c) out = zadd(out, zmul(g.beta, C[i + j * g.ldc]));
           if (use_shift && i == j) out = zadd(out, sh);
@@ -261,7 +266,10 @@ __global__ __launch_bounds__(256, OCC) void zgemm_mfma_kernel(
 // distinct banks.  Conjugation is a per-lane sign.
 constexpr int LTK = 8;
 
-template <int OPA, int OPB, int NBUF, int T>
+// F32: the FP32 form of the same kernel -- the staged FP64 operands rounded
+// to FP32 in registers and multiplied on v_mfma_f32_16x16x4_f32 (twice the
+// FP64 MFMA rate) with FP32 accumulation; same LDS traffic.
+template <int OPA, int OPB, int NBUF, int T, bool F32 = false>
 __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
   static_assert(T == 64 || T == 32, "tile of 64 or 32");
   constexpr int FM = T / 32, FN = T / 32;   // 16 x 16 MFMA blocks per wave and direction
@@ -334,13 +342,14 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     }
   };
 
-  dbl4 acc_re[FM][FN], acc_im[FM][FN];
+  typedef typename std::conditional<F32, flt4, dbl4>::type AccV;
+  AccV acc_re[FM][FN], acc_im[FM][FN];
 #pragma unroll
   for (int a = 0; a < FM; ++a)
 #pragma unroll
     for (int b = 0; b < FN; ++b) {
-      acc_re[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-      acc_im[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+      acc_re[a][b] = AccV{0, 0, 0, 0};
+      acc_im[a][b] = AccV{0, 0, 0, 0};
     }
 
   const int kq = lane >> 4, li = lane & 15, ri = kq & 1;
@@ -380,8 +389,15 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
       for (int r = 0; r < FM; ++r)
 #pragma unroll
         for (int c = 0; c < FN; ++c) {
-          acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[r], b1[c], acc_re[r][c], 0, 0, 0);
-          acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b2[c], acc_im[r][c], 0, 0, 0);
+          if constexpr (F32) {
+            acc_re[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)am[r], (float)b1[c],
+                                                                acc_re[r][c], 0, 0, 0);
+            acc_im[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[r], (float)b2[c],
+                                                                acc_im[r][c], 0, 0, 0);
+          } else {
+            acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[r], b1[c], acc_re[r][c], 0, 0, 0);
+            acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b2[c], acc_im[r][c], 0, 0, 0);
+          }
         }
     }
   };
@@ -408,8 +424,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     if (kt + NBUF - 1 < nk) issue(kt + NBUF - 1);
     compute_stage(kt % NBUF);
   }
-  store_tile<FM, FN>(g, inst, i_in, i_out, i0 + wm * 16 * FM, j0 + wn * 16 * FN, lane, acc_re,
-                     acc_im, G.tri && tm > tn);
+  store_tile<FM, FN, AccV, F32>(g, inst, i_in, i_out, i0 + wm * 16 * FM, j0 + wn * 16 * FN, lane,
+                                acc_re, acc_im, G.tri && tm > tn);
 }
 
 // Tile variants (autotuner choice, negf_gemm_variant).  0: zgemm_lds_kernel
@@ -455,7 +471,7 @@ bool lds_eligible(const GemmGroup& G, int T) {
   return true;
 }
 
-template <int OPA, int OPB, int NBUF, int T>
+template <int OPA, int OPB, int NBUF, int T, bool F32 = false>
 void launch_lds(GemmGroup G, hipStream_t s) {
   G.tiles_m = G.p[0].m / T;
   G.tiles_n = G.p[0].n / T;
@@ -464,8 +480,8 @@ void launch_lds(GemmGroup G, hipStream_t s) {
                                : (long long)G.tiles_m * G.tiles_n) * G.p[0].batch;
   G.total = G.tiles_per_problem * G.count;
   G.grid_padded = (G.total + 7) / 8 * 8;
-  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF, T>), dim3((unsigned)G.grid_padded), dim3(256),
-                     0, s, G);
+  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF, T, F32>), dim3((unsigned)G.grid_padded),
+                     dim3(256), 0, s, G);
 }
 
 template <int OPA, int OPB>
@@ -476,6 +492,10 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
   }
   const GemmArgs& g0 = G.p[0];
   const long long tiles64 = (long long)(g0.m / 64) * (g0.n / 64) * g0.batch * G.count;
+  if (g0.prec == 1) {   // the FP32 form where the LDS-DMA kernel applies
+    if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64, true>(G, s);
+    if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32, true>(G, s);
+  }
   switch (g_variant) {
     case 0:
       // fewer than two 64 x 64 tiles per CU (one instance of a 512^3 product
@@ -515,6 +535,7 @@ int check_args(const GemmArgs& g) {
   }
   if (g.inner < 0) return Status::BAD_ARG;
   if (g.herm != 0 && (g.m != g.n || (g.herm != 1 && g.herm != -1))) return Status::BAD_ARG;
+  if (g.prec < 0 || g.prec > 2) return Status::BAD_ARG;
   return Status::OK;
 }
 
@@ -537,6 +558,22 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
     int rc = check_args(g[i]);
     if (rc) return rc;
   }
+  if (g[0].prec == 2) {
+    // the emulated product, one problem after another (they share the
+    // residue workspace); k == 0 has no product and runs as FP64
+    for (int i = 0; i < count; ++i) {
+      GemmArgs gi = g[i];
+      int rc;
+      if (gi.k == 0) {
+        gi.prec = 0;
+        rc = zgemm_group_launch(&gi, 1, s);
+      } else {
+        rc = zgemm_emu_launch(gi, s);
+      }
+      if (rc) return rc;
+    }
+    return Status::OK;
+  }
   // one launch per run of problems with the same shape and op codes; a
   // problem whose output another problem of the run reads (or writes) starts
   // a new run, since the problems of one launch run concurrently
@@ -553,7 +590,7 @@ int zgemm_group_launch(const GemmArgs* g, int count, hipStream_t s) {
   for (int i = 1; i < count; ++i) {
     if (g[i].m != g[0].m || g[i].n != g[0].n || g[i].k != g[0].k || g[i].opA != g[0].opA ||
         g[i].opB != g[0].opB || g[i].batch != g[0].batch || g[0].herm != 0 || g[i].herm != 0 ||
-        conflicts(i)) {
+        g[i].prec != g[0].prec || conflicts(i)) {
       int rc = zgemm_group_launch(g, i, s);
       if (rc) return rc;
       return zgemm_group_launch(g + i, count - i, s);

@@ -47,6 +47,9 @@ SIGNATURES = {
                              i64, vp]),
     "negf_zgemm_strided_batched": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, i64, vp,
                                              i64, i64, c_double_p, vp, i64, i64, i32, vp]),
+    "negf_zgemm_prec_strided_batched": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, i64,
+                                                  vp, i64, i64, c_double_p, vp, i64, i64, i32, i32,
+                                                  i32, vp]),
     "negf_zgetrf": (C.c_int, [i32, vp, i64, vp, vp, vp]),
     "negf_zgetrf_strided_batched": (C.c_int, [i32, vp, i64, i64, vp, i64, vp, i32, vp]),
     "negf_zgetrs": (C.c_int, [ch, i32, i32, vp, i64, vp, vp, i64, vp]),
@@ -70,6 +73,10 @@ SIGNATURES = {
     "negf_rgf_keldysh_energies": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d, d, vp, vp,
                                             vp, vp, i32, i32, vp]),
     "negf_rgf_keldysh_energies_workspace": (C.c_int64, [i32, i32, i32, i32, i32]),
+    "negf_rgf_keldysh_energies_mixed": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d, d, vp,
+                                                  vp, vp, vp, i32, i32, i32, i32, d, vp, vp, vp,
+                                                  vp]),
+    "negf_rgf_keldysh_energies_mixed_workspace": (C.c_int64, [i32, i32, i32, i32, i32, i32, i32]),
     "negf_rgf_keldysh_energies_graph_create": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d,
                                                          d, vp, vp, vp, vp, i32, i32,
                                                          C.POINTER(vp)]),

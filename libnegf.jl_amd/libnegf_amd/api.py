@@ -247,6 +247,37 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
     return info
 
 
+PRECISIONS = {"fp64": 0, "fp32": 1, "emulated": 2}
+
+
+def rgf_keldysh_energies_mixed(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float,
+                               mu_L: float, mu_R: float, kT: float, precision: str = "emulated",
+                               tol: float = 1e-8, n_moduli: int = 0,
+                               GR_diag: torch.Tensor | None = None,
+                               Gless_diag: torch.Tensor | None = None,
+                               transmission: torch.Tensor | None = None, wave: int = 0,
+                               streams: int = 0, check: bool = True):
+    """``rgf_keldysh_energies`` with the complex products in FP32 ("fp32") or
+    FP64 emulated on the int8 matrix cores ("emulated"); the diagonal-block
+    inverses stay FP64.  Energies whose error estimate exceeds ``tol`` are
+    computed again in FP64.  Returns (info, est, fallback): the device info
+    array, the per-energy estimates and re-run flags (numpy)."""
+    _require_gpu()
+    nE = int(energies.numel())
+    info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
+    est = np.zeros(nE, dtype=np.float64)
+    fb = np.zeros(nE, dtype=np.int32)
+    nfb = C.c_int32(0)
+    call("negf_rgf_keldysh_energies_mixed", H.N, H.b, _ptr(H.H_diag), _ptr(H.H_upper),
+         _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta), float(mu_L),
+         float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag), _ptr(transmission), _ptr(info),
+         int(wave), int(streams), PRECISIONS[precision], int(n_moduli), float(tol),
+         est.ctypes.data_as(C.c_void_p), fb.ctypes.data_as(C.c_void_p), C.byref(nfb), _stream())
+    if check:
+        _check_info("rgf_keldysh_energies_mixed", info)
+    return info, est, fb
+
+
 class EnergiesGraph:
     """The energy job of ``rgf_keldysh_energies`` captured once into a HIP
     graph (negf_rgf_keldysh_energies_graph_create) and replayed by
