@@ -46,10 +46,14 @@ def parse():
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="rgf_keldysh",
-                    choices=["rgf_keldysh", "ddrgf_uneven", "single_energy", "cpu_pool"],
+                    choices=["rgf_keldysh", "ddrgf_uneven", "single_energy", "cpu_pool", "mixed"],
                     help="rgf_keldysh: the headline workload; ddrgf_uneven: three uneven devices "
                          "by DD-RGF (8 partitions each) vs RGF")
     ap.add_argument("--dd-energies", type=int, default=0, help="energies per device (ddrgf_uneven)")
+    ap.add_argument("--chunk", type=int, default=64,
+                    help="energies per call with resident outputs (mixed)")
+    ap.add_argument("--save-profile", action="store_true",
+                    help="mixed: write the per-energy record under profiles/")
     ap.add_argument("--procs", type=int, default=0,
                     help="host processes of the cpu_pool workload (0 = min(cores, 16))")
     ap.add_argument("--graph", action="store_true",
@@ -352,6 +356,149 @@ def ddrgf_uneven(a, torch, api):
     print(json.dumps(out))
 
 
+def mixed(a, torch, api):
+    """The mixed-precision form on its own grid: N = 64, b = 512, eta = 1e-4,
+    512 energies = 480 on linspace(-1, 1) + 32 within 1e-3 of the two lead
+    band edges (extrema of eig(H_11 + H_12 e^{ik} + H_12^H e^{-ik})), kT =
+    0.025, mu_L = +0.1, mu_R = -0.1.  The grid runs in chunks of --chunk
+    energies whose outputs stay resident (all 512 energies' G^R and G^<
+    diagonal blocks would need 550 GB).  Per chunk, each timed: the plain FP64
+    job (the baseline on this grid), the emulated-FP64 products (int8 MFMA,
+    Ozaki scheme II) and the FP32-MFMA products, both with tol = 1e-8, their
+    FP64 re-runs of flagged energies included.  Untimed, on the GPU: each
+    energy's largest relative Frobenius error over its G^R and G^< blocks and
+    the error of T against the FP64 job."""
+    import numpy as np
+    from libnegf_amd import synthetic
+    from libnegf_amd.flops import flops_rgf_keldysh_energy
+    cfg = synthetic.CONFIGS["mixed_precision"]
+    N, b, eta = cfg["N"], cfg["b"], cfg["eta"]
+    kT, muL, muR = cfg["kT"], cfg["mu_L"], cfg["mu_R"]
+    nE = a.energies if a.energies != 256 else cfg["n_energies"]
+    n_edge = min(cfg["n_edge"], nE // 4)
+    dev = synthetic.device_hamiltonian(N, b, seed=cfg["seed"])
+    Es = synthetic.near_edge_grid(dev, nE, n_edge)
+    edges = synthetic.band_edge_energies(dev)
+    near = np.minimum(np.abs(Es - edges[0]), np.abs(Es - edges[1])) <= 1e-3
+    H = api.DeviceHamiltonianGPU(dev)
+    chunk = min(a.chunk, nE)
+    GRf = torch.empty((chunk, N * b * b), dtype=torch.complex128, device="cuda")
+    GLf = torch.empty_like(GRf)
+    GRm = torch.empty_like(GRf)
+    GLm = torch.empty_like(GRf)
+    Tf = torch.empty(chunk, dtype=torch.float64, device="cuda")
+    Tm = torch.empty_like(Tf)
+    forms = ["emulated", "fp32"]
+    secs = {"fp64": 0.0, "emulated": 0.0, "fp32": 0.0}
+    err = {f: np.zeros(nE) for f in forms}       # after the FP64 re-runs (tol = 1e-8)
+    terr = {f: np.zeros(nE) for f in forms}
+    own = {f: np.zeros(nE) for f in forms}       # the form's own result (no re-run)
+    est = {f: np.zeros(nE) for f in forms}
+    fb = {f: np.zeros(nE, dtype=np.int32) for f in forms}
+
+    def block_err(X, R):
+        """per-energy max over blocks of ||X_i - R_i|| / ||R_i|| (blocks of b*b)"""
+        out = torch.zeros(X.shape[0], dtype=torch.float64, device="cuda")
+        for i in range(N):
+            sl = slice(i * b * b, (i + 1) * b * b)
+            d = torch.linalg.vector_norm(X[:, sl] - R[:, sl], dim=1)
+            r = torch.linalg.vector_norm(R[:, sl], dim=1).clamp_min(1e-300)
+            out = torch.maximum(out, d / r)
+        return out
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = f()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    # warm-up: the first chunk's shapes once per form (workspaces, code objects)
+    E0 = torch.tensor(Es[:chunk], dtype=torch.float64, device="cuda")
+    api.rgf_keldysh_energies(H, E0, eta, muL, muR, kT, GRf, GLf, Tf, check=False)
+    for f in forms:
+        api.rgf_keldysh_energies_mixed(H, E0[:min(8, chunk)], eta, muL, muR, kT, f, 1e-8,
+                                       GR_diag=GRm, Gless_diag=GLm, transmission=Tm, check=False)
+    for c0 in range(0, nE, chunk):
+        n = min(chunk, nE - c0)
+        E = torch.tensor(Es[c0:c0 + n], dtype=torch.float64, device="cuda")
+        dt, _ = timed(lambda: api.rgf_keldysh_energies(H, E, eta, muL, muR, kT, GRf[:n], GLf[:n],
+                                                       Tf[:n]))
+        secs["fp64"] += dt
+        for f in forms:
+            dt, r = timed(lambda: api.rgf_keldysh_energies_mixed(
+                H, E, eta, muL, muR, kT, f, 1e-8, GR_diag=GRm[:n], Gless_diag=GLm[:n],
+                transmission=Tm[:n]))
+            secs[f] += dt
+            est[f][c0:c0 + n] = r[1]
+            fb[f][c0:c0 + n] = r[2]
+            e = torch.maximum(block_err(GRm[:n], GRf[:n]), block_err(GLm[:n], GLf[:n]))
+            err[f][c0:c0 + n] = e.cpu().numpy()
+            terr[f][c0:c0 + n] = ((Tm[:n] - Tf[:n]).abs() /
+                                  Tf[:n].abs().clamp_min(1e-300)).cpu().numpy()
+            # untimed: the form's own result for every energy (no re-run)
+            api.rgf_keldysh_energies_mixed(H, E, eta, muL, muR, kT, f, float("inf"),
+                                           GR_diag=GRm[:n], Gless_diag=GLm[:n],
+                                           transmission=Tm[:n])
+            e = torch.maximum(block_err(GRm[:n], GRf[:n]), block_err(GLm[:n], GLf[:n]))
+            t = (Tm[:n] - Tf[:n]).abs() / Tf[:n].abs().clamp_min(1e-300)
+            own[f][c0:c0 + n] = torch.maximum(e, t).cpu().numpy()
+        print(f"# chunk {c0}: fp64 {secs['fp64']:.2f}s emulated {secs['emulated']:.2f}s "
+              f"fp32 {secs['fp32']:.2f}s", file=sys.stderr, flush=True)
+    fl = flops_rgf_keldysh_energy(N, b)
+
+    def summary(f):
+        ratio = est[f] / np.maximum(own[f], 1e-300)
+        return {"energies_per_s": round(nE / secs[f], 3), "seconds": round(secs[f], 3),
+                "speedup_vs_fp64": round(secs["fp64"] / secs[f], 3),
+                "fallback_rate": round(float(fb[f].mean()), 4),
+                "fallback_near_edge": int(fb[f][near].sum()),
+                "max_relerr_vs_fp64": float(max(err[f].max(), terr[f].max())),
+                "energies_above_1e-8": int(np.sum(np.maximum(err[f], terr[f]) > 1e-8)),
+                "own_max_relerr": float(own[f].max()),
+                "own_median_relerr": float(np.median(own[f])),
+                "own_max_relerr_near_edge": float(own[f][near].max()),
+                "own_energies_above_1e-8": int(np.sum(own[f] > 1e-8)),
+                "estimate_bounds_own_error": bool(np.all(ratio >= 1.0)),
+                "est_over_err": {"min": float(ratio.min()), "median": float(np.median(ratio))}}
+
+    res = {f: summary(f) for f in forms}
+    eps = nE / secs["emulated"]
+    out = {
+        "metric": METRIC, "value": round(eps, 4), "unit": "energies/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(secs["emulated"] * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 storage; products int8-emulated FP64 (Ozaki II, 13 moduli), FP64 inverses",
+        "data": "synthetic",
+        "config": {"workload": f"mixed_precision: N={N} b={b} energies={nE} "
+                               f"({nE - n_edge} linspace(-1,1) + {n_edge} within 1e-3 of the lead "
+                               f"band edges {edges[0]:.4f}, {edges[1]:.4f}) eta={eta} kT={kT} "
+                               f"muL={muL} muR={muR}", "chunk": chunk, "tol": 1e-8,
+                   "flops_per_energy_credited": fl},
+        "fp64_same_grid": {"energies_per_s": round(nE / secs["fp64"], 3),
+                           "seconds": round(secs["fp64"], 3)},
+        "forms": res,
+    }
+    prof = ROOT / "profiles" / f"mixed_precision_N{N}_b{b}_E{nE}.json"
+    if a.save_profile:
+        def worst(f, k=8):
+            idx = np.argsort(-own[f])[:k]
+            return [{"E": round(float(Es[i]), 6), "near_edge": bool(near[i]),
+                     "own_relerr": float(own[f][i]), "estimate": float(est[f][i]),
+                     "fallback": int(fb[f][i])} for i in idx]
+
+        def tightest(f, k=4):
+            idx = np.argsort(est[f] / np.maximum(own[f], 1e-300))[:k]
+            return [{"E": round(float(Es[i]), 6), "own_relerr": float(own[f][i]),
+                     "estimate": float(est[f][i])} for i in idx]
+        prof.write_text(json.dumps({
+            "command": "python bench.py --workload mixed --save-profile",
+            "summary": out,
+            "largest_errors": {f: worst(f) for f in forms},
+            "tightest_estimates": {f: tightest(f) for f in forms}}, indent=1) + "\n")
+    print(json.dumps(out))
+
+
 def _pool_energy(idx, E, N, b, barrier, queue):
     """One worker of the cpu_pool workload: the oracle's RGF + RKD + T for one
     energy with one BLAS thread, timed between a common start barrier and its
@@ -447,6 +594,9 @@ def main():
         return
     if a.workload == "single_energy":
         single_energy(a, torch, api)
+        return
+    if a.workload == "mixed":
+        mixed(a, torch, api)
         return
     cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
     N, b, nE = a.N, a.b, a.energies

@@ -259,7 +259,7 @@ int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t 
  * precision 1 = FP32 MFMA (4 real products, FP32 accumulation), 2 = FP64
  * emulated on the int8 matrix cores (Ozaki scheme II, n_moduli moduli, 0 =
  * 13), 0 = FP64.  After the pass, each energy's error estimate
- * est = 2 u sum_i ||g_i||_F (u = 2^-24 for FP32, 2^-beta for the emulation,
+ * est = 8 u sum_i ||g_i||_F (u = 2^-24 for FP32, 2^-beta for the emulation,
  * g_i the upward pass's diagonal-block inverses; calibrated against the
  * oracle, see the tests) decides: energies with est > tol (or a non-finite
  * est) are computed again in FP64 and their outputs replace the first ones.

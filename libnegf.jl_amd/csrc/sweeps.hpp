@@ -52,7 +52,7 @@ struct EnergyOut {
 
 // Arithmetic of the energy job's products (GemmArgs::prec; the diagonal-block
 // inverses stay FP64) and the per-energy error estimate of a reduced-precision
-// run: est[e] = 2 u sum_i ||g_i||_F, u the unit of the products (2^-24 for
+// run: est[e] = 8 u sum_i ||g_i||_F, u the unit of the products (2^-24 for
 // FP32, 2^-beta for the emulation), g_i = S_i^{-1} the upward pass's inverses.
 struct EnergyPrec {
   int prec;        // 0 = FP64, 1 = FP32 MFMA, 2 = int8-emulated FP64
