@@ -98,6 +98,14 @@ function be_ctranspose!(Mout::ROCArray, Min::ROCArray, td::TimingData, cd::Count
            "be_ctranspose!")
 end
 
+# non-mutating adjoint (src/backend_cpu.jl:70-80): a new device matrix; bm_convert
+# reads Hermitian storage's lower blocks through it (src/blockmatrix.jl:108)
+function be_ctranspose(Min::ROCArray, td::TimingData, cd::CountingData)::ROCArray
+    Mout = be_zero_array((size(Min, 2), size(Min, 1)))
+    be_ctranspose!(Mout, Min, td, cd)
+    return Mout
+end
+
 function be_zero_lu(n::Int)::AmdLU
     return AmdLU(be_zero_array((n, n)), AMDGPU.ones(Int32, n))
 end

@@ -1,9 +1,12 @@
 // The reference's C ABI (src/c_interface/libnegf_c_interface.h:12-38,
 // libnegf_c_interface.jl:85-142): a 0-based CSC matrix in, the CSC matrix of
 // the block band of its inverse out (malloc'd arrays the caller frees).
-// Host-side conversion CSC -> packed blocks (bm_convert, src/blockmatrix.jl:
-// 34-72) and back (:82-127), the inverse on the GPU by DD-RGF (uniform
-// blocks; the reference's choice, libnegf_c_interface.jl:119) or RGF.
+// The CSC arrays go to the device as they are; the conversion to packed
+// blocks (bm_convert, src/blockmatrix.jl:34-72) and back (:82-127) runs there
+// (one wave per column), the inverse by DD-RGF (uniform blocks; the
+// reference's choice, libnegf_c_interface.jl:119) or RGF.  The host only
+// checks the arguments, sums the output's per-column counts into colptr and
+// copies the result into malloc'd arrays.
 #include "../../include/negf_hip.h"
 #include "negf_internal.hpp"
 #include "sweeps.hpp"
@@ -16,6 +19,86 @@
 using namespace negf;
 
 namespace {
+
+// Block geometry on the device: blk[c] = block of row / column c, off[i] =
+// first row of block i, sizes[i], od / oo = packed offsets of diagonal /
+// off-diagonal block i.
+struct Geo {
+  const int* blk;
+  const long long* off;
+  const int* sizes;
+  const long long* od;
+  const long long* oo;
+  int N;
+};
+
+// CSC -> packed band: wave w of the grid takes column c = w, its lanes the
+// column's entries; entries outside the band are not read (bm_convert copies
+// band blocks only); with Hermitian storage the lower blocks are skipped here
+// and filled as adjoints afterwards.
+__global__ __launch_bounds__(256) void csc_to_band_kernel(int n, const int32_t* colptr,
+                                                          const int32_t* rowval,
+                                                          const double* nzval, Geo g, int herm,
+                                                          zc* d, zc* u, zc* l) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  const int bj = g.blk[c];
+  const long long lc = c - g.off[bj];
+  for (int k = colptr[c] + lane; k < colptr[c + 1]; k += 64) {
+    const int r = rowval[k];
+    const int bi = g.blk[r];
+    const long long lr = r - g.off[bi];
+    const zc v = zmake(nzval[2LL * k], nzval[2LL * k + 1]);
+    if (bi == bj) d[g.od[bi] + lr + lc * g.sizes[bi]] = v;
+    else if (bj == bi + 1) u[g.oo[bi] + lr + lc * g.sizes[bi]] = v;
+    else if (bi == bj + 1 && !herm) l[g.oo[bj] + lr + lc * g.sizes[bi]] = v;
+  }
+}
+
+// Value of band entry (row r, column c) for rows of the blocks bj - 1 .. bj + 1.
+__device__ __forceinline__ zc band_at(const Geo& g, const zc* d, const zc* u, const zc* l, int bi,
+                                      int bj, long long lr, long long lc) {
+  if (bi == bj) return d[g.od[bi] + lr + lc * g.sizes[bi]];
+  if (bj == bi + 1) return u[g.oo[bi] + lr + lc * g.sizes[bi]];
+  return l[g.oo[bj] + lr + lc * g.sizes[bi]];
+}
+
+// Per column: the band's nonzero count (cnt) or, with colptr given, its
+// nonzeros written in increasing row order at colptr[c] (exact zeros
+// dropped, as sparse() of a dense block does).  Wave per column; the lanes
+// walk the column's band rows in chunks of 64 and compact with a ballot.
+__global__ __launch_bounds__(256) void band_to_csc_kernel(int n, Geo g, const zc* d, const zc* u,
+                                                          const zc* l, int32_t* cnt,
+                                                          const int32_t* colptr, int32_t* rowval,
+                                                          double* nzval) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  const int bj = g.blk[c];
+  const long long lc = c - g.off[bj];
+  const int b0 = bj > 0 ? bj - 1 : bj, b1 = bj + 1 < g.N ? bj + 1 : bj;
+  const long long r0 = g.off[b0], r1 = g.off[b1] + g.sizes[b1];
+  long long pos = colptr ? colptr[c] : 0;
+  for (long long rb = r0; rb < r1; rb += 64) {
+    const long long r = rb + lane;
+    zc v = zmake(0.0, 0.0);
+    if (r < r1) {
+      const int bi = g.blk[r];
+      v = band_at(g, d, u, l, bi, bj, r - g.off[bi], lc);
+    }
+    const bool nz = v.x != 0.0 || v.y != 0.0;
+    const unsigned long long m = __ballot(nz);
+    if (colptr && nz) {
+      const long long p = pos + __popcll(m & ((1ull << lane) - 1ull));
+      rowval[p] = (int32_t)r;
+      nzval[2 * p] = v.x;
+      nzval[2 * p + 1] = v.y;
+    }
+    pos += __popcll(m);
+  }
+  if (!colptr && lane == 0) cnt[c] = (int32_t)pos;
+}
 
 int pick_ddrgf_bs(int N) {
   // aim at ~8 tasks (enough parallel partitions); 0 = plain RGF
@@ -55,55 +138,88 @@ int negf_run_bndiag_csc(int32_t m, int32_t n, int32_t nnz_in, const int32_t* col
     off[i + 1] = off[i] + sizes[i];
   }
   if (off[N] != n) return NEGF_BAD_ARG;
+  // the CSC structure is checked here (O(n + nnz) integer reads, no values)
+  if (colptr_in[0] != 0 || colptr_in[n] != nnz_in) return NEGF_BAD_ARG;
+  for (int c = 0; c < n; ++c)
+    if (colptr_in[c + 1] < colptr_in[c]) return NEGF_BAD_ARG;
+  for (int k = 0; k < nnz_in; ++k)
+    if (rowval_in[k] < 0 || rowval_in[k] >= n) return NEGF_BAD_ARG;
   Layout L(N, sizes.data());
   std::vector<int> blk(n);
   for (int i = 0; i < N; ++i)
     for (long long r = off[i]; r < off[i + 1]; ++r) blk[r] = i;
-  const size_t nd = (size_t)L.nd, no = (size_t)(L.no > 0 ? L.no : 0);
-  std::vector<zc> hd(nd, zmake(0, 0)), hu(no > 0 ? no : 1, zmake(0, 0)),
-      hl(no > 0 ? no : 1, zmake(0, 0));
-  for (int c = 0; c < n; ++c) {
-    if (colptr_in[c] < 0 || colptr_in[c + 1] < colptr_in[c] || colptr_in[c + 1] > nnz_in)
-      return NEGF_BAD_ARG;
-    const int bj = blk[c];
-    for (int k = colptr_in[c]; k < colptr_in[c + 1]; ++k) {
-      const int r = rowval_in[k];
-      if (r < 0 || r >= n) return NEGF_BAD_ARG;
-      const int bi = blk[r];
-      const zc v = zmake(nzval_in[2 * k], nzval_in[2 * k + 1]);
-      const long long lr = r - off[bi], lc = c - off[bj];
-      if (bi == bj) hd[L.od[bi] + lr + lc * sizes[bi]] = v;
-      else if (bj == bi + 1) hu[L.oo[bi] + lr + lc * sizes[bi]] = v;
-      else if (bi == bj + 1 && !is_hermitian) hl[L.oo[bj] + lr + lc * sizes[bi]] = v;
-      // entries outside the band are not read (bm_convert copies band blocks only)
-    }
+  const size_t nd = (size_t)L.nd, nos = (size_t)(L.no > 0 ? L.no : 1);
+  hipStream_t s = (hipStream_t)stream;
+  // one device allocation: band in / out, CSC in, geometry, counts, CSC out
+  // (the output's nnz is at most the band's element count)
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t band = nd + 2 * nos;
+  size_t max_out = 0;
+  for (int j = 0; j < N; ++j) {
+    const int b0 = j > 0 ? j - 1 : j, b1 = j + 1 < N ? j + 1 : j;
+    max_out += (size_t)(off[b1 + 1] - off[b0]) * sizes[j];
   }
-  if (is_hermitian) {
-    // Hermitian storage: A_{i+1,i} = A_{i,i+1}^H (blockmatrix.jl:105-109)
-    for (int i = 0; i + 1 < N; ++i) {
-      const int bi = sizes[i], bj = sizes[i + 1];
-      for (int c = 0; c < bi; ++c)
-        for (int r = 0; r < bj; ++r) hl[L.oo[i] + r + (long long)c * bj] = zconj(hu[L.oo[i] + c + (long long)r * bi]);
-    }
-  }
-  // device buffers: inputs then outputs
-  const size_t tot = nd + 2 * (no > 0 ? no : 1);
-  zc* dev = nullptr;
-  if (hipMalloc(&dev, 2 * tot * sizeof(zc)) != hipSuccess) {
+  const size_t nnz_alloc = nnz_in > 0 ? (size_t)nnz_in : 1;
+  const size_t bytes = up(2 * band * sizeof(zc)) + up(sizeof(int32_t) * (n + 1)) +
+                       up(sizeof(int32_t) * nnz_alloc) + up(sizeof(double) * 2 * nnz_alloc) +
+                       up(sizeof(int) * n) + 3 * up(sizeof(long long) * (N + 1)) +
+                       up(sizeof(int) * N) + up(sizeof(int32_t)) + up(sizeof(int32_t) * (n + 1)) +
+                       up(sizeof(int32_t) * max_out) + up(sizeof(double) * 2 * max_out);
+  char* dev = nullptr;
+  if (hipMalloc((void**)&dev, bytes) != hipSuccess) {
     (void)hipGetLastError();
     return NEGF_NO_MEMORY;
   }
-  zc* din = dev;
-  zc* dout = dev + tot;
-  const size_t nos = (no > 0 ? no : 1);
-  hipStream_t s = (hipStream_t)stream;
+  char* p = dev;
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += up(b);
+    return r;
+  };
+  zc* din = (zc*)take(2 * band * sizeof(zc));
+  zc* dout = din + band;
+  int32_t* dcp = (int32_t*)take(sizeof(int32_t) * (n + 1));
+  int32_t* drv = (int32_t*)take(sizeof(int32_t) * nnz_alloc);
+  double* dnz = (double*)take(sizeof(double) * 2 * nnz_alloc);
+  int* dblk = (int*)take(sizeof(int) * n);
+  long long* doff = (long long*)take(sizeof(long long) * (N + 1));
+  long long* dod = (long long*)take(sizeof(long long) * (N + 1));
+  long long* doo = (long long*)take(sizeof(long long) * (N + 1));
+  int* dsz = (int*)take(sizeof(int) * N);
+  int32_t* dinfo = (int32_t*)take(sizeof(int32_t));
+  int32_t* dcnt = (int32_t*)take(sizeof(int32_t) * (n + 1));
+  int32_t* orv = (int32_t*)take(sizeof(int32_t) * max_out);
+  double* onz = (double*)take(sizeof(double) * 2 * max_out);
+  std::vector<long long> oo(N + 1, 0), od(L.od.begin(), L.od.end());
+  for (int i = 0; i + 1 < N; ++i) oo[i] = L.oo[i];
   int rc = NEGF_OK;
-  int32_t* dinfo = nullptr;
-  if (hipMemcpy(din, hd.data(), nd * sizeof(zc), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(din + nd, hu.data(), nos * sizeof(zc), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(din + nd + nos, hl.data(), nos * sizeof(zc), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMalloc(&dinfo, sizeof(int32_t)) != hipSuccess)
-    rc = NEGF_HIP_ERROR;
+  auto H2D = [&](void* dst, const void* src, size_t b) {
+    if (rc == NEGF_OK && b > 0 && hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = NEGF_HIP_ERROR;
+  };
+  H2D(dcp, colptr_in, sizeof(int32_t) * (n + 1));
+  H2D(drv, rowval_in, sizeof(int32_t) * nnz_in);
+  H2D(dnz, nzval_in, sizeof(double) * 2 * nnz_in);
+  H2D(dblk, blk.data(), sizeof(int) * n);
+  H2D(doff, off.data(), sizeof(long long) * (N + 1));
+  H2D(dod, od.data(), sizeof(long long) * (N + 1));
+  H2D(doo, oo.data(), sizeof(long long) * (N + 1));
+  H2D(dsz, sizes.data(), sizeof(int) * N);
+  const Geo geo{dblk, doff, dsz, dod, doo, N};
+  const unsigned gcol = (unsigned)((n + 3) / 4);
+  if (rc == NEGF_OK)
+    rc = zfill_launch((int)band, 1, zmake(0, 0), zmake(0, 0), din, (long long)band, 0, 1, s);
+  if (rc == NEGF_OK) {
+    hipLaunchKernelGGL(csc_to_band_kernel, dim3(gcol), dim3(256), 0, s, n, dcp, drv, dnz, geo,
+                       is_hermitian ? 1 : 0, din, din + nd, din + nd + nos);
+    if (hipGetLastError() != hipSuccess) rc = NEGF_HIP_ERROR;
+  }
+  if (rc == NEGF_OK && is_hermitian) {
+    // Hermitian storage: A_{i+1,i} = A_{i,i+1}^H (blockmatrix.jl:105-109)
+    for (int i = 0; i + 1 < N && rc == NEGF_OK; ++i)
+      rc = zadjoint_launch(sizes[i], sizes[i + 1], din + nd + L.oo[i], sizes[i], 0,
+                           din + nd + nos + L.oo[i], sizes[i + 1], 0, 1, s);
+  }
   bool uniform = true;
   for (int i = 1; i < N; ++i) uniform = uniform && sizes[i] == sizes[0];
   const int bs = (method == 1 || !uniform) ? 0 : pick_ddrgf_bs(N);
@@ -116,54 +232,58 @@ int negf_run_bndiag_csc(int32_t m, int32_t n, int32_t nnz_in, const int32_t* col
                     (int64_t)nos, dout, dout + nd, dout + nd + nos, 1, dinfo, stream);
   }
   int32_t hinfo = 0;
+  std::vector<int32_t> colptr(n + 1, 0);
   if (rc == NEGF_OK) {
-    if (hipStreamSynchronize(s) != hipSuccess ||
-        hipMemcpy(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(hd.data(), dout, nd * sizeof(zc), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(hu.data(), dout + nd, nos * sizeof(zc), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(hl.data(), dout + nd + nos, nos * sizeof(zc), hipMemcpyDeviceToHost) != hipSuccess)
+    hipLaunchKernelGGL(band_to_csc_kernel, dim3(gcol), dim3(256), 0, s, n, geo, dout, dout + nd,
+                       dout + nd + nos, dcnt, nullptr, nullptr, nullptr);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(colptr.data() + 1, dcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
       rc = NEGF_HIP_ERROR;
   }
-  (void)hipFree(dev);
-  if (dinfo) (void)hipFree(dinfo);
-  if (rc != NEGF_OK) return rc;
-  if (hinfo != 0) return hinfo;   // > 0: singular layer
-  // band -> CSC, column by column in increasing row order, exact zeros dropped
-  std::vector<int32_t> colptr(n + 1, 0), rowval;
-  std::vector<zc> vals;
-  for (int c = 0; c < n; ++c) {
-    const int bj = blk[c];
-    const long long lc = c - off[bj];
-    for (int bi = bj - 1; bi <= bj + 1; ++bi) {
-      if (bi < 0 || bi >= N) continue;
-      for (long long lr = 0; lr < sizes[bi]; ++lr) {
-        zc v;
-        if (bi == bj) v = hd[L.od[bi] + lr + lc * sizes[bi]];
-        else if (bj == bi + 1) v = hu[L.oo[bi] + lr + lc * sizes[bi]];
-        else v = hl[L.oo[bj] + lr + lc * sizes[bi]];
-        if (v.x != 0.0 || v.y != 0.0) {
-          rowval.push_back((int32_t)(off[bi] + lr));
-          vals.push_back(v);
-        }
-      }
-    }
-    colptr[c + 1] = (int32_t)rowval.size();
+  if (rc == NEGF_OK && hinfo != 0) rc = hinfo;   // > 0: singular layer
+  size_t nnz = 0;
+  if (rc == NEGF_OK) {
+    for (int c = 0; c < n; ++c) colptr[c + 1] += colptr[c];
+    nnz = (size_t)colptr[n];
+    if (hipMemcpyAsync(dcp, colptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+      rc = NEGF_HIP_ERROR;
   }
-  const size_t nnz = rowval.size();
+  if (rc == NEGF_OK) {
+    hipLaunchKernelGGL(band_to_csc_kernel, dim3(gcol), dim3(256), 0, s, n, geo, dout, dout + nd,
+                       dout + nd + nos, nullptr, dcp, orv, onz);
+    if (hipGetLastError() != hipSuccess) rc = NEGF_HIP_ERROR;
+  }
+  if (rc == NEGF_OK) {
+    out->colptr = (int32_t*)std::malloc(sizeof(int32_t) * (n + 1));
+    out->rowval = (int32_t*)std::malloc(sizeof(int32_t) * (nnz > 0 ? nnz : 1));
+    out->nzval = (double*)std::malloc(sizeof(double) * 2 * (nnz > 0 ? nnz : 1));
+    if (!out->colptr || !out->rowval || !out->nzval) {
+      rc = NEGF_NO_MEMORY;
+    } else {
+      std::memcpy(out->colptr, colptr.data(), sizeof(int32_t) * (n + 1));
+      if (nnz > 0 &&
+          (hipMemcpyAsync(out->rowval, orv, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s) !=
+               hipSuccess ||
+           hipMemcpyAsync(out->nzval, onz, sizeof(double) * 2 * nnz, hipMemcpyDeviceToHost, s) !=
+               hipSuccess))
+        rc = NEGF_HIP_ERROR;
+      if (hipStreamSynchronize(s) != hipSuccess) rc = NEGF_HIP_ERROR;
+    }
+    if (rc != NEGF_OK) {
+      std::free(out->colptr); std::free(out->rowval); std::free(out->nzval);
+      std::memset(out, 0, sizeof(*out));
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(dev);
+  if (rc != NEGF_OK) return rc;
   out->m = m;
   out->n = n;
   out->nnz = (int32_t)nnz;
-  out->colptr = (int32_t*)std::malloc(sizeof(int32_t) * (n + 1));
-  out->rowval = (int32_t*)std::malloc(sizeof(int32_t) * (nnz > 0 ? nnz : 1));
-  out->nzval = (double*)std::malloc(sizeof(double) * 2 * (nnz > 0 ? nnz : 1));
-  if (!out->colptr || !out->rowval || !out->nzval) {
-    std::free(out->colptr); std::free(out->rowval); std::free(out->nzval);
-    std::memset(out, 0, sizeof(*out));
-    return NEGF_NO_MEMORY;
-  }
-  std::memcpy(out->colptr, colptr.data(), sizeof(int32_t) * (n + 1));
-  std::memcpy(out->rowval, rowval.data(), sizeof(int32_t) * nnz);
-  std::memcpy(out->nzval, vals.data(), sizeof(double) * 2 * nnz);
   return NEGF_OK;
 }
 

@@ -368,8 +368,13 @@ def be_copy_in_hw(Mout: torch.Tensor, Min: torch.Tensor) -> torch.Tensor:
     return Mout
 
 
-def be_ctranspose(Mout: torch.Tensor, Min: torch.Tensor) -> torch.Tensor:
-    """``adjoint!`` (``src/backend_cpu.jl:58-68``)."""
+def be_ctranspose(Mout: torch.Tensor, Min: torch.Tensor | None = None) -> torch.Tensor:
+    """``be_ctranspose!(Mout, Min)`` = ``adjoint!`` (``src/backend_cpu.jl:58-68``);
+    with one argument the non-mutating ``be_ctranspose(Min)`` (``:70-80``): a
+    new device matrix holding Min^H."""
+    if Min is None:
+        Min = Mout
+        Mout = be_zero_array((Min.shape[1], Min.shape[0]), device=Min.device)
     call("negf_zadjoint_strided_batched", Min.shape[0], Min.shape[1], _ptr(Min), _ld(Min), 0,
          _ptr(Mout), _ld(Mout), 0, 1, _stream())
     return Mout

@@ -78,3 +78,91 @@ def _herm(dev):
     d = [np.asfortranarray(5 * np.eye(b) - h) for h in dev.H_diag]
     u = [np.asfortranarray(-x) for x in dev.H_upper]
     return BlockTridiag(d, u, None).to_dense()
+
+
+def _example_csc(N, b, seed):
+    """The reference C example's input (src/c_interface/libnegf_c_example.c:
+    30-75): every element of the block band, 0-based CSC, values
+    U[0,1) + i U[0,1) (numpy's stream instead of rand())."""
+    n = N * b
+    rng = np.random.default_rng(seed)
+    bc = np.arange(n) // b
+    lo = np.maximum(bc - 1, 0) * b
+    hi = (np.minimum(bc + 1, N - 1) + 1) * b
+    counts = (hi - lo).astype(np.int64)
+    colptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    rowval = np.concatenate([np.arange(l, h) for l, h in zip(lo, hi)]).astype(np.int32)
+    nz = rng.random(len(rowval)) + 1j * rng.random(len(rowval))
+    return colptr, rowval, nz
+
+
+def test_run_bndiag_wrapper_reference_symbol_by_value():
+    """The exported reference-signature symbol itself: CCscMatrix returned by
+    value, malloc'd arrays freed with libc free, on the C example's shape
+    (N = 160 blocks of 64).  Uniform blocks take DD-RGF, as the reference's
+    wrapper does (libnegf_c_interface.jl:119); its bar against RGF is the
+    reference's own 2.5e-10 (test/test_matinvertndiag_ddrgf.jl:17)."""
+    import ctypes as C
+    from libnegf_amd._lib import load
+    from libnegf_amd.synthetic import BlockTridiag
+    lib = load()
+    N, b = 160, 64
+    n = N * b
+    colptr, rowval, nz = _example_csc(N, b, seed=160)
+    cp_in, rv_in = colptr.copy(), rowval.copy()
+    nzd = np.ascontiguousarray(nz.view(np.float64))
+    sizes = np.full(N, b, dtype=np.int32)
+    keys = (C.c_char_p * 2)(b"in", b"out")
+    vals = (C.c_int32 * 2)(3, 3)
+    p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    print(f"run_bndiag_wrapper N={N} b={b} nnz={len(rowval)}", flush=True)
+    out = lib.run_bndiag_wrapper(n, n, len(rowval), p32(colptr), p32(rowval),
+                                 nzd.ctypes.data_as(C.POINTER(C.c_double)), N, p32(sizes), 2, keys,
+                                 vals, 0)
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    try:
+        assert (out.m, out.n) == (n, n)
+        assert out.nnz == (3 * N - 2) * b * b          # a dense band: no exact zeros
+        ocp = np.ctypeslib.as_array(out.colptr, shape=(n + 1,)).copy()
+        orv = np.ctypeslib.as_array(out.rowval, shape=(out.nnz,)).copy()
+        onz = np.ctypeslib.as_array(out.nzval, shape=(2 * out.nnz,)).copy().view(np.complex128)
+    finally:
+        for ptr in (out.colptr, out.rowval, out.nzval):
+            libc.free(C.cast(ptr, C.c_void_p))
+    # the inputs are read only
+    assert np.array_equal(colptr, cp_in) and np.array_equal(rowval, rv_in)
+    # same structure as the input band, rows increasing within each column
+    assert np.array_equal(ocp, colptr) and np.array_equal(orv, rowval)
+    S = sp.csc_matrix((nz, rowval, colptr), shape=(n, n))
+    A = BlockTridiag([S[i * b:(i + 1) * b, i * b:(i + 1) * b].toarray(order="F") for i in range(N)],
+                     [S[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b].toarray(order="F")
+                      for i in range(N - 1)],
+                     [S[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b].toarray(order="F")
+                      for i in range(N - 1)])
+    ref = bndiag_of_inv_rgf_local(A)
+    G = sp.csc_matrix((onz, orv, ocp), shape=(n, n))
+    got = {k: G[k[0] * b:(k[0] + 1) * b, k[1] * b:(k[1] + 1) * b].toarray() for k in ref}
+    err = max_block_err(got, ref)
+    print(f"  max block relerr vs oracle RGF {err:.2e}")
+    assert err < 2.5e-10
+
+
+def test_run_bndiag_wrapper_error_returns_empty_struct():
+    """The reference's C interface has no error channel: on a bad argument
+    (block sizes not summing to n) the symbol returns m, n and a zero nnz with
+    null arrays (nothing to free)."""
+    import ctypes as C
+    from libnegf_amd._lib import load
+    lib = load()
+    colptr, rowval, nz = _example_csc(4, 8, seed=1)
+    nzd = np.ascontiguousarray(nz.view(np.float64))
+    sizes = np.full(4, 7, dtype=np.int32)      # 28 != 32
+    keys = (C.c_char_p * 2)(b"in", b"out")
+    vals = (C.c_int32 * 2)(3, 3)
+    p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    out = lib.run_bndiag_wrapper(32, 32, len(rowval), p32(colptr), p32(rowval),
+                                 nzd.ctypes.data_as(C.POINTER(C.c_double)), 4, p32(sizes), 2, keys,
+                                 vals, 0)
+    assert (out.m, out.n, out.nnz) == (32, 32, 0)
+    assert not out.colptr and not out.rowval and not out.nzval

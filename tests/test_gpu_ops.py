@@ -221,10 +221,12 @@ def test_zinv_exact_ties(n):
 
 
 @pytest.mark.parametrize("trans", ["N", "T", "C"])
-def test_zgetrs(trans):
-    print(f"zgetrs {trans}", flush=True)
+@pytest.mark.parametrize("n,nrhs", [(150, 33), (512, 512)])
+def test_zgetrs(trans, n, nrhs):
+    """be_lu + be_mldivide as the unchanged reference sweeps call them; 512 x
+    512 with 512 right-hand sides is the production block's solve."""
+    print(f"zgetrs {trans} n={n} nrhs={nrhs}", flush=True)
     rng = np.random.default_rng(7)
-    n, nrhs = 150, 33
     A = _rand(rng, n, n)
     B = _rand(rng, n, nrhs)
     lu = api.be_lu(None, _dev(A))
@@ -264,6 +266,7 @@ def test_zgetri_identity_pivots_and_adjoint_fill():
     out = api.be_zero_array((71, 33))
     api.be_ctranspose(out, _dev(B))
     assert np.array_equal(_host(out), B.conj().T)
+    assert np.array_equal(_host(api.be_ctranspose(_dev(B))), B.conj().T)   # non-mutating form
     I = api.be_identity(17)
     assert np.array_equal(_host(I), np.eye(17))
     api.be_fill(I, 2 - 1j)
