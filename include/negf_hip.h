@@ -386,6 +386,26 @@ int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow
                        const int32_t* bcol, void* const* blocks, int64_t ld, int64_t stride,
                        int32_t batch, void* stream);
 
+/* Block_factorization! with a choice of pivoting inside the diagonal blocks:
+ * mode 0 = none, 1 = the reference's (partial pivoting, ipiv discarded:
+ * right only when no row swap happens), 2 = partial pivoting with the row
+ * permutation applied to the block row: S_ii = P_i L_ii U_ii, the U blocks
+ * become L_ii^{-1} P A_ij, and the pivots are kept in ipiv (device int32
+ * [batch][npl][b], LAPACK's 1-based rows) for negf_block_inverse_pivoted.
+ * Mode 2 is correct for any nonsingular diagonal Schur complements; the
+ * reference's own routine (src/selected_inverse.jl:9-55 with
+ * src/backend_cpu.jl:255-265) is mode 1. */
+int negf_block_factorization_pivoted(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                                     const int32_t* bcol, void* const* blocks, int64_t ld,
+                                     int64_t stride, int32_t batch, int32_t mode, int32_t* ipiv,
+                                     int32_t* info, void* stream);
+/* Block_inverse! after negf_block_factorization_pivoted: ipiv from mode 2
+ * (every L_ii^{-1} of the inversion becomes L_ii^{-1} P), or NULL after
+ * modes 0 and 1 (then identical to negf_block_inverse). */
+int negf_block_inverse_pivoted(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                               const int32_t* bcol, void* const* blocks, int64_t ld,
+                               int64_t stride, int32_t batch, const int32_t* ipiv, void* stream);
+
 /* --------------------------------------------------------- input assembly */
 
 /* M = E S - H - Se for a batch of energies, on packed block-tridiagonal

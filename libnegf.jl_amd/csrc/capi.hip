@@ -777,7 +777,20 @@ int negf_block_factorization(int32_t npl, int32_t b, int32_t nnzb, const int32_t
   int rc = workspace(stream, selinv_work_bytes(b, batch, 1), &w);
   if (rc) return rc;
   return block_factorization(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch,
-                             ref_pivoting != 0, info, w, st(stream));
+                             ref_pivoting != 0 ? 1 : 0, nullptr, info, w, st(stream));
+}
+
+int negf_block_factorization_pivoted(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                                     const int32_t* bcol, void* const* blocks, int64_t ld,
+                                     int64_t stride, int32_t batch, int32_t mode, int32_t* ipiv,
+                                     int32_t* info, void* stream) {
+  if (!info || b <= 0 || b > 2048 || batch <= 0 || mode < 0 || mode > 2 || (mode == 2 && !ipiv))
+    return NEGF_BAD_ARG;
+  void* w = nullptr;
+  int rc = workspace(stream, selinv_work_bytes(b, batch, 1), &w);
+  if (rc) return rc;
+  return block_factorization(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, mode, ipiv, info,
+                             w, st(stream));
 }
 
 int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
@@ -791,7 +804,23 @@ int negf_block_inverse(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow
   void* w = nullptr;
   int rc = workspace(stream, selinv_work_bytes(b, batch, maxr), &w);
   if (rc) return rc;
-  return block_inverse(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, w, st(stream));
+  return block_inverse(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, nullptr, w,
+                       st(stream));
+}
+
+int negf_block_inverse_pivoted(int32_t npl, int32_t b, int32_t nnzb, const int32_t* brow,
+                               const int32_t* bcol, void* const* blocks, int64_t ld,
+                               int64_t stride, int32_t batch, const int32_t* ipiv, void* stream) {
+  if (b <= 0 || b > 2048 || batch <= 0 || npl <= 0 || nnzb <= 0 || !brow || !bcol)
+    return NEGF_BAD_ARG;
+  int maxr = 0;
+  std::vector<int> cnt(npl, 0);
+  for (int k = 0; k < nnzb; ++k)
+    if (bcol[k] > brow[k] && brow[k] >= 0 && brow[k] < npl) maxr = ++cnt[brow[k]] > maxr ? cnt[brow[k]] : maxr;
+  void* w = nullptr;
+  int rc = workspace(stream, selinv_work_bytes(b, batch, maxr), &w);
+  if (rc) return rc;
+  return block_inverse(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, ipiv, w, st(stream));
 }
 
 // ------------------------------------------------------------ input assembly

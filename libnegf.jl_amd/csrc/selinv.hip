@@ -6,13 +6,21 @@
 // triangular inverses U_ii^{-1}, L_ii^{-1} (ztrtri), after which every
 // triangular solve of the reference (trsm, getri) is one MFMA GEMM.
 //
-// Differences from the reference, on purpose:
-//   * the diagonal blocks are factored WITHOUT pivoting by default, which is
-//     the factorization the block elimination is consistent with; the
-//     reference runs getrf with partial pivoting and discards ipiv
-//     (src/backend_cpu.jl:255-265), which agrees with this whenever no row
-//     swap happens (diagonally dominant blocks, its test regime) and is wrong
-//     otherwise.  `ref_pivoting` = 1 reproduces the reference exactly.
+// Pivoting of the diagonal blocks (mode):
+//   0  none: the factorization the block elimination is consistent with when
+//      no pivoting is needed;
+//   1  the reference's: getrf with partial pivoting whose ipiv is discarded
+//      (src/backend_cpu.jl:255-265) -- right when no row swap happens
+//      (diagonally dominant blocks, the reference's test regime), wrong
+//      otherwise; kept to reproduce the reference exactly;
+//   2  correct partial pivoting within each diagonal block (the default of
+//      the Python API): S_ii = P_i L_ii U_ii; L_ji = A_ji U_ii^{-1} as before,
+//      and every L_ii^{-1} becomes (P_i L_ii)^{-1} = L_ii^{-1} P (LAPACK's P,
+//      P A = L U): U_ij = L_ii^{-1} P A_ij in the factorization and the same
+//      matrix in the inverse's products, so the Schur complements and the
+//      Takahashi recurrences are those of the pivoted block.  The pivots are
+//      kept in a caller array for the inverse.
+// Also different from the reference, on purpose:
 //   * any number of off-diagonal blocks per row (the reference keeps two
 //     accumulators, :146-148).
 // The pattern must be symmetric and contain the factorization's fill-in
@@ -71,6 +79,36 @@ __global__ void zero_kernel(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// Lp = Li P for instance blockIdx.x: LAPACK's swaps (1-based ipiv, applied in
+// order) give the row order pi of P A (row r of P A is row pi[r] of A), so
+// (Li P)[:, c] = Li[:, pi^{-1}[c]].  b <= 2048.
+__global__ __launch_bounds__(256) void times_perm_kernel(int b, const zc* Li, long long sLi,
+                                                          zc* Lp, long long sLp, const int* ipiv,
+                                                          long long sP) {
+  __shared__ int pi[2048], pinv[2048];
+  const int e = blockIdx.x;
+  const int* ip = ipiv + (long long)e * sP;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < b; ++k) pi[k] = k;
+    for (int k = 0; k < b; ++k) {
+      const int q = ip[k] - 1;
+      const int t = pi[k];
+      pi[k] = pi[q];
+      pi[q] = t;
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < b; r += blockDim.x) pinv[pi[r]] = r;
+  __syncthreads();
+  const zc* src = Li + (long long)e * sLi;
+  zc* dst = Lp + (long long)e * sLp;
+  const long long n = (long long)b * b;
+  for (long long k = threadIdx.x; k < n; k += blockDim.x) {
+    const int c = (int)(k / b), r = (int)(k - (long long)c * b);
+    dst[k] = src[r + (long long)pinv[c] * b];
+  }
+}
+
 }  // namespace
 
 std::vector<std::pair<int, int>> block_fill_pattern(int npl, const std::vector<std::pair<int, int>>& in) {
@@ -91,7 +129,7 @@ size_t selinv_work_bytes(int b, int batch, int max_row_nnz) {
   // (block_inverse) 2 per off-diagonal of a row and the accumulator; (both)
   // U_ii^{-1}, L_ii^{-1}, a product temporary and the triangular-inverse
   // scratch; (block_factorization) the pivots and info words in front
-  return bb * (2 * (size_t)max_row_nnz + 1 + 3) + ztrtri_work_bytes(b, batch) +
+  return bb * (2 * (size_t)max_row_nnz + 1 + 4) + ztrtri_work_bytes(b, batch) +
          sizeof(int) * ((size_t)b * batch + 2 * batch) + 8192;
 }
 
@@ -125,7 +163,8 @@ static int build(int npl, int b, int nnzb, const int* brow, const int* bcol, voi
 
 int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bcol,
                         void* const* blocks, long long ld, long long stride, int batch,
-                        bool ref_pivoting, int* info, void* work, hipStream_t s) {
+                        int mode, int* ipiv_out, int* info, void* work, hipStream_t s) {
+  if (mode < 0 || mode > 2 || (mode == 2 && !ipiv_out) || b > 2048) return Status::BAD_ARG;
   Pattern P;
   RC(build(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, &P));
   // fill-in must be present
@@ -141,7 +180,8 @@ int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bc
   zc* Ui = (zc*)zp;
   zc* Li = Ui + tb * batch;
   zc* Tm = Li + tb * batch;
-  zc* twork = Tm + tb * batch;
+  zc* Lp = Tm + tb * batch;
+  zc* twork = Lp + tb * batch;
   auto prod = [&](const zc* A, long long lda, long long sA, const zc* B, long long ldb,
                   long long sB, zc* D, long long ldd, long long sD) {
     GemmArgs g{};
@@ -157,20 +197,31 @@ int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bc
   hipLaunchKernelGGL(zero_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
   for (int i = 0; i < npl; ++i) {
     zc* Aii = P.at(i, i);
-    RC(zgetrf_launch(b, Aii, ld, stride, ipiv, b, linfo, batch, s, ref_pivoting));
+    // mode 2: this layer's pivots in the caller's array [batch][npl][b]
+    int* piv = mode == 2 ? ipiv_out + (size_t)i * b : ipiv;
+    const long long sP = mode == 2 ? (long long)npl * b : b;
+    RC(zgetrf_launch(b, Aii, ld, stride, piv, sP, linfo, batch, s, mode != 0));
     hipLaunchKernelGGL(fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, linfo, info, i,
                        batch);
     // the triangular solves of :28-34 as products with the explicit inverses
     bool any = false;
     for (int j = i + 1; j < npl; ++j) any = any || P.has(j, i) || P.has(i, j);
-    if (any) RC(tri_inverses(Aii, ld, stride, b, batch, Ui, Li, twork, s));
+    zc* Lu = Li;   // the matrix U_ij = Lu A_ij multiplies with: L^{-1}, or L^{-1} P
+    if (any) {
+      RC(tri_inverses(Aii, ld, stride, b, batch, Ui, Li, twork, s));
+      if (mode == 2) {
+        hipLaunchKernelGGL(times_perm_kernel, dim3(batch), dim3(256), 0, s, b, Li, tb, Lp, tb, piv,
+                           sP);
+        Lu = Lp;
+      }
+    }
     for (int j = i + 1; j < npl; ++j) {
       if (P.has(j, i)) {   // L_ji = A_ji U_ii^{-1}
         RC(prod(P.at(j, i), ld, stride, Ui, b, tb, Tm, b, tb));
         RC(zcopy2d_launch(b, b, Tm, b, tb, P.at(j, i), ld, stride, batch, s));
       }
-      if (P.has(i, j)) {   // U_ij = L_ii^{-1} A_ij
-        RC(prod(Li, b, tb, P.at(i, j), ld, stride, Tm, b, tb));
+      if (P.has(i, j)) {   // U_ij = L_ii^{-1} A_ij (mode 2: L_ii^{-1} P A_ij)
+        RC(prod(Lu, b, tb, P.at(i, j), ld, stride, Tm, b, tb));
         RC(zcopy2d_launch(b, b, Tm, b, tb, P.at(i, j), ld, stride, batch, s));
       }
     }
@@ -188,9 +239,11 @@ int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bc
 }
 
 int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, void* const* blocks,
-                  long long ld, long long stride, int batch, void* work, hipStream_t s) {
+                  long long ld, long long stride, int batch, const int* ipiv, void* work,
+                  hipStream_t s) {
   Pattern P;
   RC(build(npl, b, nnzb, brow, bcol, blocks, ld, stride, batch, &P));
+  if (b > 2048) return Status::BAD_ARG;
   int maxr = 0;
   for (int i = 0; i < npl; ++i) {
     int c = 0;
@@ -214,7 +267,17 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
   p += sizeof(zc) * tb * batch;
   zc* Tm = (zc*)p;
   p += sizeof(zc) * tb * batch;
+  zc* Lp = (zc*)p;
+  p += sizeof(zc) * tb * batch;
   zc* twork = (zc*)p;
+  // with pivots (mode 2 of the factorization) every right factor L_ii^{-1}
+  // below is (P_i L_ii)^{-1} = L_ii^{-1} P
+  auto lower_inverse = [&](int i) -> zc* {
+    if (!ipiv) return Li;
+    hipLaunchKernelGGL(times_perm_kernel, dim3(batch), dim3(256), 0, s, b, Li, tb, Lp, tb,
+                       ipiv + (size_t)i * b, (long long)npl * b);
+    return Lp;
+  };
   // temporaries use ld = b; the pattern's blocks use ld = P.ld
   auto gemm_mixed = [&](char ta, char tb_, zc alpha, const zc* A, long long lda, long long sA,
                         const zc* B, long long ldb, long long sB, zc beta, zc* C, long long ldc,
@@ -235,11 +298,13 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
   {
     zc* Ann = P.at(npl - 1, npl - 1);
     RC(tri_inverses(Ann, ld, bs, b, batch, Ui, Li, twork, s));
-    RC(gemm_mixed('N', 'N', ONE, Ui, b, tbs, Li, b, tbs, ZERO, Ann, ld, bs));
+    zc* Lr = lower_inverse(npl - 1);
+    RC(gemm_mixed('N', 'N', ONE, Ui, b, tbs, Lr, b, tbs, ZERO, Ann, ld, bs));
   }
   for (int i = npl - 2; i >= 0; --i) {
     zc* Aii = P.at(i, i);
     RC(tri_inverses(Aii, ld, bs, b, batch, Ui, Li, twork, s));
+    zc* Lr = lower_inverse(i);
     std::vector<int> js;
     for (int j = i + 1; j < npl; ++j) {
       if (!P.has(i, j)) continue;
@@ -258,7 +323,7 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
       // temporary, whose buffer then takes the operand's place
       RC(gemm_mixed('N', 'N', MONE, Ui, b, tbs, U[v], b, tbs, ZERO, Tm, b, tbs));
       std::swap(U[v], Tm);
-      RC(gemm_mixed('N', 'N', MONE, Lt[v], b, tbs, Li, b, tbs, ZERO, Tm, b, tbs));
+      RC(gemm_mixed('N', 'N', MONE, Lt[v], b, tbs, Lr, b, tbs, ZERO, Tm, b, tbs));
       std::swap(Lt[v], Tm);
       js.push_back(j);
     }
@@ -274,14 +339,14 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
     }
     if (first) RC(zfill_launch(b, b, ZERO, ZERO, Aii_acc, b, tbs, batch, s));
     // X_ii = U^{-1} L^{-1} - (sum_j X_ij L_ji) L^{-1}  (:195-199)
-    RC(gemm_mixed('N', 'N', ONE, Aii_acc, b, tbs, Li, b, tbs, ZERO, Tm, b, tbs));
+    RC(gemm_mixed('N', 'N', ONE, Aii_acc, b, tbs, Lr, b, tbs, ZERO, Tm, b, tbs));
     {
       GemmArgs g{};
       g.opA = OP_N; g.opB = OP_N;
       g.m = g.n = g.k = b;
       g.alpha = ONE; g.beta = MONE;
       g.A = Ui; g.lda = b; g.strideA = tbs;
-      g.B = Li; g.ldb = b; g.strideB = tbs;
+      g.B = Lr; g.ldb = b; g.strideB = tbs;
       g.C = Tm; g.ldc = b; g.strideC = tbs;
       g.D = Aii; g.ldd = ld; g.strideD = bs;
       g.batch = batch;

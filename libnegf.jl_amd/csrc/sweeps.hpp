@@ -95,11 +95,15 @@ int ddrgf(int N, int b, const BtdIn& A, const BtdOut& G, int batch, const int* l
 std::vector<std::pair<int, int>> block_fill_pattern(int npl,
                                                     const std::vector<std::pair<int, int>>& in);
 size_t selinv_work_bytes(int b, int batch, int max_row_nnz);
+// mode: 0 no pivoting, 1 the reference's (pivots discarded), 2 partial
+// pivoting within each diagonal block (pivots into ipiv [batch][npl][b],
+// which block_inverse then takes; null there = no pivots)
 int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bcol,
-                        void* const* blocks, long long ld, long long stride, int batch,
-                        bool ref_pivoting, int* info, void* work, hipStream_t s);
+                        void* const* blocks, long long ld, long long stride, int batch, int mode,
+                        int* ipiv, int* info, void* work, hipStream_t s);
 int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, void* const* blocks,
-                  long long ld, long long stride, int batch, void* work, hipStream_t s);
+                  long long ld, long long stride, int batch, const int* ipiv, void* work,
+                  hipStream_t s);
 
 int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,

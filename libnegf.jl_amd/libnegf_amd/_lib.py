@@ -91,6 +91,10 @@ SIGNATURES = {
                                            C.POINTER(vp), i64, i64, i32, i32, vp, vp]),
     "negf_block_inverse": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p, C.POINTER(vp), i64,
                                      i64, i32, vp]),
+    "negf_block_factorization_pivoted": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p,
+                                                   C.POINTER(vp), i64, i64, i32, i32, vp, vp, vp]),
+    "negf_block_inverse_pivoted": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p, C.POINTER(vp),
+                                             i64, i64, i32, vp, vp]),
     "negf_build_M_from_HS": (C.c_int, [i32, c_int32_p, vp, i32] + [vp, i64] * 9 + [vp, vp, vp, vp]),
 }
 

@@ -43,6 +43,8 @@ class BlockSparse:
         # (batch, nnzb, b*b): block k of instance m is column-major at [m, k]
         self.data = torch.zeros((self.batch, len(self.pattern), b * b), dtype=torch.complex128,
                                 device=device)
+        # pivots of a "partial" factorization ([batch, npl * b], 1-based rows)
+        self.ipiv = None
 
     def isassigned(self, i, j) -> bool:
         return (i, j) in self.index
@@ -53,6 +55,8 @@ class BlockSparse:
     def copy(self) -> "BlockSparse":
         out = BlockSparse(self.npl, self.b, self.pattern, self.batch, self.data.device)
         out.data.copy_(self.data)
+        ipiv = getattr(self, "ipiv", None)
+        out.ipiv = None if ipiv is None else ipiv.clone()
         return out
 
     def with_pattern(self, pattern) -> "BlockSparse":
@@ -109,30 +113,50 @@ def set_sparse_Block(bind, rind, cind, symmetric=True, seed=0, dominant=True, de
     return BlockSparse.from_host(blocks, npl, device)
 
 
-def Block_factorization_(T: BlockSparse, ref_pivoting: bool = False) -> BlockSparse:
+PIVOTING = {"none": 0, "reference": 1, "partial": 2}
+
+
+def Block_factorization_(T: BlockSparse, ref_pivoting: bool = False,
+                         pivoting: str | None = None) -> BlockSparse:
     """In place.  The pattern must already hold the fill-in (use
-    ``Block_factorization``, which adds it)."""
+    ``Block_factorization``, which adds it).
+
+    ``pivoting``: "partial" (default) pivots inside every diagonal block and
+    applies the permutation to its U row blocks, keeping the pivots on the
+    container for ``Block_inverse``; "none" factors without row swaps;
+    "reference" reproduces the reference (partial pivoting whose pivots are
+    discarded, ``src/backend_cpu.jl:255-265``: right only without swaps).
+    ``ref_pivoting = True`` is "reference"."""
+    if pivoting is None:
+        pivoting = "reference" if ref_pivoting else "partial"
+    mode = PIVOTING[pivoting]
     n, br, bc, ptrs, b, stride = T._args()
     info = torch.zeros(T.batch, dtype=torch.int32, device=T.data.device)
-    call("negf_block_factorization", T.npl, b, n, br, bc, ptrs, b, stride, T.batch,
-         int(ref_pivoting), C.c_void_p(info.data_ptr()),
-         C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    T.ipiv = torch.zeros((T.batch, T.npl * b), dtype=torch.int32,
+                         device=T.data.device) if mode == 2 else None
+    call("negf_block_factorization_pivoted", T.npl, b, n, br, bc, ptrs, b, stride, T.batch, mode,
+         C.c_void_p(T.ipiv.data_ptr()) if T.ipiv is not None else None,
+         C.c_void_p(info.data_ptr()), C.c_void_p(torch.cuda.current_stream().cuda_stream))
     bad = info.cpu()
     if bool((bad != 0).any()):
         raise SingularError("Block_factorization", bad.tolist())
     return T
 
 
-def Block_factorization(T: BlockSparse, ref_pivoting: bool = False) -> BlockSparse:
+def Block_factorization(T: BlockSparse, ref_pivoting: bool = False,
+                        pivoting: str | None = None) -> BlockSparse:
     """Copy with the fill-in blocks added, then factor
     (``src/selected_inverse.jl:124-128``)."""
     F = T.with_pattern(fill_pattern(T.npl, T.pattern))
-    return Block_factorization_(F, ref_pivoting)
+    return Block_factorization_(F, ref_pivoting, pivoting)
 
 
 def Block_inverse_(T: BlockSparse) -> BlockSparse:
+    """In place; uses the pivots a "partial" factorization kept on T."""
     n, br, bc, ptrs, b, stride = T._args()
-    call("negf_block_inverse", T.npl, b, n, br, bc, ptrs, b, stride, T.batch,
+    ipiv = getattr(T, "ipiv", None)
+    call("negf_block_inverse_pivoted", T.npl, b, n, br, bc, ptrs, b, stride, T.batch,
+         C.c_void_p(ipiv.data_ptr()) if ipiv is not None else None,
          C.c_void_p(torch.cuda.current_stream().cuda_stream))
     return T
 

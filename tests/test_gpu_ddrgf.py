@@ -120,27 +120,32 @@ def _band_residual(A, G):
 def test_ddrgf_uneven_devices_8_partitions(N, b):
     """The three devices of the uneven DD-RGF workload at their real shapes,
     each split into 8 partitions (blockSizeD1 = 15 / 7 / 3), eta = 1e-4, two
-    energies.  At this conditioning the two FP64 algorithms disagree by more
-    than the reference's 2.5e-10 (the oracle's own DD-RGF vs RGF on the
-    (128, 256) device: 1.8e-8 at E = -0.35), so the full-size bar is the
+    energies.  At this conditioning the reference's DD-RGF (no pivoting
+    across the separators' Schur complement) and RGF disagree by more than
+    the reference's 2.5e-10 (the oracle on the (128, 256) device: 1.8e-8 at
+    E = -0.35, tests/test_fp64_floors.py).  So the bar is that spread,
+    measured here by the oracle on the same input: the GPU DD-RGF is held to
+    5 x the oracle's DD-RGF-vs-RGF spread from the GPU RGF; besides, the
     normwise residual of the returned band, which does not depend on the
-    conditioning: RGF's is ~1e-14; DD-RGF's (the reference's algorithm, no
-    pivoting across the separator Schur complement) is 1.6e-10 in the
-    oracle on that device, and a 1e-6 relative change of one block raises it
-    to 1.6e-8."""
+    conditioning (RGF ~1e-14; DD-RGF 1.6e-10 in the oracle on that device, a
+    1e-6 relative change of one block raises it to 1.6e-8)."""
     from libnegf_amd import api
+    from oracle.ddrgf import bndiag_of_inv_ddrgf as dd_ref
+    from oracle.rgf import bndiag_of_inv_rgf_local as rgf_ref
     levels = dd.levels_for_tasks(N, 8)
     print(f"ddrgf device N={N} b={b} levels={levels}", flush=True)
     dev = device_hamiltonian(N, b, seed=N + b)
-    Min = DeviceBlockTridiag.from_host([assemble_A(dev, E, 1e-4) for E in (-0.35, 0.3)])
+    mats = [assemble_A(dev, E, 1e-4) for E in (-0.35, 0.3)]
+    Min = DeviceBlockTridiag.from_host(mats)
     Gd = Min.similar()
     dd.bndiag_of_inv_ddrgf(Gd, Min, dd.allocate_aux_data_DDRGF(Min, levels=levels))
     Gr = Min.similar()
     api.bndiag_of_inv_rgf_local(Gr, Min, api.allocate_aux_data_RGF(Min))
     err = _gpu_block_relerr(Gd, Gr)
+    spread = max(max_block_err(dd_ref(A, levels), rgf_ref(A)) for A in mats)
     rr, rd = _band_residual(Min, Gr), _band_residual(Min, Gd)
-    print(f"  DD-RGF vs RGF max block relerr {err:.2e}; residual RGF {rr:.2e} DD-RGF {rd:.2e}",
-          flush=True)
+    print(f"  DD-RGF vs RGF max block relerr {err:.2e} (oracle spread {spread:.2e}); "
+          f"residual RGF {rr:.2e} DD-RGF {rd:.2e}", flush=True)
     assert rr < 1e-12
     assert rd < 2e-9
-    assert err < 1e-6
+    assert err < 5 * spread

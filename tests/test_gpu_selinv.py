@@ -95,3 +95,54 @@ def test_selinv_production_block_size_matches_rgf():
     err = max(relfro(X[k], G[k]) for k in G)
     print(f"  max block relerr {err:.2e}", flush=True)
     assert err < 1e-11
+
+
+def _nondominant(npl, b, seed):
+    """Random blocks on the tridiagonal pattern, with every diagonal block's
+    rows shuffled so that partial pivoting must swap rows (the reference's
+    pivot-discarding factorization is then wrong)."""
+    rng = np.random.default_rng(seed)
+    pat = [(i, i) for i in range(npl)] + [(i, i + 1) for i in range(npl - 1)] + \
+          [(i + 1, i) for i in range(npl - 1)]
+    blocks = {}
+    for (i, j) in pat:
+        blk = rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b))
+        if i == j:
+            blk = (blk + 3 * np.sqrt(b) * np.eye(b))[rng.permutation(b)]
+        blocks[(i, j)] = np.asfortranarray(blk)
+    return blocks
+
+
+@pytest.mark.parametrize("b", [8, 64, 130])
+def test_partial_pivoting_matches_dense_inverse_where_reference_fails(b):
+    print(f"selinv pivoting modes b={b}", flush=True)
+    npl = 5
+    blocks = _nondominant(npl, b, seed=b)
+    T = si.BlockSparse.from_host(blocks, npl)
+    A = si.full(T)
+    inv = np.linalg.inv(A)
+    got = {}
+    for mode in ("partial", "reference", "none"):
+        X = si.Block_inverse(si.Block_factorization(T, pivoting=mode))
+        got[mode] = X.to_host()
+    err = {m: max(relfro(blk, inv[i * b:(i + 1) * b, j * b:(j + 1) * b])
+                  for (i, j), blk in got[m].items()) for m in got}
+    print(f"  vs dense inverse: {err}")
+    assert err["partial"] < 1e-12
+    # the reference's semantics: its pivots discarded, hence wrong here ...
+    assert err["reference"] > 1e-2
+    # ... and equal to the oracle's restatement of the reference
+    Xo = ref.Block_inverse(ref.Block_factorization(T.to_host(), npl), npl)
+    assert max(relfro(blk, Xo[k]) for k, blk in got["reference"].items()) < 1e-10
+
+
+def test_pivoting_modes_agree_on_the_reference_pattern():
+    """The reference's own test pattern (test/test_selected_inverse.jl:13-15,
+    diagonally dominant: no row swaps): every mode gives the same blocks,
+    and the reference-semantics mode equals the oracle's restatement."""
+    npl, b = 5, 40
+    T = si.set_sparse_Block([b] * npl, RIND, CIND, True, seed=6)
+    Xo = ref.Block_inverse(ref.Block_factorization(T.to_host(), npl), npl)
+    for mode in ("partial", "reference", "none"):
+        X = si.Block_inverse(si.Block_factorization(T, pivoting=mode)).to_host()
+        assert max(relfro(blk, Xo[k]) for k, blk in X.items()) < 1e-12, mode
