@@ -186,6 +186,30 @@ int negf_rgf(int32_t nblocks, const int32_t* block_sizes,
              void* g_diag, void* g_upper, void* g_lower,
              int32_t batch, int32_t* info, void* stream);
 
+/* The w-band of A^{-1} for a block (2w+1)-diagonal A by RGF, natively
+ * (bndiag_of_inv_general_rgf!, src/matinvertndiag_general_rgf.jl:31-146:
+ * kmax x kmax Schur updates of the band above each eliminated layer, then
+ * the downward w-band fill).  Band layout of A_band and G_band: the blocks
+ * (i, j) with |i - j| <= w in row-major order of (i, j), each b_i x b_j
+ * column-major, packed; G_band's batch stride is the band's element count
+ * (negf_general_rgf_band_elements).  The input is not modified (the
+ * reference writes its Schur updates into Min, :86). */
+int negf_general_rgf(int32_t nblocks, const int32_t* block_sizes, int32_t w, const void* A_band,
+                     int64_t A_bstride, void* G_band, int32_t batch, int32_t* info,
+                     void* stream);
+/* Complex elements of one instance's band in that layout (0 on bad input). */
+int64_t negf_general_rgf_band_elements(int32_t nblocks, const int32_t* block_sizes, int32_t w);
+
+/* Every block of A^{-1} by RGF, the reference's buildFullInv option
+ * (src/matinvertndiag_rgf.jl:185-199): the band, then G_{i,j} =
+ * -G_{i,j-1} t_{j-1} and G_{j,i} = -l_j G_{j-1,i} for j >= i + 2.  g_full:
+ * per instance a dense column-major n x n matrix (n = sum of block_sizes,
+ * ld n, batch stride n^2).  Other arguments as negf_rgf. */
+int negf_rgf_full(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                  int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                  const void* lower, int64_t lower_bstride, void* g_full, int32_t batch,
+                  int32_t* info, void* stream);
+
 /* negf_rgf without any device allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,

@@ -328,6 +328,46 @@ int negf_rgf(int32_t nblocks, const int32_t* block_sizes, const void* diag, int6
                       nullptr, nullptr, 0, nullptr, 0, nullptr, batch, info, stream);
 }
 
+int negf_rgf_full(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                  int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                  const void* lower, int64_t lower_bstride, void* g_full, int32_t batch,
+                  int32_t* info, void* stream) {
+  if (!valid_sizes(nblocks, block_sizes) || batch < 0) return NEGF_BAD_ARG;
+  if (batch == 0) return NEGF_OK;
+  if (!diag || !g_full || (nblocks > 1 && !upper)) return NEGF_BAD_ARG;
+  Layout L(nblocks, block_sizes);
+  void* w = nullptr;
+  const size_t sw = sweep_work_bytes(L, batch, false);
+  int rc = workspace(stream, sw + sizeof(int32_t) * (size_t)batch + 256, &w);
+  if (rc) return rc;
+  int32_t* inf = info ? info : (int32_t*)((char*)w + sw);
+  BtdIn A{(const zc*)diag, diag_bstride, (const zc*)upper, upper_bstride, (const zc*)lower,
+          lower_bstride};
+  return sweep_rgf_full(L, A, (zc*)g_full, batch, inf, w, st(stream));
+}
+
+int negf_general_rgf(int32_t nblocks, const int32_t* block_sizes, int32_t w, const void* A_band,
+                     int64_t A_bstride, void* G_band, int32_t batch, int32_t* info,
+                     void* stream) {
+  if (!valid_sizes(nblocks, block_sizes) || w < 1 || batch < 0) return NEGF_BAD_ARG;
+  if (batch == 0) return NEGF_OK;
+  if (!A_band || !G_band || A_bstride < 0) return NEGF_BAD_ARG;
+  void* wk = nullptr;
+  const size_t bytes = general_rgf_work_bytes(nblocks, block_sizes, w, batch);
+  int rc = workspace(stream, bytes + sizeof(int32_t) * (size_t)batch + 256, &wk);
+  if (rc) return rc;
+  int32_t* inf = info ? info : (int32_t*)((char*)wk + bytes);
+  return sweep_general_rgf(nblocks, block_sizes, w, (const zc*)A_band, A_bstride, (zc*)G_band,
+                           batch, inf, wk, st(stream));
+}
+
+int64_t negf_general_rgf_band_elements(int32_t nblocks, const int32_t* block_sizes, int32_t w) {
+  if (!valid_sizes(nblocks, block_sizes) || w < 1) return 0;
+  long long total = 0;
+  band_offsets(nblocks, block_sizes, w, &total);
+  return (int64_t)total;
+}
+
 int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t n_energies,
                                             int32_t wave, int32_t streams) {
   if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0 || streams <= 0) return 0;

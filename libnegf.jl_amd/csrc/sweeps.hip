@@ -576,6 +576,196 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
   return Status::OK;
 }
 
+// Every block of A^{-1} (the reference's buildFullInv, src/matinvertndiag_rgf.jl:
+// 185-199) into a dense column-major n x n matrix per instance (batch stride
+// n^2): the band from the downward pass written in place, then block
+// diagonal d = 2 .. N-1 from diagonal d - 1:
+//   G_{i,i+d} = -G_{i,i+d-1} t_{i+d-1},   G_{i+d,i} = -l_{i+d} G_{i+d-1,i},
+// one launch per diagonal and direction over (i, instance) when the blocks
+// are uniform, one per block otherwise.
+int sweep_rgf_full(const Layout& L, const BtdIn& A, zc* Gf, int batch, int* info, void* work,
+                   hipStream_t s) {
+  SweepWork w;
+  carve(work, L, batch, &w);
+  RC(set_one(w.ones, s));
+  Source S{};
+  S.diag = A.diag; S.diag_bs = A.diag_bs;
+  S.up = A.up; S.up_bs = A.up_bs;
+  S.low = A.low; S.low_bs = A.low_bs;
+  S.sLow = ONE; S.sUp = ONE;
+  RC(upward(L, S, w, batch, info, s));
+  const int N = L.N;
+  std::vector<long long> off(N + 1, 0);
+  for (int i = 0; i < N; ++i) off[i + 1] = off[i] + L.b[i];
+  const long long n = off[N], nn = n * n;
+  auto Gb = [&](int i, int j) { return MW{Gf + off[i] + off[j] * n, n, nn}; };
+  auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], L.no}; };        // t_i: b_i x b_{i+1}
+  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], L.no}; };    // l_i: b_i x b_{i-1}
+  const M none{nullptr, 1, 0};
+  // band: G_00 = g_0, then G_{i-1,i}, G_{i,i-1}, G_ii = g_i - l_i G_{i-1,i}
+  RC(zcopy2d_launch(L.b[0], L.b[0], w.g, L.b[0], L.nd, Gf, n, nn, batch, s));
+  for (int i = 1; i < N; ++i) {
+    const int bp = L.b[i - 1], bi = L.b[i];
+    RC(gemm(OP_N, OP_N, bp, bi, bp, MONE, ro(Gb(i - 1, i - 1)), t(i - 1), ZERO, none, Gb(i - 1, i),
+            batch, s));
+    RC(gemm(OP_N, OP_N, bi, bp, bp, MONE, l(i), ro(Gb(i - 1, i - 1)), ZERO, none, Gb(i, i - 1),
+            batch, s));
+    RC(gemm(OP_N, OP_N, bi, bi, bp, MONE, l(i), ro(Gb(i - 1, i)), ONE,
+            M{w.g + L.od[i], bi, L.nd}, Gb(i, i), batch, s));
+  }
+  bool uniform = true;
+  for (int i = 1; i < N; ++i) uniform = uniform && L.b[i] == L.b[0];
+  for (int d = 2; d < N; ++d) {
+    if (uniform) {
+      const int b = L.b[0], cnt = N - d;
+      const long long diag_step = (long long)b * (n + 1), bb = (long long)b * b;
+      GemmArgs g{};
+      apply_prec(g);
+      g.opA = OP_N; g.opB = OP_N; g.m = g.n = g.k = b;
+      g.alpha = MONE; g.beta = ZERO;
+      g.batch = cnt * batch;
+      g.inner = cnt;
+      // upper: A = G_{i,i+d-1}, B = t_{i+d-1}, D = G_{i,i+d}
+      g.A = Gf + (d - 1) * b * n; g.lda = n; g.strideA = diag_step; g.outerA = nn;
+      g.B = w.t + (d - 1) * bb; g.ldb = b; g.strideB = bb; g.outerB = L.no;
+      g.D = Gf + (long long)d * b * n; g.ldd = n; g.strideD = diag_step; g.outerD = nn;
+      RC(zgemm_launch(g, s));
+      // lower: A = l_{i+d}, B = G_{i+d-1,i}, D = G_{i+d,i}
+      g.A = w.l + (d - 1) * bb; g.lda = b; g.strideA = bb; g.outerA = L.no;
+      g.B = Gf + (long long)(d - 1) * b; g.ldb = n; g.strideB = diag_step; g.outerB = nn;
+      g.D = Gf + (long long)d * b; g.ldd = n; g.strideD = diag_step; g.outerD = nn;
+      RC(zgemm_launch(g, s));
+    } else {
+      for (int i = 0; i + d < N; ++i) {
+        const int j = i + d;
+        RC(gemm(OP_N, OP_N, L.b[i], L.b[j], L.b[j - 1], MONE, ro(Gb(i, j - 1)), t(j - 1), ZERO, none,
+                Gb(i, j), batch, s));
+        RC(gemm(OP_N, OP_N, L.b[j], L.b[i], L.b[j - 1], MONE, l(j), ro(Gb(j - 1, i)), ZERO, none,
+                Gb(j, i), batch, s));
+      }
+    }
+  }
+  return Status::OK;
+}
+
+// ---------------------------------------------------------------------------
+// Block (2w+1)-diagonal RGF, natively: bndiag_of_inv_general_rgf!
+// (src/matinvertndiag_general_rgf.jl:31-146) with the LU solves as products
+// with explicit inverses g_i.  Band layout: blocks (i, j), |i - j| <= w, in
+// row-major order of (i, j), each b_i x b_j column-major (band_offsets).
+//   upward, ix = N-2 .. 0, kmax = min(w, ix + 1), layers r, c in
+//   ix - kmax + 1 .. ix:
+//     T_{r,ix+1} = M_{r,ix+1} g_{ix+1},  L_{ix+1,c} = g_{ix+1} M_{ix+1,c}   (:58-69)
+//     M_{r,c} -= M_{r,ix+1} L_{ix+1,c};  g_ix = M_{ix,ix}^{-1}             (:72-89)
+//   downward, ix = 1 .. N-1, m, p over the kmax = min(w, ix) layers above:
+//     G_{p,ix} = -sum_m G_{p,m} T_{m,ix},  G_{ix,p} = -sum_m L_{ix,m} G_{m,p}
+//     G_{ix,ix} = g_ix - sum_m L_{ix,m} G_{m,ix}                           (:100-128)
+// M is a working copy of the input band (the reference updates its input in
+// place, :86); T and L share one band buffer (T above the diagonal, L below).
+std::vector<long long> band_offsets(int N, const int* b, int w, long long* total) {
+  std::vector<long long> off((size_t)N * (2 * w + 1), -1);
+  long long o = 0;
+  for (int i = 0; i < N; ++i)
+    for (int j = (i - w > 0 ? i - w : 0); j <= (i + w < N - 1 ? i + w : N - 1); ++j) {
+      off[(size_t)i * (2 * w + 1) + (j - i + w)] = o;
+      o += (long long)b[i] * b[j];
+    }
+  *total = o;
+  return off;
+}
+
+size_t general_rgf_work_bytes(int N, const int* b, int w, int batch) {
+  long long band = 0;
+  band_offsets(N, b, w, &band);
+  int bmax = 0;
+  long long nd = 0;
+  for (int i = 0; i < N; ++i) {
+    bmax = b[i] > bmax ? b[i] : bmax;
+    nd += (long long)b[i] * b[i];
+  }
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  return up(sizeof(zc) * 2 * band * batch) + up(sizeof(zc) * nd * batch) +
+         up(sizeof(int) * batch) + up(zinv_work_bytes(bmax, batch)) + 4096;
+}
+
+int sweep_general_rgf(int N, const int* b, int w, const zc* Ain, long long a_bs, zc* Gout,
+                      int batch, int* info, void* work, hipStream_t s) {
+  long long band = 0;
+  const std::vector<long long> off = band_offsets(N, b, w, &band);
+  auto at = [&](int i, int j) { return off[(size_t)i * (2 * w + 1) + (j - i + w)]; };
+  std::vector<long long> od(N + 1, 0);
+  int bmax = 0;
+  for (int i = 0; i < N; ++i) {
+    od[i + 1] = od[i] + (long long)b[i] * b[i];
+    bmax = b[i] > bmax ? b[i] : bmax;
+  }
+  const long long nd = od[N];
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  char* p = (char*)work;
+  zc* Mw = (zc*)p;                  // working band copy
+  zc* TL = Mw + band * batch;       // multipliers
+  p += up(sizeof(zc) * 2 * band * batch);
+  zc* g = (zc*)p;                   // g_i, packed diagonal layout
+  p += up(sizeof(zc) * nd * batch);
+  int* linfo = (int*)p;
+  p += up(sizeof(int) * batch);
+  InvWork iw;
+  zinv_work_carve(p, bmax, batch, &iw);
+  auto MB = [&](zc* base, int i, int j) { return MW{base + at(i, j), b[i], band}; };
+  auto Mr = [&](const zc* base, int i, int j) { return M{base + at(i, j), b[i], band}; };
+  auto G = [&](int i, int j) { return MW{Gout + at(i, j), b[i], band}; };
+  auto gi = [&](int i) { return M{g + od[i], b[i], nd}; };
+  const M none{nullptr, 1, 0};
+  hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+  RC(zcopy2d_launch((int)band, 1, Ain, band, a_bs, Mw, band, band, batch, s));
+  auto invert = [&](int i, const zc* src) -> int {
+    RC(zinv_launch(b[i], src, b[i], band, g + od[i], b[i], nd, batch, iw, linfo, s));
+    hipLaunchKernelGGL(fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, linfo, info,
+                       i, batch);
+    return Status::OK;
+  };
+  RC(invert(N - 1, Mw + at(N - 1, N - 1)));
+  for (int ix = N - 2; ix >= 0; --ix) {
+    const int nx = ix + 1, kmax = (w < ix + 1 ? w : ix + 1);
+    for (int k = 1; k <= kmax; ++k) {
+      const int t = ix - k + 1;
+      RC(gemm(OP_N, OP_N, b[t], b[nx], b[nx], ONE, Mr(Mw, t, nx), gi(nx), ZERO, none, MB(TL, t, nx),
+              batch, s));
+      RC(gemm(OP_N, OP_N, b[nx], b[t], b[nx], ONE, gi(nx), Mr(Mw, nx, t), ZERO, none, MB(TL, nx, t),
+              batch, s));
+    }
+    for (int i = 1; i <= kmax; ++i)
+      for (int j = 1; j <= kmax; ++j) {
+        const int r = ix - i + 1, c = ix - j + 1;
+        RC(gemm(OP_N, OP_N, b[r], b[c], b[nx], MONE, Mr(Mw, r, nx), Mr(TL, nx, c), ONE, Mr(Mw, r, c),
+                MB(Mw, r, c), batch, s));
+      }
+    RC(invert(ix, Mw + at(ix, ix)));
+  }
+  RC(zcopy2d_launch(b[0], b[0], g, b[0], nd, Gout + at(0, 0), b[0], band, batch, s));
+  for (int ix = 1; ix < N; ++ix) {
+    const int kmax = (w < ix ? w : ix);
+    for (int k = 1; k <= kmax; ++k) {
+      const int tp = ix - k;
+      for (int k2 = 1; k2 <= kmax; ++k2) {
+        const int m = ix - k2;
+        const zc beta = k2 == 1 ? ZERO : ONE;
+        RC(gemm(OP_N, OP_N, b[tp], b[ix], b[m], MONE, ro(G(tp, m)), Mr(TL, m, ix), beta,
+                ro(G(tp, ix)), G(tp, ix), batch, s));
+        RC(gemm(OP_N, OP_N, b[ix], b[tp], b[m], MONE, Mr(TL, ix, m), ro(G(m, tp)), beta,
+                ro(G(ix, tp)), G(ix, tp), batch, s));
+      }
+    }
+    for (int k2 = 1; k2 <= kmax; ++k2) {
+      const int m = ix - k2;
+      const M cin = k2 == 1 ? gi(ix) : ro(G(ix, ix));
+      RC(gemm(OP_N, OP_N, b[ix], b[ix], b[m], MONE, Mr(TL, ix, m), ro(G(m, ix)), ONE, cin, G(ix, ix),
+              batch, s));
+    }
+  }
+  return Status::OK;
+}
+
 // One stream's share of the energy grid: energies [e_begin, e_end) in waves.
 static size_t energies_group_fp64_bytes(const Layout& L, int wave) {
   return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +

@@ -66,6 +66,16 @@ size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams,
                            const EnergyPrec* pr = nullptr);
 
+// Block (2w+1)-diagonal RGF (band layout: blocks (i, j), |i - j| <= w, in
+// row-major order, each b_i x b_j column-major; band_offsets gives offsets).
+std::vector<long long> band_offsets(int N, const int* b, int w, long long* total);
+size_t general_rgf_work_bytes(int N, const int* b, int w, int batch);
+int sweep_general_rgf(int N, const int* b, int w, const zc* A, long long a_bstride, zc* G,
+                      int batch, int* info, void* work, hipStream_t s);
+// Every block of A^{-1} into a dense column-major n x n matrix per instance
+// (n = sum of the block sizes, batch stride n^2).
+int sweep_rgf_full(const Layout& L, const BtdIn& A, zc* Gfull, int batch, int* info, void* work,
+                   hipStream_t s);
 // inplace_inverse: the diagonal-block inverses never take the split steps
 // (which run on a second stream the library creates on first use).
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,

@@ -66,6 +66,9 @@ SIGNATURES = {
     "negf_zfill_strided_batched": (C.c_int, [i32, i32, c_double_p, c_double_p, vp, i64, i64, i32,
                                              vp]),
     "negf_rgf": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32, vp, vp]),
+    "negf_general_rgf": (C.c_int, [i32, c_int32_p, i32, vp, i64, vp, i32, vp, vp]),
+    "negf_general_rgf_band_elements": (C.c_int64, [i32, c_int32_p, i32]),
+    "negf_rgf_full": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, i32, vp, vp]),
     "negf_rgf_noalloc": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32, vp,
                                    vp]),
     "negf_rgf_rkd": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, vp,

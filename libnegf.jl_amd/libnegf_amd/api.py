@@ -60,10 +60,13 @@ def _check_info(where, info: torch.Tensor):
 class AuxDataRGF:
     """``AuxDataRGF`` (``src/matinvertndiag_rgf.jl:10-14``).  The device
     sweep keeps its buffers in the library's cached workspace; the holder
-    carries the per-instance info array and the ``buildFullInv`` flag."""
+    carries the per-instance info array, the ``buildFullInv`` flag and, when
+    set, ``fullInv``: every block of the inverse as a dense column-major
+    matrix per instance, shape (batch, n, n) indexed [instance, column, row]."""
 
     info: torch.Tensor
     buildFullInv: bool = False
+    fullInv: torch.Tensor | None = None
 
 
 def allocate_aux_data_RGF(M: DeviceBlockTridiag) -> AuxDataRGF:
@@ -78,14 +81,33 @@ def bndiag_of_inv_rgf_local(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag,
     Min may hold a batch of instances of one block structure; Mout must match.
     Raises SingularError where the reference would print and exit()."""
     _require_gpu()
-    if auxData is not None and auxData.buildFullInv:
-        raise NotImplementedError("buildFullInv: full inverse blocks are produced by the DD-RGF "
-                                  "subdomain solver, not by this entry point")
     if Mout.layout.b != Min.layout.b or Mout.batch != Min.batch or Mout.lower is None:
         raise ValueError("Mout must be a non-Hermitian BlockMatrix shaped like Min")
     L = Min.layout
     info = auxData.info if auxData is not None else torch.zeros(Min.batch, dtype=torch.int32,
                                                                  device=Min.diag.device)
+    if auxData is not None and auxData.buildFullInv:
+        # every block (:185-199) into auxData.fullInv, the band copied into Mout
+        n = sum(L.b)
+        full = torch.empty((Min.batch, n, n), dtype=torch.complex128, device=Min.diag.device)
+        call("negf_rgf_full", L.N, L.sizes_c(), _ptr(Min.diag), L.nd, _ptr(Min.upper),
+             max(L.no, 1), _ptr(Min.lower) if Min.lower is not None else None, max(L.no, 1),
+             _ptr(full), Min.batch, _ptr(info), _stream())
+        _check_info("bndiag_of_inv_rgf_local (buildFullInv)", info)
+        off = [0]
+        for x in L.b:
+            off.append(off[-1] + x)
+
+        def blk(i, j):   # column-major flat block (i, j) of every instance
+            return full[:, off[j]:off[j + 1], off[i]:off[i + 1]].reshape(Min.batch, -1)
+
+        for i in range(L.N):
+            Mout.diag[:, L.od[i]:L.od[i + 1]] = blk(i, i)
+        for i in range(L.N - 1):
+            Mout.upper[:, L.oo[i]:L.oo[i + 1]] = blk(i, i + 1)
+            Mout.lower[:, L.oo[i]:L.oo[i + 1]] = blk(i + 1, i)
+        auxData.fullInv = full
+        return Mout
     call("negf_rgf", L.N, L.sizes_c(), _ptr(Min.diag), L.nd, _ptr(Min.upper), max(L.no, 1),
          _ptr(Min.lower) if Min.lower is not None else None, max(L.no, 1),
          _ptr(Mout.diag), _ptr(Mout.upper), _ptr(Mout.lower), Min.batch, _ptr(info), _stream())

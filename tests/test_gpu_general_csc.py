@@ -15,14 +15,32 @@ from oracle.rgf import bndiag_of_inv_rgf_local  # noqa: E402
 
 
 @pytest.mark.parametrize("N,w", [(10, 2), (9, 3), (7, 1)])
-def test_general_rgf_fused_route_vs_oracle(N, w):
-    print(f"general rgf N={N} w={w}", flush=True)
+@pytest.mark.parametrize("method", ["native", "fused"])
+def test_general_rgf_vs_oracle(N, w, method):
+    """The native w-band recursion and the fused route against the oracle's
+    restatement of bndiag_of_inv_general_rgf! and the dense inverse's band."""
+    print(f"general rgf N={N} w={w} {method}", flush=True)
     sizes = [16 + (i % 3) for i in range(N)]
     A = random_ndiag(sizes, w, seed=7 * N + w)
-    G = general.bndiag_of_inv_general_rgf(A, sizes, w)
+    G = general.bndiag_of_inv_general_rgf(A, sizes, w, method=method)
     ref = bndiag_of_inv_general_rgf(A, sizes, w)
     assert set(G) == set(ref)
     assert max_block_err(G, ref) < 1e-12
+    dense = band_of_dense(np.linalg.inv(ndiag_to_dense(A, sizes)), sizes, w)
+    assert max_block_err(G, dense) < 1e-12
+
+
+def test_general_rgf_native_batched_and_matches_fused():
+    """A batch of three instances in one call, uneven sizes, w = 2 (and the
+    reference's own check: general == fused,
+    test/test_matinvertndiag_general_rgf_fused.jl:39-90)."""
+    sizes = [20, 24, 17, 30, 22, 19, 25, 21]
+    mats = [random_ndiag(sizes, 2, seed=s) for s in (1, 2, 3)]
+    G = general.bndiag_of_inv_general_rgf(mats, sizes, 2)
+    for a, g in zip(mats, G):
+        f = general.bndiag_of_inv_general_rgf(a, sizes, 2, method="fused")
+        assert max_block_err(g, f) < 1e-12
+        assert max_block_err(g, bndiag_of_inv_general_rgf(a, sizes, 2)) < 1e-12
 
 
 def _csc_of(A):

@@ -320,3 +320,30 @@ def test_rgf_uneven_layers_smaller_block_wider_panel(sizes, batch, inv_variant):
         err = max_block_err(_band(out.to_host(k)), rgf_ref(mats[k]))
         print(f"  instance {k}: max block relerr {err:.2e}")
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("sizes,batch", [([32] * 8, 2), ([20, 24, 17, 30, 22, 19], 1), ([40, 40], 1)])
+def test_rgf_build_full_inverse(sizes, batch):
+    """auxData.buildFullInv (src/matinvertndiag_rgf.jl:185-199): every block of
+    the inverse, uniform blocks (one launch per block diagonal over all
+    blocks and instances) and uneven blocks; against the oracle's
+    full_inverse and the dense inverse; the band in Mout as without the flag."""
+    from libnegf_amd.synthetic import BlockTridiag
+    print(f"rgf full inverse sizes={sizes} batch={batch}", flush=True)
+    mats = _uneven_instances(sizes, [complex(E, 5e-2) for E in np.linspace(-0.5, 0.5, batch)], 9)
+    Min = DeviceBlockTridiag.from_host(mats)
+    Mout = Min.similar()
+    aux = api.allocate_aux_data_RGF(Min)
+    aux.buildFullInv = True
+    api.bndiag_of_inv_rgf_local(Mout, Min, aux)
+    n = sum(sizes)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for e, A in enumerate(mats):
+        F = aux.fullInv[e].cpu().numpy().T           # [column, row] -> dense
+        assert relfro(F, np.linalg.inv(A.to_dense())) < 1e-12
+        ref = rgf_ref(A, full_inverse=True)
+        for (i, j), blk in ref.items():
+            assert relfro(F[off[i]:off[i + 1], off[j]:off[j + 1]], blk) < 1e-12, (i, j)
+        band = _band(Mout.to_host(e))
+        for k, blk in band.items():
+            assert np.array_equal(blk, F[off[k[0]]:off[k[0] + 1], off[k[1]]:off[k[1] + 1]])
