@@ -295,6 +295,14 @@ def single_energy(a, torch, api):
             tf += (t1 - t0) / reps
             ti += (t2 - t1) / reps
     print(f"# selected inverse {1e3 * (tf + ti):.1f} ms", file=sys.stderr, flush=True)
+    # the reference's sweep op by op through the backend binding (what
+    # backend_amd.jl runs under the unchanged bndiag_of_inv_rgf_local!)
+    from libnegf_amd import perop
+    Mp = M1.similar(zero=False)
+    paux = perop.AuxDataRGFPerOp(M1)
+    t_perop = timed(lambda: perop.bndiag_of_inv_rgf_local_perop(Mp, M1, paux))
+    print(f"# per-op sweep {t_perop * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    Gp = Mp.to_host(0)
     Xh = X.to_host(0)
     band = {}
     for i in range(N):
@@ -304,6 +312,11 @@ def single_energy(a, torch, api):
         band[(i + 1, i)] = G1.lower[i]
     rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
     si_err = max(rel(Xh[k], band[k]) for k in band)
+    perop_band = {(i, i): Gp.diag[i] for i in range(N)}
+    for i in range(N - 1):
+        perop_band[(i, i + 1)] = Gp.upper[i]
+        perop_band[(i + 1, i)] = Gp.lower[i]
+    perop_err = max(rel(perop_band[k], band[k]) for k in band)
     cpu = None
     rgf_err = None
     if not a.no_cpu_baseline:
@@ -319,27 +332,37 @@ def single_energy(a, torch, api):
     out = {"workload": "single_energy: N=64 b=512 E=0.3 eta=1e-4 (negf_rgf, selected inverse)",
            "rgf": rgf, "selected_inverse_ms": round(1e3 * (tf + ti), 1),
            "selected_inverse_split_ms": {"factorization": round(1e3 * tf, 1), "inverse": round(1e3 * ti, 1)},
+           "perop_sweep_ms": round(1e3 * t_perop, 1),
+           "perop_sweep_note": "bndiag_of_inv_rgf_local! op by op through the backend binding: "
+                               "be_lu / be_mldivide (C and N) / be_ctranspose / be_copy_in_hw / "
+                               "be_gemm per layer, the LU's info read on the host after each be_lu",
            "rgf_relerr_vs_cpu_oracle": rgf_err, "selinv_relerr_vs_gpu_rgf": si_err,
+           "perop_relerr_vs_gpu_rgf": perop_err,
            "reps": reps, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 
 def ddrgf_uneven(a, torch, api):
     """Uneven batched DD-RGF: devices (N, b) = (128, 256), (64, 512),
-    (32, 1024), each split into 8 partitions (blockSizeD1 = 15, 7, 3), 128
-    energies on linspace(-1, 1) each, eta = 1e-4; band(A(E)^{-1}) per energy.
-    A(E)'s diagonal blocks are assembled per wave on the device inside the
-    timed region (zI - H'); the coupling blocks are shared by all energies
-    (batch stride 0).  Outputs of a wave overwrite the previous wave's (the
-    full band of 128 energies of the largest device is 192 GiB).  Reported
-    per device: DD-RGF and plain RGF energies/s, and all three devices at once
-    on three streams."""
+    (32, 1024), 128 energies on linspace(-1, 1) each, eta = 1e-4; band(A(E)^{-1})
+    per energy, in waves of 8 energies per device.  A(E)'s diagonal blocks are
+    assembled per wave on the device inside the timed region (zI - H'); the
+    coupling blocks are shared by all energies (batch stride 0).  Outputs of a
+    wave overwrite the previous wave's (the full band of 128 energies of the
+    largest device is 192 GiB).
+
+    Per device, timed eagerly: DD-RGF with 8 partitions (the configuration's
+    named split, blockSizeD1 = 15 / 7 / 3), plain RGF, and the tuner's choice
+    (the tune table's entry for (N, b, wave), else timed now).  Then all three
+    devices with their tuned methods at once on three streams."""
     import numpy as np
-    from libnegf_amd import synthetic, ddrgf as dd
+    from libnegf_amd import synthetic, ddrgf as dd, tune
     from libnegf_amd.flops import flops_rgf as F_rgf
     from libnegf_amd._lib import call
     cfg = synthetic.CONFIGS["ddrgf_uneven"]
     nE, eta, wave = a.dd_energies or cfg["n_energies"], cfg["eta"], a.wave or 8
+    if nE % wave:
+        raise SystemExit("--dd-energies must be a multiple of the wave")
     Es = np.linspace(-1.0, 1.0, nE)
     p = lambda t: C.c_void_p(t.data_ptr())
 
@@ -363,65 +386,90 @@ def ddrgf_uneven(a, torch, api):
             self.Gu = torch.empty((wave, (N - 1) * bb), dtype=torch.complex128, device="cuda")
             self.Gl = torch.empty_like(self.Gu)
             self.info = torch.zeros(wave, dtype=torch.int32, device="cuda")
-            self.levels = dd.levels_for_tasks(N, cfg["parts"])
+            self.bad = torch.zeros(wave, dtype=torch.int32, device="cuda")
+            self.sizes = (C.c_int32 * N)(*([b] * N))
             self.z = torch.tensor(Es + 1j * eta, dtype=torch.complex128, device="cuda")
+            self.zw = torch.empty(wave, dtype=torch.complex128, device="cuda")
+            self.levels8 = dd.levels_for_tasks(N, cfg["parts"])
+            tuned = tune.lookup("bndiag", (N, b, wave))
+            self.tune_source = "table" if tuned is not None else "timed now"
+            self.tune_times = None
+            if tuned is None:
+                tuned, self.tune_times = tune.tune_bndiag([b] * N, wave, persist=False)
+            self.tuned = tuned
 
-        def assemble(self, e0, n):
-            self.Ad[:n].copy_(self.minus_Hp.expand(n, -1))
-            self.Ad[:n, self.didx] += self.z[e0:e0 + n, None]
-
-        def run_wave(self, e0, method, stream):
-            n = min(wave, nE - e0)
-            self.assemble(e0, n)
-            bb = self.b * self.b
-            sz = (C.c_int32 * self.N)(*([self.b] * self.N))
-            if method == "ddrgf":
-                lv = (C.c_int32 * len(self.levels))(*self.levels)
-                call("negf_ddrgf", self.N, self.b, p(self.Ad), self.N * bb, p(self.up), 0,
-                     p(self.low), 0, p(self.Gd), p(self.Gu), p(self.Gl), n, lv, len(self.levels),
-                     p(self.info), C.c_void_p(stream.cuda_stream))
+        def body(self, levels, stream):
+            """Assembly from self.zw and the solve of one wave (no host
+            synchronisation)."""
+            self.Ad.copy_(self.minus_Hp.expand(wave, -1))
+            self.Ad[:, self.didx] += self.zw[:, None]
+            s = C.c_void_p(stream.cuda_stream)
+            if levels[0] == 0:
+                call("negf_rgf", self.N, self.sizes, p(self.Ad), self.N * self.b * self.b,
+                     p(self.up), 0, p(self.low), 0, p(self.Gd), p(self.Gu), p(self.Gl), wave,
+                     p(self.info), s)
             else:
-                call("negf_rgf", self.N, sz, p(self.Ad), self.N * bb, p(self.up), 0, p(self.low), 0,
-                     p(self.Gd), p(self.Gu), p(self.Gl), n, p(self.info),
-                     C.c_void_p(stream.cuda_stream))
+                lv = (C.c_int32 * len(levels))(*levels)
+                call("negf_ddrgf", self.N, self.b, p(self.Ad), self.N * self.b * self.b,
+                     p(self.up), 0, p(self.low), 0, p(self.Gd), p(self.Gu), p(self.Gl), wave, lv,
+                     len(levels), p(self.info), s)
+            torch.maximum(self.bad, self.info.abs(), out=self.bad)
+
+        def run_wave(self, e0, levels, stream):
+            self.zw.copy_(self.z[e0:e0 + wave])
+            self.body(levels, stream)
+
+        def check(self, what):
+            if bool((self.bad != 0).any()):
+                raise RuntimeError(f"singular block in {what} N={self.N} b={self.b}")
 
     devs = [Dev(N, b) for N, b in cfg["devices"]]
     streams = [torch.cuda.Stream() for _ in devs]
     res = []
     for d, st in zip(devs, streams):
-        row = {"N": d.N, "b": d.b, "blockSizeD1": d.levels, "partitions": cfg["parts"],
-               "energies": nE, "wave": wave}
-        for method in ("ddrgf", "rgf"):
+        row = {"N": d.N, "b": d.b, "partitions_named": cfg["parts"], "blockSizeD1_named": d.levels8,
+               "tuned_choice": "rgf" if d.tuned[0] == 0 else d.tuned,
+               "tune_source": d.tune_source, "energies": nE, "wave": wave}
+        if d.tune_times:
+            row["tune_times_ms"] = {k: round(v * 1e3, 2) for k, v in d.tune_times.items()}
+        for name, lv in (("ddrgf8", d.levels8), ("rgf", [0]), ("tuned", d.tuned)):
             with torch.cuda.stream(st):
-                d.run_wave(0, method, st)                      # warmup
+                d.run_wave(0, lv, st)                      # warmup
                 st.synchronize()
                 t0 = time.perf_counter()
                 for e0 in range(0, nE, wave):
-                    d.run_wave(e0, method, st)
+                    d.run_wave(e0, lv, st)
                 st.synchronize()
                 dt = time.perf_counter() - t0
-            if bool((d.info != 0).any()):
-                raise RuntimeError(f"singular Schur complement in {method} N={d.N} b={d.b}")
-            row[f"{method}_energies_per_s"] = round(nE / dt, 3)
-            row[f"{method}_rgf_equiv_tflops"] = round(nE * F_rgf(d.N, d.b) / dt / 1e12, 2)
-            print(f"# {method} N={d.N} b={d.b}: {nE / dt:.2f} energies/s", file=sys.stderr,
+            d.check(name)
+            row[f"{name}_energies_per_s"] = round(nE / dt, 3)
+            row[f"{name}_rgf_equiv_tflops"] = round(nE * F_rgf(d.N, d.b) / dt / 1e12, 2)
+            print(f"# {name} N={d.N} b={d.b}: {nE / dt:.2f} energies/s", file=sys.stderr,
                   flush=True)
         res.append(row)
+
+    # all three devices at once, eagerly on three streams
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for e0 in range(0, nE, wave):
         for d, st in zip(devs, streams):
             with torch.cuda.stream(st):
-                d.run_wave(e0, "ddrgf", st)
+                d.run_wave(e0, d.tuned, st)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    out = {"workload": "ddrgf_uneven: devices (128,256),(64,512),(32,1024), 8 partitions each, "
-                       f"{nE} energies on linspace(-1,1) each, eta={eta}",
+    dt_eager = time.perf_counter() - t0
+    for d in devs:
+        d.check("concurrent eager")
+    print(f"# three devices eager: {dt_eager:.3f} s", file=sys.stderr, flush=True)
+
+    seq_named = sum(nE / r["ddrgf8_energies_per_s"] for r in res)
+    seq_tuned = sum(nE / r["tuned_energies_per_s"] for r in res)
+    out = {"workload": "ddrgf_uneven: devices (128,256),(64,512),(32,1024), "
+                       f"{nE} energies on linspace(-1,1) each, eta={eta}, waves of {wave}",
            "form": "plain FP64", "devices": res,
-           "concurrent_3_devices": {"seconds": round(dt, 3),
-                                    "device_energies_per_s": round(3 * nE / dt, 3),
-                                    "sequential_ddrgf_seconds": round(
-                                        sum(nE / r["ddrgf_energies_per_s"] for r in res), 3)}}
+           "three_devices": {"sequential_ddrgf8_seconds": round(seq_named, 3),
+                             "sequential_tuned_seconds": round(seq_tuned, 3),
+                             "concurrent_eager_seconds": round(dt_eager, 3),
+                             "concurrent_device_energies_per_s": round(3 * nE / dt_eager, 3)}}
     print(json.dumps(out))
 
 

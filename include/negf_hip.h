@@ -344,7 +344,8 @@ typedef struct {
  * (src/c_interface/libnegf_c_interface.h:22-38, libnegf_c_interface.jl:
  * 85-142): the CSC matrix of the block band of A^{-1} for a CSC block-
  * tridiagonal A (ndiag keys "in"/"out" must be 3), computed on the GPU by
- * DD-RGF (uniform blocks, ~8 partitions) or RGF.  On failure it returns
+ * DD-RGF or RGF, whichever the tune table (negf_tune_*) holds for the shape,
+ * else DD-RGF with ~8 partitions.  On failure it returns
  * nnz = 0 and null arrays (the reference has no error channel). */
 CCscMatrix run_bndiag_wrapper(int32_t m, int32_t n, int32_t nnz_in, int32_t* colptr_in,
                               int32_t* rowval_in, double* nzval_in, int32_t n_blocks,
@@ -380,6 +381,50 @@ int negf_ddrgf(int32_t nblocks, int32_t b, const void* diag, int64_t diag_bstrid
 /* Device workspace bytes negf_ddrgf needs (0 when the sizes do not split). */
 int64_t negf_ddrgf_workspace(int32_t nblocks, int32_t b, int32_t batch,
                              const int32_t* block_sizes_d1, int32_t nlevels);
+
+/* negf_ddrgf for a chain whose block sizes differ (the reference's own DD-RGF
+ * test workload, bm_create_synthetic_random: src/blockmatrix.jl:878-919,
+ * test/test_matinvertndiag_ddrgf.jl:13-14).  Inputs and outputs in the packed
+ * layout of negf_rgf (block_sizes[i] per layer).  Uniform sizes run
+ * negf_ddrgf as is; otherwise (nblocks <= 1024) each block is embedded in a
+ * bmax x bmax block (identity on the diagonal blocks' padding), which leaves
+ * the band of A^{-1} unchanged in the leading parts, and the padded chain is
+ * solved by the same batched partitions.  Replaces bndiag_of_inv_ddrgf!
+ * (src/matinvertndiag_ddrgf_core.jl:719-758) on uneven BlockMatrix inputs. */
+int negf_ddrgf_blocksizes(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                          int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                          const void* lower, int64_t lower_bstride, void* g_diag, void* g_upper,
+                          void* g_lower, int32_t batch, const int32_t* block_sizes_d1,
+                          int32_t nlevels, int32_t* info, void* stream);
+
+/* Device workspace bytes negf_ddrgf_blocksizes needs (0: invalid sizes or levels). */
+int64_t negf_ddrgf_blocksizes_workspace(int32_t nblocks, const int32_t* block_sizes, int32_t batch,
+                                        const int32_t* block_sizes_d1, int32_t nlevels);
+
+/* -------------------------------------------------------------- tuning */
+
+/* The tune table: measured choices per problem shape, consulted by the
+ * launchers whenever no variant is forced (negf_gemm_variant /
+ * negf_inv_variant = 0); shapes it does not hold keep the built-in
+ * thresholds.  The role of the reference's timing-based tuner opt_params
+ * (src/matinvertndiag_ddrgf_utils.jl:357-453), which picks DD-RGF levels by
+ * timing them, extended to the GEMM tile and the inverse schedule.
+ *   kind 0 (gemm):   key {m, n, k, problems per launch} -> {tile variant}
+ *   kind 1 (inv):    key {n, batch}                     -> {inverse variant}
+ *   kind 2 (bndiag): key {N, bmax, batch}               -> {blockSizeD1 per level}, {0} = RGF
+ * A lookup matches every key but the last exactly and takes the entry with
+ * the largest last key <= the request, else the smallest above it.
+ * negf_tune_set replaces an entry with the same key; negf_tune_lookup
+ * returns the number of values written (0: nothing for the key); the file
+ * of negf_tune_load holds one entry per line, "gemm|inv|bndiag keys...
+ * values...", '#' starting a comment, and the call returns the entries read. */
+int negf_tune_set(int32_t kind, const int32_t* key, int32_t nkey, const int32_t* choice,
+                  int32_t nchoice);
+int negf_tune_lookup(int32_t kind, const int32_t* key, int32_t nkey, int32_t* choice,
+                     int32_t maxchoice);
+int negf_tune_clear(void);
+int negf_tune_count(void);
+int negf_tune_load(const char* path);
 
 /* ------------------------------------------------------- selected inverse */
 

@@ -17,6 +17,14 @@ import LinearAlgebra
 
 const libnegf = joinpath(@__DIR__, "libnegf_amd", "libnegf_hip.so")
 
+# the tuner's measured choices (GEMM tiles, inverse schedules, RGF vs DD-RGF levels)
+function __init_tune_table__()
+    path = get(ENV, "LIBNEGF_AMD_TUNE_FILE", joinpath(@__DIR__, "libnegf_amd", "tune_table.txt"))
+    if !isempty(path) && isfile(path)
+        ccall((:negf_tune_load, libnegf), Cint, (Cstring,), path)
+    end
+end
+
 # ----------------------------------------------------------------------------
 # types (src/backend_cpu.jl:7-20).  LibNEGF.jl:48 must become
 #   const ArrayWithType = ROCArray{FieldType,2}
@@ -188,6 +196,21 @@ end
 # sweep-level replacements (one call per sweep instead of one per block op).
 # They take BlockMatrix objects whose blocks are packed contiguously; see
 # INTEGRATION.md for the packing helper bm_pack_amd.
+# bndiag_of_inv_ddrgf! on any blockSizes (levels: the tuner's blockSizeD1 per level)
+function bndiag_of_inv_ddrgf_amd!(gd, gu, gl, ad, au, al, blockSizes::Vector{Int32}, batch::Int,
+    levels::Vector{Int32})
+    info = AMDGPU.zeros(Int32, batch)
+    nd = sum(Int64.(blockSizes) .^ 2)
+    no = sum(Int64(blockSizes[i]) * blockSizes[i+1] for i in 1:length(blockSizes)-1; init=0)
+    _check(ccall((:negf_ddrgf_blocksizes, libnegf), Cint,
+                 (Int32, Ptr{Int32}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+                  Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Int32}, Int32, Ptr{Int32}, Ptr{Cvoid}),
+                 length(blockSizes), blockSizes, pointer(ad), nd, pointer(au), max(no, 1),
+                 pointer(al), max(no, 1), pointer(gd), pointer(gu), pointer(gl), batch, levels,
+                 length(levels), pointer(info), _stream()), "bndiag_of_inv_ddrgf_amd!")
+    _check_info(info, "bndiag_of_inv_ddrgf_amd!")
+end
+
 function bndiag_of_inv_rgf_amd!(gd, gu, gl, ad, au, al, blockSizes::Vector{Int32}, batch::Int)
     info = AMDGPU.zeros(Int32, batch)
     nd = sum(Int64.(blockSizes) .^ 2)

@@ -249,7 +249,19 @@ int negf_zinv_strided_batched(int32_t n, const void* A, int64_t lda, int64_t str
   if (n < 0 || batch < 0) return NEGF_BAD_ARG;
   if (n == 0 || batch == 0) return NEGF_OK;
   if (!A || !Ainv || !info) return NEGF_BAD_ARG;
-  if (n > 2048) return NEGF_TOO_LARGE;
+  if (n > kLuMaxN) return NEGF_TOO_LARGE;
+  if (n > 2048) {
+    // beyond the Gauss-Jordan kernels: LU, triangular inverses, product
+    if (batch > 1 && (strideA < (int64_t)lda * n || strideInv < (int64_t)ldinv * n))
+      return NEGF_BAD_ARG;
+    void* w = nullptr;
+    int rc = workspace(stream, zinv_large_work_bytes(n), &w);
+    if (rc) return rc;
+    for (int e = 0; e < batch && rc == NEGF_OK; ++e)
+      rc = zinv_large_launch(n, (const zc*)A + e * strideA, lda, (zc*)Ainv + e * strideInv, ldinv,
+                             w, info + e, st(stream));
+    return rc;
+  }
   void* w = nullptr;
   int rc = workspace(stream, zinv_work_bytes(n, batch), &w);
   if (rc) return rc;
@@ -787,6 +799,104 @@ int negf_ddrgf(int32_t nblocks, int32_t b, const void* diag, int64_t diag_bstrid
   BtdIn A{(const zc*)diag, diag_bstride, (const zc*)upper, upper_bstride, (const zc*)low, low_bs};
   BtdOut G{(zc*)g_diag, (zc*)g_upper, (zc*)g_lower};
   return ddrgf(nblocks, b, A, G, batch, block_sizes_d1, nlevels, info, w, s);
+}
+
+namespace {
+#define CAPI_RC(x)         \
+  do {                     \
+    const int rc_ = (x);   \
+    if (rc_) return rc_;   \
+  } while (0)
+
+bool pad_table(int32_t nblocks, const int32_t* sizes, PadTable* t, bool* uniform) {
+  if (!valid_sizes(nblocks, sizes)) return false;
+  int bmax = 0;
+  *uniform = true;
+  for (int i = 0; i < nblocks; ++i) {
+    bmax = sizes[i] > bmax ? sizes[i] : bmax;
+    *uniform = *uniform && sizes[i] == sizes[0];
+  }
+  if (*uniform) {
+    t->N = nblocks;
+    t->bmax = bmax;
+    return true;
+  }
+  if (nblocks > kPadMaxBlocks) return false;
+  t->N = nblocks;
+  t->bmax = bmax;
+  for (int i = 0; i < nblocks; ++i) t->b[i] = (unsigned short)sizes[i];
+  return true;
+}
+}  // namespace
+
+int64_t negf_ddrgf_blocksizes_workspace(int32_t nblocks, const int32_t* block_sizes, int32_t batch,
+                                        const int32_t* block_sizes_d1, int32_t nlevels) {
+  PadTable t;
+  bool uniform = true;
+  if (batch <= 0 || !pad_table(nblocks, block_sizes, &t, &uniform)) return 0;
+  if (uniform) return negf_ddrgf_workspace(nblocks, t.bmax, batch, block_sizes_d1, nlevels);
+  if (!block_sizes_d1 || nlevels <= 0) return 0;
+  const size_t w = ddrgf_work_bytes(nblocks, t.bmax, batch, block_sizes_d1, nlevels);
+  if (w == 0) return 0;
+  const size_t BB = (size_t)t.bmax * t.bmax;
+  const size_t padded = sizeof(zc) * (size_t)(3 * nblocks - 2) * BB * batch;   // A or G
+  return (int64_t)(w + 2 * padded + 3 * 256 + 4096);
+}
+
+int negf_ddrgf_blocksizes(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                          int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                          const void* lower, int64_t lower_bstride, void* g_diag, void* g_upper,
+                          void* g_lower, int32_t batch, const int32_t* block_sizes_d1,
+                          int32_t nlevels, int32_t* info, void* stream) {
+  PadTable t;
+  bool uniform = true;
+  if (batch <= 0 || !block_sizes_d1 || nlevels <= 0 || !info ||
+      !pad_table(nblocks, block_sizes, &t, &uniform))
+    return NEGF_BAD_ARG;
+  if (uniform)
+    return negf_ddrgf(nblocks, t.bmax, diag, diag_bstride, upper, upper_bstride, lower,
+                      lower_bstride, g_diag, g_upper, g_lower, batch, block_sizes_d1, nlevels,
+                      info, stream);
+  if (!diag || !g_diag || (nblocks > 1 && (!upper || !g_upper || !g_lower)) || batch > 65535)
+    return NEGF_BAD_ARG;
+  const int64_t need = negf_ddrgf_blocksizes_workspace(nblocks, block_sizes, batch,
+                                                       block_sizes_d1, nlevels);
+  if (need == 0) return NEGF_BAD_ARG;   // a level's block size does not split its chain
+  void* w = nullptr;
+  int rc = workspace(stream, (size_t)need, &w);
+  if (rc) return rc;
+  hipStream_t s = st(stream);
+  const int N = nblocks;
+  const long long BB = (long long)t.bmax * t.bmax;
+  const long long pd = (long long)N * BB, po = (long long)(N - 1) * BB;   // padded batch strides
+  long long nd = 0, no = 0;                                               // packed output strides
+  for (int i = 0; i < N; ++i) nd += (long long)block_sizes[i] * block_sizes[i];
+  for (int i = 0; i + 1 < N; ++i) no += (long long)block_sizes[i] * block_sizes[i + 1];
+  auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+  char* p = (char*)w;
+  zc* Ad = (zc*)p;  p += align(sizeof(zc) * (size_t)pd * batch);
+  zc* Au = (zc*)p;  p += align(sizeof(zc) * (size_t)po * batch);
+  zc* Al = (zc*)p;  p += align(sizeof(zc) * (size_t)po * batch);
+  zc* Gd = (zc*)p;  p += align(sizeof(zc) * (size_t)pd * batch);
+  zc* Gu = (zc*)p;  p += align(sizeof(zc) * (size_t)po * batch);
+  zc* Gl = (zc*)p;  p += align(sizeof(zc) * (size_t)po * batch);
+  CAPI_RC(btd_pad_launch(t, 0, (const zc*)diag, diag_bstride, Ad, pd, batch, s));
+  if (N > 1) {
+    CAPI_RC(btd_pad_launch(t, 1, (const zc*)upper, upper_bstride, Au, po, batch, s));
+    if (lower)
+      CAPI_RC(btd_pad_launch(t, 2, (const zc*)lower, lower_bstride, Al, po, batch, s));
+    else
+      CAPI_RC(btd_pad_launch(t, 3, (const zc*)upper, upper_bstride, Al, po, batch, s));
+  }
+  BtdIn A{Ad, pd, Au, po, Al, po};
+  BtdOut G{Gd, Gu, Gl};
+  CAPI_RC(ddrgf(N, t.bmax, A, G, batch, block_sizes_d1, nlevels, info, p, s));
+  CAPI_RC(btd_unpad_launch(t, 0, Gd, pd, (zc*)g_diag, nd, batch, s));
+  if (N > 1) {
+    CAPI_RC(btd_unpad_launch(t, 1, Gu, po, (zc*)g_upper, no, batch, s));
+    CAPI_RC(btd_unpad_launch(t, 2, Gl, po, (zc*)g_lower, no, batch, s));
+  }
+  return NEGF_OK;
 }
 
 // ------------------------------------------------------------ selected inverse

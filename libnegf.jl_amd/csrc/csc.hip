@@ -220,13 +220,27 @@ int negf_run_bndiag_csc(int32_t m, int32_t n, int32_t nnz_in, const int32_t* col
       rc = zadjoint_launch(sizes[i], sizes[i + 1], din + nd + L.oo[i], sizes[i], 0,
                            din + nd + nos + L.oo[i], sizes[i + 1], 0, 1, s);
   }
-  bool uniform = true;
-  for (int i = 1; i < N; ++i) uniform = uniform && sizes[i] == sizes[0];
-  const int bs = (method == 1 || !uniform) ? 0 : pick_ddrgf_bs(N);
+  // method 0: the tuned choice for (N, bmax, one instance) when the tune
+  // table holds one, else ~8 partitions; 1: RGF; 2: DD-RGF (tuned levels,
+  // else ~8 partitions).  Uneven sizes run DD-RGF on the padded chain.
+  int bmax = 0;
+  for (int i = 0; i < N; ++i) bmax = sizes[i] > bmax ? sizes[i] : bmax;
+  int32_t lv[4] = {0, 0, 0, 0};
+  int nlv = 0;
+  if (method != 1) {
+    const int32_t key[3] = {N, bmax, 1};
+    nlv = negf_tune_lookup(2, key, 3, lv, 4);
+    if (nlv <= 0 || (lv[0] == 0 && method == 2)) {
+      lv[0] = pick_ddrgf_bs(N);
+      nlv = 1;
+    }
+    if (lv[0] == 0) nlv = 0;
+  }
   if (rc == NEGF_OK) {
-    if (bs > 0)
-      rc = negf_ddrgf(N, sizes[0], din, (int64_t)nd, din + nd, (int64_t)nos, din + nd + nos,
-                      (int64_t)nos, dout, dout + nd, dout + nd + nos, 1, &bs, 1, dinfo, stream);
+    if (nlv > 0)
+      rc = negf_ddrgf_blocksizes(N, sizes.data(), din, (int64_t)nd, din + nd, (int64_t)nos,
+                                 din + nd + nos, (int64_t)nos, dout, dout + nd, dout + nd + nos, 1,
+                                 lv, nlv, dinfo, stream);
     else
       rc = negf_rgf(N, sizes.data(), din, (int64_t)nd, din + nd, (int64_t)nos, din + nd + nos,
                     (int64_t)nos, dout, dout + nd, dout + nd + nos, 1, dinfo, stream);

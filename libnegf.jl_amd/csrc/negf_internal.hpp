@@ -197,6 +197,15 @@ int ztrtri_launch(char uplo, char diag, int n, const zc* A, long long lda, long 
                   zc* X, long long ldx, long long strideX, int batch, zc* work, hipStream_t s);
 int zgetri_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
                   long long strideP, int batch, zc* work, hipStream_t s);
+// Largest n of zgetrf_launch (above 2048: two-level blocking, 512-thread panels).
+constexpr int kLuMaxN = 16384;
+// Explicit inverse of one n x n block for n beyond the Gauss-Jordan kernels
+// (n <= kLuMaxN): A = P L U by zgetrf, inv(U) and inv(L) by the recursive
+// triangular inverses, their product on the matrix cores, then the column
+// interchanges of P.  work: zinv_large_work_bytes(n).  info as zgetrf's.
+size_t zinv_large_work_bytes(int n);
+int zinv_large_launch(int n, const zc* A, long long lda, zc* Ainv, long long ldinv, void* work,
+                      int* info, hipStream_t s);
 
 // ----------------------------------------------------------------------------
 // Elementwise helpers (all batched, column-major).
@@ -224,5 +233,30 @@ int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, c
 int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                   const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                   long long strideB, int batch, hipStream_t s);
+
+// Tuned variants for a shape (0: none tuned; the launcher's thresholds apply).
+int tune_gemm_variant(int m, int n, int k, long long problems);
+int tune_inv_variant(int n, int batch);
+
+// Uniform view of a packed block-tridiagonal matrix whose block sizes differ:
+// block (i, j) of b_i x b_j becomes the leading part of a bmax x bmax block,
+// the rest 0 except 1 on the diagonal of a diagonal block.  Up to a
+// permutation the padded matrix is A (+) I, so the band of its inverse holds
+// the band of A^{-1} in the same leading parts, and the padded rows never
+// mix with the original ones.  The sizes travel as kernel arguments.
+constexpr int kPadMaxBlocks = 1024;
+struct PadTable {
+  int N, bmax;
+  unsigned short b[kPadMaxBlocks];
+};
+// kind: 0 diagonal blocks, 1 upper (i, i+1), 2 lower (i+1, i), 3 lower from
+// the upper blocks of Hermitian storage (A_{i+1,i} = A_{i,i+1}^H).  Packed
+// side: blocks back to back, tight ld, batch stride sbs (0 broadcasts);
+// padded side: block i at i * bmax^2, batch stride dbs.
+int btd_pad_launch(const PadTable& t, int kind, const zc* packed, long long sbs, zc* padded,
+                   long long dbs, int batch, hipStream_t s);
+// kind 0, 1, 2: the leading parts of the padded blocks back into packed form.
+int btd_unpad_launch(const PadTable& t, int kind, const zc* padded, long long sbs, zc* packed,
+                     long long dbs, int batch, hipStream_t s);
 
 }  // namespace negf

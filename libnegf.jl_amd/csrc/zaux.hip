@@ -235,6 +235,91 @@ int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, c
   return ok();
 }
 
+namespace {
+
+// Dimensions and packed offset of block i of `kind` (see btd_pad_launch).
+// i is uniform over the workgroup, so the prefix sum runs on scalar registers.
+__device__ inline long long pad_block(const PadTable& t, int kind, int i, int* m, int* n) {
+  long long o = 0;
+  if (kind == 0) {
+    for (int k = 0; k < i; ++k) o += (long long)t.b[k] * t.b[k];
+    *m = *n = t.b[i];
+  } else {
+    for (int k = 0; k < i; ++k) o += (long long)t.b[k] * t.b[k + 1];
+    *m = kind == 1 ? t.b[i] : t.b[i + 1];
+    *n = kind == 1 ? t.b[i + 1] : t.b[i];
+  }
+  return o;
+}
+
+// grid: (chunks of a padded block, block i, instance)
+__global__ __launch_bounds__(256) void btd_pad_kernel(PadTable t, int kind,
+                                                      const zc* __restrict__ src, long long sbs,
+                                                      zc* __restrict__ dst, long long dbs) {
+  int m, n;
+  const int i = blockIdx.y;
+  const long long off = pad_block(t, kind, i, &m, &n);
+  const int B = t.bmax, BB = t.bmax * t.bmax;
+  const zc* sp = src + blockIdx.z * sbs + off;
+  zc* dp = dst + blockIdx.z * dbs + (long long)i * BB;
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < BB; p += gridDim.x * 256) {
+    const int r = p % B, c = p / B;
+    zc v = zmake(0.0, 0.0);
+    if (r < m && c < n)   // kind 3: the upper block i is n x m, read transposed
+      v = kind == 3 ? zconj(sp[c + (long long)r * n]) : sp[r + (long long)c * m];
+    else if (kind == 0 && r == c)
+      v = zmake(1.0, 0.0);
+    dp[p] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void btd_unpad_kernel(PadTable t, int kind,
+                                                        const zc* __restrict__ src, long long sbs,
+                                                        zc* __restrict__ dst, long long dbs) {
+  int m, n;
+  const int i = blockIdx.y;
+  const long long off = pad_block(t, kind, i, &m, &n);
+  const int B = t.bmax;
+  const zc* sp = src + blockIdx.z * sbs + (long long)i * B * B;
+  zc* dp = dst + blockIdx.z * dbs + off;
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < m * n; p += gridDim.x * 256) {
+    const int r = p % m, c = p / m;
+    dp[p] = sp[r + (long long)c * B];
+  }
+}
+
+int pad_grid(const PadTable& t, int kind, int batch, dim3* g) {
+  if (t.N < 1 || t.N > kPadMaxBlocks || t.bmax < 1 || batch < 0 || kind < 0 || kind > 3)
+    return Status::BAD_ARG;
+  if (batch > 65535) return Status::TOO_LARGE;
+  const int nblk = kind == 0 ? t.N : t.N - 1;
+  long long bx = ((long long)t.bmax * t.bmax + 255) / 256;
+  if (bx > 64) bx = 64;
+  *g = dim3((unsigned)bx, (unsigned)nblk, (unsigned)batch);
+  return Status::OK;
+}
+
+}  // namespace
+
+int btd_pad_launch(const PadTable& t, int kind, const zc* packed, long long sbs, zc* padded,
+                   long long dbs, int batch, hipStream_t s) {
+  dim3 g;
+  int rc = pad_grid(t, kind, batch, &g);
+  if (rc || g.y == 0 || batch == 0) return rc;
+  hipLaunchKernelGGL(btd_pad_kernel, g, dim3(256), 0, s, t, kind, packed, sbs, padded, dbs);
+  return ok();
+}
+
+int btd_unpad_launch(const PadTable& t, int kind, const zc* padded, long long sbs, zc* packed,
+                     long long dbs, int batch, hipStream_t s) {
+  dim3 g;
+  if (kind == 3) return Status::BAD_ARG;
+  int rc = pad_grid(t, kind, batch, &g);
+  if (rc || g.y == 0 || batch == 0) return rc;
+  hipLaunchKernelGGL(btd_unpad_kernel, g, dim3(256), 0, s, t, kind, padded, sbs, packed, dbs);
+  return ok();
+}
+
 int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP, const zc* Q,
                     long long ldq, long long strideQ, double* out, int batch, hipStream_t s) {
   if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;

@@ -496,7 +496,10 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
     if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64, true>(G, s);
     if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32, true>(G, s);
   }
-  switch (g_variant) {
+  // a forced variant, else the tuned one for this shape, else the thresholds
+  const int var = g_variant ? g_variant
+                            : tune_gemm_variant(g0.m, g0.n, g0.k, (long long)g0.batch * G.count);
+  switch (var) {
     case 0:
       // fewer than two 64 x 64 tiles per CU (one instance of a 512^3 product
       // has 64), or m, n multiples of 32 only: 32 x 32 tiles
@@ -509,7 +512,7 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
     case 10: if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32>(G, s); break;
     default: break;
   }
-  switch (g_variant) {
+  switch (var) {
     case 0: case 6: case 7: case 8: case 9: case 10: launch_cfg<OPA, OPB, 2, 2, false, 0>(G, s); break;
     case 1: launch_cfg<OPA, OPB, 2, 2, true, 0>(G, s); break;
     case 2: launch_cfg<OPA, OPB, 2, 4, false, 0>(G, s); break;

@@ -915,7 +915,8 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   // instances (zinv_work_carve); the schedule's W buffers must fit in it
   if (n > w.nmax || batch > w.batchmax) return Status::BAD_ARG;
   int S = 0, PS = 0, nbuf = 0;
-  int split = zinv_schedule(n, batch, g_inv_variant, &S, &PS, &nbuf);
+  const int var = g_inv_variant ? g_inv_variant : tune_inv_variant(n, batch);
+  int split = zinv_schedule(n, batch, var, &S, &PS, &nbuf);
   if (split < 0) return split;
   if (split && w.inplace_only)   // the automatic in-place panel choice instead
     split = zinv_schedule(n, batch, batch >= 32 ? 2 : 1, &S, &PS, &nbuf);

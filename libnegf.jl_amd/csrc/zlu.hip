@@ -21,8 +21,8 @@ namespace {
 constexpr int PT = 256;
 constexpr int TB = 16;   // trsm diagonal block
 
-template <int RPT, int NB>
-__global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restrict__ A,
+template <int RPT, int NB, int NT = PT>
+__global__ __launch_bounds__(NT) void lu_panel_kernel(int n, int k0, zc* __restrict__ A,
                                                       long long lda, long long strideA,
                                                       int* __restrict__ ipiv, long long strideP,
                                                       int* __restrict__ info, bool pivot) {
@@ -33,15 +33,15 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
   int* ip = ipiv + inst * strideP;
   const int nbe = (n - k0) < NB ? (n - k0) : NB;
 
-  __shared__ double red_v[PT / 64];
-  __shared__ int red_i[PT / 64];
+  __shared__ double red_v[NT / 64];
+  __shared__ int red_i[NT / 64];
   __shared__ zc rowcur[NB];
   __shared__ zc rowpiv[NB];
 
   zc P[RPT][NB];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int r = k0 + tid + PT * q;
+    const int r = k0 + tid + NT * q;
 #pragma unroll
     for (int c = 0; c < NB; ++c)
       P[q][c] = (r < n && c < nbe) ? Ai[r + (long long)(k0 + c) * lda] : zmake(0.0, 0.0);
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
       int bq = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        const int r = k0 + tid + PT * q;
+        const int r = k0 + tid + NT * q;
         // without pivoting the "search" sees only the diagonal row
         if (r < n && (pivot ? r >= cur : r == cur)) {
           const double v = zabs1(P[q][jj]);
@@ -70,15 +70,15 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
 #pragma unroll
       for (int q = RPT - 1; q >= 0; --q) {
         const unsigned long long m = __ballot(bv >= 0.0 && bv == wmax && bq == q);
-        if (m != 0ull) win = k0 + 64 * wave + (__ffsll(m) - 1) + PT * q;
+        if (m != 0ull) win = k0 + 64 * wave + (__ffsll(m) - 1) + NT * q;
       }
       if (lane == 0) { red_v[wave] = wmax; red_i[wave] = win; }
       __syncthreads();
       // all waves' pairs in one round of LDS reads, then a branch-free pick
-      double rv[PT / 64];
-      int rix[PT / 64];
+      double rv[NT / 64];
+      int rix[NT / 64];
 #pragma unroll
-      for (int w = 0; w < PT / 64; ++w) {
+      for (int w = 0; w < NT / 64; ++w) {
         rv[w] = red_v[w];
         rix[w] = red_i[w];
       }
@@ -87,10 +87,10 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
       // compile to scalar branches, the values being wave-uniform
       bv = rv[0];
 #pragma unroll
-      for (int w = 1; w < PT / 64; ++w) bv = fmax(bv, rv[w]);
+      for (int w = 1; w < NT / 64; ++w) bv = fmax(bv, rv[w]);
       int bi = INT_MAX;
 #pragma unroll
-      for (int w = 0; w < PT / 64; ++w) bi = min(bi, rv[w] == bv ? rix[w] : INT_MAX);
+      for (int w = 0; w < NT / 64; ++w) bi = min(bi, rv[w] == bv ? rix[w] : INT_MAX);
       const int p = bi;
       if (tid == 0) {
         ip[cur] = p + 1;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
       }
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        const int r = k0 + tid + PT * q;
+        const int r = k0 + tid + NT * q;
         if (r == cur) {
 #pragma unroll
           for (int c = 0; c < NB; ++c) rowcur[c] = P[q][c];
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
       const zc inv = nz ? zrcp(pv) : zmake(0.0, 0.0);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        const int r = k0 + tid + PT * q;
+        const int r = k0 + tid + NT * q;
         if (r < n && r >= cur) {
           if (r == cur) {
 #pragma unroll
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(int n, int k0, zc* __restr
   }
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int r = k0 + tid + PT * q;
+    const int r = k0 + tid + NT * q;
     if (r < n) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
@@ -339,13 +339,82 @@ int getrf_run(int n, zc* A, long long lda, long long strideA, int* ipiv, long lo
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
+// Two-level blocking for n > 2048, where the register panel of one
+// NT-thread workgroup holds RPT x NB elements per thread: outer panels of
+// OB columns are factored NB columns at a time (the panel kernel on rows
+// j0..n-1, the interchanges and the trsm / rank-NB update confined to the
+// outer panel's columns), then the outer panel's interchanges go to the
+// columns left and right of it, U12 = L11^{-1} A12 (OB rows) and the trailing
+// update A22 -= L21 U12 runs as one GEMM with k = OB on the matrix cores.
+// The rank-NB updates touch only n x OB elements, so the HBM traffic is that
+// of the OB-wide blocking, not of NB.
+template <int NT, int RPT, int NB, int OB>
+int getrf_blocked(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
+                  int* info, int batch, bool pivot, hipStream_t s) {
+  hipLaunchKernelGGL(zero_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+  for (int k0 = 0; k0 < n; k0 += OB) {
+    const int ob = (n - k0) < OB ? (n - k0) : OB;
+    for (int j0 = k0; j0 < k0 + ob; j0 += NB) {
+      const int nbe = (k0 + ob - j0) < NB ? (k0 + ob - j0) : NB;
+      hipLaunchKernelGGL((lu_panel_kernel<RPT, NB, NT>), dim3(batch), dim3(NT), 0, s, n, j0, A,
+                         lda, strideA, ipiv, strideP, info, pivot);
+      const int right = k0 + ob - (j0 + nbe);   // columns of the outer panel right of j0
+      if (pivot && j0 > k0)
+        hipLaunchKernelGGL(laswp_kernel, dim3((j0 - k0 + 255) / 256, batch), dim3(256), 0, s,
+                           j0 - k0, A + (long long)k0 * lda, lda, strideA, ipiv, strideP, j0,
+                           j0 + nbe, 1);
+      if (right > 0) {
+        zc* A12 = A + j0 + (long long)(j0 + nbe) * lda;
+        if (pivot)
+          hipLaunchKernelGGL(laswp_kernel, dim3((right + 255) / 256, batch), dim3(256), 0, s,
+                             right, A + (long long)(j0 + nbe) * lda, lda, strideA, ipiv, strideP,
+                             j0, j0 + nbe, 1);
+        int rc = ztrsm_launch('L', 'L', 'N', 'U', nbe, right, zmake(1.0, 0.0),
+                              A + j0 + (long long)j0 * lda, lda, strideA, A12, lda, strideA, batch,
+                              s);
+        if (rc) return rc;
+        if (n - j0 - nbe > 0) {
+          rc = gemm_sub(OP_N, n - j0 - nbe, right, nbe, A + (j0 + nbe) + (long long)j0 * lda, lda,
+                        strideA, A12, lda, strideA, OP_N,
+                        A + (j0 + nbe) + (long long)(j0 + nbe) * lda, lda, strideA, batch, s);
+          if (rc) return rc;
+        }
+      }
+    }
+    if (pivot && k0 > 0)
+      hipLaunchKernelGGL(laswp_kernel, dim3((k0 + 255) / 256, batch), dim3(256), 0, s, k0, A, lda,
+                         strideA, ipiv, strideP, k0, k0 + ob, 1);
+    const int rest = n - k0 - ob;
+    if (rest > 0) {
+      zc* A12 = A + k0 + (long long)(k0 + ob) * lda;
+      if (pivot)
+        hipLaunchKernelGGL(laswp_kernel, dim3((rest + 255) / 256, batch), dim3(256), 0, s, rest,
+                           A + (long long)(k0 + ob) * lda, lda, strideA, ipiv, strideP, k0, k0 + ob,
+                           1);
+      int rc = ztrsm_launch('L', 'L', 'N', 'U', ob, rest, zmake(1.0, 0.0),
+                            A + k0 + (long long)k0 * lda, lda, strideA, A12, lda, strideA, batch, s);
+      if (rc) return rc;
+      rc = gemm_sub(OP_N, rest, rest, ob, A + (k0 + ob) + (long long)k0 * lda, lda, strideA, A12,
+                    lda, strideA, OP_N, A + (k0 + ob) + (long long)(k0 + ob) * lda, lda, strideA,
+                    batch, s);
+      if (rc) return rc;
+    }
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
 }  // namespace
 
 int zgetrf_launch(int n, zc* A, long long lda, long long strideA, int* ipiv, long long strideP,
                   int* info, int batch, hipStream_t s, bool pivot) {
   if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1)) return Status::BAD_ARG;
   if (n == 0 || batch == 0) return Status::OK;
-  if (n > 2048) return Status::TOO_LARGE;
+  if (n > kLuMaxN) return Status::TOO_LARGE;
+  // 512-thread panels (two waves per SIMD, so the 32 complex values per
+  // thread stay in registers)
+  if (n > 8192) return getrf_blocked<512, 32, 1, 64>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
+  if (n > 4096) return getrf_blocked<512, 16, 2, 64>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
+  if (n > 2048) return getrf_blocked<512, 8, 4, 64>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
   if (n <= 256) return getrf_run<1, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
   if (n <= 512) return getrf_run<2, 16>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
   if (n <= 1024) return getrf_run<4, 8>(n, A, lda, strideA, ipiv, strideP, info, batch, pivot, s);
@@ -497,6 +566,78 @@ int zgetri_launch(int n, zc* A, long long lda, long long strideA, const int* ipi
   rc = zfill_launch(n, n, zmake(0.0, 0.0), zmake(1.0, 0.0), A, lda, strideA, batch, s);
   if (rc) return rc;
   return zgetrs_launch('N', n, n, work, n, ws, ipiv, strideP, A, lda, strideA, batch, s);
+}
+
+namespace {
+
+// perm[j]: the column of inv(U) inv(L) that becomes column j of inv(A), from
+// LAPACK's interchanges (getri applies them to the columns from the last to
+// the first); one thread, n steps.
+__global__ void perm_of_ipiv_kernel(int n, const int* __restrict__ ipiv, int* __restrict__ perm) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int j = 0; j < n; ++j) perm[j] = j;
+  for (int j = n - 1; j >= 0; --j) {
+    const int p = ipiv[j] - 1;
+    if (p != j) {
+      const int t = perm[j];
+      perm[j] = perm[p];
+      perm[p] = t;
+    }
+  }
+}
+
+// out[:, j] = X[:, perm[j]]; grid (row chunks of 256, columns)
+__global__ __launch_bounds__(256) void gather_cols_kernel(int n, const zc* __restrict__ X,
+                                                          const int* __restrict__ perm,
+                                                          zc* __restrict__ out, long long ldo) {
+  const int j = blockIdx.y;
+  const zc* src = X + (long long)perm[j] * n;
+  zc* dst = out + (long long)j * ldo;
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) dst[r] = src[r];
+}
+
+}  // namespace
+
+size_t zinv_large_work_bytes(int n) {
+  const size_t nn = (size_t)n * n * sizeof(zc);
+  const size_t ints = ((size_t)2 * n * sizeof(int) + 255) / 256 * 256;
+  return 3 * ((nn + 255) / 256 * 256) + ints + ztrtri_work_bytes(n, 1) + 256;
+}
+
+int zinv_large_launch(int n, const zc* A, long long lda, zc* Ainv, long long ldinv, void* work,
+                      int* info, hipStream_t s) {
+  if (n <= 0 || n > kLuMaxN || lda < n || ldinv < n || !work) return Status::BAD_ARG;
+  const size_t nn = ((size_t)n * n * sizeof(zc) + 255) / 256 * 256;
+  char* p = (char*)work;
+  zc* LU = (zc*)p;  p += nn;
+  zc* Li = (zc*)p;  p += nn;
+  zc* Ui = (zc*)p;  p += nn;
+  int* ipiv = (int*)p;
+  int* perm = ipiv + n;
+  p += ((size_t)2 * n * sizeof(int) + 255) / 256 * 256;
+  zc* tw = (zc*)p;
+  int rc = zcopy2d_launch(n, n, A, lda, 0, LU, n, 0, 1, s);
+  if (rc) return rc;
+  rc = zgetrf_launch(n, LU, n, 0, ipiv, 0, info, 1, s, true);
+  if (rc) return rc;
+  rc = ztrtri_launch('L', 'U', n, LU, n, 0, Li, n, 0, 1, tw, s);
+  if (rc) return rc;
+  rc = ztrtri_launch('U', 'N', n, LU, n, 0, Ui, n, 0, 1, tw, s);
+  if (rc) return rc;
+  // inv(U) inv(L) into the LU buffer (no longer needed)
+  GemmArgs g{};
+  g.opA = OP_N; g.opB = OP_N;
+  g.m = g.n = g.k = n;
+  g.alpha = zmake(1.0, 0.0); g.beta = zmake(0.0, 0.0);
+  g.A = Ui; g.lda = n; g.B = Li; g.ldb = n;
+  g.C = nullptr; g.ldc = n; g.D = LU; g.ldd = n;
+  g.batch = 1;
+  rc = zgemm_launch(g, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(perm_of_ipiv_kernel, dim3(1), dim3(64), 0, s, n, ipiv, perm);
+  hipLaunchKernelGGL(gather_cols_kernel, dim3((n + 255) / 256 < 16 ? (n + 255) / 256 : 16, n),
+                     dim3(256), 0, s, n, LU, perm, Ainv, ldinv);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
 }  // namespace negf

@@ -88,6 +88,14 @@ SIGNATURES = {
     "negf_ddrgf": (C.c_int, [i32, i32, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32, c_int32_p, i32,
                              vp, vp]),
     "negf_ddrgf_workspace": (C.c_int64, [i32, i32, i32, c_int32_p, i32]),
+    "negf_ddrgf_blocksizes": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32,
+                                        c_int32_p, i32, vp, vp]),
+    "negf_ddrgf_blocksizes_workspace": (C.c_int64, [i32, c_int32_p, i32, c_int32_p, i32]),
+    "negf_tune_set": (C.c_int, [i32, c_int32_p, i32, c_int32_p, i32]),
+    "negf_tune_lookup": (C.c_int, [i32, c_int32_p, i32, c_int32_p, i32]),
+    "negf_tune_clear": (C.c_int, []),
+    "negf_tune_count": (C.c_int, []),
+    "negf_tune_load": (C.c_int, [C.c_char_p]),
     "negf_block_fill_pattern": (C.c_int, [i32, i32, c_int32_p, c_int32_p, i32, c_int32_p,
                                           c_int32_p]),
     "negf_block_factorization": (C.c_int, [i32, i32, i32, c_int32_p, c_int32_p,
@@ -136,6 +144,9 @@ def load(path: pathlib.Path | None = None) -> C.CDLL:
         fn.restype = res
         fn.argtypes = args
     _lib = lib
+    # the measured choices of the tuner (libnegf_amd/tune.py), when a table exists
+    from . import tune
+    tune.load_table()
     return lib
 
 

@@ -28,6 +28,7 @@ __all__ = [
     "rgf_keldysh_energies", "GpuLU", "be_gemm", "be_lu", "be_mldivide", "be_getrf", "be_getri",
     "be_trsm", "be_ctranspose", "be_copy_in_hw", "be_fill", "be_identity", "be_zero_array",
     "be_inv", "col_major", "release_workspace", "build_M_from_HS", "EnergiesGraph",
+    "bndiag_of_inv_direct",
 ]
 
 
@@ -471,8 +472,36 @@ def be_trsm(side: str, ul: str, tA: str, dA: str, alpha, A: torch.Tensor,
     return B
 
 
+def bndiag_of_inv_direct(M, block_sizes, ndiag: dict | None = None):
+    """``bndiag_of_inv_direct`` (``src/matinvertndiag_direct.jl:17-86``): the
+    dense inverse of M by ``be_inv`` on the device, then its block
+    n-diagonal (``ndiag["out"]``) as a scipy CSC matrix holding exactly the
+    band blocks.  M: a scipy sparse or numpy matrix (copied to the device, as
+    the reference's be_copy_to_hw) or a column-major device tensor."""
+    import scipy.sparse as sp
+    _require_gpu()
+    ndiag = ndiag or {"in": 3, "out": 3}
+    if isinstance(M, torch.Tensor):
+        Md = col_major(M.to(torch.complex128))
+    else:
+        dense = M.toarray() if sp.issparse(M) else np.asarray(M)
+        Md = col_major(torch.from_numpy(np.asarray(dense, dtype=np.complex128)).to("cuda"))
+    Minv = be_inv(Md).cpu().numpy()
+    sizes = [int(x) for x in block_sizes]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    w = (int(ndiag["out"]) - 1) // 2
+    N = len(sizes)
+    out = sp.lil_matrix(Minv.shape, dtype=np.complex128)
+    for i in range(N):
+        for j in range(max(0, i - w), min(N, i + w + 1)):
+            out[off[i]:off[i + 1], off[j]:off[j + 1]] = Minv[off[i]:off[i + 1], off[j]:off[j + 1]]
+    return out.tocsc()
+
+
 def be_inv(M: torch.Tensor) -> torch.Tensor:
-    """Explicit inverse (``src/backend_cpu.jl:143-145``) by Gauss-Jordan."""
+    """Explicit inverse (``src/backend_cpu.jl:143-145``): Gauss-Jordan up to
+    n = 2048, above that LU + triangular inverses + their product (up to
+    n = 16384)."""
     n = M.shape[0]
     out = be_zero_array((n, n), M.device)
     info = torch.zeros(1, dtype=torch.int32, device=M.device)

@@ -2,11 +2,12 @@
 ``src/matinvertndiag_ddrgf_{core,setup,utils}.jl``.
 
 ``bndiag_of_inv_ddrgf_get_nr_tasks`` is the reference's task split
-(utils.jl:278-312); ``allocate_aux_data_DDRGF`` picks the levels (the role of
-the reference's timing-based tuner ``opt_params``, utils.jl:357-453, here a
-deterministic choice: the block size that gives the requested task count,
-or the one minimising a GEMM-count cost model); ``bndiag_of_inv_ddrgf`` runs
-negf_ddrgf, whose partitions are solved as one batch on the GPU.
+(utils.jl:278-312); ``allocate_aux_data_DDRGF`` picks the levels the way the
+reference's ``opt_params`` (utils.jl:357-453) does, by timing: the tune
+table's entry for the shape, else the candidates (plain RGF among them) timed
+on the device by ``tune.tune_bndiag``; ``bndiag_of_inv_ddrgf`` runs
+negf_ddrgf_blocksizes (any block sizes; the partitions of every instance are
+solved as one batch) or, when the tuned choice is RGF, negf_rgf.
 """
 from __future__ import annotations
 
@@ -18,8 +19,8 @@ import torch
 from ._lib import call, SingularError
 from .btd import DeviceBlockTridiag
 
-__all__ = ["bndiag_of_inv_ddrgf_get_nr_tasks", "levels_for_tasks", "cost_ddrgf_gpu",
-           "AuxDataDDRGF", "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf"]
+__all__ = ["bndiag_of_inv_ddrgf_get_nr_tasks", "levels_for_tasks", "AuxDataDDRGF",
+           "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf"]
 
 
 def bndiag_of_inv_ddrgf_get_nr_tasks(npl: int, blockSizeD1: int, blockSizeD2: int = 1):
@@ -47,72 +48,59 @@ def levels_for_tasks(npl: int, tasks: int) -> list:
     raise ValueError(f"no blockSizeD1 splits {npl} layers into {tasks} tasks")
 
 
-def cost_ddrgf_gpu(npl: int, levels: list) -> float:
-    """GEMM-equivalents on the critical path of one level-recursion (each
-    batched launch counted once, whatever its batch): the chunk chains are
-    O(bs) long, the reduced chain O(tasks)."""
-    cost, n = 0.0, npl
-    for bs in levels:
-        nr, last = bndiag_of_inv_ddrgf_get_nr_tasks(n, bs)
-        if nr <= 1:
-            return cost + 4.0 * n
-        cost += 14.0 * bs + 12.0
-        n = nr
-    return cost + 4.0 * n
-
-
 @dataclasses.dataclass
 class AuxDataDDRGF:
-    """Levels (blockSizeD1 per level) and the per-instance info array; the
-    library keeps the buffers (AuxDataDDRGF, core:1-30)."""
+    """Levels (blockSizeD1 per level; [0] = plain RGF, the tuner's choice
+    where it is faster) and the per-instance info array; the library keeps
+    the buffers (AuxDataDDRGF, core:1-30)."""
 
     levels: list
     info: torch.Tensor
+    times: dict | None = None   # the tuner's measurements when it chose the levels
+
+    @property
+    def rgf(self) -> bool:
+        return not self.levels or self.levels[0] == 0
 
 
-def allocate_aux_data_DDRGF(M: DeviceBlockTridiag, levels=None, tasks=None,
-                            max_levels: int = 3) -> AuxDataDDRGF:
+def allocate_aux_data_DDRGF(M: DeviceBlockTridiag, levels=None, tasks=None, max_levels: int = 2,
+                            persist: bool = False) -> AuxDataDDRGF:
     """``allocate_aux_data_DDRGF`` (setup.jl:4-51).  `levels` fixes the
     blockSizeD1 per level; `tasks` asks for a task count at one level; else
-    the levels minimising ``cost_ddrgf_gpu`` over blockSizeD1 in 1..16 and up
-    to `max_levels` levels."""
-    npl = M.nblocks
+    the tune table's choice for (N, bmax, batch), else the fastest candidate
+    timed now on a matrix of M's shape (recorded in the table, and in the
+    table file with `persist`)."""
+    from . import tune
+    info = torch.zeros(M.batch, dtype=torch.int32, device=M.diag.device)
+    times = None
     if levels is None:
         if tasks is not None:
-            levels = levels_for_tasks(npl, tasks)
+            levels = levels_for_tasks(M.nblocks, tasks)
         else:
-            best = (cost_ddrgf_gpu(npl, [npl]), [npl])
-            frontier = [([], npl)]
-            for _ in range(max_levels):
-                nxt = []
-                for lv, n in frontier:
-                    for bs in range(1, 17):
-                        nr, _ = bndiag_of_inv_ddrgf_get_nr_tasks(n, bs)
-                        if nr < 2:
-                            continue
-                        cand = lv + [bs]
-                        c = cost_ddrgf_gpu(npl, cand)
-                        if c < best[0]:
-                            best = (c, cand)
-                        nxt.append((cand, nr))
-                frontier = nxt
-            levels = best[1]
-    return AuxDataDDRGF(list(levels), torch.zeros(M.batch, dtype=torch.int32,
-                                                  device=M.diag.device))
+            levels = tune.lookup("bndiag", (M.nblocks, max(M.block_sizes), M.batch))
+            if levels is None:
+                levels, times = tune.tune_bndiag(M.block_sizes, M.batch, max_levels=max_levels,
+                                                 persist=persist)
+    return AuxDataDDRGF(list(levels), info, times)
 
 
-def bndiag_of_inv_ddrgf(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag, aux: AuxDataDDRGF):
-    """band(Min^{-1}) into Mout (core:719-758).  Uniform block sizes."""
+def bndiag_of_inv_ddrgf(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag, aux: AuxDataDDRGF,
+                        check: bool = True):
+    """band(Min^{-1}) into Mout (core:719-758), any block sizes."""
     L = Min.layout
-    if len(set(L.b)) != 1:
-        raise ValueError("the GPU DD-RGF takes uniform block sizes")
-    lv = (C.c_int32 * len(aux.levels))(*aux.levels)
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
-    call("negf_ddrgf", L.N, L.b[0], p(Min.diag), L.nd, p(Min.upper), max(L.no, 1),
-         p(Min.lower), max(L.no, 1), p(Mout.diag), p(Mout.upper), p(Mout.lower), Min.batch, lv,
-         len(aux.levels), p(aux.info), s)
-    bad = aux.info.cpu()
-    if bool((bad != 0).any()):
-        raise SingularError("bndiag_of_inv_ddrgf", bad.tolist())
+    if aux.rgf:
+        call("negf_rgf", L.N, L.sizes_c(), p(Min.diag), L.nd, p(Min.upper), max(L.no, 1),
+             p(Min.lower), max(L.no, 1), p(Mout.diag), p(Mout.upper), p(Mout.lower), Min.batch,
+             p(aux.info), s)
+    else:
+        lv = (C.c_int32 * len(aux.levels))(*aux.levels)
+        call("negf_ddrgf_blocksizes", L.N, L.sizes_c(), p(Min.diag), L.nd, p(Min.upper),
+             max(L.no, 1), p(Min.lower), max(L.no, 1), p(Mout.diag), p(Mout.upper),
+             p(Mout.lower), Min.batch, lv, len(aux.levels), p(aux.info), s)
+    if check:
+        bad = aux.info.cpu()
+        if bool((bad != 0).any()):
+            raise SingularError("bndiag_of_inv_ddrgf", bad.tolist())
     return Mout

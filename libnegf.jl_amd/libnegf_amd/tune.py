@@ -1,0 +1,289 @@
+"""Autotuner: times candidate schedules on the device, keeps the fastest per
+problem shape in the library's tune table (negf_tune_*) and persists it as a
+text table that the library loads again.
+
+It plays the part of the reference's ``opt_params``
+(``src/matinvertndiag_ddrgf_utils.jl:357-453``), which picks the DD-RGF
+levels from a cost model fed by timed LU / mldivide rates; here the
+candidates themselves are timed, and plain RGF is one of them, so DD-RGF is
+chosen only where it is measured faster.  The same table holds the GEMM tile
+variant per (m, n, k, problems per launch) and the batched-inverse schedule
+per (n, batch), which replace the launchers' built-in thresholds wherever an
+entry exists.
+
+Table file: one entry per line, ``gemm m n k problems variant``,
+``inv n batch variant`` or ``bndiag N bmax batch L1 [L2 [L3]]`` (L1 = 0:
+RGF), then ``#`` and the measured times.  The file next to the library
+(``tune_table.txt``) is loaded on first use; ``LIBNEGF_AMD_TUNE_FILE`` names
+another one (an empty value loads none).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+from . import _lib
+
+__all__ = ["table_path", "load_table", "lookup", "record", "time_device",
+           "tune_gemm", "tune_inverse", "tune_bndiag", "bndiag_candidates", "KIND"]
+
+KIND = {"gemm": 0, "inv": 1, "bndiag": 2}
+NKEY = {"gemm": 4, "inv": 2, "bndiag": 3}
+DEFAULT_TABLE = pathlib.Path(__file__).resolve().with_name("tune_table.txt")
+
+
+def table_path() -> pathlib.Path | None:
+    env = os.environ.get("LIBNEGF_AMD_TUNE_FILE")
+    if env is None:
+        return DEFAULT_TABLE
+    return pathlib.Path(env) if env else None
+
+
+def load_table(path=None) -> int:
+    """Load a table file into the library; returns the entries read (0 when
+    there is no file)."""
+    p = pathlib.Path(path) if path else table_path()
+    if p is None or not p.exists():
+        return 0
+    return _lib.call("negf_tune_load", str(p).encode())
+
+
+def _i32(v):
+    return (C.c_int32 * len(v))(*[int(x) for x in v])
+
+
+def lookup(kind: str, key) -> list | None:
+    """The table's choice for ``key`` (nearest batch key, as the launchers
+    see it), or None."""
+    out = (C.c_int32 * 4)()
+    n = _lib.call("negf_tune_lookup", KIND[kind], _i32(key), len(key), out, 4)
+    return [int(out[i]) for i in range(n)] if n > 0 else None
+
+
+def record(kind: str, key, choice, times: dict | None = None, path=None, persist=True):
+    """Put an entry into the library's table and, with ``persist``, into the
+    table file (replacing a line with the same kind and key)."""
+    key, choice = [int(x) for x in key], [int(x) for x in choice]
+    if len(key) != NKEY[kind]:
+        raise ValueError(f"{kind} takes {NKEY[kind]} keys")
+    _lib.call("negf_tune_set", KIND[kind], _i32(key), len(key), _i32(choice), len(choice))
+    p = pathlib.Path(path) if path else table_path()
+    if not persist or p is None:
+        return
+    head = " ".join([kind] + [str(x) for x in key])
+    line = " ".join([head] + [str(x) for x in choice])
+    if times:
+        line += "  # " + " ".join(f"{k}:{v * 1e3:.3f}ms" for k, v in times.items())
+    lines = p.read_text().splitlines() if p.exists() else [
+        "# libnegf_amd tune table: kind keys... choice...  # measured times (see tune.py)"]
+    out, done = [], False
+    for ln in lines:
+        body = ln.split("#", 1)[0].split()
+        if len(body) > NKEY[kind] and " ".join(body[:1 + NKEY[kind]]) == head:
+            if not done:
+                out.append(line)
+                done = True
+            continue
+        out.append(ln)
+    if not done:
+        out.append(line)
+    p.write_text("\n".join(out) + "\n")
+
+
+def time_device(fn, reps: int = 3, warmup: int = 1) -> float:
+    """Best of ``reps`` timings (seconds) of ``fn`` on the current stream,
+    measured with device events after ``warmup`` untimed calls."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    return best
+
+
+# A candidate replaces the launcher's built-in choice (variant 0) only when it
+# is at least this much faster in isolation: the run-to-run spread of these
+# timings, and the in-pipeline difference of choices closer than this was
+# within noise.
+MARGIN = 0.03
+
+
+def _choose(times: dict) -> str:
+    best = min(times, key=times.get)
+    if "v0" in times and times[best] > (1.0 - MARGIN) * times["v0"]:
+        return "v0"
+    return best
+
+
+def _rand(shape, seed):
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    re = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64)
+    im = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64)
+    return torch.complex(re - 0.5, im - 0.5)
+
+
+def tune_gemm(m: int, n: int, k: int, batch: int, variants=(9, 10, 8, 6), persist=True,
+              path=None):
+    """Times D = A B over ``batch`` instances with the built-in choice (v0)
+    and each tile variant, and records the fastest for (m, n, k, batch) (0:
+    keep the built-in choice, see MARGIN).  Returns (variant, times)."""
+    import torch
+    A, B = _rand((batch, k, m), 1), _rand((batch, n, k), 2)   # column-major blocks
+    D = torch.empty((batch, n, m), dtype=torch.complex128, device="cuda")
+    one, zero = _lib.cplx(1.0), _lib.cplx(0.0)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+
+    def run():
+        _lib.call("negf_zgemm_strided_batched", b"N", b"N", m, n, k, one, p(A), m, m * k, p(B), k,
+                  k * n, zero, p(D), m, m * n, batch, s)
+
+    times = {}
+    record("gemm", (m, n, k, batch), (0,), persist=False)   # v0: the built-in thresholds
+    try:
+        for v in (0,) + tuple(variants):
+            _lib.call("negf_gemm_variant", int(v))
+            times[f"v{v}"] = time_device(run)
+    finally:
+        _lib.call("negf_gemm_variant", 0)
+    best = _choose(times)
+    record("gemm", (m, n, k, batch), (int(best[1:]),), times, path=path, persist=persist)
+    return int(best[1:]), times
+
+
+def tune_inverse(n: int, batch: int, variants=(1, 2, 3, 4, 5), persist=True, path=None):
+    """Times the batched inverse of ``batch`` n x n blocks with the built-in
+    schedule choice (v0) and each variant (split-step variants apply to
+    n <= 512), and records the fastest for (n, batch) (0: keep the built-in
+    choice, see MARGIN).  Returns (variant, times)."""
+    import torch
+    A = _rand((batch, n, n), 3)
+    A += n * torch.eye(n, dtype=torch.complex128, device="cuda")
+    X = torch.empty_like(A)
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+
+    def run():
+        _lib.call("negf_zinv_strided_batched", n, p(A), n, n * n, p(X), n, n * n, batch, p(info), s)
+
+    times = {}
+    record("inv", (n, batch), (0,), persist=False)   # v0: the built-in schedule choice
+    try:
+        for v in (0,) + tuple(variants):
+            if v >= 3 and n > 512:
+                continue
+            _lib.call("negf_inv_variant", int(v))
+            times[f"v{v}"] = time_device(run)
+    finally:
+        _lib.call("negf_inv_variant", 0)
+    if bool((info != 0).any()):
+        raise RuntimeError("tune_inverse: singular test matrix")
+    best = _choose(times)
+    record("inv", (n, batch), (int(best[1:]),), times, path=path, persist=persist)
+    return int(best[1:]), times
+
+
+def bndiag_candidates(npl: int, max_levels: int = 2) -> list:
+    """RGF ([0]) and DD-RGF levels giving 2 to 32 tasks at the first level
+    and, for two levels, 2 to 4 tasks at the second (blockSizeD1 lists)."""
+    from .ddrgf import bndiag_of_inv_ddrgf_get_nr_tasks as nr_tasks
+    cands = [[0]]
+    firsts = {}
+    for bs in range(1, npl):
+        nr, _ = nr_tasks(npl, bs)
+        if 2 <= nr <= 32 and nr in (2, 3, 4, 6, 8, 12, 16, 24, 32) and nr not in firsts:
+            firsts[nr] = bs
+    for nr, bs in sorted(firsts.items()):
+        cands.append([bs])
+        if max_levels >= 2 and nr >= 8:
+            seconds = {}
+            for bs2 in range(1, nr):
+                nr2, _ = nr_tasks(nr, bs2)
+                if 2 <= nr2 <= 4 and nr2 not in seconds:
+                    seconds[nr2] = bs2
+            cands.extend([bs, b2] for _, b2 in sorted(seconds.items()))
+    return cands
+
+
+def tune_bndiag(block_sizes, batch: int, candidates=None, max_levels: int = 2, persist=True,
+                reps: int = 2, path=None):
+    """Times band(A^{-1}) for a chain of ``block_sizes`` over ``batch``
+    instances with every candidate (RGF and DD-RGF levels) on a
+    well-conditioned random input of that shape (diagonal blocks shifted by
+    their size; the timing does not depend on the values) and records the
+    fastest under (N, bmax, batch).  Returns (levels, times); [0] is RGF."""
+    import torch
+    from .btd import DeviceBlockTridiag
+    from .ddrgf import AuxDataDDRGF, bndiag_of_inv_ddrgf
+    from . import api
+    sizes = [int(x) for x in block_sizes]
+    N, bmax = len(sizes), max(sizes)
+    M = DeviceBlockTridiag(sizes, batch)
+    L = M.layout
+    M.diag.copy_(_rand(M.diag.shape, 4))
+    M.upper.copy_(0.3 * _rand(M.upper.shape, 5))
+    M.lower.copy_(0.3 * _rand(M.lower.shape, 6))
+    for inst in range(batch):
+        for i in range(N):
+            M.block("diag", i, inst).diagonal().add_(float(L.b[i]))
+    G = M.similar()
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    cands = candidates if candidates is not None else bndiag_candidates(N, max_levels)
+    times = {}
+    for lv in cands:
+        aux = AuxDataDDRGF(list(lv), info)
+        name = "rgf" if not lv[0] else "dd" + "-".join(str(x) for x in lv)
+        times[name] = time_device(lambda: bndiag_of_inv_ddrgf(G, M, aux, check=False), reps=reps)
+    api.release_workspace()
+    best = min(times, key=times.get)
+    if best != "rgf" and times[best] > (1.0 - MARGIN) * times["rgf"]:
+        best = "rgf"   # DD-RGF only where it is clearly faster (RGF is the more accurate)
+    levels = [0] if best == "rgf" else [int(x) for x in best[2:].split("-")]
+    record("bndiag", (N, bmax, batch), levels, times, path=path, persist=persist)
+    return levels, times
+
+
+def main(argv=None):
+    """Tune the shapes of the production workloads on this device and write
+    them into the table file: the GEMM tile and inverse schedule at block
+    sizes 256, 512 and 1024 over batches 1..128, and the band method of the
+    three uneven devices (128 x 256, 64 x 512, 32 x 1024; 8 energies per
+    call) and of the C interface example (160 x 64, one instance)."""
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", default=None,
+                    help="table file (default: LIBNEGF_AMD_TUNE_FILE, else the package's)")
+    ap.add_argument("--skip-kernels", action="store_true")
+    ap.add_argument("--skip-bndiag", action="store_true")
+    a = ap.parse_args(argv)
+    path = pathlib.Path(a.out) if a.out else (table_path() or DEFAULT_TABLE)
+    if not a.skip_kernels:
+        for b in (256, 512, 1024):
+            for batch in (1, 2, 4, 8, 16, 32, 64, 96, 128):
+                v, t = tune_gemm(b, b, b, batch, path=path)
+                print(f"gemm {b} batch {batch}: v{v} " + " ".join(
+                    f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
+                v, t = tune_inverse(b, batch, path=path)
+                print(f"inv {b} batch {batch}: v{v} " + " ".join(
+                    f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
+    if not a.skip_bndiag:
+        for N, b, batch in ((128, 256, 8), (64, 512, 8), (32, 1024, 8), (160, 64, 1)):
+            lv, t = tune_bndiag([b] * N, batch, path=path)
+            print(f"bndiag N={N} b={b} batch={batch}: {lv} " + " ".join(
+                f"{k}={x * 1e3:.1f}ms" for k, x in t.items()), flush=True)
+    print(f"table: {path}", file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -219,3 +219,58 @@ def test_rgf_noalloc_refuses_beyond_its_scratch():
                                 None) == -1
     assert lib.negf_rgf_noalloc(2, None, dummy, 0, dummy, 0, dummy, 0, dummy, None, None, 1, None,
                                 None) == -1
+
+
+def test_tune_table_file_and_lookup_rules(tmp_path):
+    """The tune table (host-side, no GPU): a file written by tune.record loads
+    back into the library; a lookup matches every key but the batch-like last
+    one exactly and takes the largest batch key <= the request, else the
+    smallest above it; malformed lines are refused."""
+    from libnegf_amd._lib import LIB_PATH, load, call
+    from libnegf_amd import tune
+    if not LIB_PATH.exists():
+        pytest.skip("libnegf_hip.so not built")
+    load()
+    p = tmp_path / "t.txt"
+    try:
+        call("negf_tune_clear")
+        tune.record("gemm", (512, 512, 512, 32), (9,), {"v9": 1e-3, "v10": 2e-3}, path=p)
+        tune.record("gemm", (512, 512, 512, 96), (6,), path=p)
+        tune.record("bndiag", (128, 256, 8), (15,), path=p)
+        tune.record("bndiag", (128, 256, 8), (0,), {"rgf": 0.5}, path=p)   # replaces the line
+        tune.record("bndiag", (64, 512, 8), (7, 2), path=p)
+        tune.record("inv", (512, 96), (4,), path=p)
+        text = p.read_text()
+        assert text.count("bndiag 128 256 8") == 1 and "bndiag 128 256 8 0  # rgf:500.000ms" in text
+        call("negf_tune_clear")
+        assert call("negf_tune_count") == 0 and tune.lookup("gemm", (512, 512, 512, 32)) is None
+        assert tune.load_table(p) == 5
+        assert tune.lookup("gemm", (512, 512, 512, 64)) == [9]     # largest <= 64
+        assert tune.lookup("gemm", (512, 512, 512, 200)) == [6]
+        assert tune.lookup("gemm", (512, 512, 512, 4)) == [9]      # none <= 4: smallest above
+        assert tune.lookup("gemm", (256, 256, 256, 64)) is None    # other keys must match
+        assert tune.lookup("bndiag", (128, 256, 1)) == [0]
+        assert tune.lookup("bndiag", (64, 512, 16)) == [7, 2]
+        assert tune.lookup("inv", (512, 8)) == [4]
+        bad = tmp_path / "bad.txt"
+        for line in ("frob 1 2 3", "gemm 512 512 512", "inv 512 -1 2", "inv 512 8 1 2 3 4 5"):
+            bad.write_text(line + "\n")
+            with pytest.raises(Exception):
+                tune.load_table(bad)
+    finally:
+        call("negf_tune_clear")
+        tune.load_table()
+
+
+def test_tuner_candidates_always_include_rgf_and_split():
+    from libnegf_amd import tune
+    from libnegf_amd.ddrgf import bndiag_of_inv_ddrgf_get_nr_tasks as nr_tasks
+    for npl in (5, 16, 32, 64, 128, 160):
+        c = tune.bndiag_candidates(npl)
+        assert c[0] == [0]
+        for lv in c[1:]:
+            n = npl
+            for bs in lv:
+                nr, _ = nr_tasks(n, bs)
+                assert nr >= 2, (npl, lv)
+                n = nr

@@ -26,7 +26,7 @@ def _band(h):
 
 def _run(mats, levels):
     Min = DeviceBlockTridiag.from_host(mats)
-    Mout = Min.similar()
+    Mout = Min.similar(hermitian=False)
     dd.bndiag_of_inv_ddrgf(Mout, Min, dd.allocate_aux_data_DDRGF(Min, levels=levels))
     return Mout
 
@@ -59,18 +59,110 @@ def test_ddrgf_batched_energies():
         assert max_block_err(_band(out.to_host(k)), bndiag_of_inv_rgf_local(A)) < 2.5e-10
 
 
-def test_ddrgf_tuner_and_task_split():
+def _shifted(A, shift):
+    """A with shift added to every diagonal element (a second instance of the
+    same block structure)."""
+    from libnegf_amd.synthetic import BlockTridiag
+    d = [np.asfortranarray(x + shift * np.eye(x.shape[0])) for x in A.diag]
+    return BlockTridiag(d, A.upper, A.lower)
+
+
+@pytest.mark.parametrize("npl,b,levels,batch,herm", [
+    (128, 64, [15], 1, False), (128, 64, [15, 1], 1, False), (40, 20, [4], 2, False),
+    (24, 16, [3], 1, True)])
+def test_ddrgf_uneven_blocks_vs_oracle(npl, b, levels, batch, herm):
+    """The reference's own DD-RGF workload: the uneven generator
+    (test/test_matinvertndiag_ddrgf.jl:13-14, src/blockmatrix.jl:878-919),
+    here N = 128 blocks of 64 +- 4 in 8 partitions (and two levels), a batch
+    of two instances, and Hermitian storage (the lower blocks read as the
+    adjoints of the upper ones).  Against the oracle's DD-RGF with the same
+    levels and its RGF: the oracle's own DD-RGF-vs-RGF spread on these
+    inputs is at most 4.2e-12."""
+    print(f"ddrgf uneven npl={npl} b={b} levels={levels} batch={batch} herm={herm}", flush=True)
+    A = bm_create_synthetic_random(npl, b, seed=npl + b, hermitian=herm)
+    assert len(set(A.block_sizes)) > 1
+    mats = [A] + [_shifted(A, 0.25 * k) for k in range(1, batch)]
+    out = _run(mats, levels)
+    for k, Ak in enumerate(mats):
+        G = _band(out.to_host(k))
+        e_dd = max_block_err(G, dd_ref(Ak, levels))
+        e_rgf = max_block_err(G, bndiag_of_inv_rgf_local(Ak))
+        print(f"  instance {k}: vs oracle DD-RGF {e_dd:.2e}, vs oracle RGF {e_rgf:.2e}")
+        assert e_dd < 1e-11 and e_rgf < 2e-11
+
+
+@pytest.fixture
+def clean_tune_table():
+    """The tuner writes into the library's table; restore the loaded table
+    (the package's file) afterwards."""
+    from libnegf_amd import tune, _lib
+    yield
+    _lib.call("negf_tune_clear")
+    tune.load_table()
+
+
+def test_ddrgf_task_split():
     assert dd.levels_for_tasks(128, 8) == [15]
     assert dd.levels_for_tasks(64, 8) == [7]
     assert dd.levels_for_tasks(32, 8) == [3]
     assert dd.bndiag_of_inv_ddrgf_get_nr_tasks(9, 3) == [-1, 0]
-    Min = DeviceBlockTridiag([16] * 40)
+
+
+def test_tuner_times_rgf_and_ddrgf_and_keeps_the_fastest(tmp_path, clean_tune_table):
+    """tune_bndiag times plain RGF and DD-RGF levels on the device, records
+    the fastest (RGF when it is measured faster), and allocate_aux_data_DDRGF
+    takes the recorded choice; the band it then computes matches the oracle."""
+    from libnegf_amd import tune
+    sizes = bm_create_synthetic_random(40, 16, seed=56).block_sizes
+    table = tmp_path / "tune.txt"
+    levels, times = tune.tune_bndiag(sizes, 2, path=table)
+    print(f"tuned {levels}: " + " ".join(f"{k}={v * 1e3:.2f}ms" for k, v in times.items()))
+    assert "rgf" in times and len(times) >= 3
+    best = min(times, key=times.get)
+    if best != "rgf" and times[best] > (1 - tune.MARGIN) * times["rgf"]:
+        best = "rgf"
+    assert levels == ([0] if best == "rgf" else [int(x) for x in best[2:].split("-")])
+    assert tune.lookup("bndiag", (40, max(sizes), 2)) == levels
+    assert f"bndiag 40 {max(sizes)} 2 " in table.read_text()
+    A = bm_create_synthetic_random(40, 16, seed=56)
+    Min = DeviceBlockTridiag.from_host([A, _shifted(A, 0.5)])
     aux = dd.allocate_aux_data_DDRGF(Min)
-    n = 40
-    for bs in aux.levels:
-        nr, _ = dd.bndiag_of_inv_ddrgf_get_nr_tasks(n, bs)
-        assert nr >= 2
-        n = nr
+    assert aux.levels == levels
+    Mout = Min.similar()
+    dd.bndiag_of_inv_ddrgf(Mout, Min, aux)
+    assert max_block_err(_band(Mout.to_host(0)), bndiag_of_inv_rgf_local(A)) < 1e-11
+    # a forced RGF choice runs negf_rgf
+    tune.record("bndiag", (40, max(sizes), 2), [0], persist=False)
+    aux = dd.allocate_aux_data_DDRGF(Min)
+    assert aux.rgf
+    dd.bndiag_of_inv_ddrgf(Mout, Min, aux)
+    assert max_block_err(_band(Mout.to_host(0)), bndiag_of_inv_rgf_local(A)) < 1e-12
+
+
+def test_tuner_gemm_and_inverse_entries_are_used(tmp_path, clean_tune_table):
+    """The GEMM-tile and inverse-schedule tuners record a measured variant;
+    with it in the table the default launches still compute the same
+    product / inverse (the variants differ in tiling, not in results beyond
+    rounding)."""
+    from libnegf_amd import tune, api
+    table = tmp_path / "tune.txt"
+    v, t = tune.tune_gemm(128, 128, 128, 4, path=table)
+    best = min(t, key=t.get)
+    assert "v0" in t and tune.lookup("gemm", (128, 128, 128, 4)) == [v]
+    assert v == int(best[1:]) or (v == 0 and t[best] > (1 - tune.MARGIN) * t["v0"])
+    vi, ti = tune.tune_inverse(128, 4, path=table)
+    best = min(ti, key=ti.get)
+    assert "v0" in ti and tune.lookup("inv", (128, 4)) == [vi]
+    assert vi == int(best[1:]) or (vi == 0 and ti[best] > (1 - tune.MARGIN) * ti["v0"])
+    g = torch.Generator(device="cuda").manual_seed(9)
+    A = torch.randn((128, 128), dtype=torch.complex128, device="cuda", generator=g)
+    B = torch.randn((128, 128), dtype=torch.complex128, device="cuda", generator=g)
+    A += 16 * torch.eye(128, dtype=torch.complex128, device="cuda")
+    C = api.be_zero_array((128, 128))
+    api.be_gemm("N", "N", 1.0, api.col_major(A), api.col_major(B), 0.0, C)
+    assert torch.linalg.norm(C - A @ B) / torch.linalg.norm(A @ B) < 1e-14
+    X = api.be_inv(api.col_major(A))
+    assert torch.linalg.norm(X @ A - torch.eye(128, dtype=torch.complex128, device="cuda")) < 1e-12
 
 
 def _gpu_block_relerr(X, R):

@@ -210,6 +210,66 @@ def test_zgetrf_exact_ties_match_lapack(n, perm_seed):
     assert np.array_equal(_host(lu.A), lu_ref)
 
 
+@pytest.mark.slow
+@pytest.mark.parametrize("n,perm_seed", [(2560, 3), (4096, None), (4100, 8)])
+def test_zgetrf_beyond_2048_exact_ties_match_lapack(n, perm_seed):
+    """The two-level blocked LU (n > 2048: 64-column outer panels factored 8 /
+    4 / 2 columns at a time by one 512-thread workgroup, the trailing update
+    on the matrix cores) on the exactly-factorable Hadamard matrix, padded
+    to a non-power of two by an identity block where n is one: ipiv and
+    the factors equal LAPACK's bit for bit."""
+    print(f"zgetrf large ties n={n} perm={perm_seed}", flush=True)
+    m = 1 << (n - 1).bit_length() if n & (n - 1) else n
+    m = m if m == n else m // 2
+    A = np.eye(n, dtype=complex)
+    A[:m, :m] = _hadamard_phased(m, perm_seed)
+    A = np.asfortranarray(A)
+    lu = api.be_lu(None, _dev(A))
+    lu_ref, piv_ref, info = sla.lapack.zgetrf(A)
+    assert info == 0
+    assert np.array_equal(lu.piv.cpu().numpy(), piv_ref + 1)
+    assert np.array_equal(_host(lu.A), lu_ref)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("n", [2100, 4500])
+def test_zinv_beyond_gauss_jordan(n):
+    """be_inv above n = 2048 (LU, triangular inverses, their product, the
+    column interchanges), on a random matrix and on a row-permuted
+    near-identity that needs every pivot right."""
+    print(f"zinv large n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = _rand(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
+    inv = _host(api.be_inv(_dev(A)))
+    assert relfro(inv, np.linalg.inv(A)) < 1e-15 * np.linalg.cond(A) * n
+    perm = rng.permutation(n)
+    B = np.asfortranarray((np.eye(n) + 1e-8 * _rand(rng, n, n))[perm])
+    inv = _host(api.be_inv(_dev(B)))
+    assert relfro(inv, np.linalg.inv(B)) < 1e-13
+
+
+@pytest.mark.slow
+def test_bndiag_of_inv_direct_beyond_2048():
+    """bndiag_of_inv_direct (src/matinvertndiag_direct.jl:17-86) on the
+    reference's uneven generator at N = 40 blocks of 64 +- 4 (n ~ 2560, the
+    large-n inverse): its band equals the oracle's RGF band."""
+    import scipy.sparse as sp
+    from libnegf_amd.synthetic import bm_create_synthetic_random
+    from oracle.rgf import bndiag_of_inv_rgf_local
+    A = bm_create_synthetic_random(40, 64, seed=40)
+    n = sum(A.block_sizes)
+    print(f"bndiag_of_inv_direct n={n}", flush=True)
+    out = api.bndiag_of_inv_direct(sp.csc_matrix(A.to_dense()), A.block_sizes).toarray()
+    ref = bndiag_of_inv_rgf_local(A)
+    off = np.concatenate([[0], np.cumsum(A.block_sizes)])
+    got = {k: out[off[k[0]]:off[k[0] + 1], off[k[1]]:off[k[1] + 1]] for k in ref}
+    assert max(relfro(got[k], ref[k]) for k in ref) < 1e-11
+    mask = np.ones((n, n), dtype=bool)
+    for (i, j) in ref:
+        mask[off[i]:off[i + 1], off[j]:off[j + 1]] = False
+    assert not out[mask].any()
+
+
 @pytest.mark.parametrize("n", [64, 512])
 def test_zinv_exact_ties(n):
     """Gauss-Jordan on the same tie-laden matrix (rows permuted): A A^H = n I,
