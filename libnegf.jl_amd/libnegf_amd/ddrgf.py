@@ -20,7 +20,7 @@ from ._lib import call, SingularError
 from .btd import DeviceBlockTridiag
 
 __all__ = ["bndiag_of_inv_ddrgf_get_nr_tasks", "levels_for_tasks", "AuxDataDDRGF",
-           "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf"]
+           "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf", "bndiag_of_inv_ddrgf_error_inv_of_T11"]
 
 
 def bndiag_of_inv_ddrgf_get_nr_tasks(npl: int, blockSizeD1: int, blockSizeD2: int = 1):
@@ -104,3 +104,79 @@ def bndiag_of_inv_ddrgf(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag, aux: 
         if bool((bad != 0).any()):
             raise SingularError("bndiag_of_inv_ddrgf", bad.tolist())
     return Mout
+
+
+def bndiag_of_inv_ddrgf_error_inv_of_T11(Min: DeviceBlockTridiag, aux: AuxDataDDRGF) -> list:
+    """``bndiag_of_inv_ddrgf_error_inv_of_T11`` (utils.jl:776-824): how well
+    the interiors' inverses T11^{-1} that DD-RGF eliminates with solve T11,
+    per instance
+
+        sqrt( sum over tasks, block rows i, block columns j of
+              ||(T11 T11^{-1})_ij - delta_ij I||_F^2  /  sum of the interior sizes ),
+
+    with T11^{-1} every block of each interior's inverse (negf_rgf_full, the
+    recursion DD-RGF runs on the interiors) and T11 the interior's
+    block-tridiagonal part of Min, at the first level's task split (any block
+    sizes; the reference assumes equal ones).  Returns one value per
+    instance."""
+    import ctypes as Cc
+    from ._lib import cplx
+    if aux.rgf:
+        raise ValueError("the tuned method is plain RGF: there is no T11")
+    L = Min.layout
+    bs = aux.levels[0]
+    nr, left = bndiag_of_inv_ddrgf_get_nr_tasks(L.N, bs)
+    if nr < 2:
+        raise ValueError("the first level's block size does not split the chain")
+    dev = Min.diag.device
+    s = Cc.c_void_p(torch.cuda.current_stream().cuda_stream)
+    num = torch.zeros(Min.batch, dtype=torch.float64, device=dev)
+    den = 0
+    for c in range(nr):
+        f = c * (bs + 1) + 1
+        Lc = left if (c == nr - 1 and left) else bs
+        Lc = min(Lc, L.N - f)
+        if Lc <= 0:
+            continue
+        sizes = [L.b[f + i] for i in range(Lc)]
+        chunk = DeviceBlockTridiag(sizes, Min.batch, False, dev, zero=False)
+        for i in range(Lc):
+            chunk.diag[:, chunk.layout.od[i]:chunk.layout.od[i + 1]] = \
+                Min.diag[:, L.od[f + i]:L.od[f + i + 1]]
+        for i in range(Lc - 1):
+            chunk.upper[:, chunk.layout.oo[i]:chunk.layout.oo[i + 1]] = \
+                Min.upper[:, L.oo[f + i]:L.oo[f + i + 1]]
+            if Min.lower is not None:
+                chunk.lower[:, chunk.layout.oo[i]:chunk.layout.oo[i + 1]] = \
+                    Min.lower[:, L.oo[f + i]:L.oo[f + i + 1]]
+            else:
+                for e in range(Min.batch):
+                    chunk.block("lower", i, e).copy_(Min.block("upper", f + i, e).conj().T)
+        n = sum(sizes)
+        X = torch.empty((Min.batch, n, n), dtype=torch.complex128, device=dev)
+        info = torch.zeros(Min.batch, dtype=torch.int32, device=dev)
+        CL = chunk.layout
+        call("negf_rgf_full", CL.N, CL.sizes_c(), Cc.c_void_p(chunk.diag.data_ptr()), CL.nd,
+             Cc.c_void_p(chunk.upper.data_ptr()), max(CL.no, 1),
+             Cc.c_void_p(chunk.lower.data_ptr()), max(CL.no, 1), Cc.c_void_p(X.data_ptr()),
+             Min.batch, Cc.c_void_p(info.data_ptr()), s)
+        # the interior as a dense column-major matrix (stored transposed: [inst, col, row])
+        T = torch.zeros((Min.batch, n, n), dtype=torch.complex128, device=dev)
+        off = [0]
+        for x in sizes:
+            off.append(off[-1] + x)
+        for e in range(Min.batch):
+            for i in range(Lc):
+                T[e, off[i]:off[i + 1], off[i]:off[i + 1]] = chunk.block("diag", i, e).T
+                if i + 1 < Lc:
+                    T[e, off[i + 1]:off[i + 2], off[i]:off[i + 1]] = chunk.block("upper", i, e).T
+                    T[e, off[i]:off[i + 1], off[i + 1]:off[i + 2]] = chunk.block("lower", i, e).T
+        R = torch.eye(n, dtype=torch.complex128, device=dev).expand(Min.batch, n, n).clone()
+        call("negf_zgemm_strided_batched", b"N", b"N", n, n, n, cplx(1.0),
+             Cc.c_void_p(T.data_ptr()), n, n * n, Cc.c_void_p(X.data_ptr()), n, n * n, cplx(-1.0),
+             Cc.c_void_p(R.data_ptr()), n, n * n, Min.batch, s)
+        num += torch.linalg.vector_norm(R.reshape(Min.batch, -1), dim=1) ** 2
+        den += n
+    if bool((num != num).any()):
+        raise SingularError("bndiag_of_inv_ddrgf_error_inv_of_T11", [])
+    return torch.sqrt(num / den).cpu().tolist()

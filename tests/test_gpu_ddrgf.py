@@ -91,6 +91,38 @@ def test_ddrgf_uneven_blocks_vs_oracle(npl, b, levels, batch, herm):
         assert e_dd < 1e-11 and e_rgf < 2e-11
 
 
+def test_error_inv_of_T11_diagnostic():
+    """bndiag_of_inv_ddrgf_error_inv_of_T11 (utils.jl:776-824): the residual
+    of the interiors' inverses, one value per instance, uneven blocks
+    included.  Against the same measure of numpy's inverses of the same
+    interiors on the host: two correct FP64 inverses, so the same order of
+    magnitude (cond(A) ~ 4e3 here), not the same digits."""
+    print("ddrgf T11 diagnostic", flush=True)
+    A = bm_create_synthetic_random(23, 16, seed=9)
+    B = _shifted(A, -0.9)                     # closer to singular interiors
+    Min = DeviceBlockTridiag.from_host([A, B])
+    aux = dd.allocate_aux_data_DDRGF(Min, levels=[4])
+    got = dd.bndiag_of_inv_ddrgf_error_inv_of_T11(Min, aux)
+    sizes = A.block_sizes
+
+    def host_measure(M):
+        num, den = 0.0, 0
+        nr, left = dd.bndiag_of_inv_ddrgf_get_nr_tasks(23, 4)
+        for c in range(nr):
+            f = c * 5 + 1
+            Lc = left if (c == nr - 1 and left) else 4
+            T = M.to_dense()[sum(sizes[:f]):sum(sizes[:f + Lc]), sum(sizes[:f]):sum(sizes[:f + Lc])]
+            R = T @ np.linalg.inv(T) - np.eye(T.shape[0])
+            num += np.linalg.norm(R) ** 2
+            den += T.shape[0]
+        return np.sqrt(num / den)
+
+    ref = [host_measure(A), host_measure(B)]
+    print(f"  GPU {got}, host {ref}")
+    for g, r in zip(got, ref):
+        assert 0 < g < 1e-12 and 0.01 * r < g < 100 * r
+
+
 @pytest.fixture
 def clean_tune_table():
     """The tuner writes into the library's table; restore the loaded table

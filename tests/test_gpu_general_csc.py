@@ -120,9 +120,15 @@ def _example_csc(N, b, seed):
 def test_run_bndiag_wrapper_reference_symbol_by_value():
     """The exported reference-signature symbol itself: CCscMatrix returned by
     value, malloc'd arrays freed with libc free, on the C example's shape
-    (N = 160 blocks of 64).  Uniform blocks take DD-RGF, as the reference's
-    wrapper does (libnegf_c_interface.jl:119); its bar against RGF is the
-    reference's own 2.5e-10 (test/test_matinvertndiag_ddrgf.jl:17)."""
+    (N = 160 blocks of 64).  The wrapper takes DD-RGF, as the reference's
+    wrapper does (libnegf_c_interface.jl:119), with the levels the tune table
+    holds for the shape (else ~8 partitions).  On this input (random U[0,1)
+    entries, no diagonal dominance) DD-RGF and RGF differ by an amount that
+    depends on the split: the oracle's own DD-RGF is 2.2e-10 from its RGF
+    with 8 tasks and 1.2e-8 with the tuned two levels (4, 7).  So the bar is
+    the reference's 2.5e-10 (test/test_matinvertndiag_ddrgf.jl:17) or 5 x
+    the oracle's DD-RGF-vs-RGF spread for the levels taken, whichever is
+    larger."""
     import ctypes as C
     from libnegf_amd._lib import load
     from libnegf_amd.synthetic import BlockTridiag
@@ -165,8 +171,13 @@ def test_run_bndiag_wrapper_reference_symbol_by_value():
     G = sp.csc_matrix((onz, orv, ocp), shape=(n, n))
     got = {k: G[k[0] * b:(k[0] + 1) * b, k[1] * b:(k[1] + 1) * b].toarray() for k in ref}
     err = max_block_err(got, ref)
-    print(f"  max block relerr vs oracle RGF {err:.2e}")
-    assert err < 2.5e-10
+    from libnegf_amd import tune
+    from oracle.ddrgf import bndiag_of_inv_ddrgf as dd_ref
+    levels = tune.lookup("bndiag", (N, b, 1)) or [19]   # the wrapper's choice (8 tasks untuned)
+    spread = 0.0 if levels[0] == 0 else max_block_err(dd_ref(A, levels), ref)
+    print(f"  max block relerr vs oracle RGF {err:.2e} (levels {levels}, oracle spread "
+          f"{spread:.2e})")
+    assert err < max(2.5e-10, 5 * spread)
 
 
 def test_run_bndiag_wrapper_error_returns_empty_struct():
