@@ -354,7 +354,8 @@ def ddrgf_uneven(a, torch, api):
     Per device, timed eagerly: DD-RGF with 8 partitions (the configuration's
     named split, blockSizeD1 = 15 / 7 / 3), plain RGF, and the tuner's choice
     (the tune table's entry for (N, b, wave), else timed now).  Then all three
-    devices with their tuned methods at once on three streams."""
+    devices with their tuned methods at once: eagerly on three streams, and
+    as one negf_bndiag_devices call per wave."""
     import numpy as np
     from libnegf_amd import synthetic, ddrgf as dd, tune
     from libnegf_amd.flops import flops_rgf as F_rgf
@@ -398,11 +399,24 @@ def ddrgf_uneven(a, torch, api):
                 tuned, self.tune_times = tune.tune_bndiag([b] * N, wave, persist=False)
             self.tuned = tuned
 
+        def assemble(self):
+            """A(E) of the wave from self.zw: -H' everywhere, + z on the diagonal."""
+            self.Ad.copy_(self.minus_Hp.expand(wave, -1))
+            self.Ad[:, self.didx] += self.zw[:, None]
+
+        def job(self, levels):
+            """This device's wave as a NegfBandJob (levels [0]: RGF)."""
+            from libnegf_amd._lib import NegfBandJob
+            self._lv = (C.c_int32 * len(levels))(*levels)
+            return NegfBandJob(self.N, C.cast(self.sizes, C.POINTER(C.c_int32)), p(self.Ad),
+                               self.N * self.b * self.b, p(self.up), 0, p(self.low), 0,
+                               p(self.Gd), p(self.Gu), p(self.Gl), wave,
+                               C.cast(self._lv, C.POINTER(C.c_int32)), len(levels), p(self.info))
+
         def body(self, levels, stream):
             """Assembly from self.zw and the solve of one wave (no host
             synchronisation)."""
-            self.Ad.copy_(self.minus_Hp.expand(wave, -1))
-            self.Ad[:, self.didx] += self.zw[:, None]
+            self.assemble()
             s = C.c_void_p(stream.cuda_stream)
             if levels[0] == 0:
                 call("negf_rgf", self.N, self.sizes, p(self.Ad), self.N * self.b * self.b,
@@ -461,6 +475,38 @@ def ddrgf_uneven(a, torch, api):
         d.check("concurrent eager")
     print(f"# three devices eager: {dt_eager:.3f} s", file=sys.stderr, flush=True)
 
+    # all three devices in ONE library call per wave (negf_bndiag_devices: the
+    # devices run concurrently on the library's streams, joined back)
+    from libnegf_amd._lib import NegfBandJob
+    jobs = (NegfBandJob * len(devs))(*[d.job(d.tuned) for d in devs])
+    s_cur = torch.cuda.current_stream()
+
+    def one_call_wave(e0):
+        for d in devs:
+            d.zw.copy_(d.z[e0:e0 + wave])
+            d.assemble()
+        call("negf_bndiag_devices", len(devs), C.cast(jobs, C.c_void_p), C.c_void_p(s_cur.cuda_stream))
+        for d in devs:
+            torch.maximum(d.bad, d.info.abs(), out=d.bad)
+
+    one_call_wave(0)                                   # warmup (the library's streams)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0 in range(0, nE, wave):
+        one_call_wave(e0)
+    torch.cuda.synchronize()
+    dt_one = time.perf_counter() - t0
+    for d in devs:
+        d.check("one call")
+    Gd_one = [d.Gd.clone() for d in devs]
+    for d, st in zip(devs, streams):
+        with torch.cuda.stream(st):
+            d.run_wave(nE - wave, d.tuned, st)
+    torch.cuda.synchronize()
+    same = all(bool(torch.equal(g, d.Gd)) for g, d in zip(Gd_one, devs))
+    print(f"# three devices, one call per wave: {dt_one:.3f} s (== separate calls: {same})",
+          file=sys.stderr, flush=True)
+
     seq_named = sum(nE / r["ddrgf8_energies_per_s"] for r in res)
     seq_tuned = sum(nE / r["tuned_energies_per_s"] for r in res)
     out = {"workload": "ddrgf_uneven: devices (128,256),(64,512),(32,1024), "
@@ -469,7 +515,10 @@ def ddrgf_uneven(a, torch, api):
            "three_devices": {"sequential_ddrgf8_seconds": round(seq_named, 3),
                              "sequential_tuned_seconds": round(seq_tuned, 3),
                              "concurrent_eager_seconds": round(dt_eager, 3),
-                             "concurrent_device_energies_per_s": round(3 * nE / dt_eager, 3)}}
+                             "concurrent_device_energies_per_s": round(3 * nE / dt_eager, 3),
+                             "one_call_per_wave_seconds": round(dt_one, 3),
+                             "one_call_device_energies_per_s": round(3 * nE / dt_one, 3),
+                             "one_call_equals_separate_calls": same}}
     print(json.dumps(out))
 
 

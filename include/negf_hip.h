@@ -401,6 +401,38 @@ int negf_ddrgf_blocksizes(int32_t nblocks, const int32_t* block_sizes, const voi
 int64_t negf_ddrgf_blocksizes_workspace(int32_t nblocks, const int32_t* block_sizes, int32_t batch,
                                         const int32_t* block_sizes_d1, int32_t nlevels);
 
+/* Several devices (chains of different block sizes and counts) in one call:
+ * the uneven batched workload of one GPU.  Each job is band(A^{-1}) over its
+ * own batch, by RGF or by DD-RGF with the given levels (block_sizes_d1 =
+ * NULL: the tune table's choice for (nblocks, max block size, batch), RGF
+ * when it holds none; {0}: RGF).  The jobs run concurrently on streams the
+ * library keeps per calling stream, forked from `stream` and joined back
+ * into it, so work queued on `stream` afterwards sees every result.  At
+ * most 8 jobs.  Arguments of a job as negf_ddrgf_blocksizes; returns the
+ * first job's error (every job is still joined).  No counterpart in the
+ * reference, which runs one BlockMatrix per call (bndiag_of_inv_ddrgf!,
+ * src/matinvertndiag_ddrgf_core.jl:719-758): several devices of uneven
+ * block size and count in one call. */
+typedef struct {
+  int32_t nblocks;
+  const int32_t* block_sizes;
+  const void* diag;
+  int64_t diag_bstride;
+  const void* upper;
+  int64_t upper_bstride;
+  const void* lower;
+  int64_t lower_bstride;
+  void* g_diag;
+  void* g_upper;
+  void* g_lower;
+  int32_t batch;
+  const int32_t* block_sizes_d1;
+  int32_t nlevels;
+  int32_t* info;
+} NegfBandJob;
+
+int negf_bndiag_devices(int32_t njobs, const NegfBandJob* jobs, void* stream);
+
 /* -------------------------------------------------------------- tuning */
 
 /* The tune table: measured choices per problem shape, consulted by the

@@ -103,6 +103,54 @@ int64_t negf_zinv_workspace(int32_t n, int32_t batch) {
   return (int64_t)zinv_work_bytes(n, batch);
 }
 
+}  // extern "C"
+
+namespace {
+// Streams of negf_bndiag_devices, per (device, calling stream): one per job
+// slot, forked from the caller's stream by an event and joined back into it.
+constexpr int kMaxJobs = 8;
+struct JobPool {
+  hipStream_t s[kMaxJobs] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxJobs] = {};
+};
+std::mutex g_jobpool_mu;
+std::map<std::pair<int, void*>, JobPool> g_jobpools;
+
+void pool_destroy(JobPool& p) {
+  for (int i = 0; i < kMaxJobs; ++i) {
+    if (p.s[i]) (void)hipStreamDestroy(p.s[i]);
+    if (p.join[i]) (void)hipEventDestroy(p.join[i]);
+  }
+  if (p.fork) (void)hipEventDestroy(p.fork);
+  p = JobPool{};
+}
+
+int pool_for(void* stream, JobPool** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return NEGF_HIP_ERROR;
+  std::lock_guard<std::mutex> lk(g_jobpool_mu);
+  auto it = g_jobpools.find({dev, stream});
+  if (it != g_jobpools.end()) {
+    *out = &it->second;
+    return NEGF_OK;
+  }
+  JobPool p;
+  bool ok = hipEventCreateWithFlags(&p.fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < kMaxJobs && ok; ++i)
+    ok = hipStreamCreateWithFlags(&p.s[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    pool_destroy(p);
+    return NEGF_HIP_ERROR;
+  }
+  *out = &(g_jobpools[{dev, stream}] = p);
+  return NEGF_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int negf_workspace_release(void) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   (void)hipDeviceSynchronize();
@@ -110,6 +158,9 @@ int negf_workspace_release(void) {
     if (kv.second.p) (void)hipFree(kv.second.p);
   g_ws.clear();
   zinv_aux_release_all();
+  std::lock_guard<std::mutex> lk2(g_jobpool_mu);
+  for (auto& kv : g_jobpools) pool_destroy(kv.second);
+  g_jobpools.clear();
   return NEGF_OK;
 }
 
@@ -897,6 +948,53 @@ int negf_ddrgf_blocksizes(int32_t nblocks, const int32_t* block_sizes, const voi
     CAPI_RC(btd_unpad_launch(t, 2, Gl, po, (zc*)g_lower, no, batch, s));
   }
   return NEGF_OK;
+}
+
+int negf_bndiag_devices(int32_t njobs, const NegfBandJob* jobs, void* stream) {
+  if (njobs < 0 || njobs > kMaxJobs || (njobs > 0 && !jobs)) return NEGF_BAD_ARG;
+  if (njobs == 0) return NEGF_OK;
+  for (int j = 0; j < njobs; ++j)
+    if (!jobs[j].info || jobs[j].batch <= 0 || !valid_sizes(jobs[j].nblocks, jobs[j].block_sizes) ||
+        (jobs[j].block_sizes_d1 && jobs[j].nlevels <= 0))
+      return NEGF_BAD_ARG;
+  JobPool* pool = nullptr;
+  int rc = pool_for(stream, &pool);
+  if (rc) return rc;
+  hipStream_t s = st(stream);
+  if (hipEventRecord(pool->fork, s) != hipSuccess) return NEGF_HIP_ERROR;
+  int first_err = NEGF_OK;
+  for (int j = 0; j < njobs; ++j) {
+    const NegfBandJob& J = jobs[j];
+    hipStream_t js = pool->s[j];
+    if (hipStreamWaitEvent(js, pool->fork, 0) != hipSuccess) return NEGF_HIP_ERROR;
+    // the method: the given levels ({0}: RGF), else the tune table's choice
+    int32_t lv[4] = {0, 0, 0, 0};
+    int nlv = 0;
+    if (J.block_sizes_d1) {
+      nlv = J.nlevels < 4 ? J.nlevels : 4;
+      for (int i = 0; i < nlv; ++i) lv[i] = J.block_sizes_d1[i];
+    } else {
+      int bmax = 0;
+      for (int i = 0; i < J.nblocks; ++i) bmax = J.block_sizes[i] > bmax ? J.block_sizes[i] : bmax;
+      const int32_t key[3] = {J.nblocks, bmax, J.batch};
+      nlv = negf_tune_lookup(2, key, 3, lv, 4);
+      if (nlv < 0) nlv = 0;
+    }
+    if (nlv > 0 && lv[0] == 0) nlv = 0;
+    const int r = nlv > 0
+        ? negf_ddrgf_blocksizes(J.nblocks, J.block_sizes, J.diag, J.diag_bstride, J.upper,
+                                J.upper_bstride, J.lower, J.lower_bstride, J.g_diag, J.g_upper,
+                                J.g_lower, J.batch, lv, nlv, J.info, (void*)js)
+        : negf_rgf(J.nblocks, J.block_sizes, J.diag, J.diag_bstride, J.upper, J.upper_bstride,
+                   J.lower, J.lower_bstride, J.g_diag, J.g_upper, J.g_lower, J.batch, J.info,
+                   (void*)js);
+    if (r != NEGF_OK && first_err == NEGF_OK) first_err = r;
+    // joined whatever happened, so that the caller's stream never runs ahead of queued work
+    if (hipEventRecord(pool->join[j], js) != hipSuccess ||
+        hipStreamWaitEvent(s, pool->join[j], 0) != hipSuccess)
+      return NEGF_HIP_ERROR;
+  }
+  return first_err;
 }
 
 // ------------------------------------------------------------ selected inverse

@@ -30,6 +30,17 @@ class CCscMatrix(C.Structure):
 
 
 # name -> (restype, argtypes); mirrors include/negf_hip.h
+class NegfBandJob(C.Structure):
+    """include/negf_hip.h NegfBandJob (one device of negf_bndiag_devices)."""
+    _fields_ = [("nblocks", C.c_int32), ("block_sizes", C.POINTER(C.c_int32)),
+                ("diag", C.c_void_p), ("diag_bstride", C.c_int64),
+                ("upper", C.c_void_p), ("upper_bstride", C.c_int64),
+                ("lower", C.c_void_p), ("lower_bstride", C.c_int64),
+                ("g_diag", C.c_void_p), ("g_upper", C.c_void_p), ("g_lower", C.c_void_p),
+                ("batch", C.c_int32), ("block_sizes_d1", C.POINTER(C.c_int32)),
+                ("nlevels", C.c_int32), ("info", C.c_void_p)]
+
+
 SIGNATURES = {
     "run_bndiag_wrapper": (CCscMatrix, [i32, i32, i32, c_int32_p, c_int32_p, c_double_p, i32,
                                         c_int32_p, i32, C.POINTER(C.c_char_p), c_int32_p, i32]),
@@ -91,6 +102,7 @@ SIGNATURES = {
     "negf_ddrgf_blocksizes": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32,
                                         c_int32_p, i32, vp, vp]),
     "negf_ddrgf_blocksizes_workspace": (C.c_int64, [i32, c_int32_p, i32, c_int32_p, i32]),
+    "negf_bndiag_devices": (C.c_int, [i32, vp, vp]),
     "negf_tune_set": (C.c_int, [i32, c_int32_p, i32, c_int32_p, i32]),
     "negf_tune_lookup": (C.c_int, [i32, c_int32_p, i32, c_int32_p, i32]),
     "negf_tune_clear": (C.c_int, []),

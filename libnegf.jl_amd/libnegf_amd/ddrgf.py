@@ -20,7 +20,8 @@ from ._lib import call, SingularError
 from .btd import DeviceBlockTridiag
 
 __all__ = ["bndiag_of_inv_ddrgf_get_nr_tasks", "levels_for_tasks", "AuxDataDDRGF",
-           "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf", "bndiag_of_inv_ddrgf_error_inv_of_T11"]
+           "allocate_aux_data_DDRGF", "bndiag_of_inv_ddrgf", "bndiag_of_inv_ddrgf_error_inv_of_T11",
+           "bndiag_devices"]
 
 
 def bndiag_of_inv_ddrgf_get_nr_tasks(npl: int, blockSizeD1: int, blockSizeD2: int = 1):
@@ -180,3 +181,37 @@ def bndiag_of_inv_ddrgf_error_inv_of_T11(Min: DeviceBlockTridiag, aux: AuxDataDD
     if bool((num != num).any()):
         raise SingularError("bndiag_of_inv_ddrgf_error_inv_of_T11", [])
     return torch.sqrt(num / den).cpu().tolist()
+
+
+def bndiag_devices(jobs, check: bool = True):
+    """Several devices in one call (negf_bndiag_devices): ``jobs`` is a list
+    of (Mout, Min, levels) with levels a blockSizeD1 list, [0] for RGF or
+    None for the tune table's choice.  The devices run concurrently on the
+    library's streams, joined back into the current stream.  Returns the
+    per-device info arrays."""
+    from ._lib import NegfBandJob
+    if len(jobs) > 8:
+        raise ValueError("at most 8 devices per call")
+    arr = (NegfBandJob * len(jobs))()
+    keep, infos = [], []
+    p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    for k, (Mout, Min, levels) in enumerate(jobs):
+        L = Min.layout
+        info = torch.zeros(Min.batch, dtype=torch.int32, device=Min.diag.device)
+        sizes = L.sizes_c()
+        lv = None if levels is None else (C.c_int32 * len(levels))(*levels)
+        keep += [sizes, lv]
+        infos.append(info)
+        arr[k] = NegfBandJob(L.N, C.cast(sizes, C.POINTER(C.c_int32)), p(Min.diag), L.nd,
+                             p(Min.upper), max(L.no, 1), p(Min.lower), max(L.no, 1),
+                             p(Mout.diag), p(Mout.upper), p(Mout.lower), Min.batch,
+                             C.cast(lv, C.POINTER(C.c_int32)) if lv is not None else None,
+                             len(levels) if levels is not None else 0, p(info))
+    call("negf_bndiag_devices", len(jobs), C.cast(arr, C.c_void_p),
+         C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if check:
+        for info in infos:
+            bad = info.cpu()
+            if bool((bad != 0).any()):
+                raise SingularError("bndiag_devices", bad.tolist())
+    return infos

@@ -123,6 +123,33 @@ def test_error_inv_of_T11_diagnostic():
         assert 0 < g < 1e-12 and 0.01 * r < g < 100 * r
 
 
+def test_bndiag_devices_one_call_equals_separate_calls():
+    """Three devices of different block size and count (uneven blocks in one
+    of them) in one negf_bndiag_devices call: DD-RGF with given levels, RGF,
+    and the tune table's choice; each equals the same method called alone
+    bit for bit (the same kernels, on the library's streams) and the
+    oracle."""
+    print("bndiag_devices three devices", flush=True)
+    devs = [bm_create_synthetic_random(20, 24, seed=1), bm_create_synthetic_random(9, 40, seed=2,
+                                                                                  uneven=False),
+            bm_create_synthetic_random(13, 16, seed=3)]
+    devs = [[A, _shifted(A, 0.3)] for A in devs]
+    levels = [[4], [0], None]
+    Mins = [DeviceBlockTridiag.from_host(m) for m in devs]
+    outs = [M.similar() for M in Mins]
+    dd.bndiag_devices([(o, m, lv) for o, m, lv in zip(outs, Mins, levels)])
+    torch.cuda.synchronize()
+    for k, (M, out, lv) in enumerate(zip(Mins, outs, levels)):
+        alone = M.similar()
+        aux = dd.allocate_aux_data_DDRGF(M, levels=lv) if lv is not None else \
+            dd.allocate_aux_data_DDRGF(M)
+        dd.bndiag_of_inv_ddrgf(alone, M, aux)
+        for name in ("diag", "upper", "lower"):
+            assert torch.equal(getattr(out, name), getattr(alone, name)), (k, name)
+        for e, A in enumerate(devs[k]):
+            assert max_block_err(_band(out.to_host(e)), bndiag_of_inv_rgf_local(A)) < 1e-11
+
+
 @pytest.fixture
 def clean_tune_table():
     """The tuner writes into the library's table; restore the loaded table
