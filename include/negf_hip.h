@@ -249,6 +249,29 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes,
                  double* transmission,
                  int32_t batch, int32_t* info, void* stream);
 
+/* negf_rgf_rkd with the block products in a reduced arithmetic: precision
+ * 1 = FP32 MFMA (FP32 accumulation), 2 = FP64 emulated on the int8 matrix
+ * cores with n_moduli moduli (0: 13), 0 = negf_rgf_rkd itself; the diagonal
+ * inverses stay FP64.  Per instance the estimate est = 8 u sum_i ||g_i||_F
+ * (u the products' unit roundoff, g_i the upward pass's inverses) of
+ * negf_rgf_keldysh_energies_mixed; instances with est > tol (or not finite)
+ * are computed again in FP64 and flagged in fallback.  est_host /
+ * fallback_host: host arrays of batch entries, or NULL; n_fallback: host.
+ * The mixed-precision variant of the bndiag_of_inv_rgf_local! /
+ * keldyshndiag! replacement (src/matinvertndiag_rgf.jl:114-200,
+ * src/keldyshndiag.jl:120-131), with the reduced precision the reference's
+ * tests name (test/common_to_test.jl:20-26). */
+int negf_rgf_rkd_mixed(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                       int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                       const void* lower, int64_t lower_bstride, const void* sigma_diag,
+                       int64_t sigma_diag_bstride, const void* sigma_upper,
+                       int64_t sigma_upper_bstride, void* g_diag, void* g_upper, void* g_lower,
+                       void* gn_diag, void* gn_upper, const void* gamma_L,
+                       int64_t gamma_L_bstride, const void* gamma_R, int64_t gamma_R_bstride,
+                       double* transmission, int32_t batch, int32_t* info, int32_t precision,
+                       int32_t n_moduli, double tol, double* est_host, int32_t* fallback_host,
+                       int32_t* n_fallback, void* stream);
+
 /* Energy-batched NEGF workload on one device Hamiltonian (no reference
  * counterpart; the reference runs one energy per call): uniform blocks of
  * size b, H_diag (N blocks, Hermitian), H_upper (N-1 blocks H_{i,i+1}),

@@ -381,6 +381,95 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes, const void* diag,
                    want_T ? &gam : nullptr, transmission, batch, inf, w, st(stream));
 }
 
+int negf_rgf_rkd_mixed(int32_t nblocks, const int32_t* block_sizes, const void* diag,
+                       int64_t diag_bstride, const void* upper, int64_t upper_bstride,
+                       const void* lower, int64_t lower_bstride, const void* sigma_diag,
+                       int64_t sigma_diag_bstride, const void* sigma_upper,
+                       int64_t sigma_upper_bstride, void* g_diag, void* g_upper, void* g_lower,
+                       void* gn_diag, void* gn_upper, const void* gamma_L,
+                       int64_t gamma_L_bstride, const void* gamma_R, int64_t gamma_R_bstride,
+                       double* transmission, int32_t batch, int32_t* info, int32_t precision,
+                       int32_t n_moduli, double tol, double* est_host, int32_t* fallback_host,
+                       int32_t* n_fallback, void* stream) {
+  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
+      !(tol >= 0.0))
+    return NEGF_BAD_ARG;
+  if (n_fallback) *n_fallback = 0;
+  if (precision == 0 || batch <= 0) {
+    for (int e = 0; e < batch; ++e) {
+      if (est_host) est_host[e] = 0.0;
+      if (fallback_host) fallback_host[e] = 0;
+    }
+    return negf_rgf_rkd(nblocks, block_sizes, diag, diag_bstride, upper, upper_bstride, lower,
+                        lower_bstride, sigma_diag, sigma_diag_bstride, sigma_upper,
+                        sigma_upper_bstride, g_diag, g_upper, g_lower, gn_diag, gn_upper, gamma_L,
+                        gamma_L_bstride, gamma_R, gamma_R_bstride, transmission, batch, info,
+                        stream);
+  }
+  if (!valid_sizes(nblocks, block_sizes)) return NEGF_BAD_ARG;
+  if (!diag || (nblocks > 1 && !upper)) return NEGF_BAD_ARG;
+  if ((g_upper || g_lower) && !g_diag) return NEGF_BAD_ARG;
+  const bool want_rkd = gn_diag != nullptr;
+  if (want_rkd && (!sigma_diag || (nblocks > 1 && (!sigma_upper || !gn_upper))))
+    return NEGF_BAD_ARG;
+  const bool want_T = transmission != nullptr;
+  if (want_T && (!gamma_L || !gamma_R)) return NEGF_BAD_ARG;
+  Layout L(nblocks, block_sizes);
+  const size_t sw = (sweep_work_bytes(L, batch, want_rkd) + 255) / 256 * 256;
+  const size_t sx = (sweep_mixed_extra_bytes(L, batch, precision, n_moduli) + 255) / 256 * 256;
+  void* w = nullptr;
+  int rc = workspace(stream, sw + sx + sizeof(double) * (size_t)batch +
+                                 sizeof(int32_t) * (size_t)batch + 512, &w);
+  if (rc) return rc;
+  double* est_dev = (double*)((char*)w + sw + sx);
+  int32_t* inf = info ? info : (int32_t*)(est_dev + batch);
+  hipStream_t s = st(stream);
+  BtdIn A{(const zc*)diag, diag_bstride, (const zc*)upper, upper_bstride, (const zc*)lower,
+          lower_bstride};
+  BtdOut G{(zc*)g_diag, (zc*)g_upper, (zc*)g_lower};
+  BtdHerm S{(const zc*)sigma_diag, sigma_diag_bstride, (const zc*)sigma_upper,
+            sigma_upper_bstride};
+  BtdOut Gn{(zc*)gn_diag, (zc*)gn_upper, nullptr};
+  GammaIn gam{(const zc*)gamma_L, gamma_L_bstride, (const zc*)gamma_R, gamma_R_bstride};
+  // 1. every instance with reduced-precision products
+  rc = sweep_rgf_mixed(L, A, G, want_rkd ? &S : nullptr, want_rkd ? &Gn : nullptr,
+                       want_T ? &gam : nullptr, transmission, batch, inf, w, precision, n_moduli,
+                       est_dev, s);
+  if (rc) return rc;
+  std::vector<double> est(batch, 0.0);
+  if (hipMemcpyAsync(est.data(), est_dev, sizeof(double) * batch, hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return NEGF_HIP_ERROR;
+  // 2. instances whose estimate exceeds tol (or is not finite) again in FP64, one at a time
+  int nfb = 0;
+  for (int e = 0; e < batch; ++e) {
+    const bool fb = !(est[e] <= tol);
+    if (est_host) est_host[e] = est[e];
+    if (fallback_host) fallback_host[e] = fb ? 1 : 0;
+    if (!fb) continue;
+    ++nfb;
+    auto at = [&](const void* p, int64_t bs) -> const zc* {
+      return p ? (const zc*)p + (long long)e * bs : nullptr;
+    };
+    auto out = [&](void* p, long long bs) -> zc* { return p ? (zc*)p + (long long)e * bs : nullptr; };
+    BtdIn Ae{at(diag, diag_bstride), diag_bstride, at(upper, upper_bstride), upper_bstride,
+             at(lower, lower_bstride), lower_bstride};
+    BtdOut Ge{out(g_diag, L.nd), out(g_upper, L.no), out(g_lower, L.no)};
+    BtdHerm Se{at(sigma_diag, sigma_diag_bstride), sigma_diag_bstride,
+               at(sigma_upper, sigma_upper_bstride), sigma_upper_bstride};
+    BtdOut Gne{out(gn_diag, L.nd), out(gn_upper, L.no), nullptr};
+    GammaIn game{at(gamma_L, gamma_L_bstride), gamma_L_bstride, at(gamma_R, gamma_R_bstride),
+                 gamma_R_bstride};
+    rc = sweep_rgf(L, Ae, Ge, want_rkd ? &Se : nullptr, want_rkd ? &Gne : nullptr,
+                   want_T ? &game : nullptr, want_T ? transmission + e : nullptr, 1, inf + e, w, s,
+                   false);
+    if (rc) return rc;
+  }
+  if (n_fallback) *n_fallback = nfb;
+  return NEGF_OK;
+}
+
 int negf_rgf(int32_t nblocks, const int32_t* block_sizes, const void* diag, int64_t diag_bstride,
              const void* upper, int64_t upper_bstride, const void* lower, int64_t lower_bstride,
              void* g_diag, void* g_upper, void* g_lower, int32_t batch, int32_t* info,

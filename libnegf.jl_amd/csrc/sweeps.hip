@@ -145,6 +145,24 @@ __global__ __launch_bounds__(256) void block_fro_kernel(const zc* g, long long g
   if (threadIdx.x == 0) nrm[(long long)e * N + i] = sqrt(red[0]);
 }
 
+// nrm[e * N + i] = ||g_i||_F for one layer i of any size (one workgroup per instance).
+__global__ __launch_bounds__(256) void layer_fro_kernel(const zc* g, long long gbs, int bi, int i,
+                                                         int N, double* nrm) {
+  const int e = blockIdx.x;
+  const zc* x = g + (long long)e * gbs;
+  const long long n = (long long)bi * bi;
+  double acc = 0.0;
+  for (long long k = threadIdx.x; k < n; k += 256) acc = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, acc));
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nrm[(long long)e * N + i] = sqrt(red[0]);
+}
+
 // est[e] = c sum_i nrm[e * N + i]
 __global__ void energy_estimate_kernel(const double* nrm, int N, int nb, double c, double* est) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,6 +592,57 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
     RC(rkd(L, w, batch, gn->diag, L.nd, gn->up, L.no, s));
   }
   return Status::OK;
+}
+
+// sweep_rgf with its products in a reduced arithmetic (prec 1: FP32 MFMA,
+// 2: FP64 emulated on the int8 matrix cores with nmod moduli), the diagonal
+// inverses in FP64, and per instance the estimate est[e] = 8 u sum_i
+// ||g_i||_F of the energy job (u the products' unit roundoff) computed right
+// after the upward pass.  work: sweep_work_bytes + sweep_mixed_extra_bytes.
+size_t sweep_mixed_extra_bytes(const Layout& L, int batch, int prec, int nmod) {
+  size_t b = sizeof(double) * (size_t)batch * L.N + 256;
+  if (prec == 2) b += (zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, batch, batch, nmod) + 255) / 256 * 256;
+  return b;
+}
+
+int sweep_rgf_mixed(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
+                    const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info,
+                    void* work, int prec, int nmod, double* est, hipStream_t s) {
+  const size_t main = (sweep_work_bytes(L, batch, sigma && gn) + 255) / 256 * 256;
+  char* extra = (char*)work + main;
+  double* nrm = (double*)extra;
+  char* emu = extra + (sizeof(double) * (size_t)batch * L.N + 255) / 256 * 256;
+  const size_t emu_bytes =
+      prec == 2 ? zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, batch, batch, nmod) : 0;
+  PrecScope scope(PrecCtx{prec, nmod, prec == 2 ? (void*)emu : nullptr, (long long)emu_bytes});
+  SweepWork w;
+  carve(work, L, batch, &w);
+  RC(set_one(w.ones, s));
+  Source S{};
+  S.diag = A.diag; S.diag_bs = A.diag_bs;
+  S.up = A.up; S.up_bs = A.up_bs;
+  S.low = A.low; S.low_bs = A.low_bs;
+  S.sLow = ONE; S.sUp = ONE;
+  RC(upward(L, S, w, batch, info, s));
+  if (est) {
+    for (int i = 0; i < L.N; ++i)
+      hipLaunchKernelGGL(layer_fro_kernel, dim3(batch), dim3(256), 0, s, w.g + L.od[i], L.nd,
+                         L.b[i], i, L.N, nrm);
+    hipLaunchKernelGGL(energy_estimate_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, nrm, L.N,
+                       batch, 8.0 * energy_prec_unit(prec, nmod, L.bmax), est);
+  }
+  if (G.diag) RC(downward(L, w, batch, G.diag, L.nd, G.up, G.low, L.no, s));
+  if (gam && T)
+    RC(transmission(L, w, batch, M{gam->gammaL, L.b[0], gam->gammaL_bs},
+                    M{gam->gammaR, L.b[L.N - 1], gam->gammaR_bs}, T, s));
+  if (sigma && gn) {
+    RC(zcopy2d_launch((int)L.nd, 1, sigma->diag, L.nd, sigma->diag_bs, gn->diag, L.nd, L.nd, batch,
+                      s));
+    if (L.N > 1)
+      RC(zcopy2d_launch((int)L.no, 1, sigma->up, L.no, sigma->up_bs, gn->up, L.no, L.no, batch, s));
+    RC(rkd(L, w, batch, gn->diag, L.nd, gn->up, L.no, s));
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
 // Every block of A^{-1} (the reference's buildFullInv, src/matinvertndiag_rgf.jl:

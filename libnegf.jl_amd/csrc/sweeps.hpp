@@ -63,6 +63,10 @@ struct EnergyPrec {
 double energy_prec_unit(int prec, int nmod, int k);
 
 size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh);
+size_t sweep_mixed_extra_bytes(const Layout& L, int batch, int prec, int nmod);
+int sweep_rgf_mixed(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
+                    const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info,
+                    void* work, int prec, int nmod, double* est, hipStream_t s);
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams,
                            const EnergyPrec* pr = nullptr);
 

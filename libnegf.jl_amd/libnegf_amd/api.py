@@ -28,7 +28,7 @@ __all__ = [
     "rgf_keldysh_energies", "GpuLU", "be_gemm", "be_lu", "be_mldivide", "be_getrf", "be_getri",
     "be_trsm", "be_ctranspose", "be_copy_in_hw", "be_fill", "be_identity", "be_zero_array",
     "be_inv", "col_major", "release_workspace", "build_M_from_HS", "EnergiesGraph",
-    "bndiag_of_inv_direct",
+    "bndiag_of_inv_direct", "bndiag_of_inv_rgf_local_mixed",
 ]
 
 
@@ -114,6 +114,42 @@ def bndiag_of_inv_rgf_local(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag,
          _ptr(Mout.diag), _ptr(Mout.upper), _ptr(Mout.lower), Min.batch, _ptr(info), _stream())
     _check_info("bndiag_of_inv_rgf_local", info)
     return Mout
+
+
+def bndiag_of_inv_rgf_local_mixed(Mout: DeviceBlockTridiag, Min: DeviceBlockTridiag,
+                                  precision: str = "emulated", tol: float = 1e-8,
+                                  n_moduli: int = 0, S: DeviceBlockTridiag | None = None,
+                                  Gn: DeviceBlockTridiag | None = None):
+    """``bndiag_of_inv_rgf_local`` (and, with S and Gn, ``keldyshndiag``'s
+    band of Min^{-1} S Min^{-H} in one pass) with the block products in FP32
+    ("fp32") or FP64 emulated on the int8 matrix cores ("emulated"); the
+    diagonal inverses stay FP64.  Instances whose error estimate exceeds
+    ``tol`` are computed again in FP64.  Returns (est, fallback) per
+    instance (numpy)."""
+    _require_gpu()
+    if Mout.layout.b != Min.layout.b or Mout.batch != Min.batch or Mout.lower is None:
+        raise ValueError("Mout must be a non-Hermitian BlockMatrix shaped like Min")
+    L = Min.layout
+    info = torch.zeros(Min.batch, dtype=torch.int32, device=Min.diag.device)
+    est = np.zeros(Min.batch, dtype=np.float64)
+    fb = np.zeros(Min.batch, dtype=np.int32)
+    nfb = C.c_int32(0)
+    if (S is None) != (Gn is None):
+        raise ValueError("S and Gn go together")
+    if S is not None and not S.hermitian:
+        raise ValueError("the central operator S must use Hermitian storage")
+    call("negf_rgf_rkd_mixed", L.N, L.sizes_c(), _ptr(Min.diag), L.nd, _ptr(Min.upper),
+         max(L.no, 1), _ptr(Min.lower) if Min.lower is not None else None, max(L.no, 1),
+         _ptr(S.diag) if S is not None else None, (L.nd if S is not None and S.batch > 1 else 0),
+         _ptr(S.upper) if S is not None else None,
+         (max(L.no, 1) if S is not None and S.batch > 1 else 0),
+         _ptr(Mout.diag), _ptr(Mout.upper), _ptr(Mout.lower),
+         _ptr(Gn.diag) if Gn is not None else None, _ptr(Gn.upper) if Gn is not None else None,
+         None, 0, None, 0, None, Min.batch, _ptr(info), PRECISIONS[precision], int(n_moduli),
+         float(tol), est.ctypes.data_as(C.c_void_p), fb.ctypes.data_as(C.c_void_p),
+         C.byref(nfb), _stream())
+    _check_info("bndiag_of_inv_rgf_local_mixed", info)
+    return est, fb
 
 
 def _full_storage(M: DeviceBlockTridiag) -> DeviceBlockTridiag:
