@@ -274,3 +274,34 @@ def test_tuner_candidates_always_include_rgf_and_split():
                 nr, _ = nr_tasks(n, bs)
                 assert nr >= 2, (npl, lv)
                 n = nr
+
+
+def test_new_entry_points_refuse_bad_arguments_without_a_gpu():
+    """Argument checks of the multi-device, mixed-precision and uneven DD-RGF
+    entries run on the host before any device call."""
+    from libnegf_amd._lib import LIB_PATH, load, NegfBandJob
+    if not LIB_PATH.exists():
+        pytest.skip("libnegf_hip.so not built")
+    lib = load()
+    BAD = -1
+    assert lib.negf_bndiag_devices(0, None, None) == 0
+    assert lib.negf_bndiag_devices(9, ctypes.cast((NegfBandJob * 9)(), ctypes.c_void_p), None) == BAD
+    assert lib.negf_bndiag_devices(-1, None, None) == BAD
+    job = (NegfBandJob * 1)()          # no info array, no sizes
+    assert lib.negf_bndiag_devices(1, ctypes.cast(job, ctypes.c_void_p), None) == BAD
+    sizes = (ctypes.c_int32 * 3)(8, 8, 8)
+    for prec, nmod, tol in ((3, 0, 1e-8), (-1, 0, 1e-8), (2, 1, 1e-8), (2, 17, 1e-8),
+                            (2, 0, float("nan")), (2, 0, -1.0)):
+        assert lib.negf_rgf_rkd_mixed(3, sizes, None, 0, None, 0, None, 0, None, 0, None, 0,
+                                      None, None, None, None, None, None, 0, None, 0, None, 1,
+                                      None, prec, nmod, tol, None, None, None, None) == BAD
+    lv = (ctypes.c_int32 * 1)(3)
+    uneven = (ctypes.c_int32 * 12)(*([16, 18] * 6))
+    uniform = (ctypes.c_int32 * 12)(*([18] * 12))
+    wu = lib.negf_ddrgf_blocksizes_workspace(12, uneven, 2, lv, 1)
+    wf = lib.negf_ddrgf_blocksizes_workspace(12, uniform, 2, lv, 1)
+    assert wu > wf > 0     # the padded copies of A and G come on top of the same sweep
+    # blockSizeD1 = 10 leaves an empty last task (utils.jl:278-312): refused
+    assert lib.negf_ddrgf_blocksizes_workspace(12, uneven, 2, (ctypes.c_int32 * 1)(10), 1) == 0
+    big = (ctypes.c_int32 * 1025)(*([4, 5] * 512 + [4]))
+    assert lib.negf_ddrgf_blocksizes_workspace(1025, big, 1, lv, 1) == 0   # > 1024 uneven blocks
