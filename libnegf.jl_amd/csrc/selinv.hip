@@ -79,34 +79,57 @@ __global__ void zero_kernel(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// Lp = Li P for instance blockIdx.x: LAPACK's swaps (1-based ipiv, applied in
-// order) give the row order pi of P A (row r of P A is row pi[r] of A), so
-// (Li P)[:, c] = Li[:, pi^{-1}[c]].  b <= 2048.
-__global__ __launch_bounds__(256) void times_perm_kernel(int b, const zc* Li, long long sLi,
-                                                          zc* Lp, long long sLp, const int* ipiv,
-                                                          long long sP) {
-  __shared__ int pi[2048], pinv[2048];
+// Li P: LAPACK's swaps (1-based ipiv, applied in order) give the row order
+// pi of P A (row r of P A is row pi[r] of A), so (Li P)[:, c] = Li[:, pinv[c]]
+// with pinv = pi^{-1}.  First pinv per instance (one workgroup each: the
+// swaps run in LDS by one thread, position k being final after step k; the
+// inversion by all), then the column gather over (8-column chunk, instance).
+// b <= 2048.
+__global__ __launch_bounds__(256) void pivot_perm_kernel(int b, const int* ipiv, long long sP,
+                                                          int* pinv, long long sI) {
+  __shared__ int pi[2048], ip[2048];
   const int e = blockIdx.x;
-  const int* ip = ipiv + (long long)e * sP;
+  for (int k = threadIdx.x; k < b; k += blockDim.x) {
+    pi[k] = k;
+    ip[k] = ipiv[(long long)e * sP + k] - 1;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 0; k < b; ++k) pi[k] = k;
     for (int k = 0; k < b; ++k) {
-      const int q = ip[k] - 1;
+      const int q = ip[k];
       const int t = pi[k];
       pi[k] = pi[q];
       pi[q] = t;
     }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < b; r += blockDim.x) pinv[pi[r]] = r;
-  __syncthreads();
+  for (int r = threadIdx.x; r < b; r += blockDim.x) pinv[(long long)e * sI + pi[r]] = r;
+}
+
+__global__ __launch_bounds__(256) void gather_cols_kernel(int b, const zc* Li, long long sLi,
+                                                           zc* Lp, long long sLp, const int* pinv,
+                                                           long long sI) {
+  const int e = blockIdx.y;
   const zc* src = Li + (long long)e * sLi;
   zc* dst = Lp + (long long)e * sLp;
-  const long long n = (long long)b * b;
-  for (long long k = threadIdx.x; k < n; k += blockDim.x) {
-    const int c = (int)(k / b), r = (int)(k - (long long)c * b);
-    dst[k] = src[r + (long long)pinv[c] * b];
+  const int* pv = pinv + (long long)e * sI;
+  for (int cc = 0; cc < 8; ++cc) {
+    const int c = blockIdx.x * 8 + cc;
+    if (c >= b) break;
+    const zc* col = src + (long long)pv[c] * b;
+    for (int r = threadIdx.x; r < b; r += blockDim.x) dst[r + (long long)c * b] = col[r];
   }
+}
+
+// Lp = Li P for every instance; pinv: b ints per instance of scratch.
+int times_perm(int b, const zc* Li, long long sLi, zc* Lp, long long sLp, const int* ipiv,
+               long long sP, int* pinv, int batch, hipStream_t s) {
+  if (batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(pivot_perm_kernel, dim3(batch), dim3(256), 0, s, b, ipiv, sP, pinv,
+                     (long long)b);
+  hipLaunchKernelGGL(gather_cols_kernel, dim3((b + 7) / 8, batch), dim3(256), 0, s, b, Li, sLi,
+                     Lp, sLp, pinv, (long long)b);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
 }  // namespace
@@ -210,8 +233,8 @@ int block_factorization(int npl, int b, int nnzb, const int* brow, const int* bc
     if (any) {
       RC(tri_inverses(Aii, ld, stride, b, batch, Ui, Li, twork, s));
       if (mode == 2) {
-        hipLaunchKernelGGL(times_perm_kernel, dim3(batch), dim3(256), 0, s, b, Li, tb, Lp, tb, piv,
-                           sP);
+        // the pivots' inverse permutation in the internal pivot array (unused in mode 2)
+        RC(times_perm(b, Li, tb, Lp, tb, piv, sP, ipiv, batch, s));
         Lu = Lp;
       }
     }
@@ -274,8 +297,11 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
   // below is (P_i L_ii)^{-1} = L_ii^{-1} P
   auto lower_inverse = [&](int i) -> zc* {
     if (!ipiv) return Li;
-    hipLaunchKernelGGL(times_perm_kernel, dim3(batch), dim3(256), 0, s, b, Li, tb, Lp, tb,
-                       ipiv + (size_t)i * b, (long long)npl * b);
+    // the permutation's scratch in the triangular inverses' workspace, free
+    // between tri_inverses calls (stream order)
+    if (times_perm(b, Li, tb, Lp, tb, ipiv + (size_t)i * b, (long long)npl * b, (int*)twork,
+                   batch, s))
+      return nullptr;
     return Lp;
   };
   // temporaries use ld = b; the pattern's blocks use ld = P.ld
@@ -299,12 +325,14 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
     zc* Ann = P.at(npl - 1, npl - 1);
     RC(tri_inverses(Ann, ld, bs, b, batch, Ui, Li, twork, s));
     zc* Lr = lower_inverse(npl - 1);
+    if (!Lr) return Status::HIP_ERROR;
     RC(gemm_mixed('N', 'N', ONE, Ui, b, tbs, Lr, b, tbs, ZERO, Ann, ld, bs));
   }
   for (int i = npl - 2; i >= 0; --i) {
     zc* Aii = P.at(i, i);
     RC(tri_inverses(Aii, ld, bs, b, batch, Ui, Li, twork, s));
     zc* Lr = lower_inverse(i);
+    if (!Lr) return Status::HIP_ERROR;
     std::vector<int> js;
     for (int j = i + 1; j < npl; ++j) {
       if (!P.has(i, j)) continue;

@@ -172,6 +172,24 @@ __global__ void laswp_kernel(int ncols, zc* __restrict__ B, long long ldb, long 
   }
 }
 
+// The interchanges of rows k1..k2-1 on every column outside [skip, skip +
+// nskip): one launch for the columns left and right of a panel.
+__global__ void laswp_outside_kernel(int ncols, int skip, int nskip, zc* __restrict__ A,
+                                     long long lda, long long strideA,
+                                     const int* __restrict__ ipiv, long long strideP, int k1,
+                                     int k2) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int inst = blockIdx.y;
+  if (col >= ncols - nskip) return;
+  if (col >= skip) col += nskip;
+  zc* Ac = A + inst * strideA + (long long)col * lda;
+  const int* ip = ipiv + inst * strideP;
+  for (int i = k1; i < k2; ++i) {
+    const int p = ip[i] - 1;
+    if (p != i) { zc t = Ac[i]; Ac[i] = Ac[p]; Ac[p] = t; }
+  }
+}
+
 __global__ void zero_info_kernel(int* info, int batch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < batch) info[i] = 0;
@@ -315,17 +333,13 @@ int getrf_run(int n, zc* A, long long lda, long long strideA, int* ipiv, long lo
     const int nbe = (n - k0) < NB ? (n - k0) : NB;
     hipLaunchKernelGGL((lu_panel_kernel<RPT, NB>), dim3(batch), dim3(PT), 0, s, n, k0, A, lda,
                        strideA, ipiv, strideP, info, pivot);
-    // interchanges on the columns left and right of the panel
-    if (k0 > 0 && pivot)
-      hipLaunchKernelGGL(laswp_kernel, dim3((k0 + 255) / 256, batch), dim3(256), 0, s, k0, A, lda,
-                         strideA, ipiv, strideP, k0, k0 + nbe, 1);
+    // interchanges on the columns left and right of the panel, one launch
+    if (pivot && n > nbe)
+      hipLaunchKernelGGL(laswp_outside_kernel, dim3((n - nbe + 255) / 256, batch), dim3(256), 0, s,
+                         n, k0, nbe, A, lda, strideA, ipiv, strideP, k0, k0 + nbe);
     const int rest = n - k0 - nbe;
     if (rest > 0) {
       zc* A12 = A + k0 + (long long)(k0 + nbe) * lda;
-      if (pivot)
-        hipLaunchKernelGGL(laswp_kernel, dim3((rest + 255) / 256, batch), dim3(256), 0, s, rest,
-                           A + (long long)(k0 + nbe) * lda, lda, strideA, ipiv, strideP, k0,
-                           k0 + nbe, 1);
       int rc = ztrsm_launch('L', 'L', 'N', 'U', nbe, rest, zmake(1.0, 0.0),
                             A + k0 + (long long)k0 * lda, lda, strideA, A12, lda, strideA, batch,
                             s);
