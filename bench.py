@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--inv-only", action="store_true",
                     help="only batched Gauss-Jordan inverses (--gemm-reps of them, b x b x wave) plus "
                          "the copy of known size (counter calibration for the inverse's HBM traffic)")
+    ap.add_argument("--gemm-prec", action="store_true",
+                    help="only the batched product in FP64, FP32 and int8-emulated FP64 "
+                         "(b x b x b x wave, --gemm-reps calls each)")
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant-kernel timing plus a plain copy of known size "
                          "(the counter-calibration run for rocprofv3 --pmc)")
@@ -119,6 +122,41 @@ def gemm_roofline(torch, api, b, batch, reps):
             "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "launch": f"{b}x{b}x{b} complex128 x {batch}", "avg_launch_us": round(dt * 1e6, 1),
             "flops_per_launch": flops}
+
+
+def gemm_precisions(torch, b, batch, reps):
+    """The batched complex product b x b x b x `batch` (both operands per
+    instance, and op(A) shared) in each arithmetic of
+    negf_zgemm_prec_strided_batched: FP64 MFMA, FP32 MFMA, FP64 emulated on
+    the int8 matrix cores (13 moduli); average per call over `reps` calls
+    after 10 untimed ones, HIP events on the launching stream."""
+    from libnegf_amd._lib import call, cplx
+    s = torch.cuda.current_stream()
+    A = torch.randn((batch, b * b), dtype=torch.complex128, device="cuda")
+    B = torch.randn((batch, b * b), dtype=torch.complex128, device="cuda")
+    D = torch.empty((batch, b * b), dtype=torch.complex128, device="cuda")
+    out = {}
+    for name, prec, nmod, share in (("fp64", 0, 0, False), ("fp32", 1, 0, False),
+                                    ("emulated_13", 2, 13, False),
+                                    ("emulated_13_shared_A", 2, 13, True)):
+        def run():
+            call("negf_zgemm_prec_strided_batched", b"N", b"N", b, b, b, cplx(1),
+                 C.c_void_p(A.data_ptr()), b, 0 if share else b * b, C.c_void_p(B.data_ptr()), b,
+                 b * b, cplx(0), C.c_void_p(D.data_ptr()), b, b * b, batch, prec, nmod,
+                 C.c_void_p(s.cuda_stream))
+        for _ in range(10):
+            run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            run()
+        e1.record(s)
+        e1.synchronize()
+        dt = e0.elapsed_time(e1) / 1e3 / reps
+        out[name] = {"us_per_call": round(dt * 1e6, 1),
+                     "fp64_equivalent_tflops": round(8.0 * b ** 3 * batch / dt / 1e12, 2)}
+        print(f"# {name}: {dt * 1e6:.1f} us", file=sys.stderr, flush=True)
+    return out
 
 
 def host_description():
@@ -766,6 +804,11 @@ def main():
         return
     cfg = synthetic.CONFIGS["rgf_keldysh_batched"]
     N, b, nE = a.N, a.b, a.energies
+    if a.gemm_prec:
+        w = a.wave or 96
+        print(json.dumps({"gemm_prec": True, "b": b, "batch": w,
+                          "forms": gemm_precisions(torch, b, w, a.gemm_reps)}))
+        return
     if a.gemm_only:
         w = a.wave or 64
         nbytes = copy_calibration(torch)

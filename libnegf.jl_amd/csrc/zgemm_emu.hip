@@ -6,7 +6,7 @@
 //   D[idx] = alpha * op(A[idx]) op(B[idx]) + beta * C[idx] (+ shift I), all
 //   operands complex128 in memory, as zgemm_launch with GemmArgs::prec = 2.
 //
-// 1. Scaling (emu_prep_kernel, once per operand).  Row i of op(A) is scaled
+// 1. Scaling (emu_rowmax_kernel and emu_resid_kernel, once per operand).  Row i of op(A) is scaled
 //    by 2^eA_i so that its largest |re| or |im| lies in [2^(beta-1), 2^beta),
 //    column j of op(B) likewise by 2^eB_j, and rounded to integers a, b
 //    (|a|, |b| <= 2^beta).  This rounding is the only error of the product:
@@ -103,103 +103,125 @@ __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
          ((unsigned)(d & 0xff) << 24);
 }
 
-// One workgroup: 64 rows of one instance, K in chunks of PKC = 64 complex
-// (128 bytes of k': one stage of the product kernel).  Each chunk is loaded
-// coalesced for the operand's layout into an LDS tile [row][k] (rows
-// contiguous in memory: a wave reads 64 rows of one k; k contiguous: a wave
-// reads 64 k of one row), then thread t works on row t / 4, complex k
-// 16 (t % 4) .. + 15 of the chunk.  Pass 1 finds each row's largest |re|,
-// |im|; pass 2 reloads the chunks and writes the residues in the k'-tiled
-// layout [stage][row][128 bytes]: 32 bytes per thread, form and modulus, a
-// wave's 16 rows one contiguous 2 KB.  Rows >= rows and k >= K are written
-// as zeros up to Rp, Kp.
+// Scaling and residues in two kernels.  emu_rowmax_kernel: one workgroup
+// per 64 rows of one instance finds each row's largest |re|, |im| (loads
+// coalesced for the operand's layout) and stores its scale exponent (kDead
+// for rows below 2^-900, whose scale would overflow: they count as zero).
+// emu_resid_kernel: one workgroup per (64 rows, 64 complex k = one 128-byte
+// stage of the product kernel, instance) loads its chunk coalesced into an
+// LDS tile [row][k], then thread t works on row t / 4, complex k 16 (t % 4)
+// .. + 15 and writes its residues in the k'-tiled layout [stage][row][128
+// bytes]: 32 bytes per thread, form and modulus, a wave's 16 rows one
+// contiguous 2 KB.  Rows >= rows and k >= K are written as zeros up to Rp,
+// Kp.  Each element is read twice from memory (once per kernel); the
+// residues are the work (per element and modulus: a multiply, a rounding,
+// a fused multiply-add and a conversion).
 constexpr int PKC = 64;
+constexpr int kDead = -(1 << 20);
 
-__global__ __launch_bounds__(256, 2) void emu_prep_kernel(const PrepArgs a, const EmuConsts c) {
-  __shared__ zc T[64][PKC + 1];
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(256) void emu_rowmax_kernel(const PrepArgs a, const EmuConsts c) {
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int inst = blockIdx.y;
   const int r0 = blockIdx.x * 64;
   const long long i_in = a.inner > 0 ? inst % a.inner : inst;
   const long long i_out = a.inner > 0 ? inst / a.inner : 0;
   const zc* X = a.X + i_in * a.stride + i_out * a.outer;
-  const int nkt = a.P.Kp / 128;   // stages of 64 complex
-  auto load_chunk = [&](int k0) {
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      int lr, lk;
-      if (a.r_major) {      // X[r + k ld]: lanes along rows
-        lr = t & 63;
-        lk = (t >> 6) + 4 * j;
-      } else {              // X[k + r ld]: lanes along k
-        lk = t & 63;
-        lr = (t >> 6) + 4 * j;
+  if (a.r_major) {
+    // X[r + k ld]: lanes along the 64 rows, the waves split k
+    const int r = r0 + lane;
+    double mx = 0.0;
+    if (r < a.rows)
+      for (int k = wave; k < a.K; k += 4) {
+        const zc v = X[r + (long long)k * a.ld];
+        mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
       }
-      const int r = r0 + lr, k = k0 + lk;
-      zc v = zmake(0.0, 0.0);
-      if (r < a.rows && k < a.K) v = prep_load(a, X, r, k);
-      T[lr][lk] = v;
-    }
-  };
-  const int my_row = t >> 2, my_part = t & 3;
-  double mx = 0.0;
-  for (int k0 = 0; k0 < a.K; k0 += PKC) {
-    __syncthreads();
-    load_chunk(k0);
-    __syncthreads();
-#pragma unroll
+    red[wave][lane] = mx;
+  } else {
+    // X[k + r ld]: lanes along k, each wave 16 rows
     for (int q = 0; q < 16; ++q) {
-      const zc v = T[my_row][16 * my_part + q];
-      mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+      const int lr = 16 * wave + q, r = r0 + lr;
+      double mx = 0.0;
+      if (r < a.rows)
+        for (int k = lane; k < a.K; k += 64) {
+          const zc v = X[k + (long long)r * a.ld];
+          mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+        }
+      mx = wave_max(mx);
+      if (lane == 0) red[0][lr] = mx;
     }
   }
-  mx = fmax(mx, __shfl_xor(mx, 1));
-  mx = fmax(mx, __shfl_xor(mx, 2));
-  // largest |re|, |im| of the row in [2^(beta-1), 2^beta) after scaling;
-  // rows below 2^-900 (whose scale would overflow) count as zero
-  const bool live = mx >= 0x1p-900;
-  const int e = live ? c.beta - (ilogb(mx) + 1) : 0;
-  const int row = r0 + my_row;
-  if (my_part == 0) a.P.expo[(long long)inst * a.P.Rp + row] = e;
-  const double sc = live ? ldexp(1.0, e) : 0.0;
-  const long long plane = (long long)a.P.Rp * a.P.Kp;
-  for (int kt = 0; kt < nkt; ++kt) {
-    const int k0 = kt * PKC;
-    __syncthreads();
-    if (k0 < a.K) load_chunk(k0);
-    __syncthreads();
-    signed char* base0 = a.P.res + (long long)inst * a.P.forms * plane +
-                         ((long long)kt * a.P.Rp + row) * 128 + 32 * my_part;
+  __syncthreads();
+  if (t < 64) {
+    double mx = red[0][t];
+    if (a.r_major) mx = fmax(fmax(mx, red[1][t]), fmax(red[2][t], red[3][t]));
+    // largest |re|, |im| of the row in [2^(beta-1), 2^beta) after scaling
+    const bool live = mx >= 0x1p-900;
+    a.P.expo[(long long)inst * a.P.Rp + r0 + t] = live ? c.beta - (ilogb(mx) + 1) : kDead;
+  }
+}
+
+__global__ __launch_bounds__(256) void emu_resid_kernel(const PrepArgs a, const EmuConsts c) {
+  __shared__ zc T[64][PKC + 1];
+  const int t = threadIdx.x;
+  const int inst = blockIdx.z;
+  const int r0 = blockIdx.x * 64;
+  const int kt = blockIdx.y, k0 = kt * PKC;
+  const long long i_in = a.inner > 0 ? inst % a.inner : inst;
+  const long long i_out = a.inner > 0 ? inst / a.inner : 0;
+  const zc* X = a.X + i_in * a.stride + i_out * a.outer;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {   // two halves of 8 complex: 16 bytes each
-      double xr[8], xi[8];
+  for (int j = 0; j < 16; ++j) {
+    int lr, lk;
+    if (a.r_major) {      // X[r + k ld]: lanes along rows
+      lr = t & 63;
+      lk = (t >> 6) + 4 * j;
+    } else {              // X[k + r ld]: lanes along k
+      lk = t & 63;
+      lr = (t >> 6) + 4 * j;
+    }
+    const int r = r0 + lr, k = k0 + lk;
+    zc v = zmake(0.0, 0.0);
+    if (r < a.rows && k < a.K) v = prep_load(a, X, r, k);
+    T[lr][lk] = v;
+  }
+  const int my_row = t >> 2, my_part = t & 3;
+  const int row = r0 + my_row;
+  const int e = a.P.expo[(long long)inst * a.P.Rp + row];
+  const double sc = e == kDead ? 0.0 : ldexp(1.0, e);
+  __syncthreads();
+  const long long plane = (long long)a.P.Rp * a.P.Kp;
+  signed char* base0 = a.P.res + (long long)inst * a.P.forms * plane +
+                       ((long long)kt * a.P.Rp + row) * 128 + 32 * my_part;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {   // two halves of 8 complex: 16 bytes each
+    double xr[8], xi[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const zc v = T[my_row][16 * my_part + 8 * h + q];
+      xr[q] = rint(v.x * sc);
+      xi[q] = rint(v.y * sc);
+    }
+    for (int l = 0; l < c.nmod; ++l) {
+      const double m = c.md[l], im = c.invmd[l];
+      int rr[8], ri[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const zc v = k0 < a.K ? T[my_row][16 * my_part + 8 * h + q] : zmake(0.0, 0.0);
-        xr[q] = rint(v.x * sc);
-        xi[q] = rint(v.y * sc);
+        rr[q] = residue(xr[q], m, im);
+        ri[q] = residue(xi[q], m, im);
       }
-      for (int l = 0; l < c.nmod; ++l) {
-        const double m = c.md[l], im = c.invmd[l];
-        int rr[8], ri[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          rr[q] = residue(xr[q], m, im);
-          ri[q] = residue(xi[q], m, im);
-        }
-        signed char* base = base0 + (long long)l * a.P.batch * a.P.forms * plane + 16 * h;
-        if (a.P.forms == 1) {   // (br, bi)
-          *(uint4*)base =
-              make_uint4(pack4(rr[0], ri[0], rr[1], ri[1]), pack4(rr[2], ri[2], rr[3], ri[3]),
-                         pack4(rr[4], ri[4], rr[5], ri[5]), pack4(rr[6], ri[6], rr[7], ri[7]));
-        } else {                // R = (ar, -ai), I = (ai, ar)
-          *(uint4*)base = make_uint4(
-              pack4(rr[0], -ri[0], rr[1], -ri[1]), pack4(rr[2], -ri[2], rr[3], -ri[3]),
-              pack4(rr[4], -ri[4], rr[5], -ri[5]), pack4(rr[6], -ri[6], rr[7], -ri[7]));
-          *(uint4*)(base + plane) = make_uint4(
-              pack4(ri[0], rr[0], ri[1], rr[1]), pack4(ri[2], rr[2], ri[3], rr[3]),
-              pack4(ri[4], rr[4], ri[5], rr[5]), pack4(ri[6], rr[6], ri[7], rr[7]));
-        }
+      signed char* base = base0 + (long long)l * a.P.batch * a.P.forms * plane + 16 * h;
+      if (a.P.forms == 1) {   // (br, bi)
+        *(uint4*)base =
+            make_uint4(pack4(rr[0], ri[0], rr[1], ri[1]), pack4(rr[2], ri[2], rr[3], ri[3]),
+                       pack4(rr[4], ri[4], rr[5], ri[5]), pack4(rr[6], ri[6], rr[7], ri[7]));
+      } else {                // R = (ar, -ai), I = (ai, ar)
+        *(uint4*)base = make_uint4(
+            pack4(rr[0], -ri[0], rr[1], -ri[1]), pack4(rr[2], -ri[2], rr[3], -ri[3]),
+            pack4(rr[4], -ri[4], rr[5], -ri[5]), pack4(rr[6], -ri[6], rr[7], -ri[7]));
+        *(uint4*)(base + plane) = make_uint4(
+            pack4(ri[0], rr[0], ri[1], rr[1]), pack4(ri[2], rr[2], ri[3], rr[3]),
+            pack4(ri[4], rr[4], ri[5], rr[5]), pack4(ri[6], rr[6], ri[7], rr[7]));
       }
     }
   }
@@ -520,10 +542,15 @@ int zgemm_emu_launch(const GemmArgs& g, hipStream_t s) {
   // op(B) columns j as rows: N -> B[k + j ldb], T / C -> B[j + k ldb]
   PrepArgs pb{g.B, g.ldb, g.strideB, g.outerB, g.inner, g.opB == OP_N ? 0 : 1,
               g.opB == OP_C ? 1 : 0, g.n, g.k, PB};
-  hipLaunchKernelGGL(emu_prep_kernel, dim3((unsigned)(Mp / 64), (unsigned)batchA), dim3(256), 0, s,
-                     pa, c);
-  hipLaunchKernelGGL(emu_prep_kernel, dim3((unsigned)(Np / 64), (unsigned)batchB), dim3(256), 0, s,
-                     pb, c);
+  if (batchA > 65535 || batchB > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(emu_rowmax_kernel, dim3((unsigned)(Mp / 64), (unsigned)batchA), dim3(256), 0,
+                     s, pa, c);
+  hipLaunchKernelGGL(emu_rowmax_kernel, dim3((unsigned)(Np / 64), (unsigned)batchB), dim3(256), 0,
+                     s, pb, c);
+  hipLaunchKernelGGL(emu_resid_kernel, dim3((unsigned)(Mp / 64), (unsigned)(Kp / EKS),
+                                            (unsigned)batchA), dim3(256), 0, s, pa, c);
+  hipLaunchKernelGGL(emu_resid_kernel, dim3((unsigned)(Np / 64), (unsigned)(Kp / EKS),
+                                            (unsigned)batchB), dim3(256), 0, s, pb, c);
   EmuGemmArgs E;
   E.g = g;
   E.A = PA;
