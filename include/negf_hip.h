@@ -124,7 +124,10 @@ int negf_zgetrf_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA
 
 /* B <- op(LU)^{-1} B, trans in {'N','T','C'} (zgetrs).
  * Replaces be_mldivide! (src/backend_cpu.jl:193-206; the copy Min->Mout is
- * the caller's negf_zcopy). */
+ * the caller's negf_zcopy).  For 64 <= n <= 2048 (or as the tune table's
+ * kind 3 says) B is formed as inv(U) (inv(L) P B), by the recursive
+ * triangular inverses and two GEMMs, in a cached per-stream workspace;
+ * otherwise by row interchanges and two blocked substitutions. */
 int negf_zgetrs(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
                 const int32_t* ipiv, void* B, int64_t ldb, void* stream);
 int negf_zgetrs_strided_batched(char trans, int32_t n, int32_t nrhs, const void* A, int64_t lda,
@@ -133,7 +136,9 @@ int negf_zgetrs_strided_batched(char trans, int32_t n, int32_t nrhs, const void*
                                 void* stream);
 
 /* Triangular solve with n or m right-hand sides (ztrsm).
- * Replaces be_trsm! (src/backend_cpu.jl:241-253). */
+ * Replaces be_trsm! (src/backend_cpu.jl:241-253).  Routed like negf_zgetrs
+ * (tune kind 4, key {order of the triangle, right-hand sides}): one GEMM with
+ * the triangle's recursive inverse, or the blocked substitution. */
 int negf_ztrsm(char side, char uplo, char transA, char diag, int32_t m, int32_t n,
                const double alpha[2], const void* A, int64_t lda, void* B, int64_t ldb,
                void* stream);
@@ -144,7 +149,9 @@ int negf_ztrsm_strided_batched(char side, char uplo, char transA, char diag, int
 
 /* Inverse from the LU factors in place (zgetri); `work` is a device buffer of
  * n*n complex128 per instance.  ipiv = NULL means identity pivots, which is
- * be_getri! (src/backend_cpu.jl:229-239). */
+ * be_getri! (src/backend_cpu.jl:229-239).  Routed as negf_zgetrs with n
+ * right-hand sides: inv(U) inv(L) by the recursive triangular inverses and
+ * one GEMM, then the column interchanges, or getrs against the identity. */
 int negf_zgetri(int32_t n, void* A, int64_t lda, const int32_t* ipiv, void* work, void* stream);
 int negf_zgetri_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA,
                                 const int32_t* ipiv, int64_t strideP, void* work,
@@ -467,11 +474,14 @@ int negf_bndiag_devices(int32_t njobs, const NegfBandJob* jobs, void* stream);
  *   kind 0 (gemm):   key {m, n, k, problems per launch} -> {tile variant}
  *   kind 1 (inv):    key {n, batch}                     -> {inverse variant}
  *   kind 2 (bndiag): key {N, bmax, batch}               -> {blockSizeD1 per level}, {0} = RGF
+ *   kind 3 (getrs):  key {n, nrhs}                      -> {1 substitution, 2 explicit inverses}
+ *                    (getri: key {n, n})
+ *   kind 4 (trsm):   key {order of the triangle, nrhs}  -> {1 substitution, 2 explicit inverse}
  * A lookup matches every key but the last exactly and takes the entry with
  * the largest last key <= the request, else the smallest above it.
  * negf_tune_set replaces an entry with the same key; negf_tune_lookup
  * returns the number of values written (0: nothing for the key); the file
- * of negf_tune_load holds one entry per line, "gemm|inv|bndiag keys...
+ * of negf_tune_load holds one entry per line, "gemm|inv|bndiag|getrs|trsm keys...
  * values...", '#' starting a comment, and the call returns the entries read. */
 int negf_tune_set(int32_t kind, const int32_t* key, int32_t nkey, const int32_t* choice,
                   int32_t nchoice);

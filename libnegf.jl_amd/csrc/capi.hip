@@ -60,6 +60,18 @@ int workspace(void* stream, size_t bytes, void** out, int tag = 0) {
 bool valid_op(char c) { return c == 'N' || c == 'T' || c == 'C' || c == 'n' || c == 't' || c == 'c'; }
 int opc(char c) { return (c == 'N' || c == 'n') ? OP_N : ((c == 'T' || c == 't') ? OP_T : OP_C); }
 
+// Route of the per-op solves (getrs, trsm, getri): the tune table's choice
+// (1 substitution, 2 explicit inverses), else the explicit inverses of the
+// triangular factors and GEMMs (the blocked substitution is 2 n / 16
+// dependent launches) for n >= 128, or n >= 64 with 64 or more right-hand
+// sides, when their workspace stays under 1 GiB.  Measured on MI355X (getrs,
+// one instance): 2.3x (n = 512, 1 column) to 5x (512 columns) faster from
+// n = 128 on, 7 % slower at n = 64 with <= 16 columns.
+bool inverse_route(int tuned, int n, int nrhs, size_t bytes) {
+  const int route = tuned ? tuned : ((n >= 128 || (n >= 64 && nrhs >= 64)) ? 2 : 1);
+  return route == 2 && bytes <= ((size_t)1 << 30);
+}
+
 bool valid_sizes(int32_t nblocks, const int32_t* sizes) {
   if (nblocks <= 0 || !sizes) return false;
   for (int i = 0; i < nblocks; ++i)
@@ -240,6 +252,16 @@ int negf_zgetrs_strided_batched(char trans, int32_t n, int32_t nrhs, const void*
                                 int64_t ldb, int64_t strideB, int32_t batch, void* stream) {
   if (!valid_op(trans)) return NEGF_BAD_ARG;
   if ((!A || !ipiv || !B) && n > 0 && nrhs > 0 && batch > 0) return NEGF_BAD_ARG;
+  if (n > 0 && nrhs > 0 && batch > 0 && batch <= 65535 && lda >= n && ldb >= n) {
+    const size_t bytes = zgetrs_inv_work_bytes(n, nrhs, batch);
+    if (n <= kPermMaxN && inverse_route(tune_getrs_route(n, nrhs), n, nrhs, bytes)) {
+      void* w = nullptr;
+      int rc = workspace(stream, bytes, &w, 3);
+      if (rc) return rc;
+      return zgetrs_inv_launch(trans, n, nrhs, (const zc*)A, lda, strideA, ipiv, strideP, (zc*)B,
+                               ldb, strideB, batch, w, st(stream));
+    }
+  }
   return zgetrs_launch(trans, n, nrhs, (const zc*)A, lda, strideA, ipiv, strideP, (zc*)B, ldb,
                        strideB, batch, st(stream));
 }
@@ -255,6 +277,19 @@ int negf_ztrsm_strided_batched(char side, char uplo, char transA, char diag, int
                                void* stream) {
   if (!alpha) return NEGF_BAD_ARG;
   if ((!A || !B) && m > 0 && n > 0 && batch > 0) return NEGF_BAD_ARG;
+  const bool left = side == 'L' || side == 'l';
+  const int na = left ? m : n, nrhs = left ? n : m;
+  if (m > 0 && n > 0 && batch > 0 && batch <= 65535 && na <= kLuMaxN && lda >= na && ldb >= m &&
+      valid_op(transA) && (left || side == 'R' || side == 'r')) {
+    const size_t bytes = ztrsm_inv_work_bytes(na, m, n, batch);
+    if (inverse_route(tune_trsm_route(na, nrhs), na, nrhs, bytes)) {
+      void* w = nullptr;
+      int rc = workspace(stream, bytes, &w, 3);
+      if (rc) return rc;
+      return ztrsm_inv_launch(side, uplo, transA, diag, m, n, zz(alpha), (const zc*)A, lda,
+                              strideA, (zc*)B, ldb, strideB, batch, w, st(stream));
+    }
+  }
   return ztrsm_launch(side, uplo, transA, diag, m, n, zz(alpha), (const zc*)A, lda, strideA,
                       (zc*)B, ldb, strideB, batch, st(stream));
 }
@@ -272,6 +307,17 @@ int negf_zgetri_strided_batched(int32_t n, void* A, int64_t lda, int64_t strideA
   if (n < 0 || batch < 0) return NEGF_BAD_ARG;
   if (n == 0 || batch == 0) return NEGF_OK;
   if (!A || !work) return NEGF_BAD_ARG;
+  if (batch <= 65535 && n <= kPermMaxN && lda >= n) {
+    // getri as getrs against the identity: the same route
+    const size_t bytes = zgetri_inv_work_bytes(n, batch);
+    if (inverse_route(tune_getrs_route(n, n), n, n, bytes)) {
+      void* w = nullptr;
+      int rc = workspace(stream, bytes, &w, 3);
+      if (rc) return rc;
+      return zgetri_inv_launch(n, (zc*)A, lda, strideA, ipiv, strideP, batch, (zc*)work, w,
+                               st(stream));
+    }
+  }
   const int32_t* ip = ipiv;
   void* ident = nullptr;
   if (!ip) {

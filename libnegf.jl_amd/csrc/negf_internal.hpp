@@ -206,6 +206,36 @@ constexpr int kLuMaxN = 16384;
 size_t zinv_large_work_bytes(int n);
 int zinv_large_launch(int n, const zc* A, long long lda, zc* Ainv, long long ldinv, void* work,
                       int* info, hipStream_t s);
+// XP = X P per instance for LAPACK's interchanges ipiv (1-based, rows swapped
+// first to last: P A = L U): XP[:, c] = X[:, pinv[c]], pinv the inverse of
+// the row order they give.  X is n x n with ld n, XP has ld ldxp; pinv: n
+// ints per instance of scratch.  n <= kPermMaxN.
+constexpr int kPermMaxN = 2048;
+int ztimes_perm_launch(int n, const zc* X, long long sX, zc* XP, long long ldxp, long long sXP,
+                       const int* ipiv, long long sP, int* pinv, int batch, hipStream_t s);
+// getrs by the factors' explicit inverses: inv(L) P and inv(U) (recursive
+// triangular inverses, column gather), then B <- inv(U) (inv(L) P B) for 'N',
+// op(inv(L) P) (op(inv(U)) B) for 'T' / 'C': two GEMMs on the matrix cores
+// instead of 2 n / 16 dependent launches of the blocked substitution.
+// n <= kPermMaxN; work: zgetrs_inv_work_bytes.
+size_t zgetrs_inv_work_bytes(int n, int nrhs, int batch);
+int zgetrs_inv_launch(char trans, int n, int nrhs, const zc* A, long long lda, long long strideA,
+                      const int* ipiv, long long strideP, zc* B, long long ldb, long long strideB,
+                      int batch, void* work, hipStream_t s);
+// trsm by the triangle's explicit inverse: B = alpha op(inv(A)) B (left) or
+// alpha B op(inv(A)) (right), one GEMM after the recursive triangular
+// inverse; work: ztrsm_inv_work_bytes(na = m or n, m, n, batch).
+size_t ztrsm_inv_work_bytes(int na, int m, int n, int batch);
+int ztrsm_inv_launch(char side, char uplo, char trans, char diag, int m, int n, zc alpha,
+                     const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                     long long strideB, int batch, void* work, hipStream_t s);
+// getri by the factors' explicit inverses: inv(A) = inv(U) inv(L) P (the
+// product into W, n x n per instance, then the column interchanges into A;
+// straight into A when ipiv is null: no interchanges).  n <= kPermMaxN;
+// work: zgetri_inv_work_bytes.
+size_t zgetri_inv_work_bytes(int n, int batch);
+int zgetri_inv_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
+                      long long strideP, int batch, zc* W, void* work, hipStream_t s);
 
 // ----------------------------------------------------------------------------
 // Elementwise helpers (all batched, column-major).
@@ -237,6 +267,8 @@ int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_s
 // Tuned variants for a shape (0: none tuned; the launcher's thresholds apply).
 int tune_gemm_variant(int m, int n, int k, long long problems);
 int tune_inv_variant(int n, int batch);
+int tune_getrs_route(int n, int nrhs);   // 0 none, 1 substitution, 2 inverses
+int tune_trsm_route(int na, int nrhs);    // the same for trsm (na: the triangle's order)
 
 // Uniform view of a packed block-tridiagonal matrix whose block sizes differ:
 // block (i, j) of b_i x b_j becomes the leading part of a bmax x bmax block,

@@ -79,57 +79,11 @@ __global__ void zero_kernel(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// Li P: LAPACK's swaps (1-based ipiv, applied in order) give the row order
-// pi of P A (row r of P A is row pi[r] of A), so (Li P)[:, c] = Li[:, pinv[c]]
-// with pinv = pi^{-1}.  First pinv per instance (one workgroup each: the
-// swaps run in LDS by one thread, position k being final after step k; the
-// inversion by all), then the column gather over (8-column chunk, instance).
-// b <= 2048.
-__global__ __launch_bounds__(256) void pivot_perm_kernel(int b, const int* ipiv, long long sP,
-                                                          int* pinv, long long sI) {
-  __shared__ int pi[2048], ip[2048];
-  const int e = blockIdx.x;
-  for (int k = threadIdx.x; k < b; k += blockDim.x) {
-    pi[k] = k;
-    ip[k] = ipiv[(long long)e * sP + k] - 1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < b; ++k) {
-      const int q = ip[k];
-      const int t = pi[k];
-      pi[k] = pi[q];
-      pi[q] = t;
-    }
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < b; r += blockDim.x) pinv[(long long)e * sI + pi[r]] = r;
-}
-
-__global__ __launch_bounds__(256) void gather_cols_kernel(int b, const zc* Li, long long sLi,
-                                                           zc* Lp, long long sLp, const int* pinv,
-                                                           long long sI) {
-  const int e = blockIdx.y;
-  const zc* src = Li + (long long)e * sLi;
-  zc* dst = Lp + (long long)e * sLp;
-  const int* pv = pinv + (long long)e * sI;
-  for (int cc = 0; cc < 8; ++cc) {
-    const int c = blockIdx.x * 8 + cc;
-    if (c >= b) break;
-    const zc* col = src + (long long)pv[c] * b;
-    for (int r = threadIdx.x; r < b; r += blockDim.x) dst[r + (long long)c * b] = col[r];
-  }
-}
-
-// Lp = Li P for every instance; pinv: b ints per instance of scratch.
+// Lp = Li P for every instance (LAPACK's column interchanges of the pivots,
+// ztimes_perm_launch); pinv: b ints per instance of scratch.
 int times_perm(int b, const zc* Li, long long sLi, zc* Lp, long long sLp, const int* ipiv,
                long long sP, int* pinv, int batch, hipStream_t s) {
-  if (batch > 65535) return Status::TOO_LARGE;
-  hipLaunchKernelGGL(pivot_perm_kernel, dim3(batch), dim3(256), 0, s, b, ipiv, sP, pinv,
-                     (long long)b);
-  hipLaunchKernelGGL(gather_cols_kernel, dim3((b + 7) / 8, batch), dim3(256), 0, s, b, Li, sLi,
-                     Lp, sLp, pinv, (long long)b);
-  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+  return ztimes_perm_launch(b, Li, sLi, Lp, b, sLp, ipiv, sP, pinv, batch, s);
 }
 
 }  // namespace

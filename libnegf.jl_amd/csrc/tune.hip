@@ -10,6 +10,10 @@
 //   gemm    m n k problems   -> tile variant (zgemm.hip's variant numbers)
 //   inv     n batch          -> inverse schedule variant (zinv.hip's)
 //   bndiag  N bmax batch     -> DD-RGF blockSizeD1 per level; [0] = RGF
+//   getrs   n nrhs           -> 1: blocked substitution, 2: triangular
+//                               inverses and two products (getri: n n)
+//   trsm    na nrhs          -> 1: blocked substitution, 2: the triangle's
+//                               inverse and one product
 // Lookup: the other keys must match; of the entries for them, the one with
 // the largest batch key <= the request, else the smallest above it.
 #include "../../include/negf_hip.h"
@@ -41,12 +45,16 @@ std::mutex g_mu;
 std::vector<Entry> g_table;
 std::atomic<int> g_count{0};
 
-int nkeys_of(int kind) { return kind == 0 ? 4 : (kind == 1 ? 2 : (kind == 2 ? 3 : 0)); }
+int nkeys_of(int kind) {
+  return kind == 0 ? 4 : (kind == 1 ? 2 : (kind == 2 ? 3 : ((kind == 3 || kind == 4) ? 2 : 0)));
+}
 
 int kind_of(const std::string& w) {
   if (w == "gemm") return 0;
   if (w == "inv") return 1;
   if (w == "bndiag") return 2;
+  if (w == "getrs") return 3;
+  if (w == "trsm") return 4;
   return -1;
 }
 
@@ -108,6 +116,18 @@ int tune_inv_variant(int n, int batch) {
   const int key[2] = {n, batch};
   int v = 0;
   return lookup(1, key, 2, &v, 1) ? v : 0;
+}
+
+int tune_getrs_route(int n, int nrhs) {
+  const int key[2] = {n, nrhs};
+  int v = 0;
+  return lookup(3, key, 2, &v, 1) ? v : 0;
+}
+
+int tune_trsm_route(int na, int nrhs) {
+  const int key[2] = {na, nrhs};
+  int v = 0;
+  return lookup(4, key, 2, &v, 1) ? v : 0;
 }
 
 }  // namespace negf

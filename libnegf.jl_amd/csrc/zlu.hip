@@ -654,4 +654,224 @@ int zinv_large_launch(int n, const zc* A, long long lda, zc* Ainv, long long ldi
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
+
+namespace {
+
+// pinv per instance (one workgroup each): LAPACK's swaps (1-based ipiv,
+// applied in order) give the row order pi of P A (row r of P A is row pi[r]
+// of A); one thread runs the swaps in LDS (position k is final after step k),
+// then all threads write pinv = pi^{-1}.  n <= kPermMaxN.
+__global__ __launch_bounds__(256) void pivot_pinv_kernel(int n, const int* __restrict__ ipiv,
+                                                         long long sP, int* __restrict__ pinv,
+                                                         long long sI) {
+  __shared__ int pi[kPermMaxN], ip[kPermMaxN];
+  const int e = blockIdx.x;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    pi[k] = k;
+    ip[k] = ipiv[(long long)e * sP + k] - 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < n; ++k) {
+      const int q = ip[k];
+      const int t = pi[k];
+      pi[k] = pi[q];
+      pi[q] = t;
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) pinv[(long long)e * sI + pi[r]] = r;
+}
+
+// XP[:, c] = X[:, pinv[c]] over (8-column chunk, instance)
+__global__ __launch_bounds__(256) void gather_cols_batched_kernel(int n, const zc* __restrict__ X,
+                                                                  long long sX, zc* __restrict__ XP,
+                                                                  long long ldxp, long long sXP,
+                                                                  const int* __restrict__ pinv,
+                                                                  long long sI) {
+  const int e = blockIdx.y;
+  const zc* src = X + (long long)e * sX;
+  zc* dst = XP + (long long)e * sXP;
+  const int* pv = pinv + (long long)e * sI;
+  for (int cc = 0; cc < 8; ++cc) {
+    const int c = blockIdx.x * 8 + cc;
+    if (c >= n) break;
+    const zc* col = src + (long long)pv[c] * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) dst[r + (long long)c * ldxp] = col[r];
+  }
+}
+
+size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+int ztimes_perm_launch(int n, const zc* X, long long sX, zc* XP, long long ldxp, long long sXP,
+                       const int* ipiv, long long sP, int* pinv, int batch, hipStream_t s) {
+  if (n < 0 || batch < 0 || ldxp < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || batch == 0) return Status::OK;
+  if (n > kPermMaxN || batch > 65535) return Status::TOO_LARGE;
+  hipLaunchKernelGGL(pivot_pinv_kernel, dim3(batch), dim3(256), 0, s, n, ipiv, sP, pinv,
+                     (long long)n);
+  hipLaunchKernelGGL(gather_cols_batched_kernel, dim3((n + 7) / 8, batch), dim3(256), 0, s, n, X,
+                     sX, XP, ldxp, sXP, pinv, (long long)n);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+size_t zgetrs_inv_work_bytes(int n, int nrhs, int batch) {
+  const size_t nn = al256(sizeof(zc) * (size_t)n * n * batch);
+  const size_t nr = al256(sizeof(zc) * (size_t)n * nrhs * batch);
+  const size_t pv = al256(sizeof(int) * (size_t)n * batch);
+  return 3 * nn + nr + pv + al256(ztrtri_work_bytes(n, batch)) + 256;
+}
+
+int zgetrs_inv_launch(char trans, int n, int nrhs, const zc* A, long long lda, long long strideA,
+                      const int* ipiv, long long strideP, zc* B, long long ldb, long long strideB,
+                      int batch, void* work, hipStream_t s) {
+  const int op = op_code(trans);
+  if (op < 0 || n < 0 || nrhs < 0 || batch < 0) return Status::BAD_ARG;
+  if (lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || nrhs == 0 || batch == 0) return Status::OK;
+  if (n > kPermMaxN || batch > 65535) return Status::TOO_LARGE;
+  if (!work) return Status::BAD_ARG;
+  const long long tn = (long long)n * n, tr = (long long)n * nrhs;
+  char* p = (char*)work;
+  zc* Li = (zc*)p;  p += al256(sizeof(zc) * (size_t)tn * batch);
+  zc* Ui = (zc*)p;  p += al256(sizeof(zc) * (size_t)tn * batch);
+  zc* M = (zc*)p;   p += al256(sizeof(zc) * (size_t)tn * batch);
+  zc* T = (zc*)p;   p += al256(sizeof(zc) * (size_t)tr * batch);
+  int* pinv = (int*)p;  p += al256(sizeof(int) * (size_t)n * batch);
+  zc* tw = (zc*)p;
+  int rc = ztrtri_launch('L', 'U', n, A, lda, strideA, Li, n, tn, batch, tw, s);
+  if (rc) return rc;
+  rc = ztrtri_launch('U', 'N', n, A, lda, strideA, Ui, n, tn, batch, tw, s);
+  if (rc) return rc;
+  // M = inv(L) P: inv(A) = inv(U) M
+  rc = ztimes_perm_launch(n, Li, tn, M, n, tn, ipiv, strideP, pinv, batch, s);
+  if (rc) return rc;
+  // 'N': T = M B, B = inv(U) T.  'T' / 'C': op(inv(A)) = op(M) op(inv(U)), so
+  // T = op(inv(U)) B, B = op(M) T.
+  auto prod = [&](int opX, const zc* X, const zc* Y, long long ldy, long long sY, zc* Z,
+                  long long ldz, long long sZ) {
+    GemmArgs g{};
+    g.opA = opX; g.opB = OP_N;
+    g.m = n; g.n = nrhs; g.k = n;
+    g.alpha = zmake(1.0, 0.0); g.beta = zmake(0.0, 0.0);
+    g.A = X; g.lda = n; g.strideA = tn;
+    g.B = Y; g.ldb = ldy; g.strideB = sY;
+    g.C = nullptr; g.ldc = ldz; g.strideC = sZ;
+    g.D = Z; g.ldd = ldz; g.strideD = sZ;
+    g.batch = batch;
+    return zgemm_launch(g, s);
+  };
+  if (op == OP_N) {
+    rc = prod(OP_N, M, B, ldb, strideB, T, n, tr);
+    if (rc) return rc;
+    rc = prod(OP_N, Ui, T, n, tr, B, ldb, strideB);
+  } else {
+    rc = prod(op, Ui, B, ldb, strideB, T, n, tr);
+    if (rc) return rc;
+    rc = prod(op, M, T, n, tr, B, ldb, strideB);
+  }
+  if (rc) return rc;
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+size_t ztrsm_inv_work_bytes(int na, int m, int n, int batch) {
+  return al256(sizeof(zc) * (size_t)na * na * batch) + al256(sizeof(zc) * (size_t)m * n * batch) +
+         al256(ztrtri_work_bytes(na, batch)) + 256;
+}
+
+int ztrsm_inv_launch(char side, char uplo, char trans, char diag, int m, int n, zc alpha,
+                     const zc* A, long long lda, long long strideA, zc* B, long long ldb,
+                     long long strideB, int batch, void* work, hipStream_t s) {
+  const int op = op_code(trans);
+  const bool left = (side == 'L' || side == 'l');
+  const bool lo = (uplo == 'L' || uplo == 'l');
+  const bool unit = (diag == 'U' || diag == 'u');
+  if (op < 0 || (!left && side != 'R' && side != 'r') || (!lo && uplo != 'U' && uplo != 'u') ||
+      (!unit && diag != 'N' && diag != 'n'))
+    return Status::BAD_ARG;
+  if (m < 0 || n < 0 || batch < 0) return Status::BAD_ARG;
+  const int na = left ? m : n;
+  if (lda < (na > 1 ? na : 1) || ldb < (m > 1 ? m : 1)) return Status::BAD_ARG;
+  if (m == 0 || n == 0 || batch == 0) return Status::OK;
+  if (batch > 65535) return Status::TOO_LARGE;
+  if (!work) return Status::BAD_ARG;
+  const long long ta = (long long)na * na, tb = (long long)m * n;
+  char* p = (char*)work;
+  zc* Ti = (zc*)p;  p += al256(sizeof(zc) * (size_t)ta * batch);
+  zc* T = (zc*)p;   p += al256(sizeof(zc) * (size_t)tb * batch);
+  zc* tw = (zc*)p;
+  int rc = ztrtri_launch(lo ? 'L' : 'U', unit ? 'U' : 'N', na, A, lda, strideA, Ti, na, ta, batch,
+                         tw, s);
+  if (rc) return rc;
+  rc = zcopy2d_launch(m, n, B, ldb, strideB, T, m, tb, batch, s);
+  if (rc) return rc;
+  // left: B = alpha op(inv(A)) B; right: B = alpha B op(inv(A))
+  GemmArgs g{};
+  g.m = m; g.n = n; g.k = na;
+  g.alpha = alpha; g.beta = zmake(0.0, 0.0);
+  if (left) {
+    g.opA = op; g.opB = OP_N;
+    g.A = Ti; g.lda = na; g.strideA = ta;
+    g.B = T; g.ldb = m; g.strideB = tb;
+  } else {
+    g.opA = OP_N; g.opB = op;
+    g.A = T; g.lda = m; g.strideA = tb;
+    g.B = Ti; g.ldb = na; g.strideB = ta;
+  }
+  g.C = nullptr; g.ldc = ldb; g.strideC = strideB;
+  g.D = B; g.ldd = ldb; g.strideD = strideB;
+  g.batch = batch;
+  rc = zgemm_launch(g, s);
+  if (rc) return rc;
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+size_t zgetri_inv_work_bytes(int n, int batch) {
+  return 2 * al256(sizeof(zc) * (size_t)n * n * batch) + al256(sizeof(int) * (size_t)n * batch) +
+         al256(ztrtri_work_bytes(n, batch)) + 256;
+}
+
+int zgetri_inv_launch(int n, zc* A, long long lda, long long strideA, const int* ipiv,
+                      long long strideP, int batch, zc* W, void* work, hipStream_t s) {
+  if (n < 0 || batch < 0 || lda < (n > 1 ? n : 1)) return Status::BAD_ARG;
+  if (n == 0 || batch == 0) return Status::OK;
+  if (n > kPermMaxN || batch > 65535) return Status::TOO_LARGE;
+  if (!work || (ipiv && !W)) return Status::BAD_ARG;
+  const long long tn = (long long)n * n;
+  char* p = (char*)work;
+  zc* Li = (zc*)p;  p += al256(sizeof(zc) * (size_t)tn * batch);
+  zc* Ui = (zc*)p;  p += al256(sizeof(zc) * (size_t)tn * batch);
+  int* pinv = (int*)p;  p += al256(sizeof(int) * (size_t)n * batch);
+  zc* tw = (zc*)p;
+  int rc = ztrtri_launch('L', 'U', n, A, lda, strideA, Li, n, tn, batch, tw, s);
+  if (rc) return rc;
+  rc = ztrtri_launch('U', 'N', n, A, lda, strideA, Ui, n, tn, batch, tw, s);
+  if (rc) return rc;
+  // inv(A) = inv(U) inv(L) P: the product into W (with interchanges) or A
+  GemmArgs g{};
+  g.opA = OP_N; g.opB = OP_N;
+  g.m = g.n = g.k = n;
+  g.alpha = zmake(1.0, 0.0); g.beta = zmake(0.0, 0.0);
+  g.A = Ui; g.lda = n; g.strideA = tn;
+  g.B = Li; g.ldb = n; g.strideB = tn;
+  g.C = nullptr;
+  if (ipiv) {
+    g.ldc = n; g.strideC = tn;
+    g.D = W; g.ldd = n; g.strideD = tn;
+  } else {
+    g.ldc = lda; g.strideC = strideA;
+    g.D = A; g.ldd = lda; g.strideD = strideA;
+  }
+  g.batch = batch;
+  rc = zgemm_launch(g, s);
+  if (rc) return rc;
+  if (ipiv) {
+    rc = ztimes_perm_launch(n, W, tn, A, lda, strideA, ipiv, strideP, pinv, batch, s);
+    if (rc) return rc;
+  }
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
 }  // namespace negf

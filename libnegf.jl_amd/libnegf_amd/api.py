@@ -395,8 +395,12 @@ def col_major(x: torch.Tensor) -> torch.Tensor:
 
 
 def _ld(x: torch.Tensor) -> int:
-    if x.stride(0) != 1:
+    if x.stride(0) != 1 and x.shape[0] > 1:
         raise ValueError("matrices must be column-major (use col_major())")
+    if x.shape[1] <= 1:
+        # one column: torch keeps any stride for it (often 1); every ld >= m
+        # addresses the same elements
+        return max(int(x.shape[0]), 1)
     return max(int(x.stride(1)), 1)
 
 

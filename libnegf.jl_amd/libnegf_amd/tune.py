@@ -28,8 +28,8 @@ from . import _lib
 __all__ = ["table_path", "load_table", "lookup", "record", "time_device",
            "tune_gemm", "tune_inverse", "tune_bndiag", "bndiag_candidates", "KIND"]
 
-KIND = {"gemm": 0, "inv": 1, "bndiag": 2}
-NKEY = {"gemm": 4, "inv": 2, "bndiag": 3}
+KIND = {"gemm": 0, "inv": 1, "bndiag": 2, "getrs": 3, "trsm": 4}
+NKEY = {"gemm": 4, "inv": 2, "bndiag": 3, "getrs": 2, "trsm": 2}
 DEFAULT_TABLE = pathlib.Path(__file__).resolve().with_name("tune_table.txt")
 
 
@@ -193,6 +193,61 @@ def tune_inverse(n: int, batch: int, variants=(1, 2, 3, 4, 5), persist=True, pat
     return int(best[1:]), times
 
 
+def tune_getrs(n: int, nrhs: int, persist=True, path=None):
+    """Times negf_zgetrs ('N') of an n x n LU against nrhs right-hand sides by
+    the blocked substitution (route 1) and by the factors' explicit inverses
+    (route 2), and records the faster for (n, nrhs).  Returns (route, times)."""
+    import torch
+    A = _rand((n, n), 7)
+    A += 2.0 * torch.eye(n, dtype=torch.complex128, device="cuda")
+    B0 = _rand((nrhs, n), 8)
+    B = torch.empty_like(B0)
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    _lib.call("negf_zgetrf", n, p(A), n, p(piv), p(info), s)
+
+    def run():
+        B.copy_(B0)
+        _lib.call("negf_zgetrs", b"N", n, nrhs, p(A), n, p(piv), p(B), n, s)
+
+    times = {}
+    for route in (1, 2):
+        record("getrs", (n, nrhs), (route,), persist=False)
+        times[f"r{route}"] = time_device(run)
+    best = min(times, key=times.get)
+    record("getrs", (n, nrhs), (int(best[1:]),), times, path=path, persist=persist)
+    return int(best[1:]), times
+
+
+def tune_trsm(na: int, nrhs: int, persist=True, path=None):
+    """Times negf_ztrsm ('L', 'U', 'N', 'N') of an na x na triangle against
+    nrhs right-hand sides by the blocked substitution (route 1) and by one
+    GEMM with the triangle's inverse (route 2), and records the faster for
+    (na, nrhs).  Returns (route, times)."""
+    import torch
+    A = _rand((na, na), 9)
+    A += 2.0 * torch.eye(na, dtype=torch.complex128, device="cuda")
+    B0 = _rand((nrhs, na), 10)
+    B = torch.empty_like(B0)
+    one = _lib.cplx(1.0)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+
+    def run():
+        B.copy_(B0)
+        _lib.call("negf_ztrsm", b"L", b"U", b"N", b"N", na, nrhs, one, p(A), na, p(B), na, s)
+
+    times = {}
+    for route in (1, 2):
+        record("trsm", (na, nrhs), (route,), persist=False)
+        times[f"r{route}"] = time_device(run)
+    best = min(times, key=times.get)
+    record("trsm", (na, nrhs), (int(best[1:]),), times, path=path, persist=persist)
+    return int(best[1:]), times
+
+
 def bndiag_candidates(npl: int, max_levels: int = 2) -> list:
     """RGF ([0]) and DD-RGF levels giving 2 to 32 tasks at the first level
     and, for two levels, 2 to 4 tasks at the second (blockSizeD1 lists)."""
@@ -256,7 +311,8 @@ def tune_bndiag(block_sizes, batch: int, candidates=None, max_levels: int = 2, p
 def main(argv=None):
     """Tune the shapes of the production workloads on this device and write
     them into the table file: the GEMM tile and inverse schedule at block
-    sizes 256, 512 and 1024 over batches 1..128, and the band method of the
+    sizes 256, 512 and 1024 over batches 1..128, the getrs and trsm routes
+    at n = 64 .. 2048 with 1 .. n right-hand sides, and the band method of the
     three uneven devices (128 x 256, 64 x 512, 32 x 1024; 8 energies per
     call) and of the C interface example (160 x 64, one instance)."""
     import argparse
@@ -266,6 +322,7 @@ def main(argv=None):
                     help="table file (default: LIBNEGF_AMD_TUNE_FILE, else the package's)")
     ap.add_argument("--skip-kernels", action="store_true")
     ap.add_argument("--skip-bndiag", action="store_true")
+    ap.add_argument("--skip-getrs", action="store_true")
     a = ap.parse_args(argv)
     path = pathlib.Path(a.out) if a.out else (table_path() or DEFAULT_TABLE)
     if not a.skip_kernels:
@@ -276,6 +333,15 @@ def main(argv=None):
                     f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
                 v, t = tune_inverse(b, batch, path=path)
                 print(f"inv {b} batch {batch}: v{v} " + " ".join(
+                    f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
+    if not a.skip_getrs:
+        for n in (64, 128, 256, 512, 1024, 2048):
+            for nrhs in (1, 16, 64, n):
+                r, t = tune_getrs(n, nrhs, path=path)
+                print(f"getrs {n} nrhs {nrhs}: route {r} " + " ".join(
+                    f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
+                r, t = tune_trsm(n, nrhs, path=path)
+                print(f"trsm {n} nrhs {nrhs}: route {r} " + " ".join(
                     f"{k}={x * 1e3:.3f}ms" for k, x in t.items()), flush=True)
     if not a.skip_bndiag:
         for N, b, batch in ((128, 256, 8), (64, 512, 8), (32, 1024, 8), (160, 64, 1)):

@@ -240,11 +240,13 @@ def test_tune_table_file_and_lookup_rules(tmp_path):
         tune.record("bndiag", (128, 256, 8), (0,), {"rgf": 0.5}, path=p)   # replaces the line
         tune.record("bndiag", (64, 512, 8), (7, 2), path=p)
         tune.record("inv", (512, 96), (4,), path=p)
+        tune.record("getrs", (512, 16), (1,), path=p)
+        tune.record("getrs", (512, 512), (2,), {"r1": 2e-3, "r2": 1e-3}, path=p)
         text = p.read_text()
         assert text.count("bndiag 128 256 8") == 1 and "bndiag 128 256 8 0  # rgf:500.000ms" in text
         call("negf_tune_clear")
         assert call("negf_tune_count") == 0 and tune.lookup("gemm", (512, 512, 512, 32)) is None
-        assert tune.load_table(p) == 5
+        assert tune.load_table(p) == 7
         assert tune.lookup("gemm", (512, 512, 512, 64)) == [9]     # largest <= 64
         assert tune.lookup("gemm", (512, 512, 512, 200)) == [6]
         assert tune.lookup("gemm", (512, 512, 512, 4)) == [9]      # none <= 4: smallest above
@@ -252,8 +254,11 @@ def test_tune_table_file_and_lookup_rules(tmp_path):
         assert tune.lookup("bndiag", (128, 256, 1)) == [0]
         assert tune.lookup("bndiag", (64, 512, 16)) == [7, 2]
         assert tune.lookup("inv", (512, 8)) == [4]
+        assert tune.lookup("getrs", (512, 100)) == [1] and tune.lookup("getrs", (512, 600)) == [2]
+        assert tune.lookup("getrs", (256, 600)) is None
         bad = tmp_path / "bad.txt"
-        for line in ("frob 1 2 3", "gemm 512 512 512", "inv 512 -1 2", "inv 512 8 1 2 3 4 5"):
+        for line in ("frob 1 2 3", "gemm 512 512 512", "inv 512 -1 2", "inv 512 8 1 2 3 4 5",
+                     "getrs 512 16"):
             bad.write_text(line + "\n")
             with pytest.raises(Exception):
                 tune.load_table(bad)
@@ -305,3 +310,16 @@ def test_new_entry_points_refuse_bad_arguments_without_a_gpu():
     assert lib.negf_ddrgf_blocksizes_workspace(12, uneven, 2, (ctypes.c_int32 * 1)(10), 1) == 0
     big = (ctypes.c_int32 * 1025)(*([4, 5] * 512 + [4]))
     assert lib.negf_ddrgf_blocksizes_workspace(1025, big, 1, lv, 1) == 0   # > 1024 uneven blocks
+
+
+def test_leading_dimension_of_single_column_and_single_row_matrices():
+    """torch gives a one-column tensor any stride for its column (often 1); the
+    binding passes ld = m for it, and ld = the column stride otherwise."""
+    import torch
+    from libnegf_amd.api import _ld, col_major
+    v = col_major(torch.zeros((64, 1), dtype=torch.complex128))
+    assert _ld(v) == 64
+    assert _ld(col_major(torch.zeros((1, 5), dtype=torch.complex128))) >= 1
+    assert _ld(col_major(torch.zeros((7, 3), dtype=torch.complex128))) == 7
+    with pytest.raises(ValueError):
+        _ld(torch.zeros((7, 3), dtype=torch.complex128))      # row-major

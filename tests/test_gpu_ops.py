@@ -280,12 +280,29 @@ def test_zinv_exact_ties(n):
     assert relfro(inv, A.conj().T / n) < 1e-15
 
 
+@pytest.fixture
+def getrs_route():
+    """Force negf_zgetrs's route through the tune table (kind 3); the loaded
+    table is restored afterwards."""
+    from libnegf_amd import tune
+
+    def force(n, nrhs, route):
+        tune.record("getrs", (n, nrhs), [route], persist=False)
+    yield force
+    call("negf_tune_clear")
+    tune.load_table()
+
+
+@pytest.mark.parametrize("route", [1, 2])
 @pytest.mark.parametrize("trans", ["N", "T", "C"])
-@pytest.mark.parametrize("n,nrhs", [(150, 33), (512, 512)])
-def test_zgetrs(trans, n, nrhs):
-    """be_lu + be_mldivide as the unchanged reference sweeps call them; 512 x
-    512 with 512 right-hand sides is the production block's solve."""
-    print(f"zgetrs {trans} n={n} nrhs={nrhs}", flush=True)
+@pytest.mark.parametrize("n,nrhs", [(150, 33), (512, 512), (64, 1), (200, 700)])
+def test_zgetrs(trans, n, nrhs, route, getrs_route):
+    """be_lu + be_mldivide as the unchanged reference sweeps call them, by the
+    blocked substitution (route 1) and by the factors' explicit inverses
+    (route 2, the default for 64 <= n <= 2048); 512 x 512 with 512
+    right-hand sides is the production block's solve."""
+    print(f"zgetrs {trans} n={n} nrhs={nrhs} route={route}", flush=True)
+    getrs_route(n, nrhs, route)
     rng = np.random.default_rng(7)
     A = _rand(rng, n, n)
     B = _rand(rng, n, nrhs)
@@ -295,22 +312,99 @@ def test_zgetrs(trans, n, nrhs):
     assert relfro(_host(X), ref) < 1e-11
 
 
+@pytest.mark.parametrize("route", [1, 2])
+@pytest.mark.parametrize("trans", ["N", "C"])
+def test_zgetrs_strided_batched_padded(trans, route, getrs_route):
+    """Three instances in one call, leading dimensions above n and instance
+    strides above the blocks, on matrices whose pivoting swaps rows (LAPACK's
+    ipiv from getrf): each instance against numpy's solve."""
+    n, nrhs, batch, lda, ldb = 96, 40, 3, 101, 99
+    print(f"zgetrs batched {trans} route={route}", flush=True)
+    getrs_route(n, nrhs, route)
+    rng = np.random.default_rng(31)
+    As = [_rand(rng, n, n) for _ in range(batch)]
+    Bs = [_rand(rng, n, nrhs) for _ in range(batch)]
+    sA, sB = lda * n + 7, ldb * nrhs + 5
+    dA = torch.zeros(sA * batch, dtype=torch.complex128, device="cuda")
+    dB = torch.zeros(sB * batch, dtype=torch.complex128, device="cuda")
+    for e in range(batch):
+        Ap = np.zeros((lda, n), complex, order="F"); Ap[:n] = As[e]
+        Bp = np.zeros((ldb, nrhs), complex, order="F"); Bp[:n] = Bs[e]
+        dA[e * sA:e * sA + lda * n] = torch.from_numpy(Ap.ravel(order="F"))
+        dB[e * sB:e * sB + ldb * nrhs] = torch.from_numpy(Bp.ravel(order="F"))
+    piv = torch.zeros(batch * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    call("negf_zgetrf_strided_batched", n, vp(dA), lda, sA, vp(piv), n, vp(info), batch, None)
+    call("negf_zgetrs_strided_batched", trans.encode(), n, nrhs, vp(dA), lda, sA, vp(piv), n,
+         vp(dB), ldb, sB, batch, None)
+    torch.cuda.synchronize()
+    assert not info.any()
+    hB = dB.cpu().numpy()
+    for e in range(batch):
+        X = hB[e * sB:e * sB + ldb * nrhs].reshape((ldb, nrhs), order="F")
+        assert relfro(X[:n], np.linalg.solve(_op(As[e], trans), Bs[e])) < 1e-12
+        # the padding rows of B are not written
+        assert not X[n:].any()
+
+
+@pytest.mark.parametrize("route", [1, 2])
+@pytest.mark.parametrize("mn", [(70, 45), (200, 130)])
 @pytest.mark.parametrize("side", ["L", "R"])
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("tA", ["N", "T", "C"])
 @pytest.mark.parametrize("diag", ["N", "U"])
-def test_ztrsm(side, uplo, tA, diag):
-    print(f"ztrsm {side}{uplo}{tA}{diag}", flush=True)
+def test_ztrsm(side, uplo, tA, diag, mn, route):
+    """Every side / uplo / op / diag by the blocked substitution (route 1)
+    and by one GEMM with the triangle's inverse (route 2), against BLAS."""
+    from libnegf_amd import tune
+    print(f"ztrsm {side}{uplo}{tA}{diag} {mn} route={route}", flush=True)
     rng = np.random.default_rng(11)
-    m, n = 70, 45
+    m, n = mn
     na = m if side == "L" else n
+    tune.record("trsm", (na, n if side == "L" else m), [route], persist=False)
     A = _rand(rng, na, na) + 4 * np.eye(na)
     B = _rand(rng, m, n)
     alpha = 1.5 - 0.5j
     X = api.be_trsm(side, uplo, tA, diag, alpha, _dev(A), _dev(B))
     ref = sla.blas.ztrsm(alpha, A, B, side=0 if side == "L" else 1, lower=uplo == "L",
                          trans_a={"N": 0, "T": 1, "C": 2}[tA], diag=diag == "U")
+    call("negf_tune_clear")
+    tune.load_table()
     assert relfro(_host(X), ref) < 1e-12
+
+
+@pytest.mark.parametrize("route", [1, 2])
+@pytest.mark.parametrize("pivots", [True, False])
+def test_zgetri_routes(route, pivots, getrs_route):
+    """getri from getrf's factors (LAPACK pivots, rows swapped) and with
+    identity pivots (be_getri!), by both routes, batched with padded leading
+    dimensions, against numpy's inverse."""
+    n, batch, lda = 150, 2, 153
+    print(f"zgetri route={route} pivots={pivots}", flush=True)
+    getrs_route(n, n, route)
+    rng = np.random.default_rng(41)
+    As = [_rand(rng, n, n) + (0 if pivots else 3 * n * np.eye(n)) for _ in range(batch)]
+    sA = lda * n
+    dA = torch.zeros(sA * batch, dtype=torch.complex128, device="cuda")
+    for e in range(batch):
+        Ap = np.zeros((lda, n), complex, order="F"); Ap[:n] = As[e]
+        dA[e * sA:(e + 1) * sA] = torch.from_numpy(Ap.ravel(order="F"))
+    piv = torch.zeros(batch * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    work = torch.zeros(batch * n * n, dtype=torch.complex128, device="cuda")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    call("negf_zgetrf_strided_batched", n, vp(dA), lda, sA, vp(piv), n, vp(info), batch, None)
+    if not pivots:
+        assert np.array_equal(piv.cpu().numpy(), np.tile(np.arange(1, n + 1), batch))
+    call("negf_zgetri_strided_batched", n, vp(dA), lda, sA, vp(piv) if pivots else None, n,
+         vp(work), batch, None)
+    torch.cuda.synchronize()
+    h = dA.cpu().numpy()
+    for e in range(batch):
+        X = h[e * sA:(e + 1) * sA].reshape((lda, n), order="F")
+        assert relfro(X[:n], np.linalg.inv(As[e])) < 1e-12
+        assert not X[n:].any()
 
 
 def test_zgetri_identity_pivots_and_adjoint_fill():
