@@ -127,8 +127,9 @@ def gemm_roofline(torch, api, b, batch, reps):
 def gemm_precisions(torch, b, batch, reps):
     """The batched complex product b x b x b x `batch` (both operands per
     instance, and op(A) shared) in each arithmetic of
-    negf_zgemm_prec_strided_batched: FP64 MFMA, FP32 MFMA, FP64 emulated on
-    the int8 matrix cores (13 moduli); average per call over `reps` calls
+    negf_zgemm_prec_strided_batched: FP64 MFMA (four and three real
+    products), FP32 MFMA, FP64 emulated on the int8 matrix cores (13
+    moduli); average per call over `reps` calls
     after 10 untimed ones, HIP events on the launching stream."""
     from libnegf_amd._lib import call, cplx
     s = torch.cuda.current_stream()
@@ -136,7 +137,8 @@ def gemm_precisions(torch, b, batch, reps):
     B = torch.randn((batch, b * b), dtype=torch.complex128, device="cuda")
     D = torch.empty((batch, b * b), dtype=torch.complex128, device="cuda")
     out = {}
-    for name, prec, nmod, share in (("fp64", 0, 0, False), ("fp32", 1, 0, False),
+    for name, prec, nmod, share in (("fp64", 0, 0, False), ("fp64_3m", 3, 0, False),
+                                    ("fp32", 1, 0, False),
                                     ("emulated_13", 2, 13, False),
                                     ("emulated_13_shared_A", 2, 13, True)):
         def run():
@@ -592,8 +594,8 @@ def mixed(a, torch, api):
     GLm = torch.empty_like(GRf)
     Tf = torch.empty(chunk, dtype=torch.float64, device="cuda")
     Tm = torch.empty_like(Tf)
-    forms = ["emulated", "fp32"]
-    secs = {"fp64": 0.0, "emulated": 0.0, "fp32": 0.0}
+    forms = ["fp64_3m", "emulated", "fp32"]
+    secs = {"fp64": 0.0, "fp64_3m": 0.0, "emulated": 0.0, "fp32": 0.0}
     err = {f: np.zeros(nE) for f in forms}       # after the FP64 re-runs (tol = 1e-8)
     terr = {f: np.zeros(nE) for f in forms}
     own = {f: np.zeros(nE) for f in forms}       # the form's own result (no re-run)
@@ -647,8 +649,9 @@ def mixed(a, torch, api):
             e = torch.maximum(block_err(GRm[:n], GRf[:n]), block_err(GLm[:n], GLf[:n]))
             t = (Tm[:n] - Tf[:n]).abs() / Tf[:n].abs().clamp_min(1e-300)
             own[f][c0:c0 + n] = torch.maximum(e, t).cpu().numpy()
-        print(f"# chunk {c0}: fp64 {secs['fp64']:.2f}s emulated {secs['emulated']:.2f}s "
-              f"fp32 {secs['fp32']:.2f}s", file=sys.stderr, flush=True)
+        print(f"# chunk {c0}: fp64 {secs['fp64']:.2f}s fp64_3m {secs['fp64_3m']:.2f}s "
+              f"emulated {secs['emulated']:.2f}s fp32 {secs['fp32']:.2f}s", file=sys.stderr,
+              flush=True)
     fl = flops_rgf_keldysh_energy(N, b)
 
     def summary(f):

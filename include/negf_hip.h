@@ -104,7 +104,10 @@ int negf_zgemm_strided_batched(char transA, char transB, int32_t m, int32_t n, i
  * otherwise; 2 = FP64 emulated on the int8 matrix cores (Ozaki scheme II:
  * per row of op(A) / column of op(B) a power-of-two scale keeping beta bits,
  * residues modulo n_moduli coprime moduli <= 256, 0 = 13, beta = 45 at
- * k = 512; error at most 2^-beta of the row / column maxima per term).
+ * k = 512; error at most 2^-beta of the row / column maxima per term);
+ * 3 = FP64 with Gauss's three real products per complex product (Re = P - Q,
+ * Im = R - P - Q; 25 % fewer MFMAs, normwise error within a small factor of
+ * the four-product form) where the LDS-DMA kernel applies, FP64 otherwise.
  * No counterpart in the reference: the reduced-precision products of the
  * mixed-precision energy path. */
 int negf_zgemm_prec_strided_batched(char transA, char transB, int32_t m, int32_t n, int32_t k,
@@ -258,7 +261,8 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes,
 
 /* negf_rgf_rkd with the block products in a reduced arithmetic: precision
  * 1 = FP32 MFMA (FP32 accumulation), 2 = FP64 emulated on the int8 matrix
- * cores with n_moduli moduli (0: 13), 0 = negf_rgf_rkd itself; the diagonal
+ * cores with n_moduli moduli (0: 13), 3 = FP64 with three real products per
+ * complex product, 0 = negf_rgf_rkd itself; the diagonal
  * inverses stay FP64.  Per instance the estimate est = 8 u sum_i ||g_i||_F
  * (u the products' unit roundoff, g_i the upward pass's inverses) of
  * negf_rgf_keldysh_energies_mixed; instances with est > tol (or not finite)
@@ -312,8 +316,10 @@ int64_t negf_rgf_keldysh_energies_workspace(int32_t nblocks, int32_t b, int32_t 
  * a reduced or emulated precision (the diagonal-block inverses stay FP64):
  * precision 1 = FP32 MFMA (4 real products, FP32 accumulation), 2 = FP64
  * emulated on the int8 matrix cores (Ozaki scheme II, n_moduli moduli, 0 =
- * 13), 0 = FP64.  After the pass, each energy's error estimate
+ * 13), 3 = FP64 with three real products per complex product (Gauss), 0 =
+ * FP64.  After the pass, each energy's error estimate
  * est = 8 u sum_i ||g_i||_F (u = 2^-24 for FP32, 2^-beta for the emulation,
+ * 2^-51 for the three-product FP64 form,
  * g_i the upward pass's diagonal-block inverses; calibrated against the
  * oracle, see the tests) decides: energies with est > tol (or a non-finite
  * est) are computed again in FP64 and their outputs replace the first ones.

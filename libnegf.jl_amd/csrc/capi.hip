@@ -202,7 +202,7 @@ int negf_zgemm_prec_strided_batched(char transA, char transB, int32_t m, int32_t
                                     int32_t batch, int32_t precision, int32_t n_moduli,
                                     void* stream) {
   if (!valid_op(transA) || !valid_op(transB) || !alpha || !beta) return NEGF_BAD_ARG;
-  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16)
+  if (precision < 0 || precision > 3 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16)
     return NEGF_BAD_ARG;
   GemmArgs g{};
   g.opA = opc(transA); g.opB = opc(transB);
@@ -437,7 +437,7 @@ int negf_rgf_rkd_mixed(int32_t nblocks, const int32_t* block_sizes, const void* 
                        double* transmission, int32_t batch, int32_t* info, int32_t precision,
                        int32_t n_moduli, double tol, double* est_host, int32_t* fallback_host,
                        int32_t* n_fallback, void* stream) {
-  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
+  if (precision < 0 || precision > 3 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
       !(tol >= 0.0))
     return NEGF_BAD_ARG;
   if (n_fallback) *n_fallback = 0;
@@ -835,7 +835,7 @@ int64_t negf_rgf_keldysh_energies_mixed_workspace(int32_t nblocks, int32_t b, in
                                                   int32_t wave, int32_t streams, int32_t precision,
                                                   int32_t n_moduli) {
   if (nblocks <= 0 || b <= 0 || n_energies <= 0 || wave <= 0 || streams <= 0) return 0;
-  if (precision < 0 || precision > 2) return 0;
+  if (precision < 0 || precision > 3) return 0;
   double dummy = 0.0;
   EnergyPrec pr{precision, n_moduli, &dummy};
   return (int64_t)mixed_layout(nblocks, b, n_energies, wave, streams, pr).total;
@@ -850,7 +850,7 @@ int negf_rgf_keldysh_energies_mixed(int32_t nblocks, int32_t b, const void* H_di
                                     int32_t streams, int32_t precision, int32_t n_moduli,
                                     double tol, double* est_host, int32_t* fallback_host,
                                     int32_t* n_fallback, void* stream) {
-  if (precision < 0 || precision > 2 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
+  if (precision < 0 || precision > 3 || n_moduli < 0 || n_moduli == 1 || n_moduli > 16 ||
       !(tol >= 0.0))
     return NEGF_BAD_ARG;
   int32_t w0 = wave, s0 = streams;

@@ -103,7 +103,9 @@ struct GemmArgs {
   // memory): 0 = FP64 MFMA; 1 = FP32 MFMA with FP32 accumulation (operands
   // rounded to FP32, the 4-real-product complex form) where the launch takes
   // the LDS-DMA kernel, FP64 otherwise; 2 = integer-emulated FP64 (Ozaki
-  // scheme II on int8 MFMA, zgemm_emu.hip)
+  // scheme II on int8 MFMA, zgemm_emu.hip); 3 = FP64 with three real
+  // products per complex product (Gauss) where the launch takes the LDS-DMA
+  // kernel, the four-product FP64 form otherwise
   int prec;
   // prec = 2: device workspace for the residue planes (zgemm_emu_work_bytes)
   // and the number of moduli (0: 13)

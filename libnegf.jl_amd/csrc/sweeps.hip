@@ -176,6 +176,10 @@ __global__ void energy_estimate_kernel(const double* nrm, int N, int nb, double 
 
 double energy_prec_unit(int prec, int nmod, int k) {
   if (prec == 1) return 0x1p-24;
+  // FP64 with three real products: the imaginary part's bound carries
+  // (|ar| + |ai|)(|br| + |bi|) instead of |ar||bi| + |ai||br|, at most 2x;
+  // two more bits of margin for the extra additions
+  if (prec == 3) return 0x1p-51;
   if (prec == 2) {
     int beta = 0;
     if (zgemm_emu_params(nmod, k, &beta)) return 1.0;

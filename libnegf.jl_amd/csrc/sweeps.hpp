@@ -55,7 +55,7 @@ struct EnergyOut {
 // run: est[e] = 8 u sum_i ||g_i||_F, u the unit of the products (2^-24 for
 // FP32, 2^-beta for the emulation), g_i = S_i^{-1} the upward pass's inverses.
 struct EnergyPrec {
-  int prec;        // 0 = FP64, 1 = FP32 MFMA, 2 = int8-emulated FP64
+  int prec;        // 0 = FP64, 1 = FP32 MFMA, 2 = int8-emulated FP64, 3 = FP64 3M
   int nmod;        // moduli of the emulation (0: 13)
   double* est;     // device [nE] or null
 };

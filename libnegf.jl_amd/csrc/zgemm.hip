@@ -266,12 +266,25 @@ __global__ __launch_bounds__(256, OCC) void zgemm_mfma_kernel(
 // distinct banks.  Conjugation is a per-lane sign.
 constexpr int LTK = 8;
 
-// F32: the FP32 form of the same kernel -- the staged FP64 operands rounded
-// to FP32 in registers and multiplied on v_mfma_f32_16x16x4_f32 (twice the
-// FP64 MFMA rate) with FP32 accumulation; same LDS traffic.
-template <int OPA, int OPB, int NBUF, int T, bool F32 = false>
+// FORM (the arithmetic, GemmArgs::prec numbering):
+//   0: FP64, the K'-form above (two real MFMAs per two complex k);
+//   1: the FP32 form of the same kernel -- the staged FP64 operands rounded
+//      to FP32 in registers and multiplied on v_mfma_f32_16x16x4_f32 (twice
+//      the FP64 MFMA rate) with FP32 accumulation; same LDS traffic;
+//   3: FP64 with Gauss's three-multiplication complex product: per four
+//      complex k each lane reads whole complex a = A[i, k], b = B[k, j]
+//      (ds_read_b128) and feeds three MFMAs, P += ar br, Q += ai bi,
+//      R += (ar + ai)(br + bi) (conjugation folded into the sums' signs),
+//      then Re = P - Q, Im = R - P - Q: 3 real products per complex product
+//      instead of 4.  Error: normwise like the 4-product form within a small
+//      factor (Higham, "Stability of a method for multiplying complex
+//      matrices with three real matrix multiplications", 1992); the imaginary
+//      part is not componentwise accurate.
+template <int OPA, int OPB, int NBUF, int T, int FORM = 0>
 __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
   static_assert(T == 64 || T == 32, "tile of 64 or 32");
+  static_assert(FORM == 0 || FORM == 1 || FORM == 3, "arithmetic form");
+  constexpr bool F32 = FORM == 1;
   constexpr int FM = T / 32, FN = T / 32;   // 16 x 16 MFMA blocks per wave and direction
   constexpr int QL = T / 32;                // DMA loads per wave, operand and stage
   const long long bid = blockIdx.x;
@@ -402,6 +415,47 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     }
   };
 
+  // the three-multiplication form: accumulators P, Q, R (acc_re, acc_im and
+  // acc_r3); the signs of the conjugated operands' imaginary parts
+  AccV acc_r3[FORM == 3 ? FM : 1][FORM == 3 ? FN : 1];
+  if constexpr (FORM == 3) {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) acc_r3[a][b] = AccV{0, 0, 0, 0};
+  }
+  constexpr double sgA = OPA == OP_C ? -1.0 : 1.0, sgB = OPB == OP_C ? -1.0 : 1.0;
+  auto compute_stage3 = [&](int buf) {
+    if constexpr (FORM == 3) {
+    const zc* __restrict__ La = &Ls[buf][0][0][0];
+    const zc* __restrict__ Lb = &Ls[buf][1][0][0];
+#pragma unroll
+    for (int ks = 0; ks < LTK / 4; ++ks) {
+      const int kr = (4 * ks + kq) * T;   // complex k-row 4 ks + kq of the stage
+      zc a[FM], b[FN];
+      double as[FM], bs[FN];
+#pragma unroll
+      for (int r = 0; r < FM; ++r) {
+        a[r] = La[kr + wm * 16 * FM + 16 * r + li];
+        as[r] = fma(sgA, a[r].y, a[r].x);
+      }
+#pragma unroll
+      for (int c = 0; c < FN; ++c) {
+        b[c] = Lb[kr + wn * 16 * FN + 16 * c + li];
+        bs[c] = fma(sgB, b[c].y, b[c].x);
+      }
+#pragma unroll
+      for (int r = 0; r < FM; ++r)
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          acc_re[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, acc_re[r][c], 0, 0, 0);
+          acc_im[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, acc_im[r][c], 0, 0, 0);
+          acc_r3[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[r], bs[c], acc_r3[r][c], 0, 0, 0);
+        }
+    }
+    }
+  };
+
   // stage kt is waited for (vmcnt counts this wave's loads, in order, 2 QL a
   // stage: those of the NBUF - 2 later stages may stay in flight), then one
   // barrier makes it visible and frees the buffer of stage kt - 1 for stage
@@ -422,7 +476,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_lds_kernel(const GemmGroup G) {
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     if (kt + NBUF - 1 < nk) issue(kt + NBUF - 1);
-    compute_stage(kt % NBUF);
+    if constexpr (FORM == 3) compute_stage3(kt % NBUF);
+    else compute_stage(kt % NBUF);
+  }
+  if constexpr (FORM == 3) {
+    // Re = P - sQ, Im = R - P - sQ with s = sgA sgB (Q holds ai bi unsigned)
+    constexpr double sq = sgA * sgB;
+#pragma unroll
+    for (int r = 0; r < FM; ++r)
+#pragma unroll
+      for (int c = 0; c < FN; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double p = acc_re[r][c][e], q = sq * acc_im[r][c][e];
+          acc_re[r][c][e] = p - q;
+          acc_im[r][c][e] = acc_r3[r][c][e] - p - q;
+        }
   }
   store_tile<FM, FN, AccV, F32>(g, inst, i_in, i_out, i0 + wm * 16 * FM, j0 + wn * 16 * FN, lane,
                                 acc_re, acc_im, G.tri && tm > tn);
@@ -471,7 +540,7 @@ bool lds_eligible(const GemmGroup& G, int T) {
   return true;
 }
 
-template <int OPA, int OPB, int NBUF, int T, bool F32 = false>
+template <int OPA, int OPB, int NBUF, int T, int FORM = 0>
 void launch_lds(GemmGroup G, hipStream_t s) {
   G.tiles_m = G.p[0].m / T;
   G.tiles_n = G.p[0].n / T;
@@ -480,7 +549,7 @@ void launch_lds(GemmGroup G, hipStream_t s) {
                                : (long long)G.tiles_m * G.tiles_n) * G.p[0].batch;
   G.total = G.tiles_per_problem * G.count;
   G.grid_padded = (G.total + 7) / 8 * 8;
-  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF, T, F32>), dim3((unsigned)G.grid_padded),
+  hipLaunchKernelGGL((zgemm_lds_kernel<OPA, OPB, NBUF, T, FORM>), dim3((unsigned)G.grid_padded),
                      dim3(256), 0, s, G);
 }
 
@@ -493,8 +562,12 @@ void launch_t(const GemmGroup& G, hipStream_t s) {
   const GemmArgs& g0 = G.p[0];
   const long long tiles64 = (long long)(g0.m / 64) * (g0.n / 64) * g0.batch * G.count;
   if (g0.prec == 1) {   // the FP32 form where the LDS-DMA kernel applies
-    if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64, true>(G, s);
-    if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32, true>(G, s);
+    if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64, 1>(G, s);
+    if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32, 1>(G, s);
+  }
+  if (g0.prec == 3) {   // FP64, three real products, where the LDS-DMA kernel applies
+    if (tiles64 >= 512 && lds_eligible(G, 64)) return launch_lds<OPA, OPB, 2, 64, 3>(G, s);
+    if (lds_eligible(G, 32)) return launch_lds<OPA, OPB, 2, 32, 3>(G, s);
   }
   // a forced variant, else the tuned one for this shape, else the thresholds
   const int var = g_variant ? g_variant
@@ -538,7 +611,7 @@ int check_args(const GemmArgs& g) {
   }
   if (g.inner < 0) return Status::BAD_ARG;
   if (g.herm != 0 && (g.m != g.n || (g.herm != 1 && g.herm != -1))) return Status::BAD_ARG;
-  if (g.prec < 0 || g.prec > 2) return Status::BAD_ARG;
+  if (g.prec < 0 || g.prec > 3) return Status::BAD_ARG;
   return Status::OK;
 }
 

@@ -306,7 +306,7 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
     return info
 
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "emulated": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "emulated": 2, "fp64_3m": 3}
 
 
 def rgf_keldysh_energies_mixed(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float,
@@ -316,8 +316,9 @@ def rgf_keldysh_energies_mixed(H: DeviceHamiltonianGPU, energies: torch.Tensor, 
                                Gless_diag: torch.Tensor | None = None,
                                transmission: torch.Tensor | None = None, wave: int = 0,
                                streams: int = 0, check: bool = True):
-    """``rgf_keldysh_energies`` with the complex products in FP32 ("fp32") or
-    FP64 emulated on the int8 matrix cores ("emulated"); the diagonal-block
+    """``rgf_keldysh_energies`` with the complex products in FP32 ("fp32"),
+    FP64 emulated on the int8 matrix cores ("emulated") or FP64 with three
+    real products per complex product ("fp64_3m"); the diagonal-block
     inverses stay FP64.  Energies whose error estimate exceeds ``tol`` are
     computed again in FP64.  Returns (info, est, fallback): the device info
     array, the per-energy estimates and re-run flags (numpy)."""

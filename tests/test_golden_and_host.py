@@ -295,7 +295,7 @@ def test_new_entry_points_refuse_bad_arguments_without_a_gpu():
     job = (NegfBandJob * 1)()          # no info array, no sizes
     assert lib.negf_bndiag_devices(1, ctypes.cast(job, ctypes.c_void_p), None) == BAD
     sizes = (ctypes.c_int32 * 3)(8, 8, 8)
-    for prec, nmod, tol in ((3, 0, 1e-8), (-1, 0, 1e-8), (2, 1, 1e-8), (2, 17, 1e-8),
+    for prec, nmod, tol in ((4, 0, 1e-8), (-1, 0, 1e-8), (2, 1, 1e-8), (2, 17, 1e-8),
                             (2, 0, float("nan")), (2, 0, -1.0)):
         assert lib.negf_rgf_rkd_mixed(3, sizes, None, 0, None, 0, None, 0, None, 0, None, 0,
                                       None, None, None, None, None, None, 0, None, 0, None, 1,

@@ -139,6 +139,47 @@ def test_fp32_form_is_fp32_accurate(tA, tB):
         assert 1e-9 < err < 1e-5
 
 
+_OPS = [(a, b) for a in "NTC" for b in "NTC"]
+
+
+@pytest.mark.parametrize("tA,tB", _OPS)
+@pytest.mark.parametrize("m,n,k,batch", [(512, 512, 512, 8), (96, 64, 40, 3), (70, 33, 24, 2)])
+def test_fp64_3m_form_is_fp64_accurate(tA, tB, m, n, k, batch):
+    """precision 3 (three real products per complex product): FP64-class
+    normwise error on random data for every op pair, at the 64 x 64 tiles
+    (512^3 x 8 = 512 tiles), the 32 x 32 tiles (96 x 64) and a ragged shape
+    that takes the four-product kernel; alpha, beta and C applied."""
+    rng = np.random.default_rng(5 + m + 3 * k)
+    A = [_rand(rng, *_shape_of(tA, m, k)) for _ in range(batch)]
+    B = [_rand(rng, *_shape_of(tB, k, n)) for _ in range(batch)]
+    Cm = [_rand(rng, m, n) for _ in range(batch)]
+    alpha, beta = 0.75 - 0.5j, -1.25 + 0.25j
+    got = _gemm(3, tA, tB, A, B, Cin=Cm, alpha=alpha, beta=beta)
+    for a, b, c, d in zip(A, B, Cm, got):
+        ref = alpha * (_op(a, tA) @ _op(b, tB)) + beta * c
+        err = relfro(d, ref)
+        assert err < 1e-14, (tA, tB, m, n, k, err)
+
+
+def test_fp64_3m_form_runs_three_products():
+    """The three-product form's imaginary part is R - P - Q with R = (ar +
+    ai)(br + bi): where |Re| >> |Im| entries cancel, so Im loses digits that
+    the four-product form keeps (Higham 1992) -- measurably, which shows the
+    three products ran -- while the normwise error stays FP64-class."""
+    rng = np.random.default_rng(17)
+    m = n = k = 512
+    A = [1e6 * rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k)) for _ in range(8)]
+    B = [1e6 * rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n)) for _ in range(8)]
+    got3 = _gemm(3, "N", "N", A, B)
+    got4 = _gemm(0, "N", "N", A, B)
+    for a, b, d3, d4 in zip(A, B, got3, got4):
+        ref = a @ b
+        im3, im4 = relfro(d3.imag, ref.imag), relfro(d4.imag, ref.imag)
+        print(f"3M: Im relerr {im3:.2e} (4M {im4:.2e}), normwise {relfro(d3, ref):.2e}")
+        assert relfro(d3, ref) < 1e-14
+        assert im4 < 1e-14 and im3 > 1e3 * im4
+
+
 # ---------------------------------------------------------------- energy job
 from libnegf_amd import api  # noqa: E402
 from libnegf_amd.synthetic import (device_hamiltonian, assemble_A, fermi,  # noqa: E402
@@ -200,7 +241,8 @@ def test_mixed_energy_job_vs_oracle_production_block():
     """N = 16, b = 512, eta = 1e-4 at two band energies and two energies
     within 1e-3 of the lead band edges.  The emulated form (no fallback: tol
     = inf) meets 1e-8 against the oracle at every energy, and its estimate
-    bounds the measured error; the FP32 form misses 1e-8 by orders, its
+    bounds the measured error; so does the three-product FP64 form (with tol
+    = 1e-8: no energy re-run); the FP32 form misses 1e-8 by orders, its
     estimate says so, and with tol = 1e-8 every energy falls back to FP64 and
     then meets the FP64 bar."""
     N, b, eta = 16, 512, 1e-4
@@ -217,6 +259,13 @@ def test_mixed_energy_job_vs_oracle_production_block():
         assert err < 1e-8 and terr < 1e-8
         assert est[e] >= err          # the estimate bounds the error
         assert est[e] < 1e4 * err     # and is not vacuous
+    GR, GL, T, est, fb = _run_job(dev, Es, eta, "fp64_3m", 1e-8)
+    assert not fb.any()
+    for e in range(len(Es)):
+        err, terr = _energy_err(GR, GL, T, e, refs[e], b)
+        print(f"  fp64_3m E={Es[e]:+.5f}: err {err:.2e} T err {terr:.2e} est {est[e]:.2e}")
+        assert err < 1e-10 and terr < 1e-10
+        assert est[e] >= err
     GR, GL, T, est, fb = _run_job(dev, Es, eta, "fp32", np.inf)
     for e in range(len(Es)):
         err, _ = _energy_err(GR, GL, T, e, refs[e], b)
