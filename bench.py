@@ -62,6 +62,13 @@ def parse():
                     help="host processes of the cpu_pool workload (0 = min(cores, 16))")
     ap.add_argument("--graph", action="store_true",
                     help="capture the energy job once into a HIP graph and time its replays")
+    ap.add_argument("--form", default="fp64_3m", choices=["fp64", "fp64_3m"],
+                    help="products of the timed energy job: FP64 with three real products per "
+                         "complex product (Gauss; per-energy estimate, FP64 re-run above 1e-8) "
+                         "or with four; the other form is timed beside unless --no-fp64-leg")
+    ap.add_argument("--no-fp64-leg", action="store_true",
+                    help="with --form fp64_3m: skip the plain FP64 form's timing and the "
+                         "all-energy comparison against it")
     ap.add_argument("--gemm-reps", type=int, default=20)
     ap.add_argument("--inv-only", action="store_true",
                     help="only batched Gauss-Jordan inverses (--gemm-reps of them, b x b x wave) plus "
@@ -70,8 +77,8 @@ def parse():
                     help="only the batched product in FP64, FP32 and int8-emulated FP64 "
                          "(b x b x b x wave, --gemm-reps calls each)")
     ap.add_argument("--gemm-only", action="store_true",
-                    help="only the dominant-kernel timing plus a plain copy of known size "
-                         "(the counter-calibration run for rocprofv3 --pmc)")
+                    help="only the dominant-kernel timing (of --form) plus a plain copy of known "
+                         "size (the counter-calibration run for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -90,38 +97,48 @@ def copy_calibration(torch, nbytes=1 << 30):
     return nbytes
 
 
-def gemm_roofline(torch, api, b, batch, reps):
-    """Average duration of the batched FP64-MFMA ZGEMM at the sweep's shape
-    (b x b x b, `batch` instances, one operand broadcast like the shared
-    coupling blocks), timed with HIP events on the launching stream after 40
+def gemm_roofline(torch, api, b, batch, reps, form="fp64"):
+    """Average duration of the batched ZGEMM at the sweep's shape (b x b x b,
+    `batch` instances, one operand broadcast like the shared coupling blocks)
+    in the job's form (FP64 with four or three real products per complex
+    product), timed with HIP events on the launching stream after 40
     untimed launches: the clocks ramp over the first ~20 launches of this size
     (1.29 ms down to 1.0 ms each from an idle GPU), and the timed launches
-    should see them as the sweep does."""
+    should see them as the sweep does.  `achieved` counts the algorithmic
+    8mnk per complex product; the three-product form executes 6mnk of them
+    on the matrix cores (`executed_tflops`)."""
     from libnegf_amd._lib import call, cplx
+    prec = 3 if form == "fp64_3m" else 0
     s = torch.cuda.current_stream()
     A = torch.randn((b * b,), dtype=torch.complex128, device="cuda")
     B = torch.randn((batch, b * b), dtype=torch.complex128, device="cuda")
     D = torch.empty((batch, b * b), dtype=torch.complex128, device="cuda")
     args = lambda: (b"N", b"N", b, b, b, cplx(1), C.c_void_p(A.data_ptr()), b, 0,
                     C.c_void_p(B.data_ptr()), b, b * b, cplx(0), C.c_void_p(D.data_ptr()), b,
-                    b * b, batch, C.c_void_p(s.cuda_stream))
+                    b * b, batch, prec, 0, C.c_void_p(s.cuda_stream))
     for _ in range(40):
-        call("negf_zgemm_strided_batched", *args())
+        call("negf_zgemm_prec_strided_batched", *args())
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(reps):
-        call("negf_zgemm_strided_batched", *args())
+        call("negf_zgemm_prec_strided_batched", *args())
     e1.record(s)
     e1.synchronize()
     dt = e0.elapsed_time(e1) / 1e3 / reps
     flops = 8.0 * b ** 3 * batch
+    executed = flops * (0.75 if prec == 3 else 1.0)
     ach = flops / dt / 1e12
-    return {"kernel": "zgemm_lds_kernel<N,N,2,64> (FP64 MFMA 16x16x4, 64x64 tiles, operands DMA'd into LDS)",
+    kern = ("zgemm_lds_kernel<N,N,2,64,3> (FP64 MFMA 16x16x4, three real products per complex "
+            "product, 64x64 tiles, operands DMA'd into LDS)" if prec == 3 else
+            "zgemm_lds_kernel<N,N,2,64,0> (FP64 MFMA 16x16x4, 64x64 tiles, operands DMA'd into LDS)")
+    return {"kernel": kern,
             "bound": "mfma",
             "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "launch": f"{b}x{b}x{b} complex128 x {batch}", "avg_launch_us": round(dt * 1e6, 1),
-            "flops_per_launch": flops}
+            "flops_per_launch": flops,
+            "executed_tflops": round(executed / dt / 1e12, 3),
+            "executed_frac": round(executed / dt / 1e12 / FP64_PEAK_TFLOPS, 4)}
 
 
 def gemm_precisions(torch, b, batch, reps):
@@ -815,7 +832,7 @@ def main():
     if a.gemm_only:
         w = a.wave or 64
         nbytes = copy_calibration(torch)
-        roof = gemm_roofline(torch, api, b, w, a.gemm_reps)
+        roof = gemm_roofline(torch, api, b, w, a.gemm_reps, a.form)
         print(json.dumps({"gemm_only": True, "copy_bytes": nbytes, "roofline": roof}))
         return
     if a.inv_only:
@@ -844,6 +861,7 @@ def main():
     GL = torch.empty_like(GR)
     T = torch.empty(nE, dtype=torch.float64, device="cuda")
     free, total = torch.cuda.mem_get_info()
+    form = "fp64" if a.graph else a.form
     wave, streams = a.wave, a.streams
     if wave == 0:
         # the largest wave whose workspace fits: waves of 96 beat 64 by ~1 % at
@@ -852,12 +870,14 @@ def main():
         cap = -(-nE // streams)
         wave = 1
         for w in (96, 64, 48, 32, 16, 8, 4, 2, 1):
-            if w <= cap and api.energies_workspace_bytes(N, b, nE, w, streams) <= 0.85 * free:
+            ws = max(api.energies_workspace_bytes(N, b, nE, w, streams),
+                     api.energies_mixed_workspace_bytes(N, b, nE, w, streams, "fp64_3m")
+                     if form == "fp64_3m" else 0)
+            if w <= cap and ws <= 0.85 * free:
                 wave = w
                 break
-    print(f"# N={N} b={b} energies={nE} wave={wave} streams={streams} free={free/2**30:.1f}GiB",
-          file=sys.stderr,
-          flush=True)
+    print(f"# N={N} b={b} energies={nE} form={form} wave={wave} streams={streams} "
+          f"free={free/2**30:.1f}GiB", file=sys.stderr, flush=True)
 
     graph = None
     if a.graph:
@@ -867,49 +887,101 @@ def main():
         print(f"# graph captured and instantiated in {time.perf_counter() - t_c:.2f}s",
               file=sys.stderr, flush=True)
 
-    info = None
+    last = {}
 
-    def step():
-        nonlocal info
+    def step(f):
         if graph is not None:
-            info = graph.launch(check=False)
+            last["info"] = graph.launch(check=False)
+        elif f == "fp64":
+            last["info"] = api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T,
+                                                    wave=wave, streams=streams, check=False)
         else:
-            info = api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
-                                            streams=streams, check=False)
+            # per-energy estimate; energies above 1e-8 computed again in FP64
+            last["info"], last["est"], last["fb"] = api.rgf_keldysh_energies_mixed(
+                H, E_dev, eta, muL, muR, kT, f, 1e-8, GR_diag=GR, Gless_diag=GL,
+                transmission=T, wave=wave, streams=streams, check=False)
 
-    for w in range(a.warmup):
-        step()
+    def timed(f):
+        for w in range(a.warmup):
+            step(f)
+            torch.cuda.synchronize()
+            print(f"# {f} warmup {w} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
-        print(f"# warmup {w} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        step()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            step(f)
+            torch.cuda.synchronize()
+            print(f"# {f} step {k} done {time.perf_counter() - t0:.2f}s", file=sys.stderr,
+                  flush=True)
         torch.cuda.synchronize()
-        print(f"# step {k} done {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.steps
+        return (time.perf_counter() - t0) / a.steps
+
+    def check_info(f):
+        # every energy of the last step must have factored (no singular block)
+        bad = int((last["info"] != 0).sum().item())
+        if bad:
+            print(f"{f}: {bad} energies reported a singular pivot", file=sys.stderr)
+            sys.exit(3)
+
+    fp64_leg = None
+    if form != "fp64" and not a.no_fp64_leg:
+        dt64 = timed("fp64")
+        check_info("fp64")
+        fp64_leg = {"energies_per_s": round(nE / dt64, 4), "ms_per_step": round(dt64 * 1e3, 2),
+                    "form": "FP64, four real products per complex product"}
+    dt = timed(form)
+    check_info(form)
     eps = nE / dt
+    fallback = None
+    if form != "fp64":
+        fallback = {"energies_rerun_fp64": int(last["fb"].sum()),
+                    "max_estimate": float(last["est"].max()), "tolerance": 1e-8}
+    # every energy of the timed form against the plain FP64 form, on the GPU:
+    # the FP64 job again in chunks of 16 energies beside the resident outputs
+    vs64 = None
+    if form != "fp64" and not a.no_fp64_leg:
+        cw = 16
+        gr = torch.empty((cw, N * b * b), dtype=torch.complex128, device="cuda")
+        gl = torch.empty_like(gr)
+        tt = torch.empty(cw, dtype=torch.float64, device="cuda")
+        worst = torch.zeros(nE, dtype=torch.float64, device="cuda")
+        for c0 in range(0, nE, cw):
+            n = min(cw, nE - c0)
+            api.rgf_keldysh_energies(H, E_dev[c0:c0 + n], eta, muL, muR, kT, gr[:n], gl[:n],
+                                     tt[:n], wave=n, check=True)
+            for X, R in ((GR, gr), (GL, gl)):
+                for i in range(N):
+                    sl = slice(i * b * b, (i + 1) * b * b)
+                    d = torch.linalg.vector_norm(X[c0:c0 + n, sl] - R[:n, sl], dim=1)
+                    r = torch.linalg.vector_norm(R[:n, sl], dim=1).clamp_min(1e-300)
+                    worst[c0:c0 + n] = torch.maximum(worst[c0:c0 + n], d / r)
+            te = (T[c0:c0 + n] - tt[:n]).abs() / tt[:n].abs().clamp_min(1e-300)
+            worst[c0:c0 + n] = torch.maximum(worst[c0:c0 + n], te)
+        del gr, gl, tt
+        w = worst.cpu().numpy()
+        vs64 = {"energies": nE, "max_relerr": float(w.max()), "median_relerr": float(np.median(w)),
+                "energies_above_1e-8": int((w > 1e-8).sum()),
+                "measure": "per energy: max over its G^R_ii and G^<_ii blocks of the relative "
+                           "Frobenius difference, and of |T - T_fp64| / |T_fp64|"}
+        print(f"# all {nE} energies vs FP64: max {w.max():.2e}", file=sys.stderr, flush=True)
+        if not w.max() <= 1e-8:
+            print(f"energies exceed 1e-8 against the FP64 form: {vs64}", file=sys.stderr)
+            sys.exit(4)
     fl = flops_rgf_keldysh_energy(N, b)
-    roof = gemm_roofline(torch, api, b, wave, a.gemm_reps)
+    roof = gemm_roofline(torch, api, b, wave, a.gemm_reps, form)
     tfile = ROOT / "profiles" / f"zgemm_{b}_x{wave}_hbm_traffic.json"
     if tfile.exists():
         tj = json.loads(tfile.read_text())
-        if tj.get("b") == b and tj.get("batch") == wave:
+        if tj.get("b") == b and tj.get("batch") == wave and tj.get("form", "fp64") == form:
             roof["traffic"] = int(tj["traffic_bytes_per_launch"])
             roof["traffic_source"] = tj["source"]
-    fx = flops_energy_executed(N, b, wave)
+    fx = flops_energy_executed(N, b, wave, 3 if form == "fp64_3m" else 4)
     roof["path_tflops_executed"] = round(eps * fx / 1e12, 3)
     roof["path_frac_of_fp64_peak"] = round(eps * fx / 1e12 / FP64_PEAK_TFLOPS, 4)
     # the reference's call sequence's flops per energy times energies/s: a
     # throughput in the reference's units, not a hardware rate (the path
-    # executes 52 % of these flops)
+    # executes a fraction of these flops)
     roof["reference_sequence_tflops_equivalent"] = round(eps * fl / 1e12, 3)
-    # every energy of the last step must have factored (no singular block)
-    bad = int((info != 0).sum().item()) if info is not None else 0
-    if bad:
-        print(f"{bad} energies reported a singular pivot", file=sys.stderr)
-        sys.exit(3)
     cpu = None
     acc = None
     if not a.no_cpu_baseline:
@@ -931,6 +1003,11 @@ def main():
         if not a.no_cpu_pool:
             cpu["pool_shape"] = cpu_pool_shape(list(Es[::max(1, nE // 16)]), N, b,
                                                cpu["cores"])
+    form_txt = {"fp64": "FP64 (four real products per complex product), FP64 Gauss-Jordan "
+                        "inverses",
+                "fp64_3m": "FP64 with three real products per complex product (Gauss), FP64 "
+                           "Gauss-Jordan inverses; per-energy estimate 8u sum ||g_i||_F, FP64 "
+                           "re-run above 1e-8"}[form]
     out = {
         "metric": METRIC, "value": round(eps, 4), "unit": "energies/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 2),
@@ -938,10 +1015,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"rgf_keldysh_batched: N={N} b={b} energies={nE} "
                                f"linspace(-1,1) eta={eta} kT={kT} muL={muL} muR={muR}",
-                   "form": "plain FP64" + (", one hipGraph launch per step" if a.graph else ""),
+                   "form": form_txt + (", one hipGraph launch per step" if a.graph else ""),
                    "wave": wave, "streams": streams,
                    "flops_per_energy_credited": fl, "flops_per_energy_executed": fx},
         "roofline": roof,
+        "fp64_form": fp64_leg,
+        "fallback": fallback,
+        "accuracy_vs_fp64_form_all_energies": vs64,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_cpu": acc,
     }

@@ -805,11 +805,19 @@ __global__ void scatter_energy_outputs_kernel(long long nd, const int* idx, cons
 
 constexpr int kFallbackWave = 32;
 
+// Workspace: [main | est | fallback energies and indices | (temporaries)].
+// The FP64 re-runs start after the whole grid's pass has finished (the host
+// reads the estimates first), so they reuse the main region: their own
+// energy job from its start, their temporary outputs right behind it when
+// both fit there (waves of 32 or more), else in a region of their own at
+// the end.
 struct MixedLayout {
-  size_t main;       // the reduced-precision job (and, later, the FP64 re-runs)
+  size_t main;       // the reduced-precision job, later the FP64 re-runs
   size_t est;        // per-energy estimates
-  size_t fb;         // fallback: energies, indices, temporary outputs
+  size_t fb;         // fallback: energies and indices
+  size_t tmp_off;    // offset of the re-runs' temporary outputs from the base
   size_t total;
+  int fw;            // energies per FP64 re-run
 };
 
 MixedLayout mixed_layout(int32_t nblocks, int32_t b, int32_t nE, int32_t wave, int32_t streams,
@@ -820,10 +828,21 @@ MixedLayout mixed_layout(int32_t nblocks, int32_t b, int32_t nE, int32_t wave, i
   auto up = [](size_t x) { return (x + 255) / 256 * 256; };
   m.main = up(energies_work_bytes(L, nE, wave, streams, &pr) + sizeof(zc) * (size_t)L.nd + 4096);
   m.est = up(sizeof(double) * (size_t)nE);
-  const int fw = wave < kFallbackWave ? wave : kFallbackWave;
-  m.fb = up(sizeof(double) * nE) + up(sizeof(int) * nE) +
-         up(sizeof(zc) * (size_t)fw * 2 * L.nd) + up(sizeof(double) * fw) + up(sizeof(int) * fw);
-  m.total = m.main + m.est + m.fb;
+  m.fb = up(sizeof(double) * nE) + up(sizeof(int) * nE);
+  m.fw = wave < kFallbackWave ? wave : kFallbackWave;
+  // the re-run's energy job: its workspace, then its copy of H' (as in
+  // energies_enqueue)
+  const size_t rerun =
+      up(up(energies_work_bytes(L, m.fw, m.fw, 1, nullptr)) + sizeof(zc) * (size_t)L.nd + 4096);
+  const size_t tmp = up(sizeof(zc) * (size_t)m.fw * 2 * L.nd) + up(sizeof(double) * m.fw) +
+                     up(sizeof(int) * m.fw);
+  if (rerun + tmp <= m.main) {
+    m.tmp_off = rerun;
+    m.total = m.main + m.est + m.fb;
+  } else {
+    m.tmp_off = m.main + m.est + m.fb;
+    m.total = m.tmp_off + tmp;
+  }
   return m;
 }
 
@@ -918,8 +937,8 @@ int negf_rgf_keldysh_energies_mixed(int32_t nblocks, int32_t b, const void* H_di
   char* fb = base + ml.main + ml.est;
   double* dE = (double*)fb;
   int32_t* dI = (int32_t*)(fb + up(sizeof(double) * n_energies));
-  zc* tG = (zc*)(fb + up(sizeof(double) * n_energies) + up(sizeof(int) * n_energies));
-  const int fw = w0 < kFallbackWave ? w0 : kFallbackWave;
+  zc* tG = (zc*)(base + ml.tmp_off);
+  const int fw = ml.fw;
   zc* tGL = tG + (size_t)fw * L.nd;
   double* tT = (double*)((char*)tG + up(sizeof(zc) * (size_t)fw * 2 * L.nd));
   int32_t* tI = (int32_t*)((char*)tT + up(sizeof(double) * fw));

@@ -385,6 +385,13 @@ def energies_workspace_bytes(N: int, b: int, nE: int, wave: int, streams: int = 
     return int(call("negf_rgf_keldysh_energies_workspace", N, b, nE, wave, streams))
 
 
+def energies_mixed_workspace_bytes(N: int, b: int, nE: int, wave: int, streams: int = 1,
+                                   precision: str = "fp64_3m", n_moduli: int = 0) -> int:
+    """Device workspace of ``rgf_keldysh_energies_mixed`` for a wave."""
+    return int(call("negf_rgf_keldysh_energies_mixed_workspace", N, b, nE, wave, streams,
+                    PRECISIONS[precision], int(n_moduli)))
+
+
 # ----------------------------------------------------------------------------- per-op
 def col_major(x: torch.Tensor) -> torch.Tensor:
     """Column-major (Fortran) copy of a 2-D tensor unless it already is one."""

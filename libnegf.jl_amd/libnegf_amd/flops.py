@@ -30,6 +30,9 @@ anti-Hermitian, so for b a multiple of 64 only the T (T + 1) / 2 of the
 T^2 64 x 64 tiles (T = b / 64) on and below the diagonal are computed and the
 rest mirrored: 7.89e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
+With Gauss's three real products per complex product (``real_products`` =
+3) the matrix cores execute 6mnk of a product's 8mnk: the GEMM part scales
+by 3/4, the inverses (Gauss-Jordan, four real products) do not.
 """
 
 
@@ -68,6 +71,8 @@ def injection_tile_fraction(b: int, wave: int | None = None) -> float:
     return t * (t + 1) / 2 / (t * t)
 
 
-def flops_energy_executed(N: int, b: int, wave: int | None = None) -> float:
+def flops_energy_executed(N: int, b: int, wave: int | None = None,
+                          real_products: int = 4) -> float:
     skipped = 8 * N * (1.0 - injection_tile_fraction(b, wave))
-    return (88 * (N - 1) + 8 * N + 48 - skipped) * b ** 3
+    gemm = 88 * (N - 1) + 48 - skipped
+    return (gemm * real_products / 4 + 8 * N) * b ** 3
