@@ -152,6 +152,7 @@ void zinv_aux_release_all();
 struct InvWork {
   zc* X;        // n*n*batch
   zc* W;        // 3*zinv_w_elems(n)*batch (W of panels J, J+1, J+2)
+  zc* R;        // zinv_w_elems(n)*batch: a panel's chained pivot rows (variant 6)
   int* piv;     // n*batch
   int* used;    // n*batch
   int nmax;     // the n and batch the buffers were carved for: zinv_launch
@@ -163,7 +164,8 @@ size_t zinv_work_bytes(int n, int batch);
 // Elements of one W buffer per instance that serve every block size <= n.
 size_t zinv_w_elems(int n);
 // Schedule of the inverse (0: automatic, 1 / 2: in place with panels of 1 /
-// 2 strips, 3 / 4 / 5: split steps with panels of 1 / 2 / 4 strips),
+// 2 strips, 3 / 4 / 5: split steps with panels of 1 / 2 / 4 strips, 6: split
+// steps with panels of 4 strips whose bulk update is a batched GEMM),
 // process-wide.
 int zinv_set_variant(int v);
 void zinv_work_carve(void* base, int n, int batch, InvWork* w);

@@ -643,6 +643,85 @@ gj_strip_update_kernel(int n, int K, int PS, int ns, int batch, int next_panel, 
   }
 }
 
+// Pivot rows of panel K in the bulk strips, chained (variant 6): for every
+// strip s outside panels K, K+1 and K+2 (left of panel K, and right of panel
+// K+2), R[u S + j, s S + c] = R_u[j, c] with
+//   R_u = X[p_u, s] + sum_{v < u} W_v[p_u, :] R_v
+// (p_u the pivots of panel K's strip u, W_v its operation), so that the
+// whole panel's operation on those columns is one product,
+//   X[:, bulk] += [W_0 .. W_{PS-1}] R[:, bulk]
+// (n x PS S times PS S x columns), done by the batched GEMM next.  R is a
+// separate buffer, so the in-place product never reads rows it has written.
+// One 256-thread workgroup per (instance, strip); full strips only.
+__global__ __launch_bounds__(256) void gj_bulk_chain_kernel(int n, int K, int PS, int batch,
+                                                             int nleft, int rstart,
+                                                             const zc* __restrict__ X,
+                                                             const zc* __restrict__ Wcur,
+                                                             const int* __restrict__ piv,
+                                                             zc* __restrict__ R) {
+  constexpr int S = 16;
+  constexpr int NPAIR = MAXPS * (MAXPS - 1) / 2;
+  // items instance-fastest within each XCD's share, as the update kernel
+  const long long k = blockIdx.x;
+  int itl, inst;
+  if ((batch & 7) == 0) {
+    const long long nit = (long long)gridDim.x / batch;
+    const long long x = k & 7, kk = k >> 3;
+    itl = (int)(kk % nit);
+    inst = (int)(x + 8 * (kk / nit));
+  } else {
+    itl = (int)(k / batch);
+    inst = (int)(k - (long long)itl * batch);
+  }
+  const int s = itl < nleft ? itl : rstart + (itl - nleft);
+  const int pk0 = PS * K, k0 = s * S;
+  const int tid = threadIdx.x;
+  const zc* __restrict__ Xi = X + (long long)inst * n * n;
+  const zc* __restrict__ Wp = Wcur + (long long)inst * PS * n * S;
+  __shared__ int pjs[MAXPS][S];
+  __shared__ zc Rs[MAXPS][S][S + 1];
+  __shared__ zc Wg[NPAIR][S][S + 1];
+  if (tid < PS * S) pjs[tid / S][tid % S] = piv[(long long)inst * n + pk0 * S + tid];
+  __syncthreads();
+  // the pivot rows of the strip, and the W rows at the pivots of later strips
+  for (int x = tid; x < PS * S * S; x += 256) {
+    const int u = x / (S * S), r = x - u * S * S, j = r & 15, c = r >> 4;
+    Rs[u][j][c] = Xi[pjs[u][j] + (long long)(k0 + c) * n];
+  }
+  const int npair = PS * (PS - 1) / 2;
+  for (int x = tid; x < npair * S * S; x += 256) {
+    const int pr = x / (S * S), r = x - pr * S * S, j = r & 15, q = r >> 4;
+    int u = 1;
+    while ((u + 1) * u / 2 <= pr) ++u;
+    const int v = pr - u * (u - 1) / 2;
+    Wg[pr][j][q] = Wp[(long long)v * n * S + pjs[u][j] + (long long)q * n];
+  }
+  __syncthreads();
+  // the chain, block u after blocks 0 .. u-1
+  for (int u = 1; u < PS; ++u) {
+    const int j = tid & 15, c = tid >> 4;
+    double re = Rs[u][j][c].x, im = Rs[u][j][c].y;
+    for (int v = 0; v < u; ++v) {
+      const int pr = u * (u - 1) / 2 + v;
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const zc wq = Wg[pr][j][q];
+        const zc rq = Rs[v][q][c];
+        re = fma(wq.x, rq.x, fma(-wq.y, rq.y, re));
+        im = fma(wq.x, rq.y, fma(wq.y, rq.x, im));
+      }
+    }
+    Rs[u][j][c] = zmake(re, im);
+    __syncthreads();
+  }
+  // R is (PS S) x n per instance, column-major
+  zc* __restrict__ Ri = R + (long long)inst * PS * S * n;
+  for (int x = tid; x < PS * S * S; x += 256) {
+    const int c = x / (PS * S), kk = x - c * PS * S;
+    Ri[kk + (long long)(k0 + c) * PS * S] = Rs[kk / S][kk % S][c];
+  }
+}
+
 // Second stream per calling stream for the split steps (update kernel beside
 // the factoring kernel), with its fork / join events; created on first use,
 // default priority, non-blocking.  A slot lives until zinv_aux_release_all:
@@ -691,7 +770,8 @@ int aux_for(hipStream_t s, InvAux* out) {
 // panel K's, hence three W buffers.  The second stream is joined at the end.
 template <int PT, int PS>
 int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long long ldinv,
-                 long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s) {
+              long long strideInv, int batch, const InvWork& w, int* info, hipStream_t s,
+              bool bulk_gemm = false) {
   constexpr int S = 16;
   InvAux axv = w.aux;
   int rc = Status::OK;
@@ -742,12 +822,57 @@ int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, lo
       if (hipEventRecord(ax->join, ax->aux) != hipSuccess) return Status::HIP_ERROR;
       partA_pending = true;
     }
-    // part B: every other item
-    const int nB = items - nA;
-    if (nB > 0)
-      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nB)),
-                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, 0, st.pk0, nA, w.X, Wc,
-                         w.piv);
+    if (!bulk_gemm) {
+      // part B: every other item
+      const int nB = items - nA;
+      if (nB > 0)
+        hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * nB)),
+                           dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, 0, st.pk0, nA, w.X, Wc,
+                           w.piv);
+      continue;
+    }
+    // part B as one product: the strips left of panel K and right of panel
+    // K+2 get panel K's whole operation, X[:, bulk] += W R[:, bulk], with R
+    // the chained pivot rows (gj_bulk_chain_kernel); then panel K's own
+    // strips get the operations of their panel's later strips
+    const int nleft = st.pk0, rstart = st.hi + nA, nright = ns - rstart;
+    if (nleft + nright > 0) {
+      hipLaunchKernelGGL(gj_bulk_chain_kernel, dim3((unsigned)((long long)batch * (nleft + nright))),
+                         dim3(256), 0, ax->aux, n, K, PS, batch, nleft, rstart, w.X, Wc, w.piv,
+                         w.R);
+      for (int side = 0; side < 2; ++side) {
+        const int c0 = side == 0 ? 0 : rstart * S, nc = (side == 0 ? nleft : nright) * S;
+        if (nc == 0) continue;
+        GemmArgs g{};
+        g.opA = OP_N;
+        g.opB = OP_N;
+        g.m = n;
+        g.n = nc;
+        g.k = PS * S;
+        g.alpha = zmake(1.0, 0.0);
+        g.beta = zmake(1.0, 0.0);
+        g.A = Wc;
+        g.lda = n;
+        g.strideA = (long long)PS * n * S;
+        g.B = w.R + (long long)c0 * PS * S;
+        g.ldb = PS * S;
+        g.strideB = (long long)PS * S * n;
+        g.C = w.X + (long long)c0 * n;
+        g.ldc = n;
+        g.strideC = (long long)n * n;
+        g.D = w.X + (long long)c0 * n;
+        g.ldd = n;
+        g.strideD = (long long)n * n;
+        g.batch = batch;
+        rc = zgemm_launch(g, ax->aux);
+        if (rc) return rc;
+      }
+    }
+    const int ncorr = items - st.nupd;
+    if (ncorr > 0)
+      hipLaunchKernelGGL((gj_strip_update_kernel<PT>), dim3((unsigned)((long long)batch * ncorr)),
+                         dim3(PT), 0, ax->aux, n, K, PS, ns, batch, 0, st.nupd, INT_MAX, 0, w.X,
+                         Wc, w.piv);
   }
   if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
     return Status::HIP_ERROR;
@@ -800,7 +925,9 @@ int nb_for(int n) { return n <= 512 ? 64 : (n <= 1024 ? 16 : 8); }
 // Variant (autotuner choice, negf_inv_variant).  In place, one launch per
 // panel with the update items in gj_step_kernel: 1 = one-strip panels, 2 =
 // two-strip panels.  Split steps (run_split, strips of 16, n <= 512): 3, 4,
-// 5 = panels of 1, 2, 4 strips.  0 = automatic: split with four-strip panels
+// 5 = panels of 1, 2, 4 strips; 6 = panels of 4 strips with the strips
+// outside panels K .. K+2 updated by gj_bulk_chain_kernel and a batched GEMM
+// (n a multiple of 64, else as 5).  0 = automatic: split with four-strip panels
 // for 256 < n <= 512 and at least 32 instances, else in place with two-strip
 // panels from 32 instances up, one-strip below.  Measured on MI355X (ms per
 // batched inverse): 64 x 512: split/4 2.98, split/2 3.80, in place/2 3.90,
@@ -848,7 +975,7 @@ void zinv_aux_release_all() {
 }
 
 int zinv_set_variant(int v) {
-  if (v < 0 || v > 5) return Status::BAD_ARG;
+  if (v < 0 || v > 6) return Status::BAD_ARG;
   g_inv_variant = v;
   return Status::OK;
 }
@@ -868,6 +995,7 @@ size_t zinv_work_bytes(int n, int batch) {
   size_t b = 0;
   b += sizeof(zc) * (size_t)n * n * batch;   // X
   b += 3 * sizeof(zc) * zinv_w_elems(n) * batch;  // W (three buffers: panels J .. J+2)
+  b += sizeof(zc) * zinv_w_elems(n) * batch + 256;  // R (variant 6)
   b += sizeof(int) * (size_t)n * batch * 2;  // piv, used
   return b + 1024;
 }
@@ -881,6 +1009,7 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
   };
   w->X = (zc*)take(sizeof(zc) * (size_t)n * n * batch);
   w->W = (zc*)take(3 * sizeof(zc) * zinv_w_elems(n) * batch);
+  w->R = (zc*)take(sizeof(zc) * zinv_w_elems(n) * batch);
   w->piv = (int*)take(sizeof(int) * (size_t)n * batch);
   w->used = (int*)take(sizeof(int) * (size_t)n * batch);
   w->nmax = n;
@@ -890,13 +1019,14 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
 }
 
 int zinv_schedule(int n, int batch, int variant, int* strip, int* panel, int* nbuf) {
-  if (n <= 0 || n > 2048 || batch <= 0 || variant < 0 || variant > 5) return Status::BAD_ARG;
+  if (n <= 0 || n > 2048 || batch <= 0 || variant < 0 || variant > 6) return Status::BAD_ARG;
   const int v = variant;
   const bool split = n <= 512 && (v >= 3 || (v == 0 && n > 256 && batch >= 32));
   *strip = n <= 512 ? 16 : (n <= 1024 ? 8 : 4);
   if (split) {
     *panel = v == 3 ? 1 : (v == 4 ? 2 : 4);
-    *nbuf = 3;
+    // variant 6 (full panels only, else variant 5): the R buffer beside the W's
+    *nbuf = (v == 6 && n % 64 == 0) ? 4 : 3;
   } else {
     // in place (v >= 3 with n > 512 falls back to the automatic panel choice)
     const bool pairs = v == 2 || ((v == 0 || v >= 3) && batch >= 32);
@@ -920,7 +1050,11 @@ int zinv_launch(int n, const zc* A, long long lda, long long strideA, zc* Ainv, 
   if (split < 0) return split;
   if (split && w.inplace_only)   // the automatic in-place panel choice instead
     split = zinv_schedule(n, batch, batch >= 32 ? 2 : 1, &S, &PS, &nbuf);
-  if ((size_t)nbuf * n * S * PS > 3 * zinv_w_elems(w.nmax)) return Status::BAD_ARG;
+  if ((size_t)nbuf * n * S * PS > 4 * zinv_w_elems(w.nmax)) return Status::BAD_ARG;
+  if (split && nbuf == 4 && n > 256)
+    return run_split<512, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, true);
+  if (split && nbuf == 4)
+    return run_split<256, 4>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s, true);
   if (split) {
     if (n <= 256) {
       if (PS == 1) return run_split<256, 1>(n, A, lda, strideA, Ainv, ldinv, strideInv, batch, w, info, s);

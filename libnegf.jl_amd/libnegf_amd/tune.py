@@ -160,7 +160,7 @@ def tune_gemm(m: int, n: int, k: int, batch: int, variants=(9, 10, 8, 6), persis
     return int(best[1:]), times
 
 
-def tune_inverse(n: int, batch: int, variants=(1, 2, 3, 4, 5), persist=True, path=None):
+def tune_inverse(n: int, batch: int, variants=(1, 2, 3, 4, 5, 6), persist=True, path=None):
     """Times the batched inverse of ``batch`` n x n blocks with the built-in
     schedule choice (v0) and each variant (split-step variants apply to
     n <= 512), and records the fastest for (n, batch) (0: keep the built-in

@@ -178,9 +178,9 @@ def test_inverse_workspace_serves_every_smaller_block():
     for nmax in (96, 300, 512, 513, 600, 1024, 1100, 2048):
         for batch in (1, 8, 32, 96):
             ws = lib.negf_zinv_workspace(nmax, batch)
-            # X (n^2), piv + used (2n ints) per instance; W is the rest
+            # X (n^2), piv + used (2n ints) per instance; W and R are the rest
             w_bytes = ws - 1024 - 16 * nmax * nmax * batch - 8 * nmax * batch
-            for v in range(6):
+            for v in range(7):
                 for n in sorted({1, 16, 200, 256, 257, 511, 512, 513, 640, 1024, 1025, nmax}):
                     if n > nmax:
                         continue
@@ -193,7 +193,9 @@ def test_inverse_workspace_serves_every_smaller_block():
     # the case that wrote past the workspace before: a 512 layer beside 600s
     assert lib.negf_zinv_schedule(512, 32, 0, out) == 1 and out[0] * out[1] == 64
     assert lib.negf_zinv_schedule(0, 1, 0, out) < 0
-    assert lib.negf_zinv_schedule(16, 1, 6, out) < 0
+    assert lib.negf_zinv_schedule(512, 32, 6, out) == 1 and out[2] == 4
+    assert lib.negf_zinv_schedule(300, 32, 6, out) == 1 and out[2] == 3
+    assert lib.negf_zinv_schedule(16, 1, 7, out) < 0
 
 
 def test_rgf_noalloc_refuses_beyond_its_scratch():

@@ -132,15 +132,18 @@ def test_zinv_needs_pivoting(n):
     assert relfro(inv, np.linalg.inv(A)) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
-@pytest.mark.parametrize("n,batch", [(300, 6), (512, 3), (64, 2), (300, 16), (512, 32)])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("n,batch", [(300, 6), (512, 3), (64, 2), (300, 16), (512, 32),
+                                     (256, 8), (448, 16)])
 def test_zinv_every_schedule(variant, n, batch):
     """Every schedule of the batched inverse (in place with panels of 1 / 2
-    strips, split steps with panels of 1 / 2 / 4 strips, automatic) on ragged
-    (n = 300: a 12-column last strip, a partial last 64-row chunk) and full
-    shapes, against numpy.  Batches that are multiples of 8 take the split
-    update's per-XCD item order; 512 x 32 is the automatic schedule's split
-    regime (the sweep's)."""
+    strips, split steps with panels of 1 / 2 / 4 strips, the same with the
+    bulk update as a GEMM, automatic) on ragged (n = 300: a 12-column last
+    strip, a partial last 64-row chunk; variant 6 falls back to 5 there) and
+    full shapes (256 and 448: the GEMM form's left-only and right-only steps
+    at other panel counts), against numpy.  Batches that are multiples of 8
+    take the split update's per-XCD item order; 512 x 32 is the automatic
+    schedule's split regime (the sweep's)."""
     print(f"zinv variant={variant} n={n} batch={batch}", flush=True)
     rng = np.random.default_rng(n + batch)
     As = [_rand(rng, n, n) + (2 * np.sqrt(n) * np.eye(n) if k % 2 else 0) for k in range(batch)]
