@@ -863,7 +863,7 @@ def main():
     GL = torch.empty_like(GR)
     T = torch.empty(nE, dtype=torch.float64, device="cuda")
     free, total = torch.cuda.mem_get_info()
-    form = "fp64" if a.graph else a.form
+    form = a.form
     wave, streams = a.wave, a.streams
     if wave == 0:
         # the largest wave whose workspace fits: waves of 96 beat 64 by ~1 % at
@@ -882,18 +882,32 @@ def main():
           f"free={free/2**30:.1f}GiB", file=sys.stderr, flush=True)
 
     graph = None
-    if a.graph:
-        t_c = time.perf_counter()
-        graph = api.EnergiesGraph(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
-                                  streams=streams)
-        print(f"# graph captured and instantiated in {time.perf_counter() - t_c:.2f}s",
-              file=sys.stderr, flush=True)
-
     last = {}
 
+    def rerun_fp64(idx):
+        # the energies whose estimate exceeds the tolerance, again in FP64
+        # (the graph does not re-run by itself), in chunks of 16
+        for c0 in range(0, len(idx), 16):
+            sel = torch.tensor(idx[c0:c0 + 16], dtype=torch.long, device="cuda")
+            n = int(sel.numel())
+            gr = torch.empty((n, N * b * b), dtype=torch.complex128, device="cuda")
+            gl = torch.empty_like(gr)
+            tt = torch.empty(n, dtype=torch.float64, device="cuda")
+            api.rgf_keldysh_energies(H, E_dev[sel].contiguous(), eta, muL, muR, kT, gr, gl, tt,
+                                     wave=n, check=False)
+            GR[sel] = gr
+            GL[sel] = gl
+            T[sel] = tt
+
     def step(f):
-        if graph is not None:
+        if graph is not None and f == form:
             last["info"] = graph.launch(check=False)
+            if form == "fp64_3m":
+                est = graph.estimates()
+                fb = ~(est <= 1e-8)
+                if fb.any():
+                    rerun_fp64(np.nonzero(fb)[0].tolist())
+                last["est"], last["fb"] = est, fb.astype(np.int32)
         elif f == "fp64":
             last["info"] = api.rgf_keldysh_energies(H, E_dev, eta, muL, muR, kT, GR, GL, T,
                                                     wave=wave, streams=streams, check=False)
@@ -931,8 +945,20 @@ def main():
         check_info("fp64")
         fp64_leg = {"energies_per_s": round(nE / dt64, 4), "ms_per_step": round(dt64 * 1e3, 2),
                     "form": "FP64, four real products per complex product"}
+    if a.graph:
+        # the eager legs' cached workspace is released before the capture
+        # (the graph holds its own)
+        api.release_workspace()
+        t_c = time.perf_counter()
+        graph = api.EnergiesGraph(H, E_dev, eta, muL, muR, kT, GR, GL, T, wave=wave,
+                                  streams=streams, precision=form)
+        print(f"# graph captured and instantiated in {time.perf_counter() - t_c:.2f}s",
+              file=sys.stderr, flush=True)
     dt = timed(form)
     check_info(form)
+    if graph is not None:
+        graph.close()   # frees its workspace before the comparisons below
+        graph = None
     eps = nE / dt
     fallback = None
     if form != "fp64":

@@ -358,7 +358,28 @@ int negf_rgf_keldysh_energies_graph_create(int32_t nblocks, int32_t b,
                                            void* GR_diag, void* Gless_diag, double* transmission,
                                            int32_t* info, int32_t wave, int32_t streams,
                                            negf_energies_graph** graph);
+/* The same capture with the complex products in `precision` (0: FP64, the
+ * call above; 3: three real products per complex product, Gauss, in FP64 --
+ * the form of negf_rgf_keldysh_energies_mixed with precision 3).  With
+ * precision 3 each launch also writes the per-energy estimates
+ * est = 8 u sum_i ||g_i||_F (u = 2^-51) into the graph's workspace;
+ * negf_energies_graph_estimates copies them to the host.  The graph does not
+ * re-run energies itself: a caller whose tolerance an estimate exceeds
+ * computes those energies again with negf_rgf_keldysh_energies.  No
+ * counterpart in the reference. */
+int negf_rgf_keldysh_energies_graph_create_prec(int32_t nblocks, int32_t b,
+                                                const void* H_diag, const void* H_upper,
+                                                const void* gamma_L, const void* gamma_R,
+                                                const double* energies, int32_t n_energies,
+                                                double eta, double mu_L, double mu_R, double kT,
+                                                void* GR_diag, void* Gless_diag,
+                                                double* transmission, int32_t* info, int32_t wave,
+                                                int32_t streams, int32_t precision,
+                                                negf_energies_graph** graph);
 int negf_energies_graph_launch(negf_energies_graph* graph, void* stream);
+/* est_host[n_energies] <- the last launch's estimates (synchronizes `stream`);
+ * NEGF_BAD_ARG for a graph captured with precision 0. */
+int negf_energies_graph_estimates(negf_energies_graph* graph, double* est_host, void* stream);
 int negf_energies_graph_destroy(negf_energies_graph* graph);
 
 /* ------------------------------------------------- CSC (the reference ABI) */

@@ -683,7 +683,21 @@ struct negf_energies_graph {
   hipGraphExec_t exec = nullptr;
   std::vector<InvAux> aux;
   std::vector<hipEvent_t> events;
+  double* est = nullptr;   // per-energy estimates (three-product form), in work
+  int32_t n_est = 0;
 };
+
+namespace {
+// Capture of the energy job into a graph (precision 0: FP64; 3: three real
+// products per complex product, with the per-energy estimates written by
+// the graph itself); defined after the mixed-precision layout.
+int graph_create_impl(int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper,
+                      const void* gamma_L, const void* gamma_R, const double* energies,
+                      int32_t n_energies, double eta, double mu_L, double mu_R, double kT,
+                      void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
+                      int32_t wave, int32_t streams, int32_t precision,
+                      negf_energies_graph** out);
+}  // namespace
 
 extern "C" {
 
@@ -710,58 +724,30 @@ int negf_rgf_keldysh_energies_graph_create(
     const void* gamma_R, const double* energies, int32_t n_energies, double eta, double mu_L,
     double mu_R, double kT, void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
     int32_t wave, int32_t streams, negf_energies_graph** out) {
-  if (!out) return NEGF_BAD_ARG;
-  *out = nullptr;
-  if (n_energies <= 0) return NEGF_BAD_ARG;
-  int rc = energies_plan(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, kT,
-                         &wave, &streams);
-  if (rc) return rc;
-  const size_t bytes =
-      (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave, streams);
-  negf_energies_graph* g = new negf_energies_graph();
-  if (hipMalloc(&g->work, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    delete g;
-    return NEGF_NO_MEMORY;
-  }
-  g->aux.assign(streams, InvAux{nullptr, nullptr, nullptr});
-  for (auto& a : g->aux)
-    if (zinv_aux_create(&a) != NEGF_OK) {
-      negf_energies_graph_destroy(g);
-      return NEGF_HIP_ERROR;
-    }
-  g->events.assign(1 + streams, nullptr);
-  for (auto& e : g->events)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-      (void)hipGetLastError();
-      negf_energies_graph_destroy(g);
-      return NEGF_HIP_ERROR;
-    }
-  hipStream_t cs = nullptr;
-  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
-    negf_energies_graph_destroy(g);
+  return graph_create_impl(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies,
+                           eta, mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave,
+                           streams, 0, out);
+}
+
+int negf_rgf_keldysh_energies_graph_create_prec(
+    int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper, const void* gamma_L,
+    const void* gamma_R, const double* energies, int32_t n_energies, double eta, double mu_L,
+    double mu_R, double kT, void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
+    int32_t wave, int32_t streams, int32_t precision, negf_energies_graph** out) {
+  if (precision != 0 && precision != 3) return NEGF_BAD_ARG;
+  return graph_create_impl(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies,
+                           eta, mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave,
+                           streams, precision, out);
+}
+
+int negf_energies_graph_estimates(negf_energies_graph* g, double* est_host, void* stream) {
+  if (!g || !est_host) return NEGF_BAD_ARG;
+  if (!g->est) return NEGF_BAD_ARG;
+  hipStream_t s = st(stream);
+  if (hipMemcpyAsync(est_host, g->est, sizeof(double) * g->n_est, hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
     return NEGF_HIP_ERROR;
-  }
-  rc = NEGF_OK;
-  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NEGF_HIP_ERROR;
-  if (rc == NEGF_OK) {
-    rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, eta,
-                          mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave, streams,
-                          g->aux.data(), g->events.data(), g->work, cs);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(cs, &graph);
-    if (rc == NEGF_OK && (e != hipSuccess || !graph)) rc = NEGF_HIP_ERROR;
-    g->graph = graph;
-    if (rc == NEGF_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
-      rc = NEGF_HIP_ERROR;
-  }
-  (void)hipStreamDestroy(cs);
-  if (rc != NEGF_OK) {
-    (void)hipGetLastError();
-    negf_energies_graph_destroy(g);
-    return rc;
-  }
-  *out = g;
   return NEGF_OK;
 }
 
@@ -844,6 +830,80 @@ MixedLayout mixed_layout(int32_t nblocks, int32_t b, int32_t nE, int32_t wave, i
     m.total = m.tmp_off + tmp;
   }
   return m;
+}
+
+int graph_create_impl(int32_t nblocks, int32_t b, const void* H_diag, const void* H_upper,
+                      const void* gamma_L, const void* gamma_R, const double* energies,
+                      int32_t n_energies, double eta, double mu_L, double mu_R, double kT,
+                      void* GR_diag, void* Gless_diag, double* transmission, int32_t* info,
+                      int32_t wave, int32_t streams, int32_t precision,
+                      negf_energies_graph** out) {
+  if (!out) return NEGF_BAD_ARG;
+  *out = nullptr;
+  if (n_energies <= 0) return NEGF_BAD_ARG;
+  int rc = energies_plan(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, kT,
+                         &wave, &streams);
+  if (rc) return rc;
+  // the three-product form: the mixed layout (its estimates after the main
+  // region; the FP64 re-run temporaries are not used by a graph)
+  double dummy = 0.0;
+  EnergyPrec pr{precision, 0, &dummy};
+  const MixedLayout ml = mixed_layout(nblocks, b, n_energies, wave, streams, pr);
+  const size_t bytes =
+      precision == 0
+          ? (size_t)negf_rgf_keldysh_energies_workspace(nblocks, b, n_energies, wave, streams)
+          : ml.total;
+  negf_energies_graph* g = new negf_energies_graph();
+  if (hipMalloc(&g->work, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete g;
+    return NEGF_NO_MEMORY;
+  }
+  if (precision != 0) {
+    g->est = (double*)((char*)g->work + ml.main);
+    g->n_est = n_energies;
+    pr.est = g->est;
+  }
+  g->aux.assign(streams, InvAux{nullptr, nullptr, nullptr});
+  for (auto& a : g->aux)
+    if (zinv_aux_create(&a) != NEGF_OK) {
+      negf_energies_graph_destroy(g);
+      return NEGF_HIP_ERROR;
+    }
+  g->events.assign(1 + streams, nullptr);
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      negf_energies_graph_destroy(g);
+      return NEGF_HIP_ERROR;
+    }
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    negf_energies_graph_destroy(g);
+    return NEGF_HIP_ERROR;
+  }
+  rc = NEGF_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NEGF_HIP_ERROR;
+  if (rc == NEGF_OK) {
+    rc = energies_enqueue(nblocks, b, H_diag, H_upper, gamma_L, gamma_R, energies, n_energies, eta,
+                          mu_L, mu_R, kT, GR_diag, Gless_diag, transmission, info, wave, streams,
+                          g->aux.data(), g->events.data(), g->work, cs,
+                          precision == 0 ? nullptr : &pr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (rc == NEGF_OK && (e != hipSuccess || !graph)) rc = NEGF_HIP_ERROR;
+    g->graph = graph;
+    if (rc == NEGF_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = NEGF_HIP_ERROR;
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc != NEGF_OK) {
+    (void)hipGetLastError();
+    negf_energies_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return NEGF_OK;
 }
 
 }  // namespace

@@ -94,7 +94,11 @@ SIGNATURES = {
     "negf_rgf_keldysh_energies_graph_create": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d, d, d,
                                                          d, vp, vp, vp, vp, i32, i32,
                                                          C.POINTER(vp)]),
+    "negf_rgf_keldysh_energies_graph_create_prec": (C.c_int, [i32, i32, vp, vp, vp, vp, vp, i32, d,
+                                                              d, d, d, vp, vp, vp, vp, i32, i32,
+                                                              i32, C.POINTER(vp)]),
     "negf_energies_graph_launch": (C.c_int, [vp, vp]),
+    "negf_energies_graph_estimates": (C.c_int, [vp, vp, vp]),
     "negf_energies_graph_destroy": (C.c_int, [vp]),
     "negf_ddrgf": (C.c_int, [i32, i32, vp, i64, vp, i64, vp, i64, vp, vp, vp, i32, c_int32_p, i32,
                              vp, vp]),

@@ -340,26 +340,42 @@ def rgf_keldysh_energies_mixed(H: DeviceHamiltonianGPU, energies: torch.Tensor, 
 
 class EnergiesGraph:
     """The energy job of ``rgf_keldysh_energies`` captured once into a HIP
-    graph (negf_rgf_keldysh_energies_graph_create) and replayed by
+    graph (negf_rgf_keldysh_energies_graph_create[_prec]) and replayed by
     ``launch()`` as one submission.  The tensors passed here are bound for the
     graph's lifetime: change energies / H / Gamma by writing into them in
-    place between launches; eta, mu and kT are fixed."""
+    place between launches; eta, mu and kT are fixed.  ``precision`` "fp64"
+    (four real products per complex product) or "fp64_3m" (three, Gauss; each
+    launch also writes the per-energy estimates, read by ``estimates()``)."""
 
     def __init__(self, H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float, mu_L: float,
                  mu_R: float, kT: float, GR_diag: torch.Tensor | None = None,
                  Gless_diag: torch.Tensor | None = None,
-                 transmission: torch.Tensor | None = None, wave: int = 0, streams: int = 0):
+                 transmission: torch.Tensor | None = None, wave: int = 0, streams: int = 0,
+                 precision: str = "fp64"):
         _require_gpu()
+        if precision not in ("fp64", "fp64_3m"):
+            raise ValueError(f"EnergiesGraph: precision {precision!r} (fp64 or fp64_3m)")
         nE = int(energies.numel())
+        self.precision = precision
         self.info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
         # keep every bound tensor alive as long as the graph
         self._bound = (H, energies, GR_diag, Gless_diag, transmission)
         h = C.c_void_p()
-        call("negf_rgf_keldysh_energies_graph_create", H.N, H.b, _ptr(H.H_diag), _ptr(H.H_upper),
-             _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta), float(mu_L),
-             float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag), _ptr(transmission),
-             _ptr(self.info), int(wave), int(streams), C.byref(h))
+        call("negf_rgf_keldysh_energies_graph_create_prec", H.N, H.b, _ptr(H.H_diag),
+             _ptr(H.H_upper), _ptr(H.gamma_L), _ptr(H.gamma_R), _ptr(energies), nE, float(eta),
+             float(mu_L), float(mu_R), float(kT), _ptr(GR_diag), _ptr(Gless_diag),
+             _ptr(transmission), _ptr(self.info), int(wave), int(streams),
+             PRECISIONS[precision], C.byref(h))
         self._h = h
+        self._nE = nE
+
+    def estimates(self) -> np.ndarray:
+        """The last launch's per-energy error estimates (three-product form)."""
+        if self._h is None:
+            raise RuntimeError("graph destroyed")
+        est = np.zeros(self._nE, dtype=np.float64)
+        call("negf_energies_graph_estimates", self._h, est.ctypes.data_as(C.c_void_p), _stream())
+        return est
 
     def launch(self, check: bool = True):
         if self._h is None:

@@ -82,27 +82,38 @@ def test_selected_inverse_config2_vs_oracle(E):
     assert err < _bar(spread)
 
 
-def test_energy_job_config3_production_schedule():
+@pytest.mark.parametrize("form", ["fp64", "fp64_3m"])
+def test_energy_job_config3_production_schedule(form):
     """Config 3's job at its real shape and schedule: 32 energies on
     linspace(-1, 1) in one wave of 32 (the batched inverse takes its split
     steps), four of them (E = -1, +1 and two inside) against the oracle's
-    RGF + keldyshndiag + T.  G^R and T: 5 x the floor.  G^<: 10 x the floor
-    of the injection form, because one alternative formulation estimates the
-    G^< floor only to a factor of a few -- on this shape the injection form
-    and the reference-structured explicit-inverse form lie 8.2e-12 and
-    1.1e-11 from the oracle at the same input; measured GPU / floor ratios
-    here: 0.8 to 6.9."""
+    RGF + keldyshndiag + T, in the four-product FP64 form and in the
+    benchmark's form (three real products per complex product, per-energy
+    estimate with tolerance 1e-8: no energy may need its FP64 re-run here).
+    G^R and T: 5 x the floor.  G^<: 10 x the floor of the injection form,
+    because one alternative formulation estimates the G^< floor only to a
+    factor of a few -- on this shape the injection form and the
+    reference-structured explicit-inverse form lie 8.2e-12 and 1.1e-11 from
+    the oracle at the same input; measured GPU / floor ratios here: 0.8 to
+    6.9."""
     N, b, eta, kT, muL, muR = 64, 512, 1e-4, 0.025, 0.1, -0.1
     nE = 32
     Es = np.linspace(-1.0, 1.0, nE)
-    print(f"energy job N={N} b={b} {nE} energies, wave {nE}", flush=True)
+    print(f"energy job {form} N={N} b={b} {nE} energies, wave {nE}", flush=True)
     dev = device_hamiltonian(N, b, seed=1234)
     H = api.DeviceHamiltonianGPU(dev)
     GR = torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda")
     GL = torch.zeros_like(GR)
     T = torch.zeros(nE, dtype=torch.float64, device="cuda")
-    api.rgf_keldysh_energies(H, torch.tensor(Es, device="cuda"), eta, muL, muR, kT, GR, GL, T,
-                             wave=nE)
+    if form == "fp64":
+        api.rgf_keldysh_energies(H, torch.tensor(Es, device="cuda"), eta, muL, muR, kT, GR, GL, T,
+                                 wave=nE)
+    else:
+        _, est, fb = api.rgf_keldysh_energies_mixed(H, torch.tensor(Es, device="cuda"), eta, muL,
+                                                    muR, kT, form, 1e-8, GR_diag=GR,
+                                                    Gless_diag=GL, transmission=T, wave=nE)
+        print(f"  estimates max {est.max():.2e}, re-runs {int(fb.sum())}", flush=True)
+        assert not fb.any() and est.max() < 1e-8
     torch.cuda.synchronize()
     for k in (0, 9, 21, nE - 1):
         E = float(Es[k])

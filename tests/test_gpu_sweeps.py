@@ -208,6 +208,55 @@ def test_energies_graph_replays_match_eager(streams):
         graph.close()
 
 
+def test_energies_graph_three_product_form_replays_match_eager():
+    """The energy job in the three-product FP64 form (the benchmark's),
+    captured into a graph with its per-energy estimates: replays equal the
+    eager call (negf_rgf_keldysh_energies_mixed, precision 3, no energy
+    re-run) bit for bit, outputs and estimates, for new energies written in
+    place between replays; b = 320 takes the LDS-DMA three-product kernel and
+    the split inverse."""
+    N, b, eta, kT, muL, muR = 3, 320, 1e-3, 0.025, 0.1, -0.1
+    nE = 32
+    print(f"energies graph three-product N={N} b={b} wave={nE}", flush=True)
+    dev = device_hamiltonian(N, b, seed=23)
+    H = api.DeviceHamiltonianGPU(dev)
+    E_graph = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    outs = [torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda") for _ in range(2)]
+    T = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    graph = api.EnergiesGraph(H, E_graph, eta, muL, muR, kT, outs[0], outs[1], T, wave=nE,
+                              precision="fp64_3m")
+    try:
+        rng = np.random.default_rng(7)
+        for rep in range(2):
+            Es = torch.tensor(np.sort(rng.uniform(-1, 1, nE)), device="cuda")
+            GR = torch.zeros_like(outs[0])
+            GL = torch.zeros_like(outs[0])
+            Te = torch.zeros_like(T)
+            _, est, fb = api.rgf_keldysh_energies_mixed(H, Es, eta, muL, muR, kT, "fp64_3m", 1e-8,
+                                                        GR_diag=GR, Gless_diag=GL,
+                                                        transmission=Te, wave=nE)
+            torch.cuda.synchronize()
+            assert not fb.any(), "no energy should need its FP64 re-run"
+            E_graph.copy_(Es)
+            for o in outs:
+                o.zero_()
+            T.zero_()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], GR), f"replay {rep}: G^R differs"
+            assert torch.equal(outs[1], GL), f"replay {rep}: G^< differs"
+            assert torch.equal(T, Te), f"replay {rep}: T differs"
+            assert np.array_equal(graph.estimates(), est), f"replay {rep}: estimates differ"
+        k = 11
+        A = assemble_A(dev, float(Es[k]), eta)
+        G = rgf_ref(A)
+        for i in range(N):
+            blk = outs[0][k, i * b * b:(i + 1) * b * b].cpu().numpy().reshape(b, b).T
+            assert relfro(blk, G[(i, i)]) < 1e-12
+    finally:
+        graph.close()
+
+
 def test_energies_graph_split_inverse_replays_match_eager():
     """Waves of 32 energies at b = 320 take the batched inverse's split steps
     (the factoring on the job's stream, the strip updates on the library's
