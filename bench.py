@@ -540,11 +540,12 @@ def ddrgf_uneven(a, torch, api):
     jobs = (NegfBandJob * len(devs))(*[d.job(d.tuned) for d in devs])
     s_cur = torch.cuda.current_stream()
 
-    def one_call_wave(e0):
+    def one_call_wave(e0, prec=0):
         for d in devs:
             d.zw.copy_(d.z[e0:e0 + wave])
             d.assemble()
-        call("negf_bndiag_devices", len(devs), C.cast(jobs, C.c_void_p), C.c_void_p(s_cur.cuda_stream))
+        call("negf_bndiag_devices_prec", len(devs), C.cast(jobs, C.c_void_p), prec,
+             C.c_void_p(s_cur.cuda_stream))
         for d in devs:
             torch.maximum(d.bad, d.info.abs(), out=d.bad)
 
@@ -558,6 +559,76 @@ def ddrgf_uneven(a, torch, api):
     for d in devs:
         d.check("one call")
     Gd_one = [d.Gd.clone() for d in devs]
+
+    # accuracy of each device's method over all its energies, untimed: the
+    # band of every wave against plain FP64 RGF (the stable method; DD-RGF
+    # without pivoting across its separators amplifies rounding at eta = 1e-4)
+    def band_err(G, R, d):
+        worst = 0.0
+        for X, Y, nb in zip(G, R, (d.N, d.N - 1, d.N - 1)):
+            x = X.reshape(wave, nb, -1)
+            y = Y.reshape(wave, nb, -1)
+            rel = torch.linalg.vector_norm(x - y, dim=2) / torch.linalg.vector_norm(y, dim=2)
+            worst = max(worst, float(rel.max()))
+        return worst
+
+    def method_errors(levels_of, prec):
+        worst = [0.0] * len(devs)
+        for e0 in range(0, nE, wave):
+            for k, d in enumerate(devs):
+                d.zw.copy_(d.z[e0:e0 + wave])
+                d.assemble()
+                call("negf_rgf", d.N, d.sizes, p(d.Ad), d.N * d.b * d.b, p(d.up), 0, p(d.low), 0,
+                     p(d.Gd), p(d.Gu), p(d.Gl), wave, p(d.info), C.c_void_p(s_cur.cuda_stream))
+                R = (d.Gd.clone(), d.Gu.clone(), d.Gl.clone())
+                one = (NegfBandJob * 1)(d.job(levels_of[k]))
+                call("negf_bndiag_devices_prec", 1, C.cast(one, C.c_void_p), prec,
+                     C.c_void_p(s_cur.cuda_stream))
+                torch.maximum(d.bad, d.info.abs(), out=d.bad)
+                worst[k] = max(worst[k], band_err((d.Gd, d.Gu, d.Gl), R, d))
+        torch.cuda.synchronize()
+        return worst
+
+    tuned_err = method_errors([d.tuned for d in devs], 0)
+    # matched error: each device's tuned method where it stays within 1e-8 of
+    # RGF on every energy, else RGF
+    matched = [d.tuned if e <= 1e-8 else [0] for d, e in zip(devs, tuned_err)]
+    jobs_m = (NegfBandJob * len(devs))(*[d.job(lv) for d, lv in zip(devs, matched)])
+
+    def timed_one_call(job_arr, prec):
+        def wave_call(e0):
+            for d in devs:
+                d.zw.copy_(d.z[e0:e0 + wave])
+                d.assemble()
+            call("negf_bndiag_devices_prec", len(devs), C.cast(job_arr, C.c_void_p), prec,
+                 C.c_void_p(s_cur.cuda_stream))
+            for d in devs:
+                torch.maximum(d.bad, d.info.abs(), out=d.bad)
+        wave_call(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for e0 in range(0, nE, wave):
+            wave_call(e0)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    dt_m64 = timed_one_call(jobs_m, 0)
+    dt_m3 = timed_one_call(jobs_m, 3)
+    for d in devs:
+        d.check("matched one call")
+    m3_err = method_errors(matched, 3)
+    for d in devs:
+        d.check("accuracy passes")
+    # the tuned methods in the three-product form, for the record
+    dt_3m = timed_one_call(jobs, 3)
+    tuned3_err = method_errors([d.tuned for d in devs], 3)
+    err = {f"({d.N},{d.b})": {"tuned_method": "rgf" if d.tuned[0] == 0 else d.tuned,
+                              "tuned_fp64_vs_rgf": e1, "tuned_three_product_vs_rgf": e3,
+                              "matched_method": "rgf" if m[0] == 0 else m,
+                              "matched_three_product_vs_rgf": e2}
+           for d, e1, e3, m, e2 in zip(devs, tuned_err, tuned3_err, matched, m3_err)}
+    print(f"# matched error (<= 1e-8 vs RGF): FP64 {dt_m64:.3f} s, three-product {dt_m3:.3f} s; "
+          f"tuned three-product {dt_3m:.3f} s; {err}", file=sys.stderr, flush=True)
     for d, st in zip(devs, streams):
         with torch.cuda.stream(st):
             d.run_wave(nE - wave, d.tuned, st)
@@ -577,7 +648,18 @@ def ddrgf_uneven(a, torch, api):
                              "concurrent_device_energies_per_s": round(3 * nE / dt_eager, 3),
                              "one_call_per_wave_seconds": round(dt_one, 3),
                              "one_call_device_energies_per_s": round(3 * nE / dt_one, 3),
-                             "one_call_equals_separate_calls": same}}
+                             "one_call_equals_separate_calls": same,
+                             "one_call_three_product_seconds": round(dt_3m, 3),
+                             "one_call_three_product_device_energies_per_s":
+                                 round(3 * nE / dt_3m, 3),
+                             "matched_error_1e-8": {
+                                 "fp64_seconds": round(dt_m64, 3),
+                                 "fp64_device_energies_per_s": round(3 * nE / dt_m64, 3),
+                                 "three_product_seconds": round(dt_m3, 3),
+                                 "three_product_device_energies_per_s": round(3 * nE / dt_m3, 3),
+                                 "rule": "per device its tuned method where every energy's band "
+                                         "stays within 1e-8 of FP64 RGF, else RGF"},
+                             "max_block_relerr_vs_fp64_rgf_all_energies": err}}
     print(json.dumps(out))
 
 

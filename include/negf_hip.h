@@ -489,6 +489,14 @@ typedef struct {
 } NegfBandJob;
 
 int negf_bndiag_devices(int32_t njobs, const NegfBandJob* jobs, void* stream);
+/* negf_bndiag_devices with every block product of the jobs (RGF and DD-RGF
+ * alike) in `precision`: 0 = FP64 (the call above), 3 = FP64 with three real
+ * products per complex product (Gauss; normwise FP64-accurate, 3/4 of the
+ * matrix-core work).  The diagonal-block inverses stay as they are.  No
+ * counterpart in the reference (its reduced-precision intent: FieldType,
+ * test/common_to_test.jl:20-26). */
+int negf_bndiag_devices_prec(int32_t njobs, const NegfBandJob* jobs, int32_t precision,
+                             void* stream);
 
 /* -------------------------------------------------------------- tuning */
 

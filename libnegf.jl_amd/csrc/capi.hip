@@ -1211,6 +1211,13 @@ int negf_bndiag_devices(int32_t njobs, const NegfBandJob* jobs, void* stream) {
   return first_err;
 }
 
+int negf_bndiag_devices_prec(int32_t njobs, const NegfBandJob* jobs, int32_t precision,
+                             void* stream) {
+  if (precision != 0 && precision != 3) return NEGF_BAD_ARG;
+  ProductScope scope(precision);
+  return negf_bndiag_devices(njobs, jobs, stream);
+}
+
 // ------------------------------------------------------------ selected inverse
 int negf_block_fill_pattern(int32_t npl, int32_t nnzb, const int32_t* brow, const int32_t* bcol,
                             int32_t capacity, int32_t* out_brow, int32_t* out_bcol) {

@@ -174,6 +174,12 @@ __global__ void energy_estimate_kernel(const double* nrm, int N, int nb, double 
 
 }  // namespace
 
+ProductScope::ProductScope(int prec)
+    : prec_(t_prec.prec), nmod_(t_prec.nmod), work_(t_prec.work), bytes_(t_prec.bytes) {
+  t_prec = PrecCtx{prec, 0, nullptr, 0};
+}
+ProductScope::~ProductScope() { t_prec = PrecCtx{prec_, nmod_, work_, bytes_}; }
+
 double energy_prec_unit(int prec, int nmod, int k) {
   if (prec == 1) return 0x1p-24;
   // FP64 with three real products: the imaginary part's bound carries
@@ -1038,6 +1044,7 @@ int gemm2(int opA, int opB, int b, zc alpha, const zc* A, long long sA, long lon
           long long sB, long long oB, zc beta, const zc* C, long long sC, long long oC, zc* D,
           long long sD, long long oD, int inner, int outer, hipStream_t s) {
   GemmArgs g{};
+  apply_prec(g);
   g.opA = opA; g.opB = opB; g.m = g.n = g.k = b;
   g.alpha = alpha; g.beta = beta;
   g.A = A; g.lda = b; g.strideA = sA; g.outerA = oA;

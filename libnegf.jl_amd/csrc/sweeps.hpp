@@ -119,6 +119,24 @@ int block_inverse(int npl, int b, int nnzb, const int* brow, const int* bcol, vo
                   long long ld, long long stride, int batch, const int* ipiv, void* work,
                   hipStream_t s);
 
+// The arithmetic of every product the sweeps issue from this host thread
+// while the scope lives: 0 = FP64 (four real products per complex product),
+// 3 = FP64 with three real products (Gauss); the previous setting comes back
+// on exit.  (The reduced forms 1 and 2 go through the energy job's own
+// scope, which also carries their workspace.)
+class ProductScope {
+ public:
+  explicit ProductScope(int prec);
+  ~ProductScope();
+  ProductScope(const ProductScope&) = delete;
+  ProductScope& operator=(const ProductScope&) = delete;
+
+ private:
+  int prec_, nmod_;
+  void* work_;
+  long long bytes_;
+};
+
 int zaxpy_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                  const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                  long long strideB, int batch, hipStream_t s);

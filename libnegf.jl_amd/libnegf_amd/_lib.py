@@ -107,6 +107,7 @@ SIGNATURES = {
                                         c_int32_p, i32, vp, vp]),
     "negf_ddrgf_blocksizes_workspace": (C.c_int64, [i32, c_int32_p, i32, c_int32_p, i32]),
     "negf_bndiag_devices": (C.c_int, [i32, vp, vp]),
+    "negf_bndiag_devices_prec": (C.c_int, [i32, vp, i32, vp]),
     "negf_rgf_rkd_mixed": (C.c_int, [i32, c_int32_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
                                      vp, vp, vp, vp, vp, vp, i64, vp, i64, vp, i32, vp, i32, i32,
                                      C.c_double, vp, vp, vp, vp]),

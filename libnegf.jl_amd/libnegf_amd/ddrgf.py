@@ -183,12 +183,15 @@ def bndiag_of_inv_ddrgf_error_inv_of_T11(Min: DeviceBlockTridiag, aux: AuxDataDD
     return torch.sqrt(num / den).cpu().tolist()
 
 
-def bndiag_devices(jobs, check: bool = True):
+def bndiag_devices(jobs, check: bool = True, precision: str = "fp64"):
     """Several devices in one call (negf_bndiag_devices): ``jobs`` is a list
     of (Mout, Min, levels) with levels a blockSizeD1 list, [0] for RGF or
     None for the tune table's choice.  The devices run concurrently on the
-    library's streams, joined back into the current stream.  Returns the
-    per-device info arrays."""
+    library's streams, joined back into the current stream.  ``precision``
+    "fp64_3m" computes every block product with three real products
+    (negf_bndiag_devices_prec).  Returns the per-device info arrays."""
+    if precision not in ("fp64", "fp64_3m"):
+        raise ValueError(f"bndiag_devices: precision {precision!r} (fp64 or fp64_3m)")
     from ._lib import NegfBandJob
     if len(jobs) > 8:
         raise ValueError("at most 8 devices per call")
@@ -207,8 +210,8 @@ def bndiag_devices(jobs, check: bool = True):
                              p(Mout.diag), p(Mout.upper), p(Mout.lower), Min.batch,
                              C.cast(lv, C.POINTER(C.c_int32)) if lv is not None else None,
                              len(levels) if levels is not None else 0, p(info))
-    call("negf_bndiag_devices", len(jobs), C.cast(arr, C.c_void_p),
-         C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    call("negf_bndiag_devices_prec", len(jobs), C.cast(arr, C.c_void_p),
+         0 if precision == "fp64" else 3, C.c_void_p(torch.cuda.current_stream().cuda_stream))
     if check:
         for info in infos:
             bad = info.cpu()

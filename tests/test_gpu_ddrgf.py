@@ -150,6 +150,46 @@ def test_bndiag_devices_one_call_equals_separate_calls():
             assert max_block_err(_band(out.to_host(e)), bndiag_of_inv_rgf_local(A)) < 1e-11
 
 
+def test_bndiag_devices_three_product_form():
+    """negf_bndiag_devices_prec: precision 0 equals negf_bndiag_devices bit
+    for bit; precision 3 (every block product by three real products, the
+    DD-RGF chunk products included) stays within FP64 rounding of it and of
+    the oracle, and differs from it (so the three-product kernels ran:
+    block sizes 64, 128, 96 take the LDS kernels)."""
+    print("bndiag_devices three-product form", flush=True)
+    devs = [bm_create_synthetic_random(20, 64, seed=5, uneven=False),
+            bm_create_synthetic_random(8, 128, seed=6, uneven=False),
+            bm_create_synthetic_random(13, 96, seed=7, uneven=False)]
+    devs = [[A, _shifted(A, 0.25)] for A in devs]
+    levels = [[4], [0], None]
+    Mins = [DeviceBlockTridiag.from_host(m) for m in devs]
+    plain = [M.similar() for M in Mins]
+    dd.bndiag_devices([(o, m, lv) for o, m, lv in zip(plain, Mins, levels)])
+    p0 = [M.similar() for M in Mins]
+    dd.bndiag_devices([(o, m, lv) for o, m, lv in zip(p0, Mins, levels)], precision="fp64")
+    p3 = [M.similar() for M in Mins]
+    dd.bndiag_devices([(o, m, lv) for o, m, lv in zip(p3, Mins, levels)], precision="fp64_3m")
+    torch.cuda.synchronize()
+    differs = False
+    for k in range(len(devs)):
+        for name in ("diag", "upper", "lower"):
+            assert torch.equal(getattr(p0[k], name), getattr(plain[k], name)), (k, name)
+            differs |= not torch.equal(getattr(p3[k], name), getattr(plain[k], name))
+        for e, A in enumerate(devs[k]):
+            b3, b64 = _band(p3[k].to_host(e)), _band(plain[k].to_host(e))
+            ref = bndiag_of_inv_rgf_local(A)
+            e3, e64 = max_block_err(b3, ref), max_block_err(b64, ref)
+            print(f"  device {k} energy {e}: three-product {e3:.2e}, four-product {e64:.2e} "
+                  f"from the oracle; apart {max_block_err(b3, b64):.2e}", flush=True)
+            # the same rounding class: within a small multiple of the
+            # four-product form's own distance to the oracle
+            assert e3 < 8 * max(e64, 1e-14), (k, e, e3, e64)
+    assert differs, "the three-product form gave the FP64 bits everywhere"
+    from libnegf_amd._lib import call, NegfBandJob
+    with pytest.raises(Exception):
+        call("negf_bndiag_devices_prec", 0, None, 1, None)
+
+
 @pytest.fixture
 def clean_tune_table():
     """The tuner writes into the library's table; restore the loaded table
