@@ -352,9 +352,9 @@ class EnergiesGraph:
                  Gless_diag: torch.Tensor | None = None,
                  transmission: torch.Tensor | None = None, wave: int = 0, streams: int = 0,
                  precision: str = "fp64"):
-        _require_gpu()
         if precision not in ("fp64", "fp64_3m"):
             raise ValueError(f"EnergiesGraph: precision {precision!r} (fp64 or fp64_3m)")
+        _require_gpu()
         nE = int(energies.numel())
         self.precision = precision
         self.info = torch.zeros(nE, dtype=torch.int32, device=energies.device)

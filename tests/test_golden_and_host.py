@@ -312,6 +312,26 @@ def test_new_entry_points_refuse_bad_arguments_without_a_gpu():
     assert lib.negf_ddrgf_blocksizes_workspace(12, uneven, 2, (ctypes.c_int32 * 1)(10), 1) == 0
     big = (ctypes.c_int32 * 1025)(*([4, 5] * 512 + [4]))
     assert lib.negf_ddrgf_blocksizes_workspace(1025, big, 1, lv, 1) == 0   # > 1024 uneven blocks
+    # the three-product forms: precision 0 or 3 only, checked before any device call
+    for prec in (1, 2, 4, -1):
+        assert lib.negf_bndiag_devices_prec(0, None, prec, None) == BAD
+        g = ctypes.c_void_p()
+        assert lib.negf_rgf_keldysh_energies_graph_create_prec(
+            4, 64, None, None, None, None, None, 8, 1e-4, 0.1, -0.1, 0.025, None, None, None,
+            None, 8, 1, prec, ctypes.byref(g)) == BAD
+        assert not g.value
+    assert lib.negf_bndiag_devices_prec(0, None, 3, None) == 0
+    assert lib.negf_energies_graph_estimates(None, None, None) == BAD
+
+
+def test_python_precision_arguments_are_checked_first():
+    """The Python wrappers refuse an unknown product form before touching a
+    device (so the error is the argument's, on any host)."""
+    from libnegf_amd import api, ddrgf as dd
+    with pytest.raises(ValueError):
+        dd.bndiag_devices([], precision="fp32")
+    with pytest.raises(ValueError):
+        api.EnergiesGraph(None, None, 1e-4, 0.1, -0.1, 0.025, precision="emulated")
 
 
 def test_leading_dimension_of_single_column_and_single_row_matrices():
