@@ -264,6 +264,10 @@ int zcopy3d_launch(long long count, int n1, int n2, int n3, zc alpha, const zc* 
 int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, const zc* H,
                      long long hbs, const zc* E, long long ebs, zc* M, long long mbs, int batch,
                      hipStream_t s);
+// Pivoted Cholesky factor of Hermitian positive semidefinite matrices:
+// F F^H = A (F = P L, zero beyond the numerical rank), n x n, ld n, `batch`
+// instances back to back; A is overwritten; perm: n ints per instance.
+int zpchol_launch(int n, zc* A, zc* F, int* perm, int batch, hipStream_t s);
 int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                   const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                   long long strideB, int batch, hipStream_t s);

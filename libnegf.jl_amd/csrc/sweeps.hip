@@ -117,13 +117,17 @@ __global__ void zero_int_kernel(int* p, int n) {
 // z_e = E_e + i eta, fL_e = f(E_e; mu_L), fR_e = f(E_e; mu_R) as complex scales,
 // fermi = 1 / (1 + exp((E - mu) / kT)) in the overflow-safe tanh form.
 __global__ void energy_scalars_kernel(int n, const double* E, double eta, double muL, double muR,
-                                      double kT, zc* z, zc* fL, zc* fR) {
+                                      double kT, zc* z, zc* fL, zc* fR, zc* sfL, zc* sfR) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double e = E[i];
   z[i] = zmake(e, eta);
-  fL[i] = zmake(0.5 * (1.0 - tanh(0.5 * (e - muL) / kT)), 0.0);
-  fR[i] = zmake(0.5 * (1.0 - tanh(0.5 * (e - muR) / kT)), 0.0);
+  const double l = 0.5 * (1.0 - tanh(0.5 * (e - muL) / kT));
+  const double r = 0.5 * (1.0 - tanh(0.5 * (e - muR) / kT));
+  fL[i] = zmake(l, 0.0);
+  fR[i] = zmake(r, 0.0);
+  sfL[i] = zmake(sqrt(l), 0.0);
+  sfR[i] = zmake(sqrt(r), 0.0);
 }
 
 // nrm[e * N + i] = ||g_i||_F for the b x b blocks g_i of instance e (one
@@ -481,94 +485,66 @@ static int rkd(const Layout& L, const SweepWork& w, int batch, zc* Sd, long long
   return Status::OK;
 }
 
-// Upward Keldysh step of the energy driver.  Sigma^n lives on the contact
-// layers only (its off-diagonal blocks are zero on entry), so the reference's
-// upward step (src/keldyshndiag.jl:230-234) reduces to Su_i = -t_i Sd_{i+1}
-// feeding Sd_i -= Su_i t_i^H only, i.e. Sd_i += t_i Sd_{i+1} t_i^H.  The step
-// is linear in Sigma; the energy pipeline runs it on Gamma_R alone (unscaled):
-// Sd_{N-1} = Gamma_R on entry, and since Sigma^n_ii = 0 for i < N-1 the
-// "+=" is an assignment, B_j = t_j B_{j+1} t_j^H written over whatever Sd_j
-// held (so the blocks need no clearing).  Its B_0 gives the transmission
-// T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_{0,N-1} Gamma_R
-// G_{0,N-1}^H), G_{0,N-1} = G_00 prod(-t)), and f_R B_j is the right-injected
-// part of G^n (rkd_inject).
-static int rkd_contacts_up(const Layout& L, const SweepWork& w, int batch, zc* Sd,
-                           long long sd_bs, hipStream_t s) {
-  const int N = L.N;
-  const long long obs = L.no;
+// Keldysh diagonal of the energy driver in injection form, factored.  With
+// Sigma^n = f_L Gamma_L on layer 0 and f_R Gamma_R on layer N-1 only (its
+// off-diagonal blocks zero), the reference's passes (src/keldyshndiag.jl:
+// 221-318) give
+//   G^n_jj = f_L G_j0 Gamma_L G_j0^H + f_R G_j,N-1 Gamma_R G_j,N-1^H,
+// and RGF's own recursions give both columns: G_j0 = -l_j G_j-1,0 and
+// G_j,N-1 = G_jj prod_k>=j (-t_k).  So the left term is C_j = l_j C_j-1 l_j^H
+// (C_0 = f_L G_00 Gamma_L G_00^H) and the right one f_R G_jj B_j G_jj^H with
+// B_j = t_j B_j+1 t_j^H, B_N-1 = Gamma_R -- the reference's upward Keldysh
+// step (keldyshndiag.jl:230-234) run on Gamma_R alone -- whose B_0 gives
+// T = Re Tr(Gamma_L G_00 B_0 G_00^H) (= Tr(Gamma_L G_0,N-1 Gamma_R
+// G_0,N-1^H)) without the first-row chain.  Carried as factors: with the
+// contacts' Gamma_R = F_R F_R^H and Gamma_L = F_L F_L^H (pivoted Cholesky,
+// once per job), B_j = E_j E_j^H with E_j = t_j E_j+1, E_N-1 = F_R, and
+// C_j = f_L F_j F_j^H with F_j = l_j F_j-1, F_0 = G_00 F_L, so
+//   G^n_jj = Z_j Z_j^H,  Z_j = [sqrt(f_R) G_jj E_j | sqrt(f_L) F_j]  (b x 2b),
+// one Hermitian product with k = 2b (its upper tiles mirrored: the output is
+// read by nothing later).  Per layer E_j, F_j, W_j = sqrt(f_R) G_jj E_j and
+// G^< = i Z Z^H: 4 GEMMs, the last at twice the depth and half the tiles,
+// against 6 for B_j (2), C_j (2), V and G^< in the unfactored form; the same
+// recursions, so the same products of t's and l's (a factor carried instead
+// of its Gram matrix), and the Gram matrices come out Hermitian positive
+// semidefinite by construction.  T = Re Tr(Gamma_L V V^H), V = G_00 E_0.
+// Z_j alternates between two buffers (F_j is formed from F_j-1).
+static int rkd_factored(const Layout& L, const SweepWork& w, int batch, zc* Ed, long long ed_bs,
+                        const zc* FL, const zc* FR, const zc* Gd, long long gd_bs,
+                        const zc* gammaL, const zc* sfL, const zc* sfR, zc* Z0, zc* Z1,
+                        double* T, zc* Gless, long long gl_bs, hipStream_t s) {
+  const int N = L.N, b = L.b[0];
   const long long xbs = (long long)L.bmax * L.bmax;
-  auto sd = [&](int i) { return MW{Sd + L.od[i], L.b[i], sd_bs}; };
-  auto t = [&](int i) { return M{w.t + L.oo[i], L.b[i], obs}; };
-  const M none{nullptr, 1, 0};
-  // upward (keldyshndiag.jl:230-234 with Su = 0 on entry)
-  for (int i = N - 2; i >= 0; --i) {
-    const int bi = L.b[i], bj = L.b[i + 1];
-    MW T1{w.T1, bi, xbs};
-    RC(gemm(OP_N, OP_N, bi, bj, bj, ONE, t(i), ro(sd(i + 1)), ZERO, none, T1, batch, s));
-    RC(gemm(OP_N, OP_C, bi, bi, bj, ONE, ro(T1), t(i), ZERO, none, sd(i), batch, s));
-  }
-  return Status::OK;
-}
-
-// T = Re Tr(Gamma_L G_00 B_0 G_00^H) = Re sum (Gamma_L G_00 B_0) .* conj(G_00).
-static int transmission_from_b0(const Layout& L, const SweepWork& w, int batch, M G00, M B0,
-                                M gammaL, double* T, hipStream_t s) {
-  const long long xbs = (long long)L.bmax * L.bmax;
-  const int b0 = L.b[0];
-  const M none{nullptr, 1, 0};
-  MW V{w.T1, b0, xbs}, P{w.T2, b0, xbs};
-  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, G00, B0, ZERO, none, V, batch, s));
-  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, gammaL, ro(V), ZERO, none, P, batch, s));
-  return zdotc_re_launch(b0, b0, P.p, b0, xbs, G00.p, G00.ld, G00.bs, T, batch, s);
-}
-
-// Keldysh diagonal of the energy driver in injection form.  With Sigma^n =
-// f_L Gamma_L on layer 0 and f_R Gamma_R on layer N-1 only,
-//   G^n_jj = f_L G_j0 Gamma_L G_j0^H + f_R G_j,N-1 Gamma_R G_j,N-1^H.
-// RGF gives both columns through its own recursions: G_j0 = -l_j G_j-1,0, so
-// C_j = G_j0 (f_L Gamma_L) G_j0^H = l_j C_j-1 l_j^H, C_0 = G_00 (f_L Gamma_L)
-// G_00^H; and G_j,N-1 = G_jj prod_k>=j (-t_k), so the right term is
-// f_R G_jj B_j G_jj^H with B_j = t_j B_j+1 t_j^H, B_N-1 = Gamma_R (the upward
-// step, already in Sd, unscaled).  Two sums of Hermitian positive
-// semidefinite terms, no cancellation.  Per layer: Q = l_j C_j-1, C_j =
-// Q l_j^H, V = f_R G_jj B_j (f_R per energy in the GEMM's epilogue), G^<_jj =
-// i V G_jj^H + i C_j (the last GEMM writes the output).  The same G^n as the reference's passes (src/keldyshndiag.jl:
-// 221-318) for this Sigma^n, in 4 GEMMs per layer instead of their 5 after
-// the upward step; with RGF (5) and the upward step (2), 11 GEMMs and one
-// inverse per layer for G^R, G^< and T (the reference's sequence: 21 and
-// three LU solves).
-static int rkd_inject(const Layout& L, const SweepWork& w, int batch, const zc* Sd,
-                      long long sd_bs, const zc* Gd, long long gd_bs, const zc* gammaL,
-                      const zc* fL, const zc* fR, zc* Gless, long long gl_bs, hipStream_t s) {
-  const int N = L.N;
-  const long long xbs = (long long)L.bmax * L.bmax;
+  const long long zbs = 2LL * b * b;
   const zc I1 = zmake(0.0, 1.0);
-  auto G = [&](int i) { return M{Gd + L.od[i], L.b[i], gd_bs}; };
-  auto l = [&](int i) { return M{w.l + L.oo[i - 1], L.b[i], L.no}; };  // l_i = S_i^{-1} A_{i,i-1}
   const M none{nullptr, 1, 0};
-  const int b0 = L.b[0];
-  // C_0: T2 = f_L Gamma_L, T1 = G_00 T2, T2 = T1 G_00^H
-  RC(zscale_launch(b0, b0, ZERO, fL, 1, gammaL, b0, 0, w.T2, b0, xbs, batch, s));
-  RC(gemm(OP_N, OP_N, b0, b0, b0, ONE, G(0), M{w.T2, b0, xbs}, ZERO, none, MW{w.T1, b0, xbs},
-          batch, s));
-  RC(gemm(OP_N, OP_C, b0, b0, b0, ONE, M{w.T1, b0, xbs}, G(0), ZERO, none, MW{w.T2, b0, xbs},
-          batch, s));
+  auto G = [&](int i) { return M{Gd + L.od[i], b, gd_bs}; };
+  auto t = [&](int i) { return M{w.t + L.oo[i], b, L.no}; };
+  auto l = [&](int i) { return M{w.l + L.oo[i - 1], b, L.no}; };   // l_i = S_i^{-1} A_{i,i-1}
+  auto E = [&](int i) { return i == N - 1 ? M{FR, b, 0} : M{Ed + L.od[i], b, ed_bs}; };
+  for (int i = N - 2; i >= 0; --i)
+    RC(gemm(OP_N, OP_N, b, b, b, ONE, t(i), E(i + 1), ZERO, none, MW{Ed + L.od[i], b, ed_bs},
+            batch, s));
+  if (T) {
+    MW V{w.T1, b, xbs}, P{w.T2, b, xbs};
+    RC(gemm(OP_N, OP_N, b, b, b, ONE, G(0), E(0), ZERO, none, V, batch, s));
+    RC(gemm(OP_N, OP_N, b, b, b, ONE, M{gammaL, b, 0}, ro(V), ZERO, none, P, batch, s));
+    RC(zdotc_re_launch(b, b, P.p, b, xbs, V.p, b, xbs, T, batch, s));
+  }
   for (int j = 0; j < N; ++j) {
-    const int bj = L.b[j];
-    if (j > 0) {
-      const int bp = L.b[j - 1];
-      RC(gemm(OP_N, OP_N, bj, bp, bp, ONE, l(j), M{w.T2, bp, xbs}, ZERO, none, MW{w.X, bj, xbs},
+    zc* Zc = (j & 1) ? Z1 : Z0;
+    const zc* Zp = (j & 1) ? Z0 : Z1;
+    const long long fo = (long long)b * b;   // F_j: the second b columns of Z_j
+    if (j == 0)
+      RC(gemm(OP_N, OP_N, b, b, b, ONE, G(0), M{FL, b, 0}, ZERO, none, MW{Zc + fo, b, zbs}, batch,
+              s, nullptr, 0, sfL, 1));
+    else
+      RC(gemm(OP_N, OP_N, b, b, b, ONE, l(j), M{Zp + fo, b, zbs}, ZERO, none, MW{Zc + fo, b, zbs},
               batch, s));
-      RC(gemm(OP_N, OP_C, bj, bj, bp, ONE, M{w.X, bj, xbs}, l(j), ZERO, none, MW{w.T2, bj, xbs},
-              batch, s));
-    }
-    RC(gemm(OP_N, OP_N, bj, bj, bj, ONE, G(j), M{Sd + L.od[j], bj, sd_bs}, ZERO, none,
-            MW{w.T1, bj, xbs}, batch, s, nullptr, 0, fR, 1));
-    // G^<_jj = i (V G_jj^H + C_j) = i G^n_jj is anti-Hermitian and only
-    // written out (nothing later reads it), so its upper triangle is taken as
-    // the mirror of the lower one
-    RC(gemm(OP_N, OP_C, bj, bj, bj, I1, M{w.T1, bj, xbs}, G(j), I1, M{w.T2, bj, xbs},
-            MW{Gless + L.od[j], bj, gl_bs}, batch, s, nullptr, 0, nullptr, 0, -1));
+    RC(gemm(OP_N, OP_N, b, b, b, ONE, G(j), E(j), ZERO, none, MW{Zc, b, zbs}, batch, s, nullptr,
+            0, sfR, 1));
+    RC(gemm(OP_N, OP_C, b, b, 2 * b, I1, M{Zc, b, zbs}, M{Zc, b, zbs}, ZERO, none,
+            MW{Gless + L.od[j], b, gl_bs}, batch, s, nullptr, 0, nullptr, 0, -1));
   }
   return Status::OK;
 }
@@ -846,27 +822,43 @@ int sweep_general_rgf(int N, const int* b, int w, const zc* Ain, long long a_bs,
 }
 
 // One stream's share of the energy grid: energies [e_begin, e_end) in waves.
+// Per wave: the sweep's workspace, then Sd (the Keldysh pass's E_j) and G^R
+// when the caller does not keep it (nd each per energy), then the factored
+// pass's two Z buffers (2 b^2 each per energy).
 static size_t energies_group_fp64_bytes(const Layout& L, int wave) {
   return (sweep_work_bytes(L, wave, true) + 255) / 256 * 256 +
-         sizeof(zc) * (size_t)wave * (2 * L.nd) + 4096;
+         sizeof(zc) * (size_t)wave * (2 * L.nd) + sizeof(zc) * (size_t)wave * 4 * L.bmax * L.bmax +
+         4096;
 }
 
+// The job's scalars and contact factors (sweep_energies' header).
+struct EnergyJob {
+  const zc *z, *fL, *fR, *sfL, *sfR;
+  const zc *FL, *FR;   // F F^H = Gamma_L / Gamma_R (null: no G^< asked for)
+  int* einfo;
+};
+
 static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
-                          int e_begin, int e_end, const zc* z, const zc* fL, const zc* fR,
-                          int* einfo, const InvAux* aux, void* work, hipStream_t s,
-                          const EnergyPrec* pr) {
+                          int e_begin, int e_end, const EnergyJob& job, const InvAux* aux,
+                          void* work, hipStream_t s, const EnergyPrec* pr) {
   const int N = L.N, b = L.b[0];
+  const zc* z = job.z;
+  int* einfo = job.einfo;
   SweepWork w;
   carve(work, L, wave, &w);
   if (aux) w.inv.aux = *aux;
   zc* Sdw = (zc*)((char*)work + (sweep_work_bytes(L, wave, true) + 255) / 256 * 256);
   zc* Gdw = Sdw + (size_t)wave * L.nd;   // G^R diagonal when the caller does not keep it
+  zc* Z0 = Gdw + (size_t)wave * L.nd;    // the factored Keldysh pass's Z_j, two buffers
+  zc* Z1 = Z0 + (size_t)wave * 2 * L.bmax * L.bmax;
   // reduced-precision products: the emulation's residue workspace and the
   // per-layer norms of the estimate behind the FP64 group workspace
   const int prec = pr ? pr->prec : 0;
   char* extra = (char*)work + (energies_group_fp64_bytes(L, wave) + 255) / 256 * 256;
   size_t emu_bytes = 0;
-  if (prec == 2) emu_bytes = zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, wave, wave, pr->nmod);
+  // (the factored Keldysh pass's G^< product has k = 2 bmax)
+  if (prec == 2)
+    emu_bytes = zgemm_emu_work_bytes(L.bmax, L.bmax, 2 * L.bmax, wave, wave, pr->nmod);
   double* nrm = (double*)(extra + (emu_bytes + 255) / 256 * 256);
   PrecScope scope(PrecCtx{prec, pr ? pr->nmod : 0, prec == 2 ? (void*)extra : nullptr,
                           (long long)emu_bytes});
@@ -897,18 +889,12 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     if (out.T && !out.Gless_diag)
       RC(transmission(L, w, nb, M{in.gammaL, b, 0}, M{in.gammaR, b, 0}, out.T + e0, s));
     if (out.Gless_diag) {
-      // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere (its
-      // off-diagonal blocks are zero and never stored).  The upward step runs
-      // on Gamma_R alone (B_j, written layer by layer from B_N-1 = Gamma_R, so
-      // no other block needs clearing), which also yields T; then G^< = i G^n
-      // is written layer by layer in injection form, f_R applied there.
-      RC(zcopy2d_launch(b, b, in.gammaR, b, 0, Sdw + L.od[N - 1], b, L.nd, nb, s));
-      RC(rkd_contacts_up(L, w, nb, Sdw, L.nd, s));
-      if (out.T)
-        RC(transmission_from_b0(L, w, nb, M{grd, b, L.nd}, M{Sdw, b, L.nd}, M{in.gammaL, b, 0},
-                                out.T + e0, s));
-      RC(rkd_inject(L, w, nb, Sdw, L.nd, grd, L.nd, in.gammaL, fL + e0, fR + e0,
-                    out.Gless_diag + (size_t)e0 * L.nd, L.nd, s));
+      // Sigma^n = fL Gamma_L (+) fR Gamma_R on the diagonal, zero elsewhere:
+      // G^< = i G^n layer by layer in the factored injection form (E_j in Sd),
+      // T from E_0
+      RC(rkd_factored(L, w, nb, Sdw, L.nd, job.FL, job.FR, grd, L.nd, in.gammaL, job.sfL + e0,
+                      job.sfR + e0, Z0, Z1, out.T ? out.T + e0 : nullptr,
+                      out.Gless_diag + (size_t)e0 * L.nd, L.nd, s));
     }
   }
   return Status::OK;
@@ -917,13 +903,22 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
 size_t energies_group_bytes(const Layout& L, int wave, const EnergyPrec* pr) {
   size_t b = (energies_group_fp64_bytes(L, wave) + 255) / 256 * 256;
   if (pr && pr->prec == 2)
-    b += (zgemm_emu_work_bytes(L.bmax, L.bmax, L.bmax, wave, wave, pr->nmod) + 255) / 256 * 256;
+    b += (zgemm_emu_work_bytes(L.bmax, L.bmax, 2 * L.bmax, wave, wave, pr->nmod) + 255) / 256 *
+         256;
   if (pr && pr->est) b += sizeof(double) * (size_t)wave * L.N + 256;
   return b;
 }
 
+// The job's header: z, f_L, f_R, sqrt(f_L), sqrt(f_R) and info per energy,
+// then the contact factors F_L, F_R, their Cholesky scratch and pivots.
+static size_t energies_header_bytes(const Layout& L, int nE) {
+  const size_t bb = (size_t)L.bmax * L.bmax;
+  return (sizeof(zc) * 5 * (size_t)nE + sizeof(int) * (size_t)nE + 255) / 256 * 256 +
+         (sizeof(zc) * 4 * bb + 255) / 256 * 256 + sizeof(int) * 2 * (size_t)L.bmax + 4096;
+}
+
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams, const EnergyPrec* pr) {
-  return sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
+  return (energies_header_bytes(L, nE) + 255) / 256 * 256 +
          (size_t)streams * ((energies_group_bytes(L, wave, pr) + 255) / 256 * 256);
 }
 
@@ -934,19 +929,32 @@ size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams, const
 int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                    int streams, const EnergyStreams& es, void* work, hipStream_t s,
                    const EnergyPrec* pr) {
-  const int nE = in.nE;
+  const int nE = in.nE, b = L.bmax;
   char* p = (char*)work;
   zc* z = (zc*)p;
   zc* fL = z + nE;
   zc* fR = fL + nE;
-  int* einfo = (int*)(fR + nE);
-  char* groups = (char*)work + (sizeof(zc) * 3 * (size_t)nE + sizeof(int) * (size_t)nE + 4096 +
-                                255) / 256 * 256;
+  zc* sfL = fR + nE;
+  zc* sfR = sfL + nE;
+  int* einfo = (int*)(sfR + nE);
+  zc* F = (zc*)(p + (sizeof(zc) * 5 * (size_t)nE + sizeof(int) * (size_t)nE + 255) / 256 * 256);
+  zc* Fw = F + 2 * (size_t)b * b;
+  int* perm = (int*)(Fw + 2 * (size_t)b * b);
+  char* groups = (char*)work + (energies_header_bytes(L, nE) + 255) / 256 * 256;
   const size_t gbytes = (energies_group_bytes(L, wave, pr) + 255) / 256 * 256;
   hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
-                     in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR);
+                     in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR, sfL, sfR);
+  EnergyJob job{z, fL, fR, sfL, sfR, nullptr, nullptr, einfo};
+  if (out.Gless_diag) {
+    // the contacts' factors, once per job: F F^H = Gamma (pivoted Cholesky)
+    RC(zcopy2d_launch(b, b, in.gammaL, b, 0, Fw, b, 0, 1, s));
+    RC(zcopy2d_launch(b, b, in.gammaR, b, 0, Fw + (size_t)b * b, b, 0, 1, s));
+    RC(zpchol_launch(b, Fw, F, perm, 2, s));
+    job.FL = F;
+    job.FR = F + (size_t)b * b;
+  }
   if (streams <= 1)
-    return energies_group(L, in, out, wave, 0, nE, z, fL, fR, einfo, es.aux, groups, s, pr);
+    return energies_group(L, in, out, wave, 0, nE, job, es.aux, groups, s, pr);
   hipStream_t* pool = es.pool;
   // fork / join events: the caller's (kept for a captured graph's lifetime),
   // else created here and destroyed on return
@@ -972,7 +980,7 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
       rc = Status::HIP_ERROR;
       break;
     }
-    rc = energies_group(L, in, out, wave, e0, e1, z, fL, fR, einfo, es.aux ? es.aux + g : nullptr,
+    rc = energies_group(L, in, out, wave, e0, e1, job, es.aux ? es.aux + g : nullptr,
                         groups + g * gbytes, pool[g], pr);
     // join group g back into the caller's stream; where the event cannot
     // carry the edge, wait for the group on the host so later work on `s`

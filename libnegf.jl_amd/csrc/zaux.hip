@@ -3,6 +3,7 @@
 // (be_copy_in_hw!), fill / identity (be_fill!, be_identity, be_zero_array),
 // scaling, and the trace reduction of the transmission.
 #include "negf_internal.hpp"
+#include <climits>
 #include "sweeps.hpp"
 
 namespace negf {
@@ -327,6 +328,124 @@ int zdotc_re_launch(int m, int n, const zc* P, long long ldp, long long strideP,
   hipLaunchKernelGGL(dotc_re_kernel, dim3(batch), dim3(256), 0, s, m, n, P, ldp, strideP, Q, ldq,
                      strideQ, out);
   return ok();
+}
+
+namespace {
+
+// Pivoted Cholesky of a Hermitian positive semidefinite matrix (diagonal
+// pivoting, as LAPACK's zpstrf): P^T A P = L L^H, stopped where the largest
+// remaining diagonal of the Schur complement falls to n * 2^-52 times the
+// largest diagonal of A (the rank r); F = P L (zero beyond column r), so
+// F F^H = A + O(u ||A||).  One 1024-thread workgroup per instance, A
+// factored in place in full storage (both triangles of the Schur complement
+// kept, so a pivot swap is a plain row and column exchange); perm: n ints
+// per instance.
+constexpr int kCholThreads = 1024;
+
+__global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restrict__ A,
+                                                              zc* __restrict__ F,
+                                                              int* __restrict__ perm_ws) {
+  constexpr int NT = kCholThreads, NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  zc* a = A + (long long)blockIdx.x * n * n;
+  zc* f = F + (long long)blockIdx.x * n * n;
+  int* perm = perm_ws + (long long)blockIdx.x * n;
+  __shared__ double rv[NW];
+  __shared__ int ri[NW];
+  __shared__ double dmax0;
+  for (int i = tid; i < n; i += NT) perm[i] = i;
+  // the largest diagonal from position k on, lowest index on ties
+  auto argmax_diag = [&](int k, double* v_out, int* i_out) {
+    double bv = -1.0;
+    int bi = INT_MAX;
+    for (int i = k + tid; i < n; i += NT) {
+      const double d = a[i + (long long)i * n].x;
+      if (d > bv) { bv = d; bi = i; }
+    }
+    const double wmax = wave_max(bv);
+    const unsigned long long m = __ballot(bv == wmax && bi != INT_MAX);
+    int wi = INT_MAX;
+    if (m) wi = __shfl(bi, __ffsll(m) - 1);
+    __syncthreads();   // the slots are free
+    if (lane == 0) { rv[wave] = wmax; ri[wave] = wi; }
+    __syncthreads();
+    double gv = rv[0];
+    for (int w = 1; w < NW; ++w) gv = fmax(gv, rv[w]);
+    int gi = INT_MAX;
+    for (int w = 0; w < NW; ++w) gi = min(gi, rv[w] == gv ? ri[w] : INT_MAX);
+    *v_out = gv;
+    *i_out = gi;
+  };
+  {
+    double v;
+    int i;
+    argmax_diag(0, &v, &i);
+    if (tid == 0) dmax0 = v;
+    __syncthreads();
+  }
+  const double tol = (double)n * 0x1p-52 * dmax0;
+  int r = n;
+  for (int k = 0; k < n; ++k) {
+    double dv;
+    int p;
+    argmax_diag(k, &dv, &p);
+    if (!(dv > tol)) {   // the rest of the Schur complement is numerically zero
+      r = k;
+      break;
+    }
+    if (p != k) {
+      for (int c = tid; c < n; c += NT) {   // rows k and p
+        const zc x = a[k + (long long)c * n];
+        a[k + (long long)c * n] = a[p + (long long)c * n];
+        a[p + (long long)c * n] = x;
+      }
+      __syncthreads();
+      for (int rr = tid; rr < n; rr += NT) {   // columns k and p
+        const zc x = a[rr + (long long)k * n];
+        a[rr + (long long)k * n] = a[rr + (long long)p * n];
+        a[rr + (long long)p * n] = x;
+      }
+      if (tid == 0) {
+        const int t = perm[k];
+        perm[k] = perm[p];
+        perm[p] = t;
+      }
+      __syncthreads();
+    }
+    const double sq = sqrt(dv), rs = 1.0 / sq;
+    for (int i = k + 1 + tid; i < n; i += NT) {
+      const zc x = a[i + (long long)k * n];
+      a[i + (long long)k * n] = zmake(x.x * rs, x.y * rs);
+    }
+    if (tid == 0) a[k + (long long)k * n] = zmake(sq, 0.0);
+    __syncthreads();
+    // Schur complement of the trailing block (both triangles)
+    const int m = n - k - 1;
+    const long long tot = (long long)m * m;
+    for (long long e = tid; e < tot; e += NT) {
+      const int i = k + 1 + (int)(e % m), j = k + 1 + (int)(e / m);
+      const zc li = a[i + (long long)k * n], lj = a[j + (long long)k * n];
+      zc x = a[i + (long long)j * n];
+      x.x -= li.x * lj.x + li.y * lj.y;   // li conj(lj)
+      x.y -= li.y * lj.x - li.x * lj.y;
+      a[i + (long long)j * n] = x;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // F = P L: row perm[i] of F is row i of L
+  for (long long e = tid; e < (long long)n * n; e += NT) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    f[perm[i] + (long long)j * n] = (j < r && i >= j) ? a[i + (long long)j * n] : zmake(0.0, 0.0);
+  }
+}
+
+}  // namespace
+
+int zpchol_launch(int n, zc* A, zc* F, int* perm, int batch, hipStream_t s) {
+  if (n <= 0 || batch <= 0) return Status::BAD_ARG;
+  hipLaunchKernelGGL(zpchol_kernel, dim3(batch), dim3(kCholThreads), 0, s, n, A, F, perm);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
 }  // namespace negf

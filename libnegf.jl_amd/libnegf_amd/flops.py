@@ -17,18 +17,20 @@ reference's call sequence (credited).
 ``flops_energy_executed`` is what the GPU energy driver actually runs per
 energy: every S_i^{-1} solve replaced by one explicit inverse (8 b^3) and
 GEMMs, G_{i,i-1} not formed (only diagonal blocks are returned), and the
-Keldysh pass specialised to a Sigma^n on the contact layers: the upward step
-runs on Gamma_R alone (B_j = t_j B_{j+1} t_j^H; its B_0 gives
-T = Re Tr(Gamma_L G_00 B_0 G_00^H) without the first-row chain), and
-G^n_jj = f_L C_j + f_R G_jj B_j G_jj^H with C_j = l_j C_{j-1} l_j^H
-(injection form, exact for this Sigma^n):
-  GEMMs 3 (N-1) RGF upward + 2 (N-1) RGF downward + 2 (N-1) Keldysh upward
-        + 2 transmission + 4 N injection = 11 (N-1) + 6, inverses N
-  -> (88 (N-1) + 8 N + 48) b^3 = 8.19e11 at b = 512, N = 64,
-less the tiles the last injection GEMM of each layer skips: G^<_jj is
+Keldysh pass specialised to a Sigma^n on the contact layers in factored
+injection form (exact for this Sigma^n): with Gamma = F F^H (once per job),
+E_j = t_j E_{j+1} from E_{N-1} = F_R (its E_0 gives T = Re Tr(Gamma_L V V^H),
+V = G_00 E_0, without the first-row chain), F_j = l_j F_{j-1} from
+F_0 = G_00 F_L, and G^n_jj = Z_j Z_j^H with Z_j = [sqrt(f_R) G_jj E_j |
+sqrt(f_L) F_j] (one product with k = 2b):
+  GEMMs 3 (N-1) RGF upward + 2 (N-1) RGF downward + (N-1) E_j + 2
+        transmission + N F_j + N W_j + 2 N G^< (k = 2b) = 10 (N-1) + 6,
+        inverses N
+  -> (80 (N-1) + 8 N + 48) b^3 = 7.52e11 at b = 512, N = 64,
+less the tiles the G^< product of each layer skips: G^<_jj is
 anti-Hermitian, so for b a multiple of 64 only the T (T + 1) / 2 of the
 T^2 64 x 64 tiles (T = b / 64) on and below the diagonal are computed and the
-rest mirrored: 7.89e11 at b = 512, N = 64.
+rest mirrored: 6.92e11 at b = 512, N = 64.
 The hardware utilisation (fraction of FP64 peak) is computed from this count.
 With Gauss's three real products per complex product (``real_products`` =
 3) the matrix cores execute 6mnk of a product's 8mnk: the GEMM part scales
@@ -73,6 +75,6 @@ def injection_tile_fraction(b: int, wave: int | None = None) -> float:
 
 def flops_energy_executed(N: int, b: int, wave: int | None = None,
                           real_products: int = 4) -> float:
-    skipped = 8 * N * (1.0 - injection_tile_fraction(b, wave))
-    gemm = 88 * (N - 1) + 48 - skipped
+    skipped = 16 * N * (1.0 - injection_tile_fraction(b, wave))
+    gemm = 80 * (N - 1) + 48 - skipped
     return (gemm * real_products / 4 + 8 * N) * b ** 3

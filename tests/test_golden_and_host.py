@@ -114,19 +114,23 @@ def test_flop_counts_of_the_path():
     assert abs(flops_rgf_keldysh_energy(64, 512) - 1.511e12) / 1.511e12 < 1e-3
     from libnegf_amd import flops_energy_executed
     # what the energy driver launches per energy at N = 64: RGF upward 3 x 63
-    # (t, l, Schur), downward 2 x 63, Keldysh upward 2 x 63, transmission 2,
-    # injection 4 x 64 = 699 GEMMs of 512^3, and 64 inverses
-    n_gemm = 3 * 63 + 2 * 63 + 2 * 63 + 2 + 4 * 64
-    assert n_gemm == 699
-    # the 64 last injection GEMMs compute 36 of their 64 tiles (anti-Hermitian G^<)
-    assert flops_energy_executed(64, 512) == (8 * n_gemm + 8 * 64 - 8 * 64 * 28 / 64) * 512 ** 3
+    # (t, l, Schur), downward 2 x 63, the factored Keldysh pass's E_j 63,
+    # transmission 2, F_j 64, W_j 64 and G^< 64 at depth 2b (2 units each) =
+    # 636 units of a 512^3 GEMM, and 64 inverses
+    n_gemm = 3 * 63 + 2 * 63 + 63 + 2 + 64 + 64 + 2 * 64
+    assert n_gemm == 636
+    # the 64 G^< products compute 36 of their 64 tiles (anti-Hermitian G^<)
+    assert flops_energy_executed(64, 512) == (8 * n_gemm + 8 * 64 - 16 * 64 * 28 / 64) * 512 ** 3
     assert flops_energy_executed(64, 512, 96) == flops_energy_executed(64, 512)
     # b = 160: 32 x 32 tiles, 15 of 25 computed; b = 512 at a wave of 4
     # (256 tiles of 64 x 64): 32 x 32 tiles, 136 of 256
-    assert flops_energy_executed(4, 160) == (8 * (11 * 3 + 6) + 8 * 4 - 8 * 4 * 10 / 25) * 160 ** 3
-    assert flops_energy_executed(64, 512, 4) == (8 * n_gemm + 8 * 64 - 8 * 64 * 120 / 256) * 512 ** 3
+    assert flops_energy_executed(4, 160) == (8 * (10 * 3 + 6) + 8 * 4 - 16 * 4 * 10 / 25) * 160 ** 3
+    assert flops_energy_executed(64, 512, 4) == (8 * n_gemm + 8 * 64 - 16 * 64 * 120 / 256) * 512 ** 3
     # b = 100: register-staged GEMM, nothing mirrored
-    assert flops_energy_executed(4, 100) == (8 * (11 * 3 + 6) + 8 * 4) * 100 ** 3
+    assert flops_energy_executed(4, 100) == (8 * (10 * 3 + 6) + 8 * 4) * 100 ** 3
+    # the three-product form: the GEMM part at 3/4, the inverses unchanged
+    assert flops_energy_executed(64, 512, 96, 3) == \
+        ((8 * n_gemm - 16 * 64 * 28 / 64) * 3 / 4 + 8 * 64) * 512 ** 3
 
 
 def test_packed_layout_roundtrip_on_cpu_tensors():
