@@ -298,7 +298,13 @@ int negf_rgf_rkd_mixed(int32_t nblocks, const int32_t* block_sizes, const void* 
  * largest of 96, 64, 48, 32, ... whose workspace fits 90 % of free memory),
  * so one group's latency-bound steps overlap another's
  * GEMMs; `stream` waits for all of them.  info: device int32[nE] or NULL
- * (layer of the first singular Schur complement). */
+ * (layer of the first singular Schur complement).  gamma_L and gamma_R must
+ * be Hermitian positive semidefinite, as the broadening i(Sigma^R - Sigma^A)
+ * of a contact is: G^< is computed from their factors Gamma = F F^H, and
+ * when G^< is asked for and either is not (an entry of Gamma - Gamma^H, or of
+ * what its pivoted Cholesky factorization leaves at the numerical rank,
+ * above 2^-26 of its largest diagonal; NaN included) info[e] = -1 for every
+ * energy and G^< and T are not valid. */
 int negf_rgf_keldysh_energies(int32_t nblocks, int32_t b,
                               const void* H_diag, const void* H_upper,
                               const void* gamma_L, const void* gamma_R,

@@ -266,8 +266,10 @@ int zassemble_launch(long long count, const zc* z, const zc* S, long long sbs, c
                      hipStream_t s);
 // Pivoted Cholesky factor of Hermitian positive semidefinite matrices:
 // F F^H = A (F = P L, zero beyond the numerical rank), n x n, ld n, `batch`
-// instances back to back; A is overwritten; perm: n ints per instance.
-int zpchol_launch(int n, zc* A, zc* F, int* perm, int batch, hipStream_t s);
+// instances back to back; A is overwritten; perm: n ints per instance;
+// bad[instance] = 1 where A is not Hermitian positive semidefinite (to 2^-26
+// of its largest diagonal), else 0.
+int zpchol_launch(int n, zc* A, zc* F, int* perm, int* bad, int batch, hipStream_t s);
 int zscale_launch(int m, int n, zc value, const zc* scale_dev, long long scale_stride,
                   const zc* A, long long lda, long long strideA, zc* B, long long ldb,
                   long long strideB, int batch, hipStream_t s);

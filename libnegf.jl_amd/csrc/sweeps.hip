@@ -835,8 +835,17 @@ static size_t energies_group_fp64_bytes(const Layout& L, int wave) {
 struct EnergyJob {
   const zc *z, *fL, *fR, *sfL, *sfR;
   const zc *FL, *FR;   // F F^H = Gamma_L / Gamma_R (null: no G^< asked for)
+  const int* gbad;     // 2 flags: Gamma_L / Gamma_R not Hermitian PSD
   int* einfo;
 };
+
+// info[e] = -1 for every energy of the wave when a contact's Gamma is not
+// Hermitian positive semidefinite (the factored Keldysh pass needs Gamma =
+// F F^H); it overrides a singular layer's code.
+__global__ void gamma_info_kernel(const int* __restrict__ gbad, int* __restrict__ info, int nb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nb && (gbad[0] | gbad[1])) info[e] = -1;
+}
 
 static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& out, int wave,
                           int e_begin, int e_end, const EnergyJob& job, const InvAux* aux,
@@ -878,6 +887,9 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     S.shift = z + e0; S.shift_stride = 1;
     int* info = out.info ? out.info + e0 : einfo + e0;
     RC(upward(L, S, w, nb, info, s));
+    if (job.gbad)
+      hipLaunchKernelGGL(gamma_info_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, job.gbad,
+                         info, nb);
     if (pr && pr->est) {
       hipLaunchKernelGGL(block_fro_kernel, dim3(N, nb), dim3(256), 0, s, w.g, L.nd, b, N, nrm);
       hipLaunchKernelGGL(energy_estimate_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, nrm, N, nb,
@@ -910,11 +922,12 @@ size_t energies_group_bytes(const Layout& L, int wave, const EnergyPrec* pr) {
 }
 
 // The job's header: z, f_L, f_R, sqrt(f_L), sqrt(f_R) and info per energy,
-// then the contact factors F_L, F_R, their Cholesky scratch and pivots.
+// then the contact factors F_L, F_R, their Cholesky scratch, pivots and the
+// two PSD flags.
 static size_t energies_header_bytes(const Layout& L, int nE) {
   const size_t bb = (size_t)L.bmax * L.bmax;
   return (sizeof(zc) * 5 * (size_t)nE + sizeof(int) * (size_t)nE + 255) / 256 * 256 +
-         (sizeof(zc) * 4 * bb + 255) / 256 * 256 + sizeof(int) * 2 * (size_t)L.bmax + 4096;
+         (sizeof(zc) * 4 * bb + 255) / 256 * 256 + sizeof(int) * (2 * (size_t)L.bmax + 2) + 4096;
 }
 
 size_t energies_work_bytes(const Layout& L, int nE, int wave, int streams, const EnergyPrec* pr) {
@@ -940,18 +953,20 @@ int sweep_energies(const Layout& L, const EnergyIn& in, const EnergyOut& out, in
   zc* F = (zc*)(p + (sizeof(zc) * 5 * (size_t)nE + sizeof(int) * (size_t)nE + 255) / 256 * 256);
   zc* Fw = F + 2 * (size_t)b * b;
   int* perm = (int*)(Fw + 2 * (size_t)b * b);
+  int* gbad = perm + 2 * (size_t)b;
   char* groups = (char*)work + (energies_header_bytes(L, nE) + 255) / 256 * 256;
   const size_t gbytes = (energies_group_bytes(L, wave, pr) + 255) / 256 * 256;
   hipLaunchKernelGGL(energy_scalars_kernel, dim3((nE + 255) / 256), dim3(256), 0, s, nE,
                      in.energies, in.eta, in.muL, in.muR, in.kT, z, fL, fR, sfL, sfR);
-  EnergyJob job{z, fL, fR, sfL, sfR, nullptr, nullptr, einfo};
+  EnergyJob job{z, fL, fR, sfL, sfR, nullptr, nullptr, nullptr, einfo};
   if (out.Gless_diag) {
     // the contacts' factors, once per job: F F^H = Gamma (pivoted Cholesky)
     RC(zcopy2d_launch(b, b, in.gammaL, b, 0, Fw, b, 0, 1, s));
     RC(zcopy2d_launch(b, b, in.gammaR, b, 0, Fw + (size_t)b * b, b, 0, 1, s));
-    RC(zpchol_launch(b, Fw, F, perm, 2, s));
+    RC(zpchol_launch(b, Fw, F, perm, gbad, 2, s));
     job.FL = F;
     job.FR = F + (size_t)b * b;
+    job.gbad = gbad;
   }
   if (streams <= 1)
     return energies_group(L, in, out, wave, 0, nE, job, es.aux, groups, s, pr);

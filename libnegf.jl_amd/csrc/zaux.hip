@@ -339,12 +339,16 @@ namespace {
 // F F^H = A + O(u ||A||).  One 1024-thread workgroup per instance, A
 // factored in place in full storage (both triangles of the Schur complement
 // kept, so a pivot swap is a plain row and column exchange); perm: n ints
-// per instance.
+// per instance.  bad[instance] = 1 when A is not Hermitian positive
+// semidefinite to 2^-26 of its largest diagonal (|a_ij - conj(a_ji)|, or an
+// entry of the Schur complement left at the rank, above it; NaN included),
+// else 0: F F^H then does not stand for A.
 constexpr int kCholThreads = 1024;
 
 __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restrict__ A,
                                                               zc* __restrict__ F,
-                                                              int* __restrict__ perm_ws) {
+                                                              int* __restrict__ perm_ws,
+                                                              int* __restrict__ bad) {
   constexpr int NT = kCholThreads, NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   zc* a = A + (long long)blockIdx.x * n * n;
@@ -353,7 +357,27 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
   __shared__ double rv[NW];
   __shared__ int ri[NW];
   __shared__ double dmax0;
+  __shared__ int flag;
   for (int i = tid; i < n; i += NT) perm[i] = i;
+  if (tid == 0) flag = 0;
+  // the largest |a_ij| over the trailing block [k0, n)^2 (Hermitian part
+  // when herm), a NaN anywhere giving NaN
+  // (one column per wave, rows over the lanes)
+  auto max_entry = [&](int k0, bool herm) {
+    double v = 0.0;
+    for (int j = k0 + wave; j < n; j += NW)
+      for (int i = k0 + lane; i < n; i += 64) {
+        zc x = a[i + (long long)j * n];
+        if (herm) {
+          const zc y = a[j + (long long)i * n];
+          x.x -= y.x;
+          x.y += y.y;
+        }
+        const double ax = fabs(x.x) > fabs(x.y) ? fabs(x.x) : fabs(x.y);
+        v = (ax > v || ax != ax) ? ax : v;
+      }
+    return v;
+  };
   // the largest diagonal from position k on, lowest index on ties
   auto argmax_diag = [&](int k, double* v_out, int* i_out) {
     double bv = -1.0;
@@ -384,6 +408,8 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
     __syncthreads();
   }
   const double tol = (double)n * 0x1p-52 * dmax0;
+  const double bound = 0x1p-26 * dmax0;
+  if (!(max_entry(0, true) <= bound)) flag = 1;   // benign race: every writer stores 1
   int r = n;
   for (int k = 0; k < n; ++k) {
     double dv;
@@ -419,32 +445,40 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
     }
     if (tid == 0) a[k + (long long)k * n] = zmake(sq, 0.0);
     __syncthreads();
-    // Schur complement of the trailing block (both triangles)
-    const int m = n - k - 1;
-    const long long tot = (long long)m * m;
-    for (long long e = tid; e < tot; e += NT) {
-      const int i = k + 1 + (int)(e % m), j = k + 1 + (int)(e / m);
-      const zc li = a[i + (long long)k * n], lj = a[j + (long long)k * n];
-      zc x = a[i + (long long)j * n];
-      x.x -= li.x * lj.x + li.y * lj.y;   // li conj(lj)
-      x.y -= li.y * lj.x - li.x * lj.y;
-      a[i + (long long)j * n] = x;
+    // Schur complement of the trailing block (both triangles): one column
+    // per wave, rows over the lanes
+    const zc* lk = a + (long long)k * n;
+    for (int j = k + 1 + wave; j < n; j += NW) {
+      const zc lj = lk[j];
+      zc* aj = a + (long long)j * n;
+      for (int i = k + 1 + lane; i < n; i += 64) {
+        const zc li = lk[i];
+        zc x = aj[i];
+        x.x -= li.x * lj.x + li.y * lj.y;   // li conj(lj)
+        x.y -= li.y * lj.x - li.x * lj.y;
+        aj[i] = x;
+      }
     }
     __syncthreads();
   }
   __syncthreads();
+  // what the rank leaves must be numerically zero (a PSD Schur complement
+  // with every diagonal <= tol has every entry <= tol)
+  if (r < n && !(max_entry(r, false) <= bound)) flag = 1;
   // F = P L: row perm[i] of F is row i of L
-  for (long long e = tid; e < (long long)n * n; e += NT) {
-    const int i = (int)(e % n), j = (int)(e / n);
-    f[perm[i] + (long long)j * n] = (j < r && i >= j) ? a[i + (long long)j * n] : zmake(0.0, 0.0);
-  }
+  for (int j = wave; j < n; j += NW)
+    for (int i = lane; i < n; i += 64)
+      f[perm[i] + (long long)j * n] =
+          (j < r && i >= j) ? a[i + (long long)j * n] : zmake(0.0, 0.0);
+  __syncthreads();
+  if (tid == 0) bad[blockIdx.x] = flag;
 }
 
 }  // namespace
 
-int zpchol_launch(int n, zc* A, zc* F, int* perm, int batch, hipStream_t s) {
+int zpchol_launch(int n, zc* A, zc* F, int* perm, int* bad, int batch, hipStream_t s) {
   if (n <= 0 || batch <= 0) return Status::BAD_ARG;
-  hipLaunchKernelGGL(zpchol_kernel, dim3(batch), dim3(kCholThreads), 0, s, n, A, F, perm);
+  hipLaunchKernelGGL(zpchol_kernel, dim3(batch), dim3(kCholThreads), 0, s, n, A, F, perm, bad);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 

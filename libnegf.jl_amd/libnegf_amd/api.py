@@ -52,6 +52,11 @@ def release_workspace():
 
 def _check_info(where, info: torch.Tensor):
     bad = info.cpu()
+    if bool((bad < 0).any()):
+        # the energy job's code for a contact broadening that is not
+        # Hermitian positive semidefinite (its G^< uses Gamma = F F^H)
+        raise ValueError(f"{where}: a contact broadening Gamma is not Hermitian positive "
+                         f"semidefinite (info = {bad.tolist()})")
     if bool((bad != 0).any()):
         raise SingularError(where, bad.tolist())
 
@@ -286,6 +291,33 @@ class DeviceHamiltonianGPU:
         self.gamma_R = _pack_blocks([dev.gamma_R], device)
 
 
+def _check_energy_job(where, H, energies, GR_diag, Gless_diag, transmission):
+    """The energy job's tensors as the C ABI reads them: energies a contiguous
+    float64 vector, G^R / G^< complex128 and T float64, contiguous, with room
+    for every energy, all on a GPU (checked before any device call: a float32
+    energy vector would otherwise be read as half as many doubles, past its
+    end)."""
+    def need(name, t, dtype, count):
+        if t is None:
+            return
+        if not isinstance(t, torch.Tensor) or t.dtype != dtype:
+            raise ValueError(f"{where}: {name} must be a {dtype} tensor")
+        if not t.is_contiguous() or t.numel() < count:
+            raise ValueError(f"{where}: {name} must be contiguous with at least {count} elements")
+    if energies is None:
+        raise ValueError(f"{where}: energies must be a torch.float64 tensor")
+    need("energies", energies, torch.float64, 0)
+    nE = int(energies.numel())
+    per = H.N * H.b * H.b
+    need("GR_diag", GR_diag, torch.complex128, nE * per)
+    need("Gless_diag", Gless_diag, torch.complex128, nE * per)
+    need("transmission", transmission, torch.float64, nE)
+    for name, t in (("energies", energies), ("GR_diag", GR_diag), ("Gless_diag", Gless_diag),
+                    ("transmission", transmission)):
+        if t is not None and t.device.type != "cuda":
+            raise ValueError(f"{where}: {name} must be on the GPU")
+
+
 def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: float,
                          mu_L: float, mu_R: float, kT: float, GR_diag: torch.Tensor | None = None,
                          Gless_diag: torch.Tensor | None = None,
@@ -294,6 +326,7 @@ def rgf_keldysh_energies(H: DeviceHamiltonianGPU, energies: torch.Tensor, eta: f
     """G^R and G^< diagonal blocks and T(E) for every energy (device float64
     tensor).  GR_diag / Gless_diag: complex128 device tensors of shape
     (nE, N*b*b) (packed diag layout per energy) or None."""
+    _check_energy_job("rgf_keldysh_energies", H, energies, GR_diag, Gless_diag, transmission)
     _require_gpu()
     nE = int(energies.numel())
     info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
@@ -322,6 +355,10 @@ def rgf_keldysh_energies_mixed(H: DeviceHamiltonianGPU, energies: torch.Tensor, 
     inverses stay FP64.  Energies whose error estimate exceeds ``tol`` are
     computed again in FP64.  Returns (info, est, fallback): the device info
     array, the per-energy estimates and re-run flags (numpy)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"rgf_keldysh_energies_mixed: precision {precision!r}")
+    _check_energy_job("rgf_keldysh_energies_mixed", H, energies, GR_diag, Gless_diag,
+                      transmission)
     _require_gpu()
     nE = int(energies.numel())
     info = torch.zeros(nE, dtype=torch.int32, device=energies.device)
@@ -354,6 +391,7 @@ class EnergiesGraph:
                  precision: str = "fp64"):
         if precision not in ("fp64", "fp64_3m"):
             raise ValueError(f"EnergiesGraph: precision {precision!r} (fp64 or fp64_3m)")
+        _check_energy_job("EnergiesGraph", H, energies, GR_diag, Gless_diag, transmission)
         _require_gpu()
         nE = int(energies.numel())
         self.precision = precision

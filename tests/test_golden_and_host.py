@@ -338,6 +338,48 @@ def test_python_precision_arguments_are_checked_first():
         api.EnergiesGraph(None, None, 1e-4, 0.1, -0.1, 0.025, precision="emulated")
 
 
+def test_energy_job_tensor_checks_before_any_device_call():
+    """The energy job's wrappers refuse tensors the C ABI would misread (a
+    float32 energy vector read as doubles, outputs too small for every
+    energy, host memory) before touching a device."""
+    import types
+    import torch
+    from libnegf_amd import api
+    H = types.SimpleNamespace(N=2, b=4)
+    e32 = torch.zeros(3, dtype=torch.float32)
+    e64 = torch.zeros(3, dtype=torch.float64)
+    GR = torch.zeros((3, 2 * 16), dtype=torch.complex128)
+    with pytest.raises(ValueError, match="float64"):
+        api.rgf_keldysh_energies(H, e32, 1e-4, 0.1, -0.1, 0.025, GR)
+    with pytest.raises(ValueError, match="at least 96"):
+        api.rgf_keldysh_energies(H, e64, 1e-4, 0.1, -0.1, 0.025, GR[:2])
+    with pytest.raises(ValueError, match="complex128"):
+        api.rgf_keldysh_energies(H, e64, 1e-4, 0.1, -0.1, 0.025, None, GR.to(torch.complex64))
+    with pytest.raises(ValueError, match="at least 3"):
+        api.rgf_keldysh_energies_mixed(H, e64, 1e-4, 0.1, -0.1, 0.025, "fp64_3m",
+                                       transmission=e64[:2])
+    with pytest.raises(ValueError, match="on the GPU"):
+        api.EnergiesGraph(H, e64, 1e-4, 0.1, -0.1, 0.025, GR, precision="fp64_3m")
+    with pytest.raises(ValueError, match="precision"):
+        api.rgf_keldysh_energies_mixed(H, e64, 1e-4, 0.1, -0.1, 0.025, "fp16")
+
+
+def test_info_codes_map_to_python_errors():
+    """A positive info (the layer of a singular Schur complement) raises
+    SingularError, the energy job's -1 (a contact broadening that is not
+    Hermitian positive semidefinite) ValueError, all zeros nothing."""
+    import torch
+    from libnegf_amd.api import _check_info
+    from libnegf_amd._lib import SingularError
+    _check_info("x", torch.zeros(3, dtype=torch.int32))
+    with pytest.raises(SingularError):
+        _check_info("x", torch.tensor([0, 2, 0], dtype=torch.int32))
+    with pytest.raises(ValueError, match="positive semidefinite"):
+        _check_info("x", torch.tensor([-1, -1], dtype=torch.int32))
+    with pytest.raises(ValueError):
+        _check_info("x", torch.tensor([-1, 3], dtype=torch.int32))
+
+
 def test_leading_dimension_of_single_column_and_single_row_matrices():
     """torch gives a one-column tensor any stride for its column (often 1); the
     binding passes ld = m for it, and ld = the column stride otherwise."""

@@ -142,6 +142,14 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams, gemm):
         torch.cuda.synchronize()
     finally:
         api.call("negf_gemm_variant", 0)
+    _check_energies_vs_oracle(dev, Es, eta, muL, muR, kT, GR, GL, T, tol)
+
+
+def _check_energies_vs_oracle(dev, Es, eta, muL, muR, kT, GR, GL, T, tol):
+    """Every energy's G^R and G^< diagonal blocks and T against the oracle's
+    RGF, keldyshndiag (Sigma^n = f_L Gamma_L on layer 0, f_R Gamma_R on
+    layer N-1) and Tr(Gamma_L G_1N Gamma_R G_1N^H)."""
+    N, b = dev.nblocks, dev.b
     GR, GL, T = GR.cpu().numpy(), GL.cpu().numpy(), T.cpu().numpy()
     for e, E in enumerate(Es):
         A = assemble_A(dev, E, eta)
@@ -161,6 +169,63 @@ def test_energies_workload_vs_oracle(N, b, eta, Es, seed, tol, streams, gemm):
             blk = GL[e, i * b * b:(i + 1) * b * b].reshape(b, b).T
             assert relfro(blk, 1j * gd[i]) < tol
         assert abs(T[e] - Tref) < 10 * tol * max(1.0, abs(Tref))
+
+
+def _energies(dev, Es, eta, muL, muR, kT, check=True):
+    N, b, nE = dev.nblocks, dev.b, len(Es)
+    H = api.DeviceHamiltonianGPU(dev)
+    GR = torch.zeros((nE, N * b * b), dtype=torch.complex128, device="cuda")
+    GL = torch.zeros_like(GR)
+    T = torch.zeros(nE, dtype=torch.float64, device="cuda")
+    info = api.rgf_keldysh_energies(H, torch.tensor(Es, dtype=torch.float64, device="cuda"), eta,
+                                    muL, muR, kT, GR, GL, T, wave=2, check=check)
+    torch.cuda.synchronize()
+    return GR, GL, T, info
+
+
+def test_energies_rank_deficient_and_zero_broadening():
+    """The contacts' Gamma enter G^< through their pivoted Cholesky factors
+    Gamma = F F^H: a rank-3 Gamma_L (factor zero past column 3) and a zero
+    Gamma_R (factor zero, no right injection, T = 0) still match the
+    oracle, which uses Gamma itself.  eta = 1e-2: with no right broadening
+    the last layer keeps only eta, and at eta = 1e-3 the oracle's RGF and a
+    dense numpy inverse of A already differ by 1.3e-12 per block (2.2e-13 at
+    1e-2), above the 1e-12 bar."""
+    kT, muL, muR, eta = 0.025, 0.1, -0.1, 1e-2
+    dev = device_hamiltonian(4, 64, seed=8)
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((64, 3)) + 1j * rng.standard_normal((64, 3))
+    g = X @ X.conj().T / 3
+    dev.gamma_L = np.asfortranarray((g + g.conj().T) / 2)
+    dev.gamma_R = np.zeros((64, 64), complex, order="F")
+    print("energies: rank-3 Gamma_L, zero Gamma_R, N=4 b=64", flush=True)
+    GR, GL, T, info = _energies(dev, _ENERGIES, eta, muL, muR, kT)
+    assert not bool((info != 0).any())
+    _check_energies_vs_oracle(dev, _ENERGIES, eta, muL, muR, kT, GR, GL, T, 1e-12)
+    assert float(T.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("kind", ["negative", "indefinite", "non_hermitian"])
+def test_energies_broadening_not_psd_is_reported(kind):
+    """A Gamma that is not Hermitian positive semidefinite cannot be written
+    F F^H: the job sets info = -1 for every energy and the Python host
+    raises ValueError (the outputs are not valid)."""
+    kT, muL, muR, eta = 0.025, 0.1, -0.1, 1e-3
+    dev = device_hamiltonian(3, 64, seed=9)
+    g = dev.gamma_R.copy()
+    if kind == "negative":
+        g = -g
+    elif kind == "indefinite":
+        g[:, :] = 0.0
+        g[0, 1] = g[1, 0] = 1.0   # eigenvalues +1 and -1, zero diagonal
+    else:
+        g[0, 1] += 1e-3 * abs(g).max()
+    dev.gamma_R = np.asfortranarray(g)
+    print(f"energies: Gamma_R {kind}", flush=True)
+    *_, info = _energies(dev, _ENERGIES, eta, muL, muR, kT, check=False)
+    assert info.cpu().tolist() == [-1] * len(_ENERGIES)
+    with pytest.raises(ValueError, match="positive semidefinite"):
+        _energies(dev, _ENERGIES, eta, muL, muR, kT)
 
 
 @pytest.mark.parametrize("streams", [1, 2])
