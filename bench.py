@@ -376,6 +376,38 @@ def single_energy(a, torch, api):
         perop_band[(i, i + 1)] = Gp.upper[i]
         perop_band[(i + 1, i)] = Gp.lower[i]
     perop_err = max(rel(perop_band[k], band[k]) for k in band)
+    # the drop-in Keldysh sweep (keldyshndiag! with the band of G^R and T
+    # from one shared upward pass) on the same A with Sigma^n = f_L Gamma_L
+    # on layer 0 and f_R Gamma_R on layer N-1, against the energy job's G^< =
+    # i G^n and T at the same energy
+    kT, muL, muR = 0.025, 0.1, -0.1
+    fL, fR = float(synthetic.fermi(E, muL, kT)), float(synthetic.fermi(E, muR, kT))
+    Sd = [np.zeros((b, b), complex, order="F") for _ in range(N)]
+    Sd[0] = np.asfortranarray(fL * dev.gamma_L)
+    Sd[-1] = np.asfortranarray(fR * dev.gamma_R)
+    Su = [np.zeros((b, b), complex, order="F") for _ in range(N - 1)]
+    Sdev = DeviceBlockTridiag.from_host(synthetic.BlockTridiag(Sd, Su, None))
+    gl = torch.from_numpy(np.asfortranarray(dev.gamma_L).ravel(order="F")).cuda().view(b, b).T
+    gr = torch.from_numpy(np.asfortranarray(dev.gamma_R).ravel(order="F")).cuda().view(b, b).T
+    Gr = M1.similar(zero=False)
+    Gn = Sdev.similar(zero=False)
+    Tk = []
+    t_rkd = timed(lambda: Tk.append(api.rgf_keldysh_transmission(M1, Sdev, gl, gr, Gr, Gn)))
+    print(f"# drop-in RGF + RKD + T {t_rkd * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    Hd = api.DeviceHamiltonianGPU(dev)
+    GLj = torch.zeros((1, N * b * b), dtype=torch.complex128, device="cuda")
+    Tj = torch.zeros(1, dtype=torch.float64, device="cuda")
+    api.rgf_keldysh_energies(Hd, torch.tensor([E], dtype=torch.float64, device="cuda"), eta,
+                             muL, muR, kT, None, GLj, Tj, wave=1)
+    torch.cuda.synchronize()
+    Gnh = Gn.to_host(0)
+    GLh = GLj.cpu().numpy()[0]
+    rkd_err = max(rel(1j * Gnh.diag[i], GLh[i * b * b:(i + 1) * b * b].reshape(b, b).T)
+                  for i in range(N))
+    T_drop = float(Tk[-1].cpu()[0])
+    T_job = float(Tj.cpu()[0])
+    del Gr, Gn, Sdev, Hd, GLj
+    torch.cuda.empty_cache()
     cpu = None
     rgf_err = None
     if not a.no_cpu_baseline:
@@ -388,7 +420,8 @@ def single_energy(a, torch, api):
         rgf_err = max(rel(band[k], ref[k]) for k in ref)
         cpu = {"value": round(1.0 / dt, 4), "unit": "energies/s", "cores": cores, "kind": "port",
                "sample": f"the same energy's RGF through scipy/OpenBLAS 0.3.29, {dt:.2f} s"}
-    out = {"workload": "single_energy: N=64 b=512 E=0.3 eta=1e-4 (negf_rgf, selected inverse)",
+    out = {"workload": "single_energy: N=64 b=512 E=0.3 eta=1e-4 (negf_rgf, selected inverse, "
+                       "negf_rgf_rkd)",
            "rgf": rgf, "selected_inverse_ms": round(1e3 * (tf + ti), 1),
            "selected_inverse_split_ms": {"factorization": round(1e3 * tf, 1), "inverse": round(1e3 * ti, 1)},
            "perop_sweep_ms": round(1e3 * t_perop, 1),
@@ -397,6 +430,12 @@ def single_energy(a, torch, api):
                                "be_gemm per layer, the LU's info read on the host after each be_lu",
            "rgf_relerr_vs_cpu_oracle": rgf_err, "selinv_relerr_vs_gpu_rgf": si_err,
            "perop_relerr_vs_gpu_rgf": perop_err,
+           "rkd_dropin_ms": round(1e3 * t_rkd, 1),
+           "rkd_dropin_note": "negf_rgf_rkd (keldyshndiag! with the band of G^R and T from one "
+                              "shared upward pass; the reference's four Keldysh passes on a "
+                              "general Sigma^n) at one energy, Sigma^n = f_L Gamma_L (+) f_R Gamma_R",
+           "rkd_dropin_vs_energy_job": {"max_block_relerr_Gless": rkd_err,
+                                        "T_relerr": abs(T_drop - T_job) / abs(T_job)},
            "reps": reps, "cpu_baseline": cpu}
     print(json.dumps(out))
 

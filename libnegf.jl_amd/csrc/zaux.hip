@@ -344,6 +344,7 @@ namespace {
 // entry of the Schur complement left at the rank, above it; NaN included),
 // else 0: F F^H then does not stand for A.
 constexpr int kCholThreads = 1024;
+constexpr int kCholMaxN = 2048;
 
 __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restrict__ A,
                                                               zc* __restrict__ F,
@@ -358,6 +359,7 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
   __shared__ int ri[NW];
   __shared__ double dmax0;
   __shared__ int flag;
+  __shared__ zc lk[kCholMaxN];   // the column of L of the current step
   for (int i = tid; i < n; i += NT) perm[i] = i;
   if (tid == 0) flag = 0;
   // the largest |a_ij| over the trailing block [k0, n)^2 (Hermitian part
@@ -441,23 +443,32 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
     const double sq = sqrt(dv), rs = 1.0 / sq;
     for (int i = k + 1 + tid; i < n; i += NT) {
       const zc x = a[i + (long long)k * n];
-      a[i + (long long)k * n] = zmake(x.x * rs, x.y * rs);
+      const zc y = zmake(x.x * rs, x.y * rs);
+      a[i + (long long)k * n] = y;
+      lk[i] = y;
     }
     if (tid == 0) a[k + (long long)k * n] = zmake(sq, 0.0);
     __syncthreads();
     // Schur complement of the trailing block (both triangles): one column
-    // per wave, rows over the lanes
-    const zc* lk = a + (long long)k * n;
+    // per wave, rows over the lanes, four rows per lane loaded before any
+    // is stored (the column of L from LDS)
     for (int j = k + 1 + wave; j < n; j += NW) {
       const zc lj = lk[j];
       zc* aj = a + (long long)j * n;
-      for (int i = k + 1 + lane; i < n; i += 64) {
-        const zc li = lk[i];
-        zc x = aj[i];
+      auto upd = [&](zc x, zc li) {
         x.x -= li.x * lj.x + li.y * lj.y;   // li conj(lj)
         x.y -= li.y * lj.x - li.x * lj.y;
-        aj[i] = x;
+        return x;
+      };
+      int i = k + 1 + lane;
+      for (; i + 192 < n; i += 256) {
+        const zc x0 = aj[i], x1 = aj[i + 64], x2 = aj[i + 128], x3 = aj[i + 192];
+        aj[i] = upd(x0, lk[i]);
+        aj[i + 64] = upd(x1, lk[i + 64]);
+        aj[i + 128] = upd(x2, lk[i + 128]);
+        aj[i + 192] = upd(x3, lk[i + 192]);
       }
+      for (; i < n; i += 64) aj[i] = upd(aj[i], lk[i]);
     }
     __syncthreads();
   }
@@ -477,7 +488,7 @@ __global__ __launch_bounds__(kCholThreads) void zpchol_kernel(int n, zc* __restr
 }  // namespace
 
 int zpchol_launch(int n, zc* A, zc* F, int* perm, int* bad, int batch, hipStream_t s) {
-  if (n <= 0 || batch <= 0) return Status::BAD_ARG;
+  if (n <= 0 || n > kCholMaxN || batch <= 0) return Status::BAD_ARG;
   hipLaunchKernelGGL(zpchol_kernel, dim3(batch), dim3(kCholThreads), 0, s, n, A, F, perm, bad);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
