@@ -54,6 +54,9 @@ def parse():
                     help="rgf_keldysh: the headline workload; ddrgf_uneven: three uneven devices "
                          "by DD-RGF (8 partitions each) vs RGF")
     ap.add_argument("--dd-energies", type=int, default=0, help="energies per device (ddrgf_uneven)")
+    ap.add_argument("--match-wave", type=int, default=16,
+                    help="ddrgf_uneven: energies per device and call of the matched-error leg "
+                         "when it runs RGF on every device")
     ap.add_argument("--chunk", type=int, default=64,
                     help="energies per call with resident outputs (mixed)")
     ap.add_argument("--save-profile", action="store_true",
@@ -453,15 +456,19 @@ def ddrgf_uneven(a, torch, api):
     named split, blockSizeD1 = 15 / 7 / 3), plain RGF, and the tuner's choice
     (the tune table's entry for (N, b, wave), else timed now).  Then all three
     devices with their tuned methods at once: eagerly on three streams, and
-    as one negf_bndiag_devices call per wave."""
+    as one negf_bndiag_devices call per wave.  At matched error (every
+    energy within 1e-8 of FP64 RGF), FP64 and three-product products, in
+    waves of --match-wave when RGF is every device's matched method."""
     import numpy as np
     from libnegf_amd import synthetic, ddrgf as dd, tune
     from libnegf_amd.flops import flops_rgf as F_rgf
     from libnegf_amd._lib import call
     cfg = synthetic.CONFIGS["ddrgf_uneven"]
     nE, eta, wave = a.dd_energies or cfg["n_energies"], cfg["eta"], a.wave or 8
-    if nE % wave:
-        raise SystemExit("--dd-energies must be a multiple of the wave")
+    mwave = a.match_wave
+    if nE % wave or nE % mwave:
+        raise SystemExit("--dd-energies must be a multiple of the waves")
+    W = max(wave, mwave)   # buffers for the larger wave; a wave of w uses the first w rows
     Es = np.linspace(-1.0, 1.0, nE)
     p = lambda t: C.c_void_p(t.data_ptr())
 
@@ -480,15 +487,15 @@ def ddrgf_uneven(a, torch, api):
             self.low = (-up.transpose(1, 2).conj()).reshape(1, -1).contiguous()
             r = torch.arange(b, device="cuda")
             self.didx = (torch.arange(N, device="cuda")[:, None] * bb + r[None, :] * (b + 1)).reshape(-1)
-            self.Ad = torch.empty((wave, N * bb), dtype=torch.complex128, device="cuda")
+            self.Ad = torch.empty((W, N * bb), dtype=torch.complex128, device="cuda")
             self.Gd = torch.empty_like(self.Ad)
-            self.Gu = torch.empty((wave, (N - 1) * bb), dtype=torch.complex128, device="cuda")
+            self.Gu = torch.empty((W, (N - 1) * bb), dtype=torch.complex128, device="cuda")
             self.Gl = torch.empty_like(self.Gu)
-            self.info = torch.zeros(wave, dtype=torch.int32, device="cuda")
-            self.bad = torch.zeros(wave, dtype=torch.int32, device="cuda")
+            self.info = torch.zeros(W, dtype=torch.int32, device="cuda")
+            self.bad = torch.zeros(W, dtype=torch.int32, device="cuda")
             self.sizes = (C.c_int32 * N)(*([b] * N))
             self.z = torch.tensor(Es + 1j * eta, dtype=torch.complex128, device="cuda")
-            self.zw = torch.empty(wave, dtype=torch.complex128, device="cuda")
+            self.zw = torch.empty(W, dtype=torch.complex128, device="cuda")
             self.levels8 = dd.levels_for_tasks(N, cfg["parts"])
             tuned = tune.lookup("bndiag", (N, b, wave))
             self.tune_source = "table" if tuned is not None else "timed now"
@@ -497,19 +504,23 @@ def ddrgf_uneven(a, torch, api):
                 tuned, self.tune_times = tune.tune_bndiag([b] * N, wave, persist=False)
             self.tuned = tuned
 
-        def assemble(self):
-            """A(E) of the wave from self.zw: -H' everywhere, + z on the diagonal."""
-            self.Ad.copy_(self.minus_Hp.expand(wave, -1))
-            self.Ad[:, self.didx] += self.zw[:, None]
+        def assemble(self, w=wave):
+            """A(E) of a wave of w from self.zw: -H' everywhere, + z on the
+            diagonal."""
+            self.Ad[:w].copy_(self.minus_Hp.expand(w, -1))
+            self.Ad[:w, self.didx] += self.zw[:w, None]
 
-        def job(self, levels):
-            """This device's wave as a NegfBandJob (levels [0]: RGF)."""
+        def job(self, levels, w=wave):
+            """This device's wave of w as a NegfBandJob (levels [0]: RGF)."""
             from libnegf_amd._lib import NegfBandJob
             self._lv = (C.c_int32 * len(levels))(*levels)
             return NegfBandJob(self.N, C.cast(self.sizes, C.POINTER(C.c_int32)), p(self.Ad),
                                self.N * self.b * self.b, p(self.up), 0, p(self.low), 0,
-                               p(self.Gd), p(self.Gu), p(self.Gl), wave,
+                               p(self.Gd), p(self.Gu), p(self.Gl), w,
                                C.cast(self._lv, C.POINTER(C.c_int32)), len(levels), p(self.info))
+
+        def fold_info(self, w=wave):
+            torch.maximum(self.bad[:w], self.info[:w].abs(), out=self.bad[:w])
 
         def body(self, levels, stream):
             """Assembly from self.zw and the solve of one wave (no host
@@ -525,10 +536,10 @@ def ddrgf_uneven(a, torch, api):
                 call("negf_ddrgf", self.N, self.b, p(self.Ad), self.N * self.b * self.b,
                      p(self.up), 0, p(self.low), 0, p(self.Gd), p(self.Gu), p(self.Gl), wave, lv,
                      len(levels), p(self.info), s)
-            torch.maximum(self.bad, self.info.abs(), out=self.bad)
+            self.fold_info()
 
         def run_wave(self, e0, levels, stream):
-            self.zw.copy_(self.z[e0:e0 + wave])
+            self.zw[:wave].copy_(self.z[e0:e0 + wave])
             self.body(levels, stream)
 
         def check(self, what):
@@ -581,12 +592,12 @@ def ddrgf_uneven(a, torch, api):
 
     def one_call_wave(e0, prec=0):
         for d in devs:
-            d.zw.copy_(d.z[e0:e0 + wave])
+            d.zw[:wave].copy_(d.z[e0:e0 + wave])
             d.assemble()
         call("negf_bndiag_devices_prec", len(devs), C.cast(jobs, C.c_void_p), prec,
              C.c_void_p(s_cur.cuda_stream))
         for d in devs:
-            torch.maximum(d.bad, d.info.abs(), out=d.bad)
+            d.fold_info()
 
     one_call_wave(0)                                   # warmup (the library's streams)
     torch.cuda.synchronize()
@@ -597,34 +608,35 @@ def ddrgf_uneven(a, torch, api):
     dt_one = time.perf_counter() - t0
     for d in devs:
         d.check("one call")
-    Gd_one = [d.Gd.clone() for d in devs]
+    Gd_one = [d.Gd[:wave].cpu() for d in devs]   # host: the device holds the workspaces
 
     # accuracy of each device's method over all its energies, untimed: the
     # band of every wave against plain FP64 RGF (the stable method; DD-RGF
     # without pivoting across its separators amplifies rounding at eta = 1e-4)
-    def band_err(G, R, d):
+    def band_err(G, R, d, w):
         worst = 0.0
         for X, Y, nb in zip(G, R, (d.N, d.N - 1, d.N - 1)):
-            x = X.reshape(wave, nb, -1)
-            y = Y.reshape(wave, nb, -1)
+            x = X[:w].reshape(w, nb, -1)
+            y = Y[:w].reshape(w, nb, -1)
             rel = torch.linalg.vector_norm(x - y, dim=2) / torch.linalg.vector_norm(y, dim=2)
             worst = max(worst, float(rel.max()))
         return worst
 
-    def method_errors(levels_of, prec):
+    def method_errors(levels_of, prec, w=wave):
         worst = [0.0] * len(devs)
-        for e0 in range(0, nE, wave):
+        for e0 in range(0, nE, w):
             for k, d in enumerate(devs):
-                d.zw.copy_(d.z[e0:e0 + wave])
-                d.assemble()
+                d.zw[:w].copy_(d.z[e0:e0 + w])
+                d.assemble(w)
                 call("negf_rgf", d.N, d.sizes, p(d.Ad), d.N * d.b * d.b, p(d.up), 0, p(d.low), 0,
-                     p(d.Gd), p(d.Gu), p(d.Gl), wave, p(d.info), C.c_void_p(s_cur.cuda_stream))
-                R = (d.Gd.clone(), d.Gu.clone(), d.Gl.clone())
-                one = (NegfBandJob * 1)(d.job(levels_of[k]))
+                     p(d.Gd), p(d.Gu), p(d.Gl), w, p(d.info), C.c_void_p(s_cur.cuda_stream))
+                R = (d.Gd[:w].clone(), d.Gu[:w].clone(), d.Gl[:w].clone())
+                one = (NegfBandJob * 1)(d.job(levels_of[k], w))
                 call("negf_bndiag_devices_prec", 1, C.cast(one, C.c_void_p), prec,
                      C.c_void_p(s_cur.cuda_stream))
-                torch.maximum(d.bad, d.info.abs(), out=d.bad)
-                worst[k] = max(worst[k], band_err((d.Gd, d.Gu, d.Gl), R, d))
+                d.fold_info(w)
+                worst[k] = max(worst[k], band_err((d.Gd, d.Gu, d.Gl), R, d, w))
+                del R
         torch.cuda.synchronize()
         return worst
 
@@ -632,30 +644,38 @@ def ddrgf_uneven(a, torch, api):
     # matched error: each device's tuned method where it stays within 1e-8 of
     # RGF on every energy, else RGF
     matched = [d.tuned if e <= 1e-8 else [0] for d, e in zip(devs, tuned_err)]
-    jobs_m = (NegfBandJob * len(devs))(*[d.job(lv) for d, lv in zip(devs, matched)])
+    # RGF batches as far as memory allows (the named DD-RGF's per-partition
+    # column buffers hold its waves to 8): the matched leg takes waves of
+    # --match-wave when every device's matched method is RGF
+    w_m = mwave if all(m[0] == 0 for m in matched) else wave
+    api.release_workspace()
+    torch.cuda.empty_cache()
+    jobs_m = (NegfBandJob * len(devs))(*[d.job(lv, w_m) for d, lv in zip(devs, matched)])
 
-    def timed_one_call(job_arr, prec):
+    def timed_one_call(job_arr, prec, w=wave):
         def wave_call(e0):
             for d in devs:
-                d.zw.copy_(d.z[e0:e0 + wave])
-                d.assemble()
+                d.zw[:w].copy_(d.z[e0:e0 + w])
+                d.assemble(w)
             call("negf_bndiag_devices_prec", len(devs), C.cast(job_arr, C.c_void_p), prec,
                  C.c_void_p(s_cur.cuda_stream))
             for d in devs:
-                torch.maximum(d.bad, d.info.abs(), out=d.bad)
+                d.fold_info(w)
         wave_call(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for e0 in range(0, nE, wave):
+        for e0 in range(0, nE, w):
             wave_call(e0)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    dt_m64 = timed_one_call(jobs_m, 0)
-    dt_m3 = timed_one_call(jobs_m, 3)
+    dt_m64 = timed_one_call(jobs_m, 0, w_m)
+    dt_m3 = timed_one_call(jobs_m, 3, w_m)
     for d in devs:
         d.check("matched one call")
-    m3_err = method_errors(matched, 3)
+    m3_err = method_errors(matched, 3, w_m)
+    api.release_workspace()
+    torch.cuda.empty_cache()
     for d in devs:
         d.check("accuracy passes")
     # the tuned methods in the three-product form, for the record
@@ -672,7 +692,7 @@ def ddrgf_uneven(a, torch, api):
         with torch.cuda.stream(st):
             d.run_wave(nE - wave, d.tuned, st)
     torch.cuda.synchronize()
-    same = all(bool(torch.equal(g, d.Gd)) for g, d in zip(Gd_one, devs))
+    same = all(bool(torch.equal(g, d.Gd[:wave].cpu())) for g, d in zip(Gd_one, devs))
     print(f"# three devices, one call per wave: {dt_one:.3f} s (== separate calls: {same})",
           file=sys.stderr, flush=True)
 
@@ -696,8 +716,10 @@ def ddrgf_uneven(a, torch, api):
                                  "fp64_device_energies_per_s": round(3 * nE / dt_m64, 3),
                                  "three_product_seconds": round(dt_m3, 3),
                                  "three_product_device_energies_per_s": round(3 * nE / dt_m3, 3),
+                                 "wave": w_m,
                                  "rule": "per device its tuned method where every energy's band "
-                                         "stays within 1e-8 of FP64 RGF, else RGF"},
+                                         "stays within 1e-8 of FP64 RGF, else RGF; waves of "
+                                         "--match-wave when every device's is RGF"},
                              "max_block_relerr_vs_fp64_rgf_all_energies": err}}
     print(json.dumps(out))
 
