@@ -159,6 +159,8 @@ struct InvWork {
   int batchmax; // rejects larger problems
   InvAux aux;   // null members: the per-calling-stream slot
   int inplace_only;  // 1: never the split steps (no second stream is used)
+  double* fro;       // null, or fro[inst * ceil(n / 64) + c] = sum of |x|^2 over
+                     // columns 64 c .. 64 c + 63 of instance inst's inverse
 };
 size_t zinv_work_bytes(int n, int batch);
 // Elements of one W buffer per instance that serve every block size <= n.

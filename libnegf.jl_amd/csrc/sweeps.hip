@@ -28,7 +28,6 @@ namespace negf {
 
 namespace {
 
-inline zc Z(double r, double i = 0.0) { return zmake(r, i); }
 const zc ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
 
 struct M {           // one block position in a batch: base pointer, ld, batch stride
@@ -130,25 +129,6 @@ __global__ void energy_scalars_kernel(int n, const double* E, double eta, double
   sfR[i] = zmake(sqrt(r), 0.0);
 }
 
-// nrm[e * N + i] = ||g_i||_F for the b x b blocks g_i of instance e (one
-// workgroup per (layer, instance); fixed summation order).
-__global__ __launch_bounds__(256) void block_fro_kernel(const zc* g, long long gbs, int b, int N,
-                                                         double* nrm) {
-  const int i = blockIdx.x, e = blockIdx.y;
-  const zc* x = g + (long long)e * gbs + (long long)i * b * b;
-  const long long n = (long long)b * b;
-  double acc = 0.0;
-  for (long long k = threadIdx.x; k < n; k += 256) acc = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, acc));
-  __shared__ double red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) nrm[(long long)e * N + i] = sqrt(red[0]);
-}
-
 // nrm[e * N + i] = ||g_i||_F for one layer i of any size (one workgroup per instance).
 __global__ __launch_bounds__(256) void layer_fro_kernel(const zc* g, long long gbs, int bi, int i,
                                                          int N, double* nrm) {
@@ -165,6 +145,22 @@ __global__ __launch_bounds__(256) void layer_fro_kernel(const zc* g, long long g
     __syncthreads();
   }
   if (threadIdx.x == 0) nrm[(long long)e * N + i] = sqrt(red[0]);
+}
+
+// est[e] = c sum_i sqrt(sum_c part[(i * nb + e) * nch + c]): the Frobenius
+// norms of the inverses g_i from the partial sums their final pass wrote
+// (InvWork::fro), layer by layer in a fixed order.
+__global__ void fro_estimate_kernel(const double* part, int N, int nb, int nch, double c,
+                                    double* est) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nb) return;
+  double s = 0.0;
+  for (int i = 0; i < N; ++i) {
+    double q = 0.0;
+    for (int k = 0; k < nch; ++k) q += part[((long long)i * nb + e) * nch + k];
+    s += sqrt(q);
+  }
+  est[e] = c * s;
 }
 
 // est[e] = c sum_i nrm[e * N + i]
@@ -224,6 +220,8 @@ struct SweepWork {
   zc* ones;   // 1 element (the shift that adds I)
   int* linfo; // batch
   InvWork inv;
+  double* fro;  // null, or per layer i the inverses' partial sums of |g_i|^2
+                // (InvWork::fro) at fro + i * batch * ceil(bmax / 64)
 };
 
 size_t sweep_work_bytes(const Layout& L, int batch, bool keldysh) {
@@ -262,6 +260,7 @@ static void carve(void* base, const Layout& L, int batch, SweepWork* w) {
   w->ones = (zc*)take(zb);
   w->linfo = (int*)take(sizeof(int) * (size_t)batch);
   zinv_work_carve(p, L.bmax, batch, &w->inv);
+  w->fro = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -285,11 +284,12 @@ static int upward(const Layout& L, const Source& S, const SweepWork& w, int batc
                   hipStream_t s) {
   const int N = L.N;
   const long long gbs = L.nd, obs = L.no;
-  const long long xbs = (long long)L.bmax * L.bmax;
   hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
   auto diagM = [&](int i) { return M{S.diag + L.od[i], L.b[i], S.diag_bs}; };
   auto invert = [&](int i, const zc* Xsrc, long long ldx, long long xs) -> int {
-    RC(zinv_launch(L.b[i], Xsrc, ldx, xs, w.g + L.od[i], L.b[i], gbs, batch, w.inv, w.linfo, s));
+    InvWork iw = w.inv;
+    if (w.fro) iw.fro = w.fro + (size_t)i * batch * ((L.bmax + 63) / 64);
+    RC(zinv_launch(L.b[i], Xsrc, ldx, xs, w.g + L.od[i], L.b[i], gbs, batch, iw, w.linfo, s));
     hipLaunchKernelGGL(fold_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, w.linfo, info,
                        i, batch);
     return Status::OK;
@@ -868,7 +868,10 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
   // (the factored Keldysh pass's G^< product has k = 2 bmax)
   if (prec == 2)
     emu_bytes = zgemm_emu_work_bytes(L.bmax, L.bmax, 2 * L.bmax, wave, wave, pr->nmod);
+  // the inverses' partial sums of |g_i|^2, written by their final pass
   double* nrm = (double*)(extra + (emu_bytes + 255) / 256 * 256);
+  const int nch = (L.bmax + 63) / 64;
+  if (pr && pr->est) w.fro = nrm;
   PrecScope scope(PrecCtx{prec, pr ? pr->nmod : 0, prec == 2 ? (void*)extra : nullptr,
                           (long long)emu_bytes});
   // est = 8 u sum_i ||g_i||_F: measured on the 512-energy grid at N = 64,
@@ -890,11 +893,9 @@ static int energies_group(const Layout& L, const EnergyIn& in, const EnergyOut& 
     if (job.gbad)
       hipLaunchKernelGGL(gamma_info_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, job.gbad,
                          info, nb);
-    if (pr && pr->est) {
-      hipLaunchKernelGGL(block_fro_kernel, dim3(N, nb), dim3(256), 0, s, w.g, L.nd, b, N, nrm);
-      hipLaunchKernelGGL(energy_estimate_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, nrm, N, nb,
+    if (pr && pr->est)
+      hipLaunchKernelGGL(fro_estimate_kernel, dim3((nb + 63) / 64), dim3(64), 0, s, nrm, N, nb, nch,
                          est_c, pr->est + e0);
-    }
     // G^R diagonal blocks (the Keldysh pass reads them)
     zc* grd = out.GR_diag ? out.GR_diag + (size_t)e0 * L.nd : Gdw;
     RC(downward(L, w, nb, grd, L.nd, nullptr, nullptr, 0, s));
@@ -917,7 +918,7 @@ size_t energies_group_bytes(const Layout& L, int wave, const EnergyPrec* pr) {
   if (pr && pr->prec == 2)
     b += (zgemm_emu_work_bytes(L.bmax, L.bmax, 2 * L.bmax, wave, wave, pr->nmod) + 255) / 256 *
          256;
-  if (pr && pr->est) b += sizeof(double) * (size_t)wave * L.N + 256;
+  if (pr && pr->est) b += sizeof(double) * (size_t)wave * L.N * ((L.bmax + 63) / 64) + 256;
   return b;
 }
 

@@ -388,12 +388,16 @@ __global__ void gj_init_kernel(int n, int* used, int* info, int batch) {
   }
 }
 
-// out[r, c] = X[p_r, q_c], q = p^{-1}; grid.x = column chunks of 64, grid.y = instance
+// out[r, c] = X[p_r, q_c], q = p^{-1}; grid.x = column chunks of 64, grid.y =
+// instance.  With fro: fro[inst * gridDim.x + chunk] = sum of |out|^2 over the
+// chunk (fixed order).
 __global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __restrict__ X,
                                                             const int* __restrict__ piv,
                                                             zc* __restrict__ out, long long ldo,
-                                                            long long strideO) {
+                                                            long long strideO,
+                                                            double* __restrict__ fro) {
   extern __shared__ int sh[];
+  __shared__ double red[256];
   int* p = sh;
   int* qinv = sh + n;
   const int inst = blockIdx.y;
@@ -406,9 +410,23 @@ __global__ __launch_bounds__(256) void gj_unscramble_kernel(int n, const zc* __r
   zc* Oi = out + inst * strideO;
   const int c0 = blockIdx.x * 64;
   const int c1 = (c0 + 64 < n) ? c0 + 64 : n;
+  double acc = 0.0;
   for (int c = c0; c < c1; ++c) {
     const zc* col = Xi + (long long)qinv[c] * n;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) Oi[r + c * ldo] = col[p[r]];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const zc v = col[p[r]];
+      Oi[r + c * ldo] = v;
+      acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+    }
+  }
+  if (fro) {   // uniform over the workgroup
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) fro[(long long)inst * gridDim.x + blockIdx.x] = red[0];
   }
 }
 
@@ -877,7 +895,7 @@ int run_split(int n, const zc* A, long long lda, long long strideA, zc* Ainv, lo
   if (hipEventRecord(ax->join, ax->aux) != hipSuccess || hipStreamWaitEvent(s, ax->join, 0) != hipSuccess)
     return Status::HIP_ERROR;
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
-                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
+                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv, w.fro);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
@@ -914,7 +932,7 @@ int run(int n, const zc* A, long long lda, long long strideA, zc* Ainv, long lon
                        dim3(PT), 0, s, n, K, ns, batch, w.X, Wc, Wn, w.piv, w.used, info, 0);
   }
   hipLaunchKernelGGL(gj_unscramble_kernel, dim3((n + 63) / 64, batch), dim3(256),
-                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv);
+                     2 * n * sizeof(int), s, n, w.X, w.piv, Ainv, ldinv, strideInv, w.fro);
   return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
 }
 
@@ -1016,6 +1034,7 @@ void zinv_work_carve(void* base, int n, int batch, InvWork* w) {
   w->batchmax = batch;
   w->aux = InvAux{nullptr, nullptr, nullptr};
   w->inplace_only = 0;
+  w->fro = nullptr;
 }
 
 int zinv_schedule(int n, int batch, int variant, int* strip, int* panel, int* nbuf) {
