@@ -293,7 +293,8 @@ def single_energy(a, torch, api):
     device with N=64, b=512, E=0.3, eta=1e-4, one call of negf_rgf (band of
     A^{-1}) and one selected inverse (Block_factorization + Block_inverse on
     the tridiagonal pattern) of the same matrix, each timed over reps after a
-    warm-up; RGF also at 4 / 16 / 64 copies per call.  Errors: RGF vs the
+    warm-up; RGF also at 4 / 16 / 64 copies per call (the automatic schedule:
+    two-sided up to 16 instances) and at one copy one-sided.  Errors: RGF vs the
     oracle's RGF on the host, and the selected inverse vs the GPU RGF; the host
     RGF's time is the CPU baseline."""
     import numpy as np
@@ -332,6 +333,15 @@ def single_energy(a, torch, api):
         print(f"# rgf batch {batch}: {t * 1e3:.1f} ms", file=sys.stderr, flush=True)
         if batch == 1:
             G1 = Mout.to_host(0)
+            # the one-sided recursion (negf_rgf_schedule 1) at the same call
+            from libnegf_amd._lib import call as lcall
+            lcall("negf_rgf_schedule", 1)
+            try:
+                t1 = timed(lambda: api.bndiag_of_inv_rgf_local(Mout, Min, aux))
+            finally:
+                lcall("negf_rgf_schedule", 0)
+            rgf["1_one_sided"] = {"ms_per_call": round(t1 * 1e3, 2)}
+            print(f"# rgf batch 1 one-sided: {t1 * 1e3:.1f} ms", file=sys.stderr, flush=True)
         del Min, Mout, aux
         torch.cuda.empty_cache()
     pat = [(i, i) for i in range(N)] + [(i, i + 1) for i in range(N - 1)] + \

@@ -74,6 +74,16 @@ int negf_gemm_variant(int32_t variant);
  * Wider panels: fewer passes over the matrix, more latency-bound factoring
  * per pass. */
 int negf_inv_variant(int32_t variant);
+/* Autotuner knob: schedule of the band sweep negf_rgf (and negf_rgf_noalloc,
+ * negf_bndiag_devices' RGF devices), process-wide.  0: automatic (default):
+ * the two-sided recursion for band(A^{-1}) alone on a chain of uniform
+ * blocks, N >= 4, at most 16 instances -- the upward recursion from both
+ * ends at once as 2 x batch instances of N/2 + 1 layers, meeting at layer
+ * N/2 (half as many dependent inversions, the same RGF identities on each
+ * side); 1: the one-sided recursion of bndiag_of_inv_rgf_local!
+ * (src/matinvertndiag_rgf.jl:114-200) always; 2: two-sided wherever the
+ * chain allows, any batch.  Other values: NEGF_BAD_ARG. */
+int negf_rgf_schedule(int32_t schedule);
 /* The schedule the inverse runs for (n, batch, variant): out[0] = column strip
  * width, out[1] = strips per panel, out[2] = W buffers of n * out[0] * out[1]
  * elements per instance.  Returns 1 for split steps, 0 in place, < 0 for bad

@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 using namespace negf;
@@ -98,6 +99,12 @@ const char* negf_status_string(int status) {
 
 int negf_gemm_variant(int32_t variant) { return zgemm_set_variant(variant); }
 int negf_inv_variant(int32_t variant) { return zinv_set_variant(variant); }
+
+int negf_rgf_schedule(int32_t schedule) {
+  if (schedule < 0 || schedule > 2) return NEGF_BAD_ARG;
+  g_rgf_schedule = schedule;
+  return NEGF_OK;
+}
 
 int negf_zinv_schedule(int32_t n, int32_t batch, int32_t variant, int32_t out[3]) {
   if (!out) return NEGF_BAD_ARG;
@@ -411,8 +418,11 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes, const void* diag,
   if (want_T && (!gamma_L || !gamma_R)) return NEGF_BAD_ARG;
   Layout L(nblocks, block_sizes);
   void* w = nullptr;
+  // band(A^{-1}) alone may take the two-sided recursion (its own workspace)
+  const bool band_only = !want_rkd && !want_T;
   // the per-sweep info accumulator lives behind the sweep workspace
-  const size_t sw = sweep_work_bytes(L, batch, want_rkd);
+  size_t sw = (sweep_work_bytes(L, batch, want_rkd) + 255) / 256 * 256;
+  if (band_only) sw = std::max(sw, (sweep_rgf_two_sided_bytes(L, batch) + 255) / 256 * 256);
   int rc = workspace(stream, sw + sizeof(int32_t) * (size_t)batch + 256, &w);
   if (rc) return rc;
   int32_t* inf = info ? info : (int32_t*)((char*)w + sw);
@@ -424,7 +434,8 @@ int negf_rgf_rkd(int32_t nblocks, const int32_t* block_sizes, const void* diag,
   BtdOut Gn{(zc*)gn_diag, (zc*)gn_upper, nullptr};
   GammaIn gam{(const zc*)gamma_L, gamma_L_bstride, (const zc*)gamma_R, gamma_R_bstride};
   return sweep_rgf(L, A, G, want_rkd ? &S : nullptr, want_rkd ? &Gn : nullptr,
-                   want_T ? &gam : nullptr, transmission, batch, inf, w, st(stream));
+                   want_T ? &gam : nullptr, transmission, batch, inf, w, st(stream), false,
+                   band_only);
 }
 
 int negf_rgf_rkd_mixed(int32_t nblocks, const int32_t* block_sizes, const void* diag,

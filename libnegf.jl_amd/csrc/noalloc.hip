@@ -69,7 +69,9 @@ int negf_rgf_noalloc(int32_t nblocks, const int32_t* block_sizes, const void* di
   if (!diag || !g_diag || (nblocks > 1 && !upper)) return NEGF_BAD_ARG;
   Layout L(nblocks, block_sizes);
   // the sweep's workspace, then the info accumulator when the caller has none
-  const size_t sw = (sweep_work_bytes(L, batch, false) + 255) / 256 * 256;
+  size_t sw = (sweep_work_bytes(L, batch, false) + 255) / 256 * 256;
+  const size_t sw2 = (sweep_rgf_two_sided_bytes(L, batch) + 255) / 256 * 256;
+  if (sw2 > sw) sw = sw2;
   const size_t need = sw + sizeof(int32_t) * (size_t)batch;
   if (need > kScratchBytes) return NEGF_TOO_LARGE;
   void* w = nullptr;
@@ -80,7 +82,7 @@ int negf_rgf_noalloc(int32_t nblocks, const int32_t* block_sizes, const void* di
           lower_bstride};
   BtdOut G{(zc*)g_diag, (zc*)g_upper, (zc*)g_lower};
   return sweep_rgf(L, A, G, nullptr, nullptr, nullptr, nullptr, batch, inf, w,
-                   (hipStream_t)stream, /*inplace_inverse=*/true);
+                   (hipStream_t)stream, /*inplace_inverse=*/true, /*two_sided=*/true);
 }
 
 }  // extern "C"

@@ -280,8 +280,9 @@ struct Source {
   zc sUp;      // A_{i,i+1} = sUp * up
 };
 
+// skip_top: stop after forming X = S_0 (in w.inv.X), before its inversion.
 static int upward(const Layout& L, const Source& S, const SweepWork& w, int batch, int* info,
-                  hipStream_t s) {
+                  hipStream_t s, bool skip_top = false) {
   const int N = L.N;
   const long long gbs = L.nd, obs = L.no;
   hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
@@ -338,6 +339,7 @@ static int upward(const Layout& L, const Source& S, const SweepWork& w, int batc
     } else {
       RC(gemm(OP_N, OP_N, bi, bi, bj, a, up, ro(l), ONE, diagM(i), X, batch, s));
     }
+    if (i == 0 && skip_top) break;
     RC(invert(i, w.inv.X, bi, (long long)bi * bi));
   }
   return Status::OK;
@@ -550,11 +552,231 @@ static int rkd_factored(const Layout& L, const SweepWork& w, int batch, zc* Ed, 
 }
 
 // ---------------------------------------------------------------------------
+// Two-sided RGF: band(A^{-1}) of a chain of uniform blocks with the upward
+// recursion run from both ends at once.  With m = N / 2, the right part
+// (layers m .. N-1) is the usual recursion (g_i connected to the layers
+// above i) and the left part, layers m, m-1, .. 0 read in reverse with its
+// couplings swapped, the same recursion on the mirrored chain (g_i connected
+// to the layers below i).  Both parts go into one batch of 2B instances of
+// n2 = N / 2 + 1 layers (the shorter padded at its far end by an identity
+// layer with zero coupling, which changes nothing), so the dependent
+// inversions number n2 instead of N.  At layer m both Schur terms meet:
+// S_m = X_right + X_left - A_mm (each X = A_mm minus its side's term), whose
+// inverse is G_mm, seeds both halves, and the usual downward pass runs
+// outward in both at once.  The same RGF identities on each side (no
+// pivoting across blocks is introduced), rounding in a different order.
+
+int g_rgf_schedule = 0;   // 0 automatic, 1 one-sided always, 2 two-sided wherever it applies
+constexpr int kTwoSidedMaxBatch = 16;
+
+bool two_sided_applies(const Layout& L, int batch) {
+  if (g_rgf_schedule == 1 || L.N < 4) return false;
+  for (int i = 1; i < L.N; ++i)
+    if (L.b[i] != L.b[0]) return false;
+  return g_rgf_schedule == 2 || batch <= kTwoSidedMaxBatch;
+}
+
+namespace {
+
+struct TwoSidedShape {
+  int N, m, n2, b, B;
+  long long bb, pbs;   // block elements; elements of one packed instance (3 n2 - 2 blocks)
+  TwoSidedShape(const Layout& L, int batch)
+      : N(L.N), m(L.N / 2), n2(L.N / 2 + 1), b(L.b[0]), B(batch),
+        bb((long long)L.b[0] * L.b[0]), pbs((3LL * (L.N / 2 + 1) - 2) * bb) {}
+};
+
+// The 2B half-chains, per instance [diag n2][up n2-1][low n2-1] blocks.
+// grid (3 n2 - 2 slots, 2B instances, column chunks of 16).
+__global__ __launch_bounds__(256) void two_sided_pack_kernel(int N, int m, int n2, int b, int B,
+                                                             const zc* __restrict__ Ad,
+                                                             long long dbs,
+                                                             const zc* __restrict__ Au,
+                                                             long long ubs,
+                                                             const zc* __restrict__ Al,
+                                                             long long lbs, zc* __restrict__ P) {
+  const int slot = blockIdx.x, k2 = blockIdx.y;
+  const bool left = k2 >= B;
+  const int k = left ? k2 - B : k2;
+  const long long bb = (long long)b * b;
+  zc* dst = P + (long long)k2 * (3LL * n2 - 2) * bb + (long long)slot * bb;
+  int mode = 0;   // 0 zero, 1 identity, 2 copy, 3 adjoint of src
+  const zc* src = nullptr;
+  if (slot < n2) {
+    const int layer = left ? m - slot : m + slot;
+    if (layer >= 0 && layer < N) {
+      mode = 2;
+      src = Ad + (long long)k * dbs + (long long)layer * bb;
+    } else {
+      mode = 1;
+    }
+  } else {
+    const bool up = slot < 2 * n2 - 1;
+    const int p = up ? slot - n2 : slot - (2 * n2 - 1);
+    // original coupling pair (i, i+1): right part positions p, p+1 are
+    // layers m+p, m+p+1 (i = m+p); left part they are m-p, m-p-1 (i = m-p-1),
+    // so its up block is A_{i+1,i} and its low block A_{i,i+1}
+    const int i = left ? m - p - 1 : m + p;
+    if (i >= 0 && i + 1 < N) {
+      const bool want_upper = up != left;   // A_{i,i+1} wanted
+      if (want_upper) {
+        mode = 2;
+        src = Au + (long long)k * ubs + (long long)i * bb;
+      } else if (Al) {
+        mode = 2;
+        src = Al + (long long)k * lbs + (long long)i * bb;
+      } else {   // Hermitian storage: A_{i+1,i} = A_{i,i+1}^H
+        mode = 3;
+        src = Au + (long long)k * ubs + (long long)i * bb;
+      }
+    }
+  }
+  const int c0 = blockIdx.z * 16, c1 = (c0 + 16) < b ? c0 + 16 : b;
+  for (int c = c0; c < c1; ++c)
+    for (int r = threadIdx.x; r < b; r += 256) {
+      zc v;
+      if (mode == 2) {
+        v = src[r + (long long)c * b];
+      } else if (mode == 3) {
+        const zc x = src[c + (long long)r * b];
+        v = zmake(x.x, -x.y);
+      } else {
+        v = zmake(mode == 1 && r == c ? 1.0 : 0.0, 0.0);
+      }
+      dst[r + (long long)c * b] = v;
+    }
+}
+
+// X[k] += X[B + k] - A_mm[k] (A_mm: position 0 of the packed right part).
+__global__ __launch_bounds__(256) void two_sided_combine_kernel(long long bb, int B, long long pbs,
+                                                                const zc* __restrict__ P,
+                                                                zc* __restrict__ X) {
+  const int k = blockIdx.y;
+  zc* x = X + (long long)k * bb;
+  const zc* xl = X + (long long)(B + k) * bb;
+  const zc* a = P + (long long)k * pbs;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < bb; e += 256LL * gridDim.x) {
+    const zc u = x[e], v = xl[e], w = a[e];
+    x[e] = zmake(u.x + v.x - w.x, u.y + v.y - w.y);
+  }
+}
+
+// The half-chains' band back into the caller's layout.  grid (3N - 2 slots:
+// diag N, up N-1, low N-1; B instances; column chunks of 16).
+__global__ __launch_bounds__(256) void two_sided_unpack_kernel(
+    int N, int m, int n2, int b, int B, const zc* __restrict__ Gd2, const zc* __restrict__ Gu2,
+    const zc* __restrict__ Gl2, zc* __restrict__ Gd, long long dbs, zc* __restrict__ Gu,
+    zc* __restrict__ Gl, long long obs) {
+  const int slot = blockIdx.x, k = blockIdx.y;
+  const long long bb = (long long)b * b;
+  const long long d2 = (long long)n2 * bb, o2 = (long long)(n2 - 1) * bb;
+  const zc* src;
+  zc* dst;
+  if (slot < N) {
+    const int i = slot;
+    src = i >= m ? Gd2 + (long long)k * d2 + (long long)(i - m) * bb
+                 : Gd2 + (long long)(B + k) * d2 + (long long)(m - i) * bb;
+    dst = Gd + (long long)k * dbs + (long long)i * bb;
+  } else if (slot < 2 * N - 1) {   // G_{i,i+1}
+    const int i = slot - N;
+    if (!Gu) return;
+    src = i >= m ? Gu2 + (long long)k * o2 + (long long)(i - m) * bb
+                 : Gl2 + (long long)(B + k) * o2 + (long long)(m - i - 1) * bb;
+    dst = Gu + (long long)k * obs + (long long)i * bb;
+  } else {                         // G_{i+1,i}
+    const int i = slot - (2 * N - 1);
+    if (!Gl) return;
+    src = i >= m ? Gl2 + (long long)k * o2 + (long long)(i - m) * bb
+                 : Gu2 + (long long)(B + k) * o2 + (long long)(m - i - 1) * bb;
+    dst = Gl + (long long)k * obs + (long long)i * bb;
+  }
+  const int c0 = blockIdx.z * 16, c1 = (c0 + 16) < b ? c0 + 16 : b;
+  for (int c = c0; c < c1; ++c)
+    for (int r = threadIdx.x; r < b; r += 256) dst[r + (long long)c * b] = src[r + (long long)c * b];
+}
+
+// info[k] = 1 + the original layer of instance k's first singular block
+// found (right part first), or 0.
+__global__ void two_sided_info_kernel(const int* __restrict__ info2, int m, int B,
+                                      int* __restrict__ info) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B) return;
+  const int r = info2[k], l = info2[B + k];
+  info[k] = r ? m + r : (l ? m - (l - 1) + 1 : 0);
+}
+
+size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+size_t sweep_rgf_two_sided_bytes(const Layout& L, int batch) {
+  if (!two_sided_applies(L, batch)) return 0;
+  const TwoSidedShape t(L, batch);
+  std::vector<int> sz(t.n2, t.b);
+  Layout Lh(t.n2, sz.data());
+  const size_t zb = sizeof(zc);
+  return up256(sweep_work_bytes(Lh, 2 * batch, false)) +
+         up256(zb * 2 * batch * (size_t)t.pbs) +               // packed halves
+         up256(zb * 2 * batch * (size_t)t.n2 * t.bb) +          // G_ii of both halves
+         2 * up256(zb * 2 * batch * (size_t)(t.n2 - 1) * t.bb) + // G_{i,i+1}, G_{i+1,i}
+         up256(sizeof(int) * 2 * (size_t)batch);
+}
+
+static int sweep_rgf_two_sided(const Layout& L, const BtdIn& A, const BtdOut& G, int batch,
+                               int* info, void* work, hipStream_t s, bool inplace_inverse) {
+  const TwoSidedShape t(L, batch);
+  const int B = batch, B2 = 2 * batch;
+  std::vector<int> sz(t.n2, t.b);
+  Layout Lh(t.n2, sz.data());
+  const size_t zb = sizeof(zc);
+  char* p = (char*)work;
+  SweepWork w;
+  carve(p, Lh, B2, &w);
+  w.inv.inplace_only = inplace_inverse ? 1 : 0;
+  p += up256(sweep_work_bytes(Lh, B2, false));
+  zc* P = (zc*)p;
+  p += up256(zb * B2 * (size_t)t.pbs);
+  zc* Gd2 = (zc*)p;
+  p += up256(zb * B2 * (size_t)t.n2 * t.bb);
+  zc* Gu2 = (zc*)p;
+  p += up256(zb * B2 * (size_t)(t.n2 - 1) * t.bb);
+  zc* Gl2 = (zc*)p;
+  p += up256(zb * B2 * (size_t)(t.n2 - 1) * t.bb);
+  int* info2 = (int*)p;
+  RC(set_one(w.ones, s));
+  const unsigned chunks = (unsigned)((t.b + 15) / 16);
+  hipLaunchKernelGGL(two_sided_pack_kernel, dim3(3 * t.n2 - 2, B2, chunks), dim3(256), 0, s, t.N,
+                     t.m, t.n2, t.b, B, A.diag, A.diag_bs, A.up, A.up_bs, A.low, A.low_bs, P);
+  Source S{};
+  S.diag = P; S.diag_bs = t.pbs;
+  S.up = P + (long long)t.n2 * t.bb; S.up_bs = t.pbs;
+  S.low = P + (2LL * t.n2 - 1) * t.bb; S.low_bs = t.pbs;
+  S.sLow = ONE; S.sUp = ONE;
+  RC(upward(Lh, S, w, B2, info2, s, /*skip_top=*/true));
+  // layer m: both sides' Schur terms, one inverse, G_mm into both halves
+  const long long blocks = (t.bb + 255) / 256;
+  hipLaunchKernelGGL(two_sided_combine_kernel, dim3((unsigned)(blocks < 1024 ? blocks : 1024), B),
+                     dim3(256), 0, s, t.bb, B, t.pbs, P, w.inv.X);
+  RC(zinv_launch(t.b, w.inv.X, t.b, t.bb, w.g, t.b, Lh.nd, B, w.inv, w.linfo, s));
+  hipLaunchKernelGGL(fold_info_kernel, dim3((B + 255) / 256), dim3(256), 0, s, w.linfo, info2, 0,
+                     B);
+  RC(zcopy2d_launch(t.b, t.b, w.g, t.b, Lh.nd, w.g + (long long)B * Lh.nd, t.b, Lh.nd, B, s));
+  RC(downward(Lh, w, B2, Gd2, Lh.nd, Gu2, G.low ? Gl2 : nullptr, Lh.no, s));
+  hipLaunchKernelGGL(two_sided_unpack_kernel, dim3(3 * t.N - 2, B, chunks), dim3(256), 0, s, t.N,
+                     t.m, t.n2, t.b, B, Gd2, Gu2, Gl2, G.diag, L.nd, G.up, G.low, L.no);
+  hipLaunchKernelGGL(two_sided_info_kernel, dim3((B + 255) / 256), dim3(256), 0, s, info2, t.m, B,
+                     info);
+  return hipGetLastError() == hipSuccess ? Status::OK : Status::HIP_ERROR;
+}
+
+// ---------------------------------------------------------------------------
 // Public sweep drivers (called by the C-ABI layer with validated arguments).
 
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
-              hipStream_t s, bool inplace_inverse) {
+              hipStream_t s, bool inplace_inverse, bool two_sided) {
+  if (two_sided && G.diag && !(sigma && gn) && !(gam && T) && two_sided_applies(L, batch))
+    return sweep_rgf_two_sided(L, A, G, batch, info, work, s, inplace_inverse);
   SweepWork w;
   carve(work, L, batch, &w);
   w.inv.inplace_only = inplace_inverse ? 1 : 0;

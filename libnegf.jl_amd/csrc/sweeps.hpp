@@ -82,9 +82,18 @@ int sweep_rgf_full(const Layout& L, const BtdIn& A, zc* Gfull, int batch, int* i
                    hipStream_t s);
 // inplace_inverse: the diagonal-block inverses never take the split steps
 // (which run on a second stream the library creates on first use).
+// two_sided: band(A^{-1}) alone (no Keldysh pass, no transmission) of a chain
+// of uniform blocks at small batch runs the two-sided recursion; the
+// workspace must then hold max(sweep_work_bytes, sweep_rgf_two_sided_bytes).
 int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* sigma,
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
-              hipStream_t s, bool inplace_inverse = false);
+              hipStream_t s, bool inplace_inverse = false, bool two_sided = false);
+// Whether the two-sided recursion applies to (L, batch) under the current
+// schedule (uniform blocks, N >= 4, batch <= 16 unless forced), and its
+// workspace (0 when it does not apply).
+extern int g_rgf_schedule;   // 0 automatic, 1 one-sided always, 2 two-sided wherever it applies
+bool two_sided_applies(const Layout& L, int batch);
+size_t sweep_rgf_two_sided_bytes(const Layout& L, int batch);
 // Streams and events of the energy job.  `pool`: the group streams (streams
 // > 1).  `aux`: one inverse aux set per group, or null for the library's
 // per-stream slots.  `events`: 1 + streams events (fork, then one join per

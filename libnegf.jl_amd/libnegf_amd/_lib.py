@@ -52,6 +52,7 @@ SIGNATURES = {
     "negf_workspace_release": (C.c_int, []),
     "negf_gemm_variant": (C.c_int, [i32]),
     "negf_inv_variant": (C.c_int, [i32]),
+    "negf_rgf_schedule": (C.c_int, [i32]),
     "negf_zinv_schedule": (C.c_int, [i32, i32, i32, c_int32_p]),
     "negf_zinv_workspace": (C.c_int64, [i32, i32]),
     "negf_zgemm": (C.c_int, [ch, ch, i32, i32, i32, c_double_p, vp, i64, vp, i64, c_double_p, vp,

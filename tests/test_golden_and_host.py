@@ -391,3 +391,15 @@ def test_leading_dimension_of_single_column_and_single_row_matrices():
     assert _ld(col_major(torch.zeros((7, 3), dtype=torch.complex128))) == 7
     with pytest.raises(ValueError):
         _ld(torch.zeros((7, 3), dtype=torch.complex128))      # row-major
+
+
+def test_rgf_schedule_knob_checks_its_argument():
+    """negf_rgf_schedule takes 0 (automatic), 1 (one-sided) or 2 (two-sided
+    wherever the chain allows); anything else is refused on the host."""
+    from libnegf_amd._lib import call, NegfError
+    with pytest.raises(NegfError):
+        call("negf_rgf_schedule", 3)
+    with pytest.raises(NegfError):
+        call("negf_rgf_schedule", -1)
+    call("negf_rgf_schedule", 1)
+    call("negf_rgf_schedule", 0)
