@@ -569,11 +569,15 @@ static int rkd_factored(const Layout& L, const SweepWork& w, int batch, zc* Ed, 
 int g_rgf_schedule = 0;   // 0 automatic, 1 one-sided always, 2 two-sided wherever it applies
 constexpr int kTwoSidedMaxBatch = 16;
 
+// Automatic: blocks up to 512 (where an inversion at a few instances is
+// latency-bound: at 1024 x 8 the two-sided form measured 354 against 344 ms
+// one-sided for the 32-layer chain, the 1024-wide inversion already filling
+// the GPU at 16 instances) and at most 16 instances.
 bool two_sided_applies(const Layout& L, int batch) {
   if (g_rgf_schedule == 1 || L.N < 4) return false;
   for (int i = 1; i < L.N; ++i)
     if (L.b[i] != L.b[0]) return false;
-  return g_rgf_schedule == 2 || batch <= kTwoSidedMaxBatch;
+  return g_rgf_schedule == 2 || (batch <= kTwoSidedMaxBatch && L.b[0] <= 512);
 }
 
 namespace {
