@@ -77,7 +77,8 @@ int negf_inv_variant(int32_t variant);
 /* Autotuner knob: schedule of the band sweep negf_rgf (and negf_rgf_noalloc,
  * negf_bndiag_devices' RGF devices), process-wide.  0: automatic (default):
  * the two-sided recursion for band(A^{-1}) alone on a chain of uniform
- * blocks of at most 512, N >= 4, at most 16 instances -- the upward recursion from both
+ * blocks, N >= 4, at most 16 instances of blocks up to 512 (4 up to 1024, 1
+ * above) -- the upward recursion from both
  * ends at once as 2 x batch instances of N/2 + 1 layers, meeting at layer
  * N/2 (half as many dependent inversions, the same RGF identities on each
  * side); 1: the one-sided recursion of bndiag_of_inv_rgf_local!

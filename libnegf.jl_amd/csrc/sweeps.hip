@@ -569,15 +569,17 @@ static int rkd_factored(const Layout& L, const SweepWork& w, int batch, zc* Ed, 
 int g_rgf_schedule = 0;   // 0 automatic, 1 one-sided always, 2 two-sided wherever it applies
 constexpr int kTwoSidedMaxBatch = 16;
 
-// Automatic: blocks up to 512 (where an inversion at a few instances is
-// latency-bound: at 1024 x 8 the two-sided form measured 354 against 344 ms
-// one-sided for the 32-layer chain, the 1024-wide inversion already filling
-// the GPU at 16 instances) and at most 16 instances.
+// Automatic while the inversions at 2B instances are still latency-bound:
+// up to 16 instances of blocks up to 512, 4 of blocks up to 1024, 1 above.
+// Measured (one-sided / two-sided, ms): 64 x 512 at 1, 4, 8 instances
+// 96.6 / 54.2, 118.3 / 76.4, 143.9 / 109.6; 128 x 256 at 8 87.1 / 54.2;
+// 32 x 1024 at 1, 4, 8 156.1 / 99.2, 230.4 / 186.0, 343.2 / 355.9.
 bool two_sided_applies(const Layout& L, int batch) {
   if (g_rgf_schedule == 1 || L.N < 4) return false;
   for (int i = 1; i < L.N; ++i)
     if (L.b[i] != L.b[0]) return false;
-  return g_rgf_schedule == 2 || (batch <= kTwoSidedMaxBatch && L.b[0] <= 512);
+  const int bmax_batch = L.b[0] <= 512 ? kTwoSidedMaxBatch : (L.b[0] <= 1024 ? 4 : 1);
+  return g_rgf_schedule == 2 || batch <= bmax_batch;
 }
 
 namespace {

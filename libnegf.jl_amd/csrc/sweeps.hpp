@@ -89,8 +89,8 @@ int sweep_rgf(const Layout& L, const BtdIn& A, const BtdOut& G, const BtdHerm* s
               const BtdOut* gn, const GammaIn* gam, double* T, int batch, int* info, void* work,
               hipStream_t s, bool inplace_inverse = false, bool two_sided = false);
 // Whether the two-sided recursion applies to (L, batch) under the current
-// schedule (uniform blocks, N >= 4, blocks <= 512 and batch <= 16 unless
-// forced), and its
+// schedule (uniform blocks, N >= 4, at most 16 instances of blocks <= 512,
+// 4 of blocks <= 1024, 1 above, unless forced), and its
 // workspace (0 when it does not apply).
 extern int g_rgf_schedule;   // 0 automatic, 1 one-sided always, 2 two-sided wherever it applies
 bool two_sided_applies(const Layout& L, int batch);
