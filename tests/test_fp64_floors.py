@@ -3,7 +3,7 @@ measured on the CPU (no GPU) on the same inputs.
 
 * RGF at N = 64, b = 512, E = 0.3, eta = 1e-4 (the single-energy drop-in's
   input): the oracle's LU-solve form and the explicit-inverse form differ by
-  2.9e-12 -- so the north star's 1e-12 per block is below what FP64 resolves
+  2.9e-12 -- so a 1e-12 bar per block is below what FP64 resolves
   on this problem, and the GPU test asserts a multiple of this spread.
 * N = 64, b = 256: 1.7e-12 (the floor grows with b).
 * DD-RGF vs RGF on the (N, b) = (128, 256) device of the uneven DD-RGF
